@@ -1,0 +1,182 @@
+"""Benchmark: 26-qubit complex128 state-vector bandwidth circuit on one MI355X.
+
+Workload: n=26 state vector (2^26 amplitudes = 1 GiB), depth-20 circuit; each
+layer is a Haar-random 2x2 unitary on every qubit, then CZ on (1,2),(3,4),...
+in even layers and (2,3),(4,5),... in odd layers: 520 dense one-qubit gates and
+250 CZ = 770 gates.  One step = one full pass of the circuit over the resident
+state.  The state lives in HBM before the timed region starts.
+
+Reported (one JSON line):
+  value      gates/s of the fused plan (LDS-tile sweeps), the headline
+  extra      gates/s and HBM GB/s with and without fusion, sweep counts
+  roofline   the dominant kernel (k_tile, one fused sweep): algorithmic bytes
+             per launch (2 x 16 x 2^26 B) / its average duration, from HIP
+             events on the context stream around launches of that kernel alone
+  cpu_baseline  the CPU oracle (cpu_ref, OpenMP) on a bounded sample of the
+             same gates at the same size, on the host cores
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "HBM GB/s & gates/s at 26q c128 SV; batched adjoint grads/s with p99 latency"
+HBM_PEAK_GBPS = 8000.0
+N, DEPTH = 26, 20
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def circuit_bytes(n, depth):
+    """Algorithmic HBM bytes of the unfused circuit: a dense 1q gate reads and
+    writes every amplitude (2*16*2^n B); CZ touches its |11> quarter (2*16*2^(n-2) B)."""
+    n1q = n * depth
+    ncz = sum(len(range(1 if l % 2 == 0 else 2, n, 2)) for l in range(depth))
+    return n1q, ncz, n1q * 32 * 2 ** n + ncz * 32 * 2 ** (n - 2)
+
+
+def time_program(ctx, prog, sv, steps, warmup, s0, s1):
+    for _ in range(warmup):
+        prog.run(sv.storage)
+    ctx.sync()
+    ctx.record(s0)
+    for _ in range(steps):
+        prog.run(sv.storage)
+    ctx.record(s1)
+    ctx.sync()
+    return ctx.elapsed_ms(s0, s1)
+
+
+def cpu_baseline(circ, n, budget_s=12.0):
+    """cpu_ref on the host cores: the circuit's gates in order on a 2^n host
+    vector until the time budget is spent; gates/s over the gates run."""
+    import oracle
+    from mossquantum_amd import lower
+    lw = lower(circ, n, False)
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[0] = 1
+    recs, pool = oracle._records(lw)
+    lib = oracle.lib()
+    t0 = time.perf_counter()
+    done = 0
+    one = (oracle.RefOp * 1)()
+    while done < len(lw.records) and time.perf_counter() - t0 < budget_s:
+        one[0] = recs[done]
+        lib.ref_run_program(oracle._p(psi), n, 0, 1, one, oracle._p(pool))
+        done += 1
+    dt = time.perf_counter() - t0
+    return done, dt, oracle.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-unfused", action="store_true", help="skip the unfused leg")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import mossquantum_amd as mq
+    from mossquantum_amd.workloads import haar_layers, random_circuit
+
+    gold = ROOT / "tests" / "golden" / "c1_haar26.npz"
+    U = np.load(gold)["unitaries"] if gold.exists() else haar_layers(N, DEPTH, 20260126)
+    circ = random_circuit(N, DEPTH, U)
+    n1q, ncz, alg_bytes = circuit_bytes(N, DEPTH)
+    ngates = n1q + ncz
+    ctx = mq.default_context()
+    log(f"workload n={N} depth={DEPTH} gates={ngates} algorithmic bytes/step={alg_bytes/1e9:.1f} GB")
+
+    sv = mq.StateVector.zeros(N, ctx)
+    fused = mq.Program(circ, N, False, ctx, fuse=True)
+    nsweeps = fused.sweeps()
+    log(f"fused plan: {nsweeps} sweeps per step")
+    ms_f = time_program(ctx, fused, sv, args.steps, args.warmup, 0, 1)
+    step_f = ms_f / args.steps
+    sweep_bytes = nsweeps * 32 * 2 ** N
+    log(f"fused: {step_f:.3f} ms/step")
+
+    extra = {"fused": {"ms_per_step": step_f, "gates_per_s": ngates / (step_f / 1e3), "sweeps": nsweeps,
+                       "hbm_gbps": sweep_bytes / (step_f / 1e3) / 1e9,
+                       "effective_gbps_unfused_bytes": alg_bytes / (step_f / 1e3) / 1e9}}
+    # roofline of the dominant kernel: every fused launch is one k_tile sweep
+    tile_avg_ms = step_f / nsweeps
+    tile_bytes = 32 * 2 ** N
+    achieved = tile_bytes / (tile_avg_ms / 1e3) / 1e9
+
+    if not args.no_unfused:
+        sv_u = mq.StateVector.zeros(N, ctx)
+        unf = mq.Program(circ, N, False, ctx, fuse=False)
+        usteps = max(1, min(args.steps, 3))
+        ms_u = time_program(ctx, unf, sv_u, usteps, 1, 2, 3)
+        step_u = ms_u / usteps
+        log(f"unfused: {step_u:.3f} ms/step")
+        # one-qubit dense sweeps alone (k_gate1): the first layer's 26 gates
+        from mossquantum_amd import QCircuit
+        layer = QCircuit(list(circ.flat())[:N])
+        g1 = mq.Program(layer, N, False, ctx, fuse=False)
+        ms_g1 = time_program(ctx, g1, sv_u, 4, 1, 4, 5)
+        g1_avg = ms_g1 / (4 * N)
+        extra["unfused"] = {"ms_per_step": step_u, "gates_per_s": ngates / (step_u / 1e3),
+                            "hbm_gbps": alg_bytes / (step_u / 1e3) / 1e9,
+                            "gate1_avg_ms": g1_avg, "gate1_gbps": tile_bytes / (g1_avg / 1e3) / 1e9}
+        # parity of the two plans after the same number of passes (1 + steps vs 1 + usteps)
+        if usteps + 1 == args.steps + args.warmup:
+            a, b = sv.to_host(), sv_u.to_host()
+            extra["fused_vs_unfused_max_abs_diff"] = float(np.max(np.abs(a - b)))
+            del a, b
+        del sv_u
+    norm = mq.dot(sv, sv).real
+    extra["final_norm_minus_one"] = norm - 1.0
+
+    cpu = None
+    if not args.no_cpu:
+        done, dt, cores = cpu_baseline(circ, N)
+        cpu = {"value": done / dt, "unit": "gates/s", "cores": cores, "kind": "port",
+               "sample": f"first {done} of the {ngates} gates of this circuit at n=26 "
+                         f"({dt:.1f} s, cpu_ref OpenMP, one sweep per gate)"}
+        log(f"cpu baseline: {done} gates in {dt:.1f} s on {cores} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": ngates / (step_f / 1e3),
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_f,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128",
+        "data": "synthetic (seeded Haar unitaries, state |0...0>)",
+        "config": {"workload": "26q c128 SV random circuit, depth 20 (Haar 1q + alternating CZ), fused plan",
+                   "n_qubits": N, "depth": DEPTH, "gates": ngates, "state_bytes": 16 * 2 ** N},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_tile",
+                     "avg_launch_ms": tile_avg_ms, "bytes_per_launch": tile_bytes},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,199 @@
+/*
+ * mossq_hip.h -- C ABI of libmossq_hip.so, the MI355X (gfx950) dense
+ * simulation backend for MOSSQuantum.
+ *
+ * The Julia side binds every entry point with `ccall((:name, libmossq_hip), ...)`;
+ * the Python host in mossquantum_amd/ binds the same symbols through ctypes.
+ *
+ * Conventions (all entry points):
+ *   - Return an int status: MOSSQ_OK (0) or a nonzero MOSSQ_E* code.  The
+ *     message of the last failure on the calling thread is mossq_last_error().
+ *   - Complex numbers are complex128, interleaved (re, im), i.e. Julia's
+ *     ComplexF64 / numpy complex128 memory layout.
+ *   - Qubit positions at this boundary are 0-based bit indices, qubit 1 of the
+ *     reference = bit 0 = least significant index bit
+ *     (reference docs/conventions.md:7-13).  The Julia side subtracts one.
+ *   - Local matrices are column-major 2^k x 2^k in the reference's canonical
+ *     "ordered" basis: positions sorted ascending, the first (lowest) position
+ *     is the least significant local bit
+ *     (reference src/quantum_circuits/gates/abstract_gates.jl:88-121,162-166).
+ *   - A density matrix of n qubits is vec(rho) of length 4^n, rho[i,j] at
+ *     i + 2^n * j (reference src/states/density_matrix.jl:12-22,74).
+ *   - Device buffers are owned by the caller (allocated with mossq_alloc);
+ *     the library keeps no host pointer after a call returns.
+ *   - Every context owns one in-order HIP stream.  Calls on one context are
+ *     ordered; calls on distinct contexts are thread-safe.
+ *   - Nothing here falls back to the CPU: with no usable device every entry
+ *     point returns MOSSQ_EDEVICE.
+ */
+#ifndef MOSSQ_HIP_H
+#define MOSSQ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOSSQ_OK 0
+#define MOSSQ_EINVAL 1   /* invalid argument (reference: ArgumentError / DimensionMismatch) */
+#define MOSSQ_EOOM 2     /* device memory or memory budget exhausted */
+#define MOSSQ_EDEVICE 3  /* HIP runtime / device failure */
+#define MOSSQ_EUNSUPPORTED 4 /* reference: UnsupportedOperation, backend_protocols.jl:16-36 */
+
+typedef struct mossq_ctx mossq_ctx;
+typedef struct mossq_program mossq_program;
+typedef struct mossq_obs mossq_obs;
+
+/* Instruction kinds of a program (a lowered QCircuit). */
+#define MOSSQ_OP_GATE 0    /* unitary gate: mat = 2^k x 2^k ordered_mat */
+#define MOSSQ_OP_CHANNEL 1 /* Kraus channel: mat = 4^k x 4^k ordered_supermat */
+
+/* One instruction of a lowered circuit.  Replaces the per-instruction loop of
+ * reference src/execution/gates.jl:21-36 (gates) and
+ * src/execution/channels.jl:8-17 (channels). */
+typedef struct mossq_op {
+    int32_t kind;   /* MOSSQ_OP_GATE or MOSSQ_OP_CHANNEL */
+    int32_t k;      /* number of register qubits the instruction acts on (1..5; channels 1..2) */
+    int32_t pos[5]; /* 0-based qubit positions, strictly increasing */
+    int32_t mat;    /* offset (in complex elements) of the matrix in the pool */
+    int32_t dmat;   /* offset of the derivative gate D = (dU/dtheta)^dagger U, or -1
+                       (reference src/quantum_circuits/gates/parametric_gates.jl:76-91) */
+    int32_t reserved;
+} mossq_op;
+
+/* ---- library / context --------------------------------------------------- */
+int mossq_version(void);
+const char *mossq_last_error(void);
+/* Device count visible to HIP (0 when no GPU). */
+int mossq_device_count(int *count);
+/* Create a context on `device` with a device-memory budget in bytes (0 = no limit).
+ * Backs the HIP storage family's executor (reference: backend(state) selection,
+ * src/api/backend_protocols.jl:38-70). */
+int mossq_ctx_create(int device, uint64_t mem_budget_bytes, mossq_ctx **out);
+int mossq_ctx_destroy(mossq_ctx *ctx);
+int mossq_sync(mossq_ctx *ctx);
+/* Bytes currently allocated through this context. */
+int mossq_mem_in_use(mossq_ctx *ctx, uint64_t *bytes);
+
+/* ---- device buffers (back the HIPArray storage family: similar/copyto!/copy,
+ *      reference src/states/array_storage.jl:15-64) ------------------------- */
+int mossq_alloc(mossq_ctx *ctx, uint64_t bytes, void **dptr);
+int mossq_free(mossq_ctx *ctx, void *dptr);
+int mossq_memset_zero(mossq_ctx *ctx, void *dptr, uint64_t bytes);
+int mossq_copy_h2d(mossq_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int mossq_copy_d2h(mossq_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int mossq_copy_d2d(mossq_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* ---- states ------------------------------------------------------------- */
+/* |0...0> (reference StateVector zero state, src/states/state_vector.jl:36-40). */
+int mossq_sv_init_zero(mossq_ctx *ctx, void *psi, int n);
+/* |0...0><0...0| (reference DensityMatrix(n), src/states/density_matrix.jl:12-22). */
+int mossq_dm_init_zero(mossq_ctx *ctx, void *rho, int n);
+/* rho = psi psi^dagger (reference DensityMatrix(::StateVector),
+ * src/states/density_matrix.jl:65-71; SV->DM promotion of execution/channels.jl:19). */
+int mossq_dm_from_sv(mossq_ctx *ctx, void *rho, const void *psi, int n);
+
+/* ---- gates --------------------------------------------------------------- */
+/* Dense k-qubit gate on a state vector of n qubits, one read+write sweep
+ * (reference _apply_gate_2!/apply_threaded!, src/backends/cpu/gates/
+ * serial_short_range.jl:2-71, threaded_short_range.jl, threaded_long_range.jl,
+ * generic.jl:20).  Diagonal and permutation matrices (CZ, CNOT, PHASE, X, SWAP,
+ * iSWAP, FREDKIN, CCPHASE, Rz ...) are detected and touch only the amplitudes
+ * they change (reference src/backends/cpu/gates/specific/ cz_gate.jl, phase_gate.jl, swap_gates.jl, fredkin_gate.jl).  k <= 5. */
+int mossq_sv_apply_gate(mossq_ctx *ctx, void *psi, int n, int k, const int32_t *pos,
+                        const double *mat);
+/* U rho U^dagger in ONE sweep of U (x) conj(U) on bits (pos, pos+n)
+ * (reference _dm_apply_threaded!, src/backends/cpu/gates/generic.jl:24-34, which
+ * takes two sweeps).  k <= 2. */
+int mossq_dm_apply_gate(mossq_ctx *ctx, void *rho, int n, int k, const int32_t *pos,
+                        const double *mat);
+/* Kraus channel via its 4^k x 4^k supermatrix on bits (pos, pos+n); dagger=1
+ * applies S^dagger (reference _qmap_apply_threaded! / _qmap_apply_dagger_threaded!,
+ * src/backends/cpu/gates/channels.jl:2-24).  k <= 2. */
+int mossq_dm_apply_channel(mossq_ctx *ctx, void *rho, int n, int k, const int32_t *pos,
+                           const double *supermat, int dagger);
+
+/* ---- observables (QubitsOperator grouped by site tuple) ------------------ */
+/* ngroups groups; group g acts on group_k[g] (<=3) sites listed consecutively in
+ * sites (0-based, increasing) with a 2^k x 2^k column-major matrix stored
+ * consecutively in mats (reference _get_mat, src/backends/cpu/operators/
+ * utils.jl:3-20; QubitsOperator data, quantum_circuits/operators/
+ * qubits_operator.jl:1-50). A group with k=0 is c * identity (1x1 matrix). */
+int mossq_obs_create(mossq_ctx *ctx, int n, int ngroups, const int32_t *group_k,
+                     const int32_t *sites, const double *mats, mossq_obs **out);
+int mossq_obs_destroy(mossq_obs *obs);
+
+/* <psi|H|psi> = sum_g sum_blocks v_b^dagger M_g v_b, complex result in out[2]
+ * (reference _cpu_expectation(::QubitsOperator, ::StateVector),
+ * src/observables/expectation.jl:12-33). */
+int mossq_sv_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *psi, double *out);
+/* <phi|H|psi> (reference _cpu_cross_expectation, src/observables/expectation.jl:48-66). */
+int mossq_sv_cross_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *phi,
+                               const void *psi, double *out);
+/* out = H psi (reference apply(::QubitsOperator, ::StateVector),
+ * src/execution/operators.jl:3-64, src/backends/cpu/operators/generic.jl:12-28). */
+int mossq_sv_apply_operator(mossq_ctx *ctx, const mossq_obs *obs, const void *psi, void *out);
+/* tr(H rho) (reference _cpu_expectation(::QubitsOperator, ::DensityMatrix),
+ * src/observables/density_matrix_expectation.jl:12-25). */
+int mossq_dm_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *rho, double *out);
+/* out = H rho, left action on the row bits (reference operators/density_matrix.jl:1-5). */
+int mossq_dm_apply_operator(mossq_ctx *ctx, const mossq_obs *obs, const void *rho, void *out);
+/* sum_i conj(a_i) b_i over len complex entries (reference LinearAlgebra.dot on storage;
+ * purity = Re dot(rho, rho), src/states/density_matrix.jl:106). */
+int mossq_dot(mossq_ctx *ctx, const void *a, const void *b, uint64_t len, double *out);
+/* tr(rho) (reference src/states/density_matrix.jl:105). */
+int mossq_dm_trace(mossq_ctx *ctx, const void *rho, int n, double *out);
+/* Marginal readout probabilities of `k` qubits (first listed = least significant
+ * output bit), out has 2^k doubles (reference probabilities,
+ * src/execution/probabilities.jl:42-87). is_density selects the diagonal of rho. */
+int mossq_probabilities(mossq_ctx *ctx, const void *state, int n, int is_density, int k,
+                        const int32_t *qubits, double *out);
+
+/* ---- programs (lowered circuits) ----------------------------------------- */
+/* Lower a circuit once: matrices are uploaded and the launch plan is built.
+ * fuse=1 lets the planner merge consecutive instructions into LDS-resident
+ * tile sweeps (one HBM read+write per sweep).  pool holds pool_len complex
+ * numbers.  Replaces the instruction loop of src/execution/gates.jl:21-36. */
+int mossq_program_create(mossq_ctx *ctx, int n, int is_density, int nops, const mossq_op *ops,
+                         const double *pool, uint64_t pool_len, int fuse, mossq_program **out);
+int mossq_program_destroy(mossq_program *prog);
+/* Number of HBM sweeps (kernel launches that stream the whole state) the plan makes. */
+int mossq_program_sweeps(const mossq_program *prog, int *sweeps);
+/* Apply the program in place (reference apply!(circuit, state),
+ * src/api/backend_protocols.jl:246-254). */
+int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state);
+
+/* ---- adjoint gradients ---------------------------------------------------- */
+/* value = Re <H> at C|in>, grads[j] for the j-th derivative gate in forward
+ * order (reference _circuit_adjoint + back_propagate,
+ * src/differentiation/circuit.jl:77-229, driven by value_and_pullback,
+ * src/differentiation/objectives.jl:183-218).  For a pure program (no
+ * channels) `in` is a state vector of n qubits; with channels `in` is a
+ * density matrix (in_is_density=1) or a state vector promoted at the first
+ * channel.  The input buffer is not modified.  The final state is written to
+ * `out_state` when it is non-NULL (size of the final representation). */
+int mossq_adjoint(mossq_ctx *ctx, const mossq_program *prog, const mossq_obs *obs,
+                  const void *in, int in_is_density, void *out_state, double *value,
+                  double *grads);
+
+/* Batched adjoint gradients of B independent programs (uneven qubit counts,
+ * own parameters, own observable), all started from |0...0>.  grads of item b
+ * start at grads[grad_offset[b]].  New capability: the reference's batch
+ * reverse is per column with one shared circuit
+ * (src/differentiation/batch.jl:35-66). */
+int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs,
+                        mossq_obs *const *obs, const int64_t *grad_offset, double *values,
+                        double *grads);
+
+/* ---- timing ----------------------------------------------------------------
+ * HIP events recorded on the context's stream (the stream every kernel of the
+ * context is launched on). */
+int mossq_event_record(mossq_ctx *ctx, int slot);
+int mossq_event_elapsed_ms(mossq_ctx *ctx, int slot0, int slot1, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOSSQ_HIP_H */

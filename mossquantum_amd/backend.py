@@ -1,0 +1,308 @@
+"""HIPBackend: the executor registered for HIPArray storage.
+
+Mirrors the reference's backend protocol (src/api/backend_protocols.jl):
+`backend(state)`, `supports(b, op, state, action)`, `apply_(op, state)` (the
+reference `apply!`: mutates and returns the resulting state, which is a new
+DensityMatrix when a channel meets a StateVector), `expectation(obs, state)`,
+`apply(obs, state)` (independent O psi / O rho), `probabilities(state, qubits)`
+and `simulate`.  Unsupported requests raise `UnsupportedOperation` before the
+payload is touched; nothing falls back to the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+from .channels import QuantumMap
+from .circuit import QCircuit, lower
+from .gates import Gate
+from .operators import QubitsOperator, QubitsTerm
+from .states import BatchDensityMatrix, BatchStateVector, DensityMatrix, HIPArray, StateVector
+
+MAX_GATE_QUBITS = 5
+MAX_CHANNEL_QUBITS = 2
+MAX_OBS_SITES = 3
+
+
+class UnsupportedOperation(Exception):
+    """reference UnsupportedOperation(backend, op, state, action), backend_protocols.jl:16-36."""
+
+    def __init__(self, backend, op, state, action):
+        super().__init__(f"{type(backend).__name__} does not support {action} of "
+                         f"{type(op).__name__} on {type(state).__name__}")
+        self.backend, self.op, self.state, self.action = backend, op, state, action
+
+
+class HIPBackend:
+    """Executor for HIPArray-backed states.  fuse=True lets circuits run as
+    fused LDS-tile sweeps; fuse=False runs one sweep per instruction."""
+
+    def __init__(self, ctx: _lib.Context | None = None, fuse: bool = True):
+        self.ctx = ctx or _lib.default_context()
+        self.fuse = fuse
+
+    def __repr__(self):
+        return f"HIPBackend(device={self.ctx.device}, fuse={self.fuse})"
+
+
+_default_backend: HIPBackend | None = None
+
+
+def backend(state=None) -> HIPBackend:
+    """The backend of a state's storage (only HIPArray storage is executable here)."""
+    global _default_backend
+    if state is not None:
+        st = getattr(state, "storage", None)
+        if state is not None and not isinstance(st, HIPArray) and not isinstance(
+                state, (BatchStateVector, BatchDensityMatrix)):
+            raise TypeError("no backend is registered for this storage")
+    if _default_backend is None:
+        _default_backend = HIPBackend()
+    return _default_backend
+
+
+# ------------------------------------------------------------------ support ----
+def _obs_groups(obs):
+    if isinstance(obs, QubitsTerm):
+        obs = QubitsOperator([obs])
+    if not isinstance(obs, QubitsOperator):
+        return None
+    return obs.groups()
+
+
+def supports(b: HIPBackend, op, state, action: str = "apply") -> bool:
+    if not isinstance(b, HIPBackend):
+        return False
+    if action == "apply":
+        if isinstance(op, Gate):
+            return op.nqubits() <= MAX_GATE_QUBITS and max(op.positions()) <= state.n
+        if isinstance(op, QuantumMap):
+            return op.nqubits() <= MAX_CHANNEL_QUBITS and max(op.positions()) <= state.n
+        if isinstance(op, QCircuit):
+            return all(supports(b, o, state, "apply") for o in op.flat())
+        if isinstance(op, (QubitsOperator, QubitsTerm)):
+            g = _obs_groups(op)
+            return all(len(k) <= MAX_OBS_SITES and (not k or max(k) <= state.n) for k, _ in g)
+        return False
+    if action == "expectation":
+        g = _obs_groups(op)
+        return g is not None and all(len(k) <= MAX_OBS_SITES and (not k or max(k) <= state.n) for k, _ in g)
+    if action == "probabilities":
+        return isinstance(state, (StateVector, DensityMatrix, BatchStateVector, BatchDensityMatrix))
+    return False
+
+
+def _require(b, op, state, action):
+    if not supports(b, op, state, action):
+        raise UnsupportedOperation(b, op, state, action)
+
+
+# ------------------------------------------------------------- observables ----
+class DeviceObservable:
+    """A QubitsOperator uploaded once (mossq_obs)."""
+
+    def __init__(self, obs, n: int, ctx: _lib.Context):
+        groups = _obs_groups(obs)
+        if groups is None or not groups:
+            raise ValueError("observable must be a non-empty QubitsOperator or QubitsTerm")
+        ks = np.array([len(k) for k, _ in groups], dtype=np.int32)
+        if ks.max() > MAX_OBS_SITES:
+            raise ValueError("observable groups wider than 3 sites are not supported on the device")
+        sites = np.array([p - 1 for k, _ in groups for p in k] or [0], dtype=np.int32)
+        mats = np.concatenate([m.reshape(-1, order="F") for _, m in groups]).astype(np.complex128)
+        self.ctx = ctx
+        self.n = n
+        h = C.c_void_p()
+        check(ctx.lib.mossq_obs_create(ctx.h, n, len(groups), _lib.iptr(ks), _lib.iptr(sites),
+                                       mats.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.ctx.lib.mossq_obs_destroy(self.h)
+            self.h = None
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- programs ----
+class Program:
+    """A lowered circuit on the device (mossq_program)."""
+
+    def __init__(self, circuit: QCircuit, n: int, is_density: bool, ctx: _lib.Context, fuse: bool = False):
+        self.lowered = lower(circuit, n, is_density)
+        recs, pool = self.lowered.arrays()
+        self.ctx = ctx
+        self.n = n
+        self.is_density = is_density
+        h = C.c_void_p()
+        check(ctx.lib.mossq_program_create(ctx.h, n, int(is_density), len(self.lowered.records), recs,
+                                           pool.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)),
+                                           pool.size, int(fuse), C.byref(h)))
+        self.h = h
+
+    def sweeps(self) -> int:
+        v = C.c_int(0)
+        check(self.ctx.lib.mossq_program_sweeps(self.h, C.byref(v)))
+        return v.value
+
+    def run(self, storage: HIPArray):
+        check(self.ctx.lib.mossq_program_run(self.ctx.h, self.h, storage.ptr))
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.ctx.lib.mossq_program_destroy(self.h)
+            self.h = None
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------- apply! ----
+def _mat_ptr(m):
+    a = np.ascontiguousarray(np.asarray(m, dtype=np.complex128).reshape(-1, order="F"))
+    return a, a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pos_arr(positions):
+    return np.array([p - 1 for p in positions], dtype=np.int32)
+
+
+def apply_(op, state, b: HIPBackend | None = None):
+    """reference apply!(op, state): mutate and return the resulting state."""
+    b = b or backend(state)
+    if isinstance(state, (BatchStateVector, BatchDensityMatrix)):
+        cols = [apply_(op, c, b) for c in state.columns]
+        if isinstance(state, BatchStateVector) and isinstance(cols[0], DensityMatrix):
+            return BatchDensityMatrix(cols)
+        state.columns = cols
+        return state
+    _require(b, op, state, "apply")
+    ctx = state.storage.ctx
+    if isinstance(op, Gate):
+        pos = _pos_arr(op.ordered_positions())
+        keep, mp = _mat_ptr(op.ordered_mat())
+        fn = ctx.lib.mossq_sv_apply_gate if isinstance(state, StateVector) else ctx.lib.mossq_dm_apply_gate
+        check(fn(ctx.h, state.storage.ptr, state.n, len(pos), _lib.iptr(pos), mp))
+        return state
+    if isinstance(op, QuantumMap):
+        if isinstance(state, StateVector):
+            state = DensityMatrix.from_state(state)
+        pos = _pos_arr(op.ordered_positions())
+        keep, mp = _mat_ptr(op.ordered_supermat())
+        check(ctx.lib.mossq_dm_apply_channel(ctx.h, state.storage.ptr, state.n, len(pos), _lib.iptr(pos), mp, 0))
+        return state
+    if isinstance(op, QCircuit):
+        flat = list(op.flat())
+        if isinstance(state, StateVector) and any(isinstance(o, QuantumMap) for o in flat):
+            first = next(i for i, o in enumerate(flat) if isinstance(o, QuantumMap))
+            if first:
+                Program(QCircuit(flat[:first]), state.n, False, ctx, b.fuse).run(state.storage)
+            state = DensityMatrix.from_state(state)
+            flat = flat[first:]
+        if flat:
+            Program(QCircuit(flat), state.n, isinstance(state, DensityMatrix), ctx, b.fuse).run(state.storage)
+        return state
+    raise UnsupportedOperation(b, op, state, "apply")
+
+
+def simulate(circuit: QCircuit, state, b: HIPBackend | None = None):
+    """Run a circuit on a copy of `state` (reference circuit * state copies first)."""
+    return apply_(circuit, state.copy(), b)
+
+
+# -------------------------------------------------------------- expectation ----
+def expectation(obs, state, b: HIPBackend | None = None):
+    """<psi|O|psi> or tr(O rho) as a complex number; a vector for batches."""
+    b = b or backend(state)
+    if isinstance(state, (BatchStateVector, BatchDensityMatrix)):
+        return np.array([expectation(obs, c, b) for c in state.columns])
+    _require(b, obs, state, "expectation")
+    ctx = state.storage.ctx
+    dob = DeviceObservable(obs, state.n, ctx)
+    out = np.zeros(2)
+    fn = ctx.lib.mossq_sv_expectation if isinstance(state, StateVector) else ctx.lib.mossq_dm_expectation
+    check(fn(ctx.h, dob.h, state.storage.ptr, _lib.dptr(out)))
+    return complex(out[0], out[1])
+
+
+def cross_expectation(phi: StateVector, obs, psi: StateVector):
+    """<phi|O|psi> (reference expectation(state_c, m, state), observables/expectation.jl:48-66)."""
+    ctx = psi.storage.ctx
+    dob = DeviceObservable(obs, psi.n, ctx)
+    out = np.zeros(2)
+    check(ctx.lib.mossq_sv_cross_expectation(ctx.h, dob.h, phi.storage.ptr, psi.storage.ptr, _lib.dptr(out)))
+    return complex(out[0], out[1])
+
+
+def apply(obs, state, b: HIPBackend | None = None):
+    """Independent O psi (or O rho, left action) (reference MOSSQuantum.apply)."""
+    b = b or backend(state)
+    _require(b, obs, state, "apply")
+    ctx = state.storage.ctx
+    dob = DeviceObservable(obs, state.n, ctx)
+    out = state.storage.similar()
+    if isinstance(state, StateVector):
+        check(ctx.lib.mossq_sv_apply_operator(ctx.h, dob.h, state.storage.ptr, out.ptr))
+        return StateVector(out, state.n)
+    check(ctx.lib.mossq_dm_apply_operator(ctx.h, dob.h, state.storage.ptr, out.ptr))
+    return DensityMatrix(out, state.n)
+
+
+def trace(rho: DensityMatrix) -> complex:
+    ctx = rho.storage.ctx
+    out = np.zeros(2)
+    check(ctx.lib.mossq_dm_trace(ctx.h, rho.storage.ptr, rho.n, _lib.dptr(out)))
+    return complex(out[0], out[1])
+
+
+def dot(a, b) -> complex:
+    """sum conj(a) b over the storage (purity = real(dot(rho, rho)))."""
+    ctx = a.storage.ctx
+    out = np.zeros(2)
+    check(ctx.lib.mossq_dot(ctx.h, a.storage.ptr, b.storage.ptr, a.storage.size, _lib.dptr(out)))
+    return complex(out[0], out[1])
+
+
+def purity(rho: DensityMatrix) -> float:
+    return dot(rho, rho).real
+
+
+# ------------------------------------------------------------ probabilities ----
+def _probability_tolerance():
+    return float(np.sqrt(np.finfo(np.float64).eps))
+
+
+def _normalize(values, tol):
+    if not np.all(np.isfinite(values)):
+        raise ValueError("probabilities must be finite")
+    if values.min() < -tol:
+        raise ValueError("probabilities must be nonnegative")
+    values = np.maximum(values, 0)
+    total = values.sum()
+    if not np.isclose(total, 1, atol=tol, rtol=tol):
+        raise ValueError(f"state probabilities must sum to one; normalize the state first (sum = {total})")
+    return values / total
+
+
+def probabilities(state, qubits=None, b: HIPBackend | None = None):
+    """Readout marginals, first requested qubit = least significant output bit
+    (reference probabilities, src/execution/probabilities.jl:42-87).  The device
+    sums the marginal; validation and normalisation follow the reference on the host."""
+    b = b or backend(state)
+    if isinstance(state, (BatchStateVector, BatchDensityMatrix)):
+        return np.stack([probabilities(c, qubits, b) for c in state.columns], axis=1)
+    _require(b, None, state, "probabilities")
+    qubits = list(range(1, state.n + 1)) if qubits is None else [int(q) for q in qubits]
+    if len(set(qubits)) != len(qubits) or not all(1 <= q <= state.n for q in qubits):
+        raise ValueError("readout qubits must be distinct and in range")
+    ctx = state.storage.ctx
+    q = np.array([x - 1 for x in qubits], dtype=np.int32)
+    out = np.zeros(1 << len(q))
+    check(ctx.lib.mossq_probabilities(ctx.h, state.storage.ptr, state.n, int(isinstance(state, DensityMatrix)),
+                                      len(q), _lib.iptr(q), _lib.dptr(out)))
+    return _normalize(out, _probability_tolerance())

@@ -1,0 +1,217 @@
+"""Circuits and their lowering to device programs.
+
+`QCircuit` mirrors the reference container (src/quantum_circuits/circuits/
+circuit.jl:98-294): an ordered list of gates, channels and nested circuits;
+active parameters are enumerated in circuit order (circuit.jl:214-239).
+
+`lower()` turns a circuit into the flat instruction records of the C ABI
+(`mossq_op`) plus a complex matrix pool: ordered positions (0-based bits),
+`ordered_mat` for gates, `ordered_supermat` for channels and one derivative
+gate (dU/dtheta)^dagger U per active parameter.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import OP_CHANNEL, OP_GATE, OpRecord
+from .channels import QuantumMap
+from .gates import Gate, QuantumGate, reorder_legs
+
+
+class QCircuit:
+    def __init__(self, ops=()):
+        self.ops = []
+        for o in ops:
+            self.append(o)
+
+    def append(self, op):
+        if not isinstance(op, (Gate, QuantumMap, QCircuit)):
+            raise TypeError(f"unsupported instruction {type(op).__name__}")
+        self.ops.append(op)
+        return self
+
+    push = append
+
+    def __iter__(self):
+        return iter(self.ops)
+
+    def __len__(self):
+        return len(self.ops)
+
+    def flat(self):
+        for o in self.ops:
+            if isinstance(o, QCircuit):
+                yield from o.flat()
+            else:
+                yield o
+
+    def has_channel(self):
+        return any(isinstance(o, QuantumMap) for o in self.flat())
+
+    def nparameters(self):
+        return sum(o.nparameters() for o in self.flat())
+
+    def active_parameters(self):
+        out = []
+        for o in self.flat():
+            out.extend(o.active_parameters() if isinstance(o, Gate) else [])
+        return out
+
+    def bind(self, values):
+        """Copy with the active parameters replaced in circuit order
+        (reference bind_parameters, src/api/parameter_binding.jl:19-63)."""
+        values = list(np.asarray(values, dtype=np.float64))
+        if len(values) != self.nparameters():
+            raise ValueError(f"expected {self.nparameters()} parameters, received {len(values)}")
+        if not all(np.isfinite(values)):
+            raise ValueError("parameter values must be finite")
+        it = iter(values)
+
+        def rebind(c):
+            out = QCircuit()
+            for o in c.ops:
+                if isinstance(o, QCircuit):
+                    out.append(rebind(o))
+                elif isinstance(o, Gate) and o.nparameters():
+                    out.append(o.with_parameters([next(it) for _ in range(o.nparameters())]))
+                else:
+                    out.append(o)
+            return out
+
+        return rebind(self)
+
+    def adjoint(self):
+        return QCircuit([o.adjoint() for o in reversed(list(self.flat()))])
+
+    def max_position(self):
+        return max((max(o.positions()) for o in self.flat()), default=0)
+
+
+def fuse_gates(circuit: QCircuit, max_qubits: int = 5) -> QCircuit:
+    """Greedy merge of adjacent non-parameterized gates while the combined
+    support stays within max_qubits (reference gate_fusion.jl:10-62)."""
+    if not 1 <= max_qubits <= 5:
+        raise ValueError("max_qubits must be between 1 and 5")
+    result = QCircuit()
+    pending = None
+    for op in circuit.ops:
+        eligible = isinstance(op, Gate) and op.parameters() is None and op.nqubits() <= max_qubits
+        if not eligible:
+            if pending is not None:
+                result.append(pending)
+                pending = None
+            result.append(fuse_gates(op, max_qubits) if isinstance(op, QCircuit) else op)
+        elif pending is None:
+            pending = op
+        else:
+            support = sorted(set(pending.positions()) | set(op.positions()))
+            if len(support) <= max_qubits:
+                combined = _embed(op, support) @ _embed(pending, support)
+                pending = QuantumGate(tuple(support), combined)
+            else:
+                result.append(pending)
+                pending = op
+    if pending is not None:
+        result.append(pending)
+    return result
+
+
+def _embed(g: Gate, support):
+    pos = g.ordered_positions()
+    data = g.ordered_mat()
+    if tuple(pos) == tuple(support):
+        return data
+    bits = [support.index(p) for p in pos]
+    dim = 1 << len(support)
+    out = np.zeros((dim, dim), dtype=np.complex128)
+    mask = sum(1 << b for b in bits)
+    for col in range(dim):
+        lc = sum(((col >> b) & 1) << j for j, b in enumerate(bits))
+        rem = col & ~mask
+        for lr in range(1 << len(bits)):
+            row = rem
+            for j, b in enumerate(bits):
+                row |= ((lr >> j) & 1) << b
+            out[row, col] = data[lr, lc]
+    return out
+
+
+class Lowered:
+    """Instruction records + matrix pool of a circuit, ready for mossq_program_create."""
+
+    def __init__(self, n, is_density):
+        self.n = n
+        self.is_density = is_density
+        self.records: list[OpRecord] = []
+        self.pool: list[np.ndarray] = []
+        self.pool_len = 0
+        self.nslots = 0
+        self.grad_slot: list[int] = []
+
+    def _push(self, m):
+        m = np.asarray(m, dtype=np.complex128)
+        off = self.pool_len
+        flat = np.asfortranarray(m).reshape(-1, order="F")
+        self.pool.append(flat)
+        self.pool_len += flat.size
+        return off
+
+    def add(self, op):
+        r = OpRecord()
+        pos = op.ordered_positions()
+        if max(pos) > self.n:
+            raise ValueError(f"position {max(pos)} outside a {self.n}-qubit register")
+        r.k = len(pos)
+        for j, p in enumerate(pos):
+            r.pos[j] = p - 1
+        r.dmat = -1
+        if isinstance(op, QuantumMap):
+            r.kind = OP_CHANNEL
+            r.mat = self._push(op.ordered_supermat())
+            self.records.append(r)
+            return
+        r.kind = OP_GATE
+        r.mat = self._push(op.ordered_mat())
+        ders = op.differentiate() or []
+        first_slot = self.nslots
+        self.nslots += len(ders)
+        if len(ders) > 1:
+            # extra derivative gates ride on identity records placed before the
+            # gate: in reverse order they see the same (U^dag y, U^dag lambda)
+            ident = np.eye(1 << r.k, dtype=np.complex128)
+            for j, dg in enumerate(ders[1:], start=1):
+                e = OpRecord()
+                e.kind, e.k = OP_GATE, r.k
+                for q in range(r.k):
+                    e.pos[q] = r.pos[q]
+                e.mat = self._push(ident)
+                e.dmat = self._push(reorder_legs(dg.positions(), dg.mat())[1])
+                self.records.append(e)
+                self.grad_slot.append(first_slot + j)
+        if ders:
+            r.dmat = self._push(reorder_legs(ders[0].positions(), ders[0].mat())[1])
+            self.grad_slot.append(first_slot)
+        self.records.append(r)
+
+    @property
+    def nderiv(self):
+        return len(self.grad_slot)
+
+    def slot_gradients(self, device_grads):
+        """Device gradient order (records with a derivative, forward order) ->
+        active-slot order of the circuit."""
+        out = np.zeros(self.nslots)
+        out[np.asarray(self.grad_slot, dtype=np.int64)] = device_grads
+        return out
+
+    def arrays(self):
+        recs = (OpRecord * max(len(self.records), 1))(*self.records)
+        pool = np.concatenate(self.pool) if self.pool else np.zeros(1, dtype=np.complex128)
+        return recs, np.ascontiguousarray(pool)
+
+
+def lower(circuit: QCircuit, n: int, is_density: bool) -> Lowered:
+    lw = Lowered(n, is_density)
+    for op in circuit.flat():
+        lw.add(op)
+    return lw

@@ -1,0 +1,232 @@
+// Context, device buffers, copies, errors and events of libmossq_hip.so.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+#include "mossq_internal.h"
+
+namespace mossq {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+Ctx *C(mossq_ctx *ctx) { return reinterpret_cast<Ctx *>(ctx); }
+
+int ctx_check(mossq_ctx *ctx) {
+    MOSSQ_CHECK(ctx != nullptr, MOSSQ_EINVAL, "null context");
+    MOSSQ_HIP(hipSetDevice(C(ctx)->device));
+    return MOSSQ_OK;
+}
+
+int dev_alloc(Ctx *c, u64 bytes, void **p) {
+    if (bytes == 0) bytes = 16;
+    if (c->budget && c->in_use + bytes > c->budget)
+        return fail(MOSSQ_EOOM, "device memory budget exceeded: " + std::to_string(c->in_use) + " + " +
+                                    std::to_string(bytes) + " > " + std::to_string(c->budget));
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MOSSQ_EOOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    c->allocs.emplace_back(*p, bytes);
+    c->in_use += bytes;
+    return MOSSQ_OK;
+}
+
+int dev_free(Ctx *c, void *p) {
+    if (!p) return MOSSQ_OK;
+    auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
+    MOSSQ_CHECK(it != c->allocs.end(), MOSSQ_EINVAL, "pointer was not allocated by this context");
+    // Memory may still be read by queued kernels: order the free after them.
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    MOSSQ_HIP(hipFree(p));
+    c->in_use -= it->second;
+    c->allocs.erase(it);
+    return MOSSQ_OK;
+}
+
+int get_scratch(Ctx *c, u64 bytes, void **p) {
+    if (c->scratch_bytes < bytes) {
+        MOSSQ_TRY(dev_free(c, c->scratch));
+        c->scratch = nullptr;
+        c->scratch_bytes = 0;
+        u64 want = std::max<u64>(bytes, 1 << 20);
+        MOSSQ_TRY(dev_alloc(c, want, &c->scratch));
+        c->scratch_bytes = want;
+    }
+    *p = c->scratch;
+    return MOSSQ_OK;
+}
+
+int stage_matrix(Ctx *c, const c128 *host, u64 count, const c128 **dev) {
+    u64 bytes = count * sizeof(c128);
+    if (c->mat_stage_bytes < bytes) {
+        MOSSQ_TRY(dev_free(c, c->mat_stage));
+        c->mat_stage = nullptr;
+        c->mat_stage_bytes = 0;
+        u64 want = std::max<u64>(bytes, 64 << 10);
+        MOSSQ_TRY(dev_alloc(c, want, &c->mat_stage));
+        c->mat_stage_bytes = want;
+    }
+    // Stream order keeps the copy behind every kernel that reads the previous
+    // matrix; the wait makes sure `host` (caller memory) has been consumed.
+    MOSSQ_HIP(hipMemcpyAsync(c->mat_stage, host, bytes, hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    *dev = reinterpret_cast<const c128 *>(c->mat_stage);
+    return MOSSQ_OK;
+}
+
+int get_pinned(Ctx *c, u64 bytes, double **p) {
+    if (c->host_pinned_bytes < bytes) {
+        if (c->host_pinned) MOSSQ_HIP(hipHostFree(c->host_pinned));
+        c->host_pinned = nullptr;
+        u64 want = std::max<u64>(bytes, 64 << 10);
+        MOSSQ_HIP(hipHostMalloc((void **)&c->host_pinned, want, hipHostMallocDefault));
+        c->host_pinned_bytes = want;
+    }
+    *p = c->host_pinned;
+    return MOSSQ_OK;
+}
+
+}  // namespace mossq
+
+using namespace mossq;
+
+extern "C" {
+
+int mossq_version(void) { return 1; }
+
+const char *mossq_last_error(void) { return g_last_error.c_str(); }
+
+int mossq_device_count(int *count) {
+    MOSSQ_CHECK(count, MOSSQ_EINVAL, "null output");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *count = n;
+    return MOSSQ_OK;
+}
+
+int mossq_ctx_create(int device, uint64_t mem_budget_bytes, mossq_ctx **out) {
+    MOSSQ_CHECK(out, MOSSQ_EINVAL, "null output");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return fail(MOSSQ_EDEVICE, "no HIP device available (libmossq_hip has no CPU fallback)");
+    }
+    MOSSQ_CHECK(device >= 0 && device < n, MOSSQ_EINVAL, "device index out of range");
+    MOSSQ_HIP(hipSetDevice(device));
+    Ctx *c = new Ctx();
+    c->device = device;
+    c->budget = mem_budget_bytes;
+    // Default priority, non-blocking w.r.t. the legacy null stream.
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MOSSQ_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    for (int i = 0; i < kNumEvents; ++i) {
+        e = hipEventCreate(&c->events[i]);
+        if (e != hipSuccess) {
+            delete c;
+            return fail(MOSSQ_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
+    *out = reinterpret_cast<mossq_ctx *>(c);
+    return MOSSQ_OK;
+}
+
+int mossq_ctx_destroy(mossq_ctx *ctx) {
+    if (!ctx) return MOSSQ_OK;
+    Ctx *c = C(ctx);
+    MOSSQ_HIP(hipSetDevice(c->device));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    for (auto &a : c->allocs) (void)hipFree(a.first);
+    c->allocs.clear();
+    if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+    for (int i = 0; i < kNumEvents; ++i)
+        if (c->events[i]) (void)hipEventDestroy(c->events[i]);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MOSSQ_OK;
+}
+
+int mossq_sync(mossq_ctx *ctx) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_HIP(hipStreamSynchronize(C(ctx)->stream));
+    return MOSSQ_OK;
+}
+
+int mossq_mem_in_use(mossq_ctx *ctx, uint64_t *bytes) {
+    MOSSQ_CHECK(ctx && bytes, MOSSQ_EINVAL, "null argument");
+    *bytes = C(ctx)->in_use;
+    return MOSSQ_OK;
+}
+
+int mossq_alloc(mossq_ctx *ctx, uint64_t bytes, void **dptr) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(dptr, MOSSQ_EINVAL, "null output");
+    return dev_alloc(C(ctx), bytes, dptr);
+}
+
+int mossq_free(mossq_ctx *ctx, void *dptr) {
+    MOSSQ_TRY(ctx_check(ctx));
+    return dev_free(C(ctx), dptr);
+}
+
+int mossq_memset_zero(mossq_ctx *ctx, void *dptr, uint64_t bytes) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(dptr || bytes == 0, MOSSQ_EINVAL, "null buffer");
+    MOSSQ_HIP(hipMemsetAsync(dptr, 0, bytes, C(ctx)->stream));
+    return MOSSQ_OK;
+}
+
+int mossq_copy_h2d(mossq_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK((dst && src) || bytes == 0, MOSSQ_EINVAL, "null buffer");
+    MOSSQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, C(ctx)->stream));
+    MOSSQ_HIP(hipStreamSynchronize(C(ctx)->stream));
+    return MOSSQ_OK;
+}
+
+int mossq_copy_d2h(mossq_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK((dst && src) || bytes == 0, MOSSQ_EINVAL, "null buffer");
+    MOSSQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C(ctx)->stream));
+    MOSSQ_HIP(hipStreamSynchronize(C(ctx)->stream));
+    return MOSSQ_OK;
+}
+
+int mossq_copy_d2d(mossq_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK((dst && src) || bytes == 0, MOSSQ_EINVAL, "null buffer");
+    MOSSQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, C(ctx)->stream));
+    return MOSSQ_OK;
+}
+
+int mossq_event_record(mossq_ctx *ctx, int slot) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(slot >= 0 && slot < kNumEvents, MOSSQ_EINVAL, "event slot out of range");
+    MOSSQ_HIP(hipEventRecord(C(ctx)->events[slot], C(ctx)->stream));
+    return MOSSQ_OK;
+}
+
+int mossq_event_elapsed_ms(mossq_ctx *ctx, int slot0, int slot1, float *ms) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(slot0 >= 0 && slot0 < kNumEvents && slot1 >= 0 && slot1 < kNumEvents && ms,
+                MOSSQ_EINVAL, "bad event arguments");
+    MOSSQ_HIP(hipEventSynchronize(C(ctx)->events[slot1]));
+    MOSSQ_HIP(hipEventElapsedTime(ms, C(ctx)->events[slot0], C(ctx)->events[slot1]));
+    return MOSSQ_OK;
+}
+
+}  // extern "C"

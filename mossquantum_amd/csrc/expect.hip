@@ -1,0 +1,503 @@
+// Observables (QubitsOperator grouped by site tuple), expectations, operator
+// action, dot products, traces and readout marginals.
+//
+// All reductions are deterministic: each workgroup writes a partial sum to a
+// fixed slot and a single-workgroup kernel adds the partials in a fixed order,
+// so repeated calls return bit-identical values (the reference's threaded
+// parallel_sum adds unordered atomics, utils/parallel_for.jl:47-80).
+#include "mossq_obs.h"
+
+namespace mossq {
+
+constexpr int kB = 256;
+constexpr int kMaxParts = 512;  // partial sums per group / per reduction
+
+static inline unsigned blocks_for(u64 work, int items, unsigned cap) {
+    u64 g = (work + u64(kB) * items - 1) / (u64(kB) * items);
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return unsigned(g);
+}
+
+__device__ __forceinline__ c128 wave_sum(c128 a) {
+    for (int o = 32; o > 0; o >>= 1) {
+        a.re += __shfl_xor(a.re, o, 64);
+        a.im += __shfl_xor(a.im, o, 64);
+    }
+    return a;
+}
+
+// Block reduction of one c128 per thread; result valid in thread 0.
+__device__ __forceinline__ c128 block_sum(c128 a) {
+    __shared__ c128 red[kB / 64];
+    a = wave_sum(a);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = a;
+    __syncthreads();
+    c128 s = mk(0, 0);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kB / 64; ++i) s = cadd(s, red[i]);
+    return s;
+}
+
+// out[g] = sum_{j < nparts} part[g * stride + j], one workgroup per g.
+__global__ __launch_bounds__(kB) void k_reduce_rows(const c128 *__restrict__ part, int nparts, int stride,
+                                                    c128 *__restrict__ out) {
+    const c128 *p = part + size_t(blockIdx.x) * stride;
+    c128 a = mk(0, 0);
+    for (int j = threadIdx.x; j < nparts; j += kB) a = cadd(a, p[j]);
+    c128 s = block_sum(a);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+int reduce_partials(Ctx *c, const c128 *part, int nparts, c128 *dev_out) {
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kB), 0, c->stream, part, nparts, nparts, dev_out);
+    MOSSQ_HIP(hipGetLastError());
+    return MOSSQ_OK;
+}
+
+// ------------------------------------------------------------ expectation ----
+// <phi|M_g|psi> summed over the blocks of group g (phi == psi for <M_g>).
+template <int K>
+__device__ c128 group_block_sum(const c128 *__restrict__ phi, const c128 *__restrict__ psi, const int *sites,
+                                const c128 *sm, u64 nbase, u64 t0, u64 step) {
+    constexpr int D = 1 << K;
+    u64 off[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        u64 o = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (c & (1 << j)) o |= u64(1) << sites[j];
+        off[c] = o;
+    }
+    c128 acc = mk(0, 0);
+    for (u64 t = t0; t < nbase; t += step) {
+        u64 base = insert0s<K>(t, sites);
+        c128 v[D], w[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            v[c] = psi[base + off[c]];
+            w[c] = phi[base + off[c]];
+        }
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            c128 mv = mk(0, 0);
+#pragma unroll
+            for (int c = 0; c < D; ++c) mv = cfma(sm[r + D * c], v[c], mv);
+            acc = cfmaconj(w[r], mv, acc);
+        }
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(kB) void k_expect_groups(const c128 *__restrict__ phi, const c128 *__restrict__ psi,
+                                                      int n, const ObsGroup *__restrict__ groups,
+                                                      const c128 *__restrict__ mats, c128 *__restrict__ part,
+                                                      int nparts) {
+    const ObsGroup g = groups[blockIdx.y];
+    __shared__ c128 sm[64];
+    const int D = 1 << g.k;
+    for (int i = threadIdx.x; i < D * D; i += kB) sm[i] = mats[g.moff + i];
+    __syncthreads();
+    const u64 nbase = u64(1) << (n - g.k);
+    const u64 t0 = u64(blockIdx.x) * kB + threadIdx.x, step = u64(gridDim.x) * kB;
+    c128 acc;
+    switch (g.k) {
+        case 0:
+            acc = group_block_sum<0>(phi, psi, g.sites, sm, nbase, t0, step);
+            break;
+        case 1:
+            acc = group_block_sum<1>(phi, psi, g.sites, sm, nbase, t0, step);
+            break;
+        case 2:
+            acc = group_block_sum<2>(phi, psi, g.sites, sm, nbase, t0, step);
+            break;
+        default:
+            acc = group_block_sum<3>(phi, psi, g.sites, sm, nbase, t0, step);
+            break;
+    }
+    c128 s = block_sum(acc);
+    if (threadIdx.x == 0) part[size_t(blockIdx.y) * nparts + blockIdx.x] = s;
+}
+
+// Per-group sums to dev_out[G] (device), deterministic.
+int expect_groups_dev(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, c128 *dev_out, c128 *part) {
+    unsigned nb = blocks_for(u64(1) << o->n, 2, kMaxParts);
+    hipLaunchKernelGGL(k_expect_groups, dim3(nb, o->G), dim3(kB), 0, c->stream, phi, psi, o->n, o->d_groups,
+                       o->d_mats, part, int(nb));
+    MOSSQ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_rows, dim3(o->G), dim3(kB), 0, c->stream, part, int(nb), int(nb), dev_out);
+    MOSSQ_HIP(hipGetLastError());
+    return MOSSQ_OK;
+}
+
+int expect_total_host(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, double *out) {
+    void *scr = nullptr;
+    u64 need = (u64(o->G) * kMaxParts + o->G) * sizeof(c128);
+    MOSSQ_TRY(get_scratch(c, need, &scr));
+    c128 *part = (c128 *)scr;
+    c128 *res = part + size_t(o->G) * kMaxParts;
+    MOSSQ_TRY(expect_groups_dev(c, o, phi, psi, res, part));
+    std::vector<c128> h(o->G);
+    MOSSQ_HIP(hipMemcpyAsync(h.data(), res, sizeof(c128) * o->G, hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    double re = 0, im = 0;
+    for (int g = 0; g < o->G; ++g) {
+        re += h[g].re;
+        im += h[g].im;
+    }
+    out[0] = re;
+    out[1] = im;
+    return MOSSQ_OK;
+}
+
+// ---------------------------------------------------------- operator action ----
+// out[i] = sum_g sum_c M_g[r_g(i), c] psi[i with the sites of g set to c].
+// nbits may exceed the observable's n (left action on density-matrix rows).
+__global__ __launch_bounds__(kB) void k_apply_op(c128 *__restrict__ out, const c128 *__restrict__ psi, u64 len,
+                                                 int G, const ObsGroup *__restrict__ groups,
+                                                 const c128 *__restrict__ mats) {
+    for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < len; i += u64(gridDim.x) * kB) {
+        c128 acc = mk(0, 0);
+        for (int gi = 0; gi < G; ++gi) {
+            const ObsGroup g = groups[gi];
+            const int D = 1 << g.k;
+            int r = 0;
+            u64 base = i;
+            for (int j = 0; j < g.k; ++j) {
+                r |= int((i >> g.sites[j]) & 1) << j;
+                base &= ~(u64(1) << g.sites[j]);
+            }
+            const c128 *m = mats + g.moff + r;
+            for (int cc = 0; cc < D; ++cc) {
+                u64 o = 0;
+                for (int j = 0; j < g.k; ++j)
+                    if (cc & (1 << j)) o |= u64(1) << g.sites[j];
+                acc = cfma(m[size_t(D) * cc], psi[base | o], acc);
+            }
+        }
+        out[i] = acc;
+    }
+}
+
+int apply_op_dev(Ctx *c, const Obs *o, const c128 *psi, c128 *out, int nbits, OpMats which) {
+    u64 len = u64(1) << nbits;
+    const c128 *m = which == OpMats::Plain ? o->d_mats : which == OpMats::Seed ? o->d_mats_seed : o->d_mats_adj;
+    hipLaunchKernelGGL(k_apply_op, dim3(blocks_for(len, 2, 1u << 20)), dim3(kB), 0, c->stream, out, psi, len,
+                       o->G, o->d_groups, m);
+    MOSSQ_HIP(hipGetLastError());
+    return MOSSQ_OK;
+}
+
+// ---------------------------------------------------- density expectation ----
+// sum_g sum_b sum_{r,c} M_g[r,c] rho[b|c, b|r]
+__global__ __launch_bounds__(kB) void k_dm_expect_groups(const c128 *__restrict__ rho, int n,
+                                                         const ObsGroup *__restrict__ groups,
+                                                         const c128 *__restrict__ mats, c128 *__restrict__ part,
+                                                         int nparts) {
+    const ObsGroup g = groups[blockIdx.y];
+    const int D = 1 << g.k;
+    const u64 N = u64(1) << n;
+    const u64 nbase = u64(1) << (n - g.k);
+    c128 acc = mk(0, 0);
+    for (u64 t = u64(blockIdx.x) * kB + threadIdx.x; t < nbase; t += u64(gridDim.x) * kB) {
+        u64 b = insert0s_dyn(t, g.sites, g.k);
+        for (int r = 0; r < D; ++r) {
+            u64 orr = 0;
+            for (int j = 0; j < g.k; ++j)
+                if (r & (1 << j)) orr |= u64(1) << g.sites[j];
+            for (int cc = 0; cc < D; ++cc) {
+                u64 oc = 0;
+                for (int j = 0; j < g.k; ++j)
+                    if (cc & (1 << j)) oc |= u64(1) << g.sites[j];
+                acc = cfma(mats[g.moff + r + D * cc], rho[(b | oc) + N * (b | orr)], acc);
+            }
+        }
+    }
+    c128 s = block_sum(acc);
+    if (threadIdx.x == 0) part[size_t(blockIdx.y) * nparts + blockIdx.x] = s;
+}
+
+// ------------------------------------------------------------ dot / trace ----
+__global__ __launch_bounds__(kB) void k_dot(const c128 *__restrict__ a, const c128 *__restrict__ b, u64 len,
+                                            u64 stride_a, c128 *__restrict__ part) {
+    c128 acc = mk(0, 0);
+    for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < len; i += u64(gridDim.x) * kB)
+        acc = cfmaconj(a[i * stride_a], b[i * stride_a], acc);
+    c128 s = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kB) void k_trace(const c128 *__restrict__ rho, u64 N, c128 *__restrict__ part) {
+    c128 acc = mk(0, 0);
+    for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < N; i += u64(gridDim.x) * kB)
+        acc = cadd(acc, rho[i + N * i]);
+    c128 s = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+static int finish_scalar(Ctx *c, c128 *part, int nb, double *out) {
+    c128 *res = part + kMaxParts;
+    MOSSQ_TRY(reduce_partials(c, part, nb, res));
+    c128 h;
+    MOSSQ_HIP(hipMemcpyAsync(&h, res, sizeof(c128), hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    out[0] = h.re;
+    out[1] = h.im;
+    return MOSSQ_OK;
+}
+
+// ----------------------------------------------------------- probabilities ----
+// part[chunk * nbins + m] = sum of |amp|^2 (or Re rho_ii) over the chunk's
+// members of readout bin m.
+__global__ __launch_bounds__(kB) void k_marginal(const c128 *__restrict__ s, int n, int is_dm, int k,
+                                                 ObsSites q, ObsSites rest, u64 nchunks, u64 chunk_len,
+                                                 double *__restrict__ part) {
+    const u64 nbins = u64(1) << k;
+    const u64 total = nbins * nchunks;
+    const u64 N = u64(1) << n;
+    for (u64 t = u64(blockIdx.x) * kB + threadIdx.x; t < total; t += u64(gridDim.x) * kB) {
+        u64 m = t % nbins, ch = t / nbins;
+        u64 hi = 0;
+        for (int j = 0; j < k; ++j)
+            if (m & (u64(1) << j)) hi |= u64(1) << q.s[j];
+        double acc = 0;
+        for (u64 r = ch * chunk_len; r < (ch + 1) * chunk_len; ++r) {
+            u64 lo = 0;
+            for (int j = 0; j < n - k; ++j)
+                if (r & (u64(1) << j)) lo |= u64(1) << rest.s[j];
+            u64 i = hi | lo;
+            if (is_dm) {
+                acc += s[i + N * i].re;
+            } else {
+                c128 a = s[i];
+                acc += a.re * a.re + a.im * a.im;
+            }
+        }
+        part[ch * nbins + m] = acc;
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_marginal_reduce(const double *__restrict__ part, u64 nbins, u64 nchunks,
+                                                        double *__restrict__ out) {
+    for (u64 m = u64(blockIdx.x) * kB + threadIdx.x; m < nbins; m += u64(gridDim.x) * kB) {
+        double acc = 0;
+        for (u64 ch = 0; ch < nchunks; ++ch) acc += part[ch * nbins + m];
+        out[m] = acc;
+    }
+}
+
+int dm_expect_groups_dev(Ctx *c, const Obs *o, const c128 *rho, c128 *dev_out, c128 *part) {
+    unsigned nb = blocks_for(u64(1) << o->n, 1, 64);
+    hipLaunchKernelGGL(k_dm_expect_groups, dim3(nb, o->G), dim3(kB), 0, c->stream, rho, o->n, o->d_groups,
+                       o->d_mats, part, int(nb));
+    MOSSQ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_rows, dim3(o->G), dim3(kB), 0, c->stream, part, int(nb), int(nb), dev_out);
+    MOSSQ_HIP(hipGetLastError());
+    return MOSSQ_OK;
+}
+
+}  // namespace mossq
+
+using namespace mossq;
+
+extern "C" {
+
+int mossq_obs_create(mossq_ctx *ctx, int n, int ngroups, const int32_t *group_k, const int32_t *sites,
+                     const double *mats, mossq_obs **out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(out && n >= 1 && n <= 40 && ngroups >= 1 && ngroups <= 4096 && group_k && mats, MOSSQ_EINVAL,
+                "bad observable arguments");
+    Obs *o = new Obs();
+    o->ctx = C(ctx);
+    o->n = n;
+    o->G = ngroups;
+    int so = 0, mo = 0;
+    const c128 *M = (const c128 *)mats;
+    for (int g = 0; g < ngroups; ++g) {
+        ObsGroup gr{};
+        gr.k = group_k[g];
+        if (gr.k < 0 || gr.k > 3 || gr.k > n) {
+            delete o;
+            return fail(MOSSQ_EINVAL, "observable group size must be 0..3 and <= n");
+        }
+        for (int j = 0; j < gr.k; ++j) {
+            gr.sites[j] = sites[so + j];
+            bool ok = gr.sites[j] >= 0 && gr.sites[j] < n && (j == 0 || gr.sites[j] > gr.sites[j - 1]);
+            if (!ok) {
+                delete o;
+                return fail(MOSSQ_EINVAL, "observable sites must be increasing and < n");
+            }
+        }
+        so += gr.k;
+        int D = 1 << gr.k;
+        gr.moff = mo;
+        for (int i = 0; i < D * D; ++i) o->h_mats.push_back(M[mo + i]);
+        // Adjoint seed matrix M + M^dagger, column-major: the seed of a real
+        // objective is (conj(z) H + z H^dagger) psi at z = 1
+        // (reference differentiation/expectation.jl:32-35).
+        for (int cc = 0; cc < D; ++cc)
+            for (int r = 0; r < D; ++r) {
+                o->h_seed.push_back(cadd(M[mo + r + D * cc], cconj(M[mo + cc + D * r])));
+                o->h_adj.push_back(cconj(M[mo + cc + D * r]));
+            }
+        mo += D * D;
+        o->h_groups.push_back(gr);
+    }
+    Ctx *c = o->ctx;
+    int st = dev_alloc(c, sizeof(ObsGroup) * ngroups, (void **)&o->d_groups);
+    if (st == MOSSQ_OK) st = dev_alloc(c, sizeof(c128) * o->h_mats.size(), (void **)&o->d_mats);
+    if (st == MOSSQ_OK) st = dev_alloc(c, sizeof(c128) * o->h_seed.size(), (void **)&o->d_mats_seed);
+    if (st == MOSSQ_OK) st = dev_alloc(c, sizeof(c128) * o->h_adj.size(), (void **)&o->d_mats_adj);
+    if (st != MOSSQ_OK) {
+        mossq_obs_destroy(reinterpret_cast<mossq_obs *>(o));
+        return st;
+    }
+    MOSSQ_HIP(hipMemcpyAsync(o->d_groups, o->h_groups.data(), sizeof(ObsGroup) * ngroups, hipMemcpyHostToDevice,
+                             c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(o->d_mats, o->h_mats.data(), sizeof(c128) * o->h_mats.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(o->d_mats_seed, o->h_seed.data(), sizeof(c128) * o->h_seed.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(o->d_mats_adj, o->h_adj.data(), sizeof(c128) * o->h_adj.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    *out = reinterpret_cast<mossq_obs *>(o);
+    return MOSSQ_OK;
+}
+
+int mossq_obs_destroy(mossq_obs *obs) {
+    if (!obs) return MOSSQ_OK;
+    Obs *o = reinterpret_cast<Obs *>(obs);
+    Ctx *c = o->ctx;
+    (void)hipSetDevice(c->device);
+    int st = MOSSQ_OK;
+    if (o->d_groups) st |= dev_free(c, o->d_groups);
+    if (o->d_mats) st |= dev_free(c, o->d_mats);
+    if (o->d_mats_seed) st |= dev_free(c, o->d_mats_seed);
+    if (o->d_mats_adj) st |= dev_free(c, o->d_mats_adj);
+    delete o;
+    return st ? MOSSQ_EDEVICE : MOSSQ_OK;
+}
+
+int mossq_sv_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *psi, double *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(obs && psi && out, MOSSQ_EINVAL, "null argument");
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    return expect_total_host(C(ctx), o, (const c128 *)psi, (const c128 *)psi, out);
+}
+
+int mossq_sv_cross_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *phi, const void *psi,
+                               double *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(obs && phi && psi && out, MOSSQ_EINVAL, "null argument");
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    return expect_total_host(C(ctx), o, (const c128 *)phi, (const c128 *)psi, out);
+}
+
+int mossq_sv_apply_operator(mossq_ctx *ctx, const mossq_obs *obs, const void *psi, void *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(obs && psi && out && psi != out, MOSSQ_EINVAL, "bad buffers (output must not alias input)");
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    return apply_op_dev(C(ctx), o, (const c128 *)psi, (c128 *)out, o->n, OpMats::Plain);
+}
+
+int mossq_dm_apply_operator(mossq_ctx *ctx, const mossq_obs *obs, const void *rho, void *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(obs && rho && out && rho != out, MOSSQ_EINVAL, "bad buffers (output must not alias input)");
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    MOSSQ_CHECK(o->n <= 20, MOSSQ_EINVAL, "density matrix too large");
+    return apply_op_dev(C(ctx), o, (const c128 *)rho, (c128 *)out, 2 * o->n, OpMats::Plain);
+}
+
+int mossq_dm_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *rho, double *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(obs && rho && out, MOSSQ_EINVAL, "null argument");
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    MOSSQ_CHECK(o->n <= 20, MOSSQ_EINVAL, "density matrix too large");
+    Ctx *c = C(ctx);
+    void *scr = nullptr;
+    MOSSQ_TRY(get_scratch(c, (u64(o->G) * 64 + o->G) * sizeof(c128), &scr));
+    c128 *part = (c128 *)scr, *res = part + size_t(o->G) * 64;
+    MOSSQ_TRY(dm_expect_groups_dev(c, o, (const c128 *)rho, res, part));
+    std::vector<c128> h(o->G);
+    MOSSQ_HIP(hipMemcpyAsync(h.data(), res, sizeof(c128) * o->G, hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    double re = 0, im = 0;
+    for (auto &v : h) {
+        re += v.re;
+        im += v.im;
+    }
+    out[0] = re;
+    out[1] = im;
+    return MOSSQ_OK;
+}
+
+int mossq_dot(mossq_ctx *ctx, const void *a, const void *b, uint64_t len, double *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(a && b && out, MOSSQ_EINVAL, "null argument");
+    Ctx *c = C(ctx);
+    void *scr = nullptr;
+    MOSSQ_TRY(get_scratch(c, (kMaxParts + 1) * sizeof(c128), &scr));
+    c128 *part = (c128 *)scr;
+    unsigned nb = blocks_for(len, 4, kMaxParts);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kB), 0, c->stream, (const c128 *)a, (const c128 *)b, len, u64(1),
+                       part);
+    MOSSQ_HIP(hipGetLastError());
+    return finish_scalar(c, part, int(nb), out);
+}
+
+int mossq_dm_trace(mossq_ctx *ctx, const void *rho, int n, double *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(rho && out && n >= 0 && n <= 20, MOSSQ_EINVAL, "bad argument");
+    Ctx *c = C(ctx);
+    void *scr = nullptr;
+    MOSSQ_TRY(get_scratch(c, (kMaxParts + 1) * sizeof(c128), &scr));
+    c128 *part = (c128 *)scr;
+    u64 N = u64(1) << n;
+    unsigned nb = blocks_for(N, 1, kMaxParts);
+    hipLaunchKernelGGL(k_trace, dim3(nb), dim3(kB), 0, c->stream, (const c128 *)rho, N, part);
+    MOSSQ_HIP(hipGetLastError());
+    return finish_scalar(c, part, int(nb), out);
+}
+
+int mossq_probabilities(mossq_ctx *ctx, const void *state, int n, int is_density, int k, const int32_t *qubits,
+                        double *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(state && out && qubits && n >= 1 && n <= 40 && k >= 1 && k <= n && k <= 24, MOSSQ_EINVAL,
+                "bad probabilities arguments");
+    MOSSQ_CHECK(!is_density || n <= 20, MOSSQ_EINVAL, "density matrix too large");
+    ObsSites q{}, rest{};
+    bool used[64] = {false};
+    for (int j = 0; j < k; ++j) {
+        MOSSQ_CHECK(qubits[j] >= 0 && qubits[j] < n && !used[qubits[j]], MOSSQ_EINVAL,
+                    "readout qubits must be distinct and in range");
+        used[qubits[j]] = true;
+        q.s[j] = qubits[j];
+    }
+    int r = 0;
+    for (int b = 0; b < n; ++b)
+        if (!used[b]) rest.s[r++] = b;
+    Ctx *c = C(ctx);
+    u64 nbins = u64(1) << k, nrest = u64(1) << (n - k);
+    // enough (bin, chunk) work items to fill the device, chunks of equal length
+    u64 nchunks = 1;
+    while (nbins * nchunks < (u64(1) << 16) && nchunks < nrest) nchunks <<= 1;
+    u64 chunk_len = nrest / nchunks;
+    void *scr = nullptr;
+    MOSSQ_TRY(get_scratch(c, (nbins * nchunks + nbins) * sizeof(double), &scr));
+    double *part = (double *)scr, *res = part + nbins * nchunks;
+    hipLaunchKernelGGL(k_marginal, dim3(blocks_for(nbins * nchunks, 1, 1u << 16)), dim3(kB), 0, c->stream,
+                       (const c128 *)state, n, is_density, k, q, rest, nchunks, chunk_len, part);
+    MOSSQ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_marginal_reduce, dim3(blocks_for(nbins, 1, 4096)), dim3(kB), 0, c->stream, part, nbins,
+                       nchunks, res);
+    MOSSQ_HIP(hipGetLastError());
+    MOSSQ_HIP(hipMemcpyAsync(out, res, nbins * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    return MOSSQ_OK;
+}
+
+}  // extern "C"

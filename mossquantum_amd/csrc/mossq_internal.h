@@ -1,0 +1,143 @@
+// Internal declarations shared by the translation units of libmossq_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mossq_hip.h"
+
+namespace mossq {
+
+using u64 = uint64_t;
+using i64 = int64_t;
+
+// ---------------------------------------------------------------- errors ----
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define MOSSQ_HIP(call)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return ::mossq::fail(MOSSQ_EDEVICE, std::string(#call) + ": " +          \
+                                                    hipGetErrorString(e_));          \
+    } while (0)
+
+#define MOSSQ_CHECK(cond, code, msg)                                                 \
+    do {                                                                             \
+        if (!(cond)) return ::mossq::fail((code), (msg));                            \
+    } while (0)
+
+#define MOSSQ_TRY(expr)                                                              \
+    do {                                                                             \
+        int s_ = (expr);                                                             \
+        if (s_ != MOSSQ_OK) return s_;                                               \
+    } while (0)
+
+// --------------------------------------------------------------- complex ----
+// complex128 as double2 (re, im); 16-byte aligned -> one dwordx4 access.
+struct __align__(16) c128 {
+    double re, im;
+};
+
+__host__ __device__ __forceinline__ c128 mk(double r, double i) { return c128{r, i}; }
+__host__ __device__ __forceinline__ c128 cadd(c128 a, c128 b) { return c128{a.re + b.re, a.im + b.im}; }
+__host__ __device__ __forceinline__ c128 csub(c128 a, c128 b) { return c128{a.re - b.re, a.im - b.im}; }
+__host__ __device__ __forceinline__ c128 cmul(c128 a, c128 b) {
+    return c128{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// acc + a*b
+__host__ __device__ __forceinline__ c128 cfma(c128 a, c128 b, c128 acc) {
+    return c128{fma(a.re, b.re, fma(-a.im, b.im, acc.re)), fma(a.re, b.im, fma(a.im, b.re, acc.im))};
+}
+// acc + conj(a)*b
+__host__ __device__ __forceinline__ c128 cfmaconj(c128 a, c128 b, c128 acc) {
+    return c128{fma(a.re, b.re, fma(a.im, b.im, acc.re)), fma(a.re, b.im, fma(-a.im, b.re, acc.im))};
+}
+__host__ __device__ __forceinline__ c128 cconj(c128 a) { return c128{a.re, -a.im}; }
+
+// ----------------------------------------------------------------- bits ----
+// Insert a zero bit at position q of x (bits >= q move up by one).
+__host__ __device__ __forceinline__ u64 insert0(u64 x, int q) {
+    u64 lo = x & ((u64(1) << q) - 1);
+    return ((x >> q) << (q + 1)) | lo;
+}
+// Insert zeros at k increasing positions.
+template <int K>
+__host__ __device__ __forceinline__ u64 insert0s(u64 x, const int *pos) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) x = insert0(x, pos[j]);
+    return x;
+}
+__host__ __device__ __forceinline__ u64 insert0s_dyn(u64 x, const int *pos, int k) {
+    for (int j = 0; j < k; ++j) x = insert0(x, pos[j]);
+    return x;
+}
+
+// ------------------------------------------------------------- context ----
+constexpr int kNumEvents = 16;
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    u64 budget = 0;  // 0 = unlimited
+    u64 in_use = 0;
+    std::vector<std::pair<void *, u64>> allocs;
+    hipEvent_t events[kNumEvents] = {};
+    // reusable device scratch (grown on demand, owned by the context)
+    void *scratch = nullptr;
+    u64 scratch_bytes = 0;
+    void *mat_stage = nullptr;  // device staging for per-call matrices
+    u64 mat_stage_bytes = 0;
+    double *host_pinned = nullptr;  // pinned host staging for results
+    u64 host_pinned_bytes = 0;
+};
+
+int ctx_check(mossq_ctx *ctx);
+Ctx *C(mossq_ctx *ctx);
+// Device allocation that respects the context's memory budget.
+int dev_alloc(Ctx *c, u64 bytes, void **p);
+int dev_free(Ctx *c, void *p);
+// Scratch of at least `bytes`, valid until the next scratch request.
+int get_scratch(Ctx *c, u64 bytes, void **p);
+// Upload `count` complex numbers to the context's matrix staging area.
+int stage_matrix(Ctx *c, const c128 *host, u64 count, const c128 **dev);
+int get_pinned(Ctx *c, u64 bytes, double **p);
+
+// --------------------------------------------------------- gate launches ----
+// Classification of a local matrix, used to pick a kernel.
+struct MatInfo {
+    bool monomial = false;  // exactly one nonzero per row and column
+    int nchanged = 0;       // rows with a non-identity action (monomial only)
+    int rows[32];           // changed rows
+    int src[32];            // source column of each changed row
+    c128 val[32];           // matrix entry of each changed row
+};
+MatInfo classify(const c128 *m, int dim);
+
+// Apply a dense matrix on `k` (sorted) bits of a 2^nbits vector; dispatches to the
+// monomial kernel when `info.monomial`.  `dmat` is the device copy of the matrix.
+int launch_dense(Ctx *c, c128 *v, int nbits, int k, const int *bits, const c128 *dmat,
+                 const MatInfo &info);
+// Reverse-pass fused kernel: applies matrix V on x and (unless y_fixed) on y,
+// and accumulates per-workgroup partials of sum_b x_b^dagger D y_b into part
+// (if D != nullptr; pair_parts(nbits) partials).  k <= 4.
+int launch_dense_pair(Ctx *c, c128 *x, c128 *y, int nbits, int k, const int *bits,
+                      const c128 *dV, const c128 *dD, c128 *part, bool y_fixed);
+int pair_parts(int nbits);
+// Host-matrix application (monomial detection, staging) on `k` sorted bits.
+int apply_host_matrix(Ctx *c, c128 *v, int nbits, int k, const int *bits, const c128 *m);
+int fill_basis0(Ctx *c, c128 *v, u64 len);
+int outer_product(Ctx *c, c128 *rho, const c128 *psi, int n);
+// kron(conj(U), U): the density-matrix action of a k-qubit gate on bits (pos, pos+n).
+void dm_superop_of_gate(const c128 *U, int k, std::vector<c128> &out);
+void conj_transpose(const c128 *m, int dim, std::vector<c128> &out);
+
+// Reduction helpers (deterministic two-stage).
+int reduce_partials(Ctx *c, const c128 *part, int nparts, c128 *dev_out);
+
+}  // namespace mossq

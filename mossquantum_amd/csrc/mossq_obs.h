@@ -1,0 +1,42 @@
+// Device form of an observable (QubitsOperator grouped by site tuple).
+#pragma once
+
+#include "mossq_internal.h"
+
+namespace mossq {
+
+struct ObsGroup {
+    int k;         // number of sites (0..3)
+    int sites[3];  // increasing 0-based sites
+    int moff;      // offset of the 2^k x 2^k column-major matrix
+};
+
+struct ObsSites {
+    int s[40];
+};
+
+struct Obs {
+    Ctx *ctx = nullptr;
+    int n = 0;
+    int G = 0;
+    std::vector<ObsGroup> h_groups;
+    std::vector<c128> h_mats;  // M_g
+    std::vector<c128> h_seed;  // M_g + M_g^dagger (pure-state adjoint seed)
+    std::vector<c128> h_adj;   // M_g^dagger (density-matrix adjoint seed)
+    ObsGroup *d_groups = nullptr;
+    c128 *d_mats = nullptr;
+    c128 *d_mats_seed = nullptr;
+    c128 *d_mats_adj = nullptr;
+};
+
+// Per-group <phi|M_g|psi> to dev_out[G]; part needs G * 512 entries.
+int expect_groups_dev(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, c128 *dev_out, c128 *part);
+// sum_g <phi|M_g|psi> copied to the host (out[0] re, out[1] im).
+int expect_total_host(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, double *out);
+// out = sum_g A_g psi on a 2^nbits vector with A_g = M_g, M_g + M_g^dagger or M_g^dagger.
+enum class OpMats { Plain, Seed, Adjoint };
+int apply_op_dev(Ctx *c, const Obs *o, const c128 *psi, c128 *out, int nbits, OpMats which);
+// Per-group tr(M_g rho) to dev_out[G]; part needs G * 64 entries.
+int dm_expect_groups_dev(Ctx *c, const Obs *o, const c128 *rho, c128 *dev_out, c128 *part);
+
+}  // namespace mossq

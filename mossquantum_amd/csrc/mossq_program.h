@@ -1,0 +1,61 @@
+// Lowered circuits: launch plans over a device matrix pool.
+#pragma once
+
+#include "mossq_obs.h"
+
+namespace mossq {
+
+// One unfused instruction, expressed on the bits of the stored vector
+// (state vector: the qubit bits; density matrix: the 2n bits of vec(rho)).
+struct Step {
+    bool channel = false;
+    int k = 0;        // register qubits of the instruction
+    int kb = 0;       // bits of the stored vector touched (k or 2k)
+    int bits[10] = {};
+    int fwd = -1;     // forward matrix offset in the device pool (2^kb x 2^kb)
+    MatInfo ifwd;
+    int rev = -1;     // reverse matrix: U^dagger (SV), its superoperator (DM), S^dagger (channel)
+    MatInfo irev;
+    int grad = -1;    // gradient matrix: D (SV) or D^dagger(x)I + I(x)D^T (DM), 2^kb x 2^kb
+    int grad_index = -1;
+    int u_src = -1;   // density 2-qubit gates: offset of U itself (for the fused row/column split)
+    // density gates wider than 2 qubits: applied as row pass then column pass
+    bool split = false;
+    int fwd_cols = -1, rev_cols = -1;
+    MatInfo ifwd_cols, irev_cols;
+};
+
+struct FusedPlan;  // LDS-tile sweep plan (tile.hip)
+
+struct Program {
+    Ctx *ctx = nullptr;
+    int n = 0;
+    int is_density = 0;
+    int nbits = 0;
+    bool has_channel = false;
+    int nderiv = 0;
+    std::vector<Step> steps;
+    std::vector<c128> h_pool;
+    c128 *d_pool = nullptr;
+    FusedPlan *fused = nullptr;
+};
+
+// Unfused forward application of steps [lo, hi).
+int run_steps(Ctx *c, const Program *p, c128 *state, int lo, int hi);
+
+// Fused plans (tile.hip).
+int fused_plan_build(Program *p);
+void fused_plan_free(Program *p);
+int fused_plan_sweeps(const Program *p);
+int fused_run(Ctx *c, const Program *p, c128 *state);
+
+// Pure-state adjoint enqueued on the stream, no host synchronisation.
+// in == nullptr starts from |0...0>.  Buffers a, b hold 2^n amplitudes.
+// value_dev receives G per-group partial values; grads_dev nderiv complex.
+struct AdjointScratch {
+    c128 *a = nullptr, *b = nullptr, *part = nullptr, *value_dev = nullptr, *grads_dev = nullptr;
+};
+int adjoint_pure_enqueue(Ctx *c, const Program *p, const Obs *o, const c128 *in, c128 *out_state,
+                         const AdjointScratch &s);
+
+}  // namespace mossq

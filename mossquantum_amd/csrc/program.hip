@@ -1,0 +1,436 @@
+// Programs (lowered circuits) and the adjoint gradient drivers.
+//
+// A program owns a device matrix pool holding, per instruction, the forward
+// matrix, the reverse (uncompute) matrix and the gradient matrix, all already
+// expressed on the bits of the stored vector:
+//   state vector   gate   : U, U^dagger, D = (dU)^dagger U      on the k qubit bits
+//   density matrix gate   : conj(U)(x)U, conj(U^dag)(x)U^dag,
+//                           D^dagger(x)I + I(x)D^T              on bits (pos, pos+n)
+//   density matrix channel: S, S^dagger                         on bits (pos, pos+n)
+// so every launch is one sweep of a dense or monomial block kernel.
+#include <algorithm>
+#include <cmath>
+
+#include "mossq_program.h"
+
+namespace mossq {
+
+constexpr int kB = 256;
+
+__global__ void k_fill_identity(c128 *__restrict__ v, int n) {
+    const u64 N = u64(1) << n, len = N * N;
+    for (u64 t = u64(blockIdx.x) * kB + threadIdx.x; t < len; t += u64(gridDim.x) * kB) {
+        u64 i = t & (N - 1), j = t >> n;
+        v[t] = mk(i == j ? 1.0 : 0.0, 0.0);
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_reduce_rows_real(const c128 *__restrict__ part, int nparts,
+                                                         c128 *__restrict__ out) {
+    __shared__ c128 red[kB / 64];
+    const c128 *p = part + size_t(blockIdx.x) * nparts;
+    c128 a = mk(0, 0);
+    for (int j = threadIdx.x; j < nparts; j += kB) a = cadd(a, p[j]);
+    for (int o = 32; o > 0; o >>= 1) {
+        a.re += __shfl_xor(a.re, o, 64);
+        a.im += __shfl_xor(a.im, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c128 s = red[0];
+        for (int w = 1; w < kB / 64; ++w) s = cadd(s, red[w]);
+        out[blockIdx.x] = s;
+    }
+}
+
+static int reduce_rows(Ctx *c, const c128 *part, int nrows, int nparts, c128 *out) {
+    if (nrows <= 0) return MOSSQ_OK;
+    hipLaunchKernelGGL(k_reduce_rows_real, dim3(nrows), dim3(kB), 0, c->stream, part, nparts, out);
+    MOSSQ_HIP(hipGetLastError());
+    return MOSSQ_OK;
+}
+
+static int push(std::vector<c128> &pool, const c128 *m, size_t count) {
+    int off = int(pool.size());
+    pool.insert(pool.end(), m, m + count);
+    return off;
+}
+
+// D^dagger (x) I + I (x) D^T in the local order (row bits low, column bits high).
+static void dm_grad_matrix(const c128 *D, int k, std::vector<c128> &out) {
+    int d = 1 << k, L = d * d;
+    out.assign(size_t(L) * L, mk(0, 0));
+    for (int a = 0; a < d; ++a)
+        for (int cc = 0; cc < d; ++cc)
+            for (int b = 0; b < d; ++b)
+                for (int e = 0; e < d; ++e) {
+                    c128 v = mk(0, 0);
+                    if (cc == e) v = cadd(v, cconj(D[b + d * a]));
+                    if (a == b) v = cadd(v, D[e + d * cc]);
+                    out[size_t(a + d * cc) + size_t(L) * (b + d * e)] = v;
+                }
+}
+
+static int apply_step_fwd(Ctx *c, const Program *p, const Step &s, c128 *v) {
+    if (s.split) {
+        MOSSQ_TRY(launch_dense(c, v, p->nbits, s.k, s.bits, p->d_pool + s.fwd, s.ifwd));
+        return launch_dense(c, v, p->nbits, s.k, s.bits + s.k, p->d_pool + s.fwd_cols, s.ifwd_cols);
+    }
+    return launch_dense(c, v, p->nbits, s.kb, s.bits, p->d_pool + s.fwd, s.ifwd);
+}
+
+static int apply_step_rev(Ctx *c, const Program *p, const Step &s, c128 *v) {
+    if (s.split) {
+        MOSSQ_TRY(launch_dense(c, v, p->nbits, s.k, s.bits, p->d_pool + s.rev, s.irev));
+        return launch_dense(c, v, p->nbits, s.k, s.bits + s.k, p->d_pool + s.rev_cols, s.irev_cols);
+    }
+    return launch_dense(c, v, p->nbits, s.kb, s.bits, p->d_pool + s.rev, s.irev);
+}
+
+int run_steps(Ctx *c, const Program *p, c128 *state, int lo, int hi) {
+    for (int i = lo; i < hi; ++i) MOSSQ_TRY(apply_step_fwd(c, p, p->steps[i], state));
+    return MOSSQ_OK;
+}
+
+// ------------------------------------------------------------- pure adjoint ----
+int adjoint_pure_enqueue(Ctx *c, const Program *p, const Obs *o, const c128 *in, c128 *out_state,
+                         const AdjointScratch &s) {
+    const u64 len = u64(1) << p->n;
+    if (in)
+        MOSSQ_HIP(hipMemcpyAsync(s.a, in, len * sizeof(c128), hipMemcpyDeviceToDevice, c->stream));
+    else
+        MOSSQ_TRY(fill_basis0(c, s.a, len));
+    // forward y = C x (reference circuit * x, differentiation/circuit.jl:83)
+    if (p->fused)
+        MOSSQ_TRY(fused_run(c, p, s.a));
+    else
+        MOSSQ_TRY(run_steps(c, p, s.a, 0, int(p->steps.size())));
+    if (out_state)
+        MOSSQ_HIP(hipMemcpyAsync(out_state, s.a, len * sizeof(c128), hipMemcpyDeviceToDevice, c->stream));
+    // value <H> and seed (H + H^dagger) y (differentiation/expectation.jl:32-35)
+    MOSSQ_TRY(expect_groups_dev(c, o, s.a, s.a, s.value_dev, s.part));
+    MOSSQ_TRY(apply_op_dev(c, o, s.a, s.b, p->n, OpMats::Seed));
+    // reverse: per instruction  lambda <- U^dag lambda, y <- U^dag y,
+    // g = Re <y|D|lambda>  (differentiation/circuit.jl:136-144)
+    const int P = pair_parts(p->n);
+    for (int i = int(p->steps.size()) - 1; i >= 0; --i) {
+        const Step &st = p->steps[i];
+        if (st.grad >= 0) {
+            MOSSQ_TRY(launch_dense_pair(c, s.a, s.b, p->n, st.kb, st.bits, p->d_pool + st.rev,
+                                        p->d_pool + st.grad, s.part + size_t(st.grad_index) * P, false));
+        } else if (st.irev.monomial || st.kb > 4) {
+            MOSSQ_TRY(apply_step_rev(c, p, st, s.a));
+            MOSSQ_TRY(apply_step_rev(c, p, st, s.b));
+        } else {
+            MOSSQ_TRY(launch_dense_pair(c, s.a, s.b, p->n, st.kb, st.bits, p->d_pool + st.rev, nullptr, nullptr,
+                                        false));
+        }
+    }
+    return reduce_rows(c, s.part, p->nderiv, P, s.grads_dev);
+}
+
+static int copy_results(Ctx *c, const Program *p, const Obs *o, const c128 *value_dev, const c128 *grads_dev,
+                        double *value, double *grads) {
+    std::vector<c128> hv(o->G), hg(p->nderiv);
+    MOSSQ_HIP(hipMemcpyAsync(hv.data(), value_dev, sizeof(c128) * o->G, hipMemcpyDeviceToHost, c->stream));
+    if (p->nderiv)
+        MOSSQ_HIP(hipMemcpyAsync(hg.data(), grads_dev, sizeof(c128) * p->nderiv, hipMemcpyDeviceToHost,
+                                 c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    double v = 0;
+    for (auto &x : hv) v += x.re;  // objective value = real(expectation)
+    if (value) *value = v;
+    if (grads)
+        for (int j = 0; j < p->nderiv; ++j) grads[j] = hg[j].re;
+    return MOSSQ_OK;
+}
+
+struct Allocs {
+    Ctx *c;
+    std::vector<void *> ptrs;
+    explicit Allocs(Ctx *cc) : c(cc) {}
+    int get(u64 bytes, void **p) {
+        int st = dev_alloc(c, bytes, p);
+        if (st == MOSSQ_OK) ptrs.push_back(*p);
+        return st;
+    }
+    ~Allocs() {
+        for (void *p : ptrs) dev_free(c, p);
+    }
+};
+
+static int adjoint_pure(Ctx *c, const Program *p, const Obs *o, const c128 *in, c128 *out_state, double *value,
+                        double *grads) {
+    const u64 len = u64(1) << p->n;
+    Allocs al(c);
+    AdjointScratch s;
+    const int P = pair_parts(p->n);
+    u64 nparts = std::max<u64>(u64(o->G) * 512, u64(std::max(p->nderiv, 1)) * P);
+    MOSSQ_TRY(al.get(len * sizeof(c128), (void **)&s.a));
+    MOSSQ_TRY(al.get(len * sizeof(c128), (void **)&s.b));
+    MOSSQ_TRY(al.get(nparts * sizeof(c128), (void **)&s.part));
+    MOSSQ_TRY(al.get(u64(o->G) * sizeof(c128), (void **)&s.value_dev));
+    MOSSQ_TRY(al.get(u64(std::max(p->nderiv, 1)) * sizeof(c128), (void **)&s.grads_dev));
+    MOSSQ_TRY(adjoint_pure_enqueue(c, p, o, in, out_state, s));
+    return copy_results(c, p, o, s.value_dev, s.grads_dev, value, grads);
+}
+
+// ---------------------------------------------------------- density adjoint ----
+// Checkpoints at segment starts (every s instructions); the reverse pass
+// recomputes each segment once from its checkpoint and keeps the states
+// before each of its instructions, so channels never need an inverse and
+// memory is (N/s + s + 2) states instead of one per channel
+// (the reference checkpoints before every channel, differentiation/circuit.jl:90-98).
+static int adjoint_density(Ctx *c, const Program *p, const Obs *o, const c128 *in, int in_is_density,
+                           c128 *out_state, double *value, double *grads) {
+    const int n = p->n, nb = p->nbits;
+    const u64 len = u64(1) << nb, bytes = len * sizeof(c128);
+    const int N = int(p->steps.size());
+    int seg = std::max(1, int(std::ceil(std::sqrt(double(std::max(N, 1))))));
+    int nseg = std::max(1, (N + seg - 1) / seg);
+    Allocs al(c);
+    c128 *delta = nullptr, *work = nullptr;
+    std::vector<c128 *> ckpt(nseg), stored(seg);
+    MOSSQ_TRY(al.get(bytes, (void **)&delta));
+    MOSSQ_TRY(al.get(bytes, (void **)&work));
+    for (auto &q : ckpt) MOSSQ_TRY(al.get(bytes, (void **)&q));
+    for (auto &q : stored) MOSSQ_TRY(al.get(bytes, (void **)&q));
+    const int P = pair_parts(nb);
+    c128 *part = nullptr, *value_dev = nullptr, *grads_dev = nullptr;
+    u64 nparts = std::max<u64>(u64(o->G) * 64, u64(std::max(p->nderiv, 1)) * P);
+    MOSSQ_TRY(al.get(nparts * sizeof(c128), (void **)&part));
+    MOSSQ_TRY(al.get(u64(o->G) * sizeof(c128), (void **)&value_dev));
+    MOSSQ_TRY(al.get(u64(std::max(p->nderiv, 1)) * sizeof(c128), (void **)&grads_dev));
+
+    if (!in)
+        MOSSQ_TRY(fill_basis0(c, work, len));
+    else if (in_is_density)
+        MOSSQ_HIP(hipMemcpyAsync(work, in, bytes, hipMemcpyDeviceToDevice, c->stream));
+    else
+        MOSSQ_TRY(outer_product(c, work, in, n));
+    for (int i = 0; i < N; ++i) {
+        if (i % seg == 0)
+            MOSSQ_HIP(hipMemcpyAsync(ckpt[i / seg], work, bytes, hipMemcpyDeviceToDevice, c->stream));
+        MOSSQ_TRY(apply_step_fwd(c, p, p->steps[i], work));
+    }
+    if (out_state) MOSSQ_HIP(hipMemcpyAsync(out_state, work, bytes, hipMemcpyDeviceToDevice, c->stream));
+    MOSSQ_TRY(dm_expect_groups_dev(c, o, work, value_dev, part));
+    // seed: z * H^dagger as a dense density matrix (differentiation/expectation.jl:69-72)
+    hipLaunchKernelGGL(k_fill_identity, dim3(4096), dim3(kB), 0, c->stream, work, n);
+    MOSSQ_HIP(hipGetLastError());
+    MOSSQ_TRY(apply_op_dev(c, o, work, delta, nb, OpMats::Adjoint));
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        int lo = sg * seg, hi = std::min(N, lo + seg);
+        MOSSQ_HIP(hipMemcpyAsync(stored[0], ckpt[sg], bytes, hipMemcpyDeviceToDevice, c->stream));
+        for (int i = lo; i + 1 < hi; ++i) {
+            MOSSQ_HIP(hipMemcpyAsync(stored[i - lo + 1], stored[i - lo], bytes, hipMemcpyDeviceToDevice, c->stream));
+            MOSSQ_TRY(apply_step_fwd(c, p, p->steps[i], stored[i - lo + 1]));
+        }
+        for (int i = hi - 1; i >= lo; --i) {
+            const Step &st = p->steps[i];
+            if (st.grad >= 0) {
+                // Delta <- U^dag Delta U; rho_before = U^dag rho U is the stored state;
+                // g = Re[<Delta, D^dag rho> + <Delta, rho D>] (differentiation/circuit.jl:155-172)
+                MOSSQ_TRY(launch_dense_pair(c, delta, stored[i - lo], nb, st.kb, st.bits, p->d_pool + st.rev,
+                                            p->d_pool + st.grad, part + size_t(st.grad_index) * P, true));
+            } else {
+                // channel: Delta <- S^dagger Delta (differentiation/circuit.jl:107-108); gate: U^dag Delta U
+                MOSSQ_TRY(apply_step_rev(c, p, st, delta));
+            }
+        }
+    }
+    MOSSQ_TRY(reduce_rows(c, part, p->nderiv, P, grads_dev));
+    return copy_results(c, p, o, value_dev, grads_dev, value, grads);
+}
+
+static int check_mat_range(u64 off, u64 count, u64 pool_len) {
+    MOSSQ_CHECK(off + count <= pool_len, MOSSQ_EINVAL, "matrix offset outside the pool");
+    return MOSSQ_OK;
+}
+
+}  // namespace mossq
+
+using namespace mossq;
+
+extern "C" {
+
+int mossq_program_create(mossq_ctx *ctx, int n, int is_density, int nops, const mossq_op *ops, const double *pool,
+                         uint64_t pool_len, int fuse, mossq_program **out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(out && (ops || nops == 0) && (pool || pool_len == 0) && nops >= 0, MOSSQ_EINVAL,
+                "bad program arguments");
+    MOSSQ_CHECK(n >= 1 && n <= (is_density ? 20 : 40), MOSSQ_EINVAL, "qubit count out of range");
+    const c128 *P = (const c128 *)pool;
+    Program *p = new Program();
+    p->ctx = C(ctx);
+    p->n = n;
+    p->is_density = is_density ? 1 : 0;
+    p->nbits = is_density ? 2 * n : n;
+    auto bad = [&](int code, const std::string &m) {
+        delete p;
+        return fail(code, m);
+    };
+    for (int i = 0; i < nops; ++i) {
+        const mossq_op &op = ops[i];
+        const int k = op.k;
+        if (op.kind != MOSSQ_OP_GATE && op.kind != MOSSQ_OP_CHANNEL) return bad(MOSSQ_EINVAL, "unknown op kind");
+        if (op.kind == MOSSQ_OP_CHANNEL && !is_density)
+            return bad(MOSSQ_EINVAL, "channels act on density matrices: promote the state before the first channel");
+        int kmax = op.kind == MOSSQ_OP_GATE ? 5 : 2;
+        if (k < 1 || k > kmax || k > n) return bad(MOSSQ_EUNSUPPORTED, "instruction size not supported");
+        for (int j = 0; j < k; ++j)
+            if (op.pos[j] < 0 || op.pos[j] >= n || (j && op.pos[j] <= op.pos[j - 1]))
+                return bad(MOSSQ_EINVAL, "positions must be increasing and in range");
+        const int d = 1 << k;
+        Step st;
+        st.k = k;
+        if (op.kind == MOSSQ_OP_CHANNEL) {
+            const int L = d * d;
+            if (check_mat_range(op.mat, u64(L) * L, pool_len) != MOSSQ_OK) return bad(MOSSQ_EINVAL, "bad matrix offset");
+            st.channel = true;
+            p->has_channel = true;
+            st.kb = 2 * k;
+            for (int j = 0; j < k; ++j) {
+                st.bits[j] = op.pos[j];
+                st.bits[j + k] = op.pos[j] + n;
+            }
+            std::vector<c128> Sd;
+            conj_transpose(P + op.mat, L, Sd);
+            st.fwd = push(p->h_pool, P + op.mat, size_t(L) * L);
+            st.ifwd = classify(P + op.mat, L);
+            st.rev = push(p->h_pool, Sd.data(), Sd.size());
+            st.irev = classify(Sd.data(), L);
+            if (op.dmat >= 0) return bad(MOSSQ_EUNSUPPORTED, "channels carry no parameters");
+            p->steps.push_back(st);
+            continue;
+        }
+        if (check_mat_range(op.mat, u64(d) * d, pool_len) != MOSSQ_OK) return bad(MOSSQ_EINVAL, "bad matrix offset");
+        if (op.dmat >= 0 && check_mat_range(op.dmat, u64(d) * d, pool_len) != MOSSQ_OK)
+            return bad(MOSSQ_EINVAL, "bad derivative offset");
+        const c128 *U = P + op.mat;
+        std::vector<c128> Ud;
+        conj_transpose(U, d, Ud);
+        if (!is_density) {
+            st.kb = k;
+            for (int j = 0; j < k; ++j) st.bits[j] = op.pos[j];
+            st.fwd = push(p->h_pool, U, size_t(d) * d);
+            st.ifwd = classify(U, d);
+            st.rev = push(p->h_pool, Ud.data(), Ud.size());
+            st.irev = classify(Ud.data(), d);
+            if (op.dmat >= 0) {
+                if (k > 4) return bad(MOSSQ_EUNSUPPORTED, "parameterized gates wider than 4 qubits");
+                st.grad = push(p->h_pool, P + op.dmat, size_t(d) * d);
+                st.grad_index = p->nderiv++;
+            }
+        } else if (k <= 2) {
+            st.kb = 2 * k;
+            for (int j = 0; j < k; ++j) {
+                st.bits[j] = op.pos[j];
+                st.bits[j + k] = op.pos[j] + n;
+            }
+            std::vector<c128> Sf, Sr;
+            dm_superop_of_gate(U, k, Sf);
+            dm_superop_of_gate(Ud.data(), k, Sr);
+            const int L = d * d;
+            st.fwd = push(p->h_pool, Sf.data(), Sf.size());
+            st.ifwd = classify(Sf.data(), L);
+            st.rev = push(p->h_pool, Sr.data(), Sr.size());
+            st.irev = classify(Sr.data(), L);
+            if (k == 2) st.u_src = push(p->h_pool, U, size_t(d) * d);
+            if (op.dmat >= 0) {
+                std::vector<c128> G;
+                dm_grad_matrix(P + op.dmat, k, G);
+                st.grad = push(p->h_pool, G.data(), G.size());
+                st.grad_index = p->nderiv++;
+            }
+        } else {
+            if (op.dmat >= 0)
+                return bad(MOSSQ_EUNSUPPORTED, "parameterized density-matrix gates wider than 2 qubits");
+            st.split = true;
+            st.kb = 2 * k;
+            for (int j = 0; j < k; ++j) {
+                st.bits[j] = op.pos[j];
+                st.bits[j + k] = op.pos[j] + n;
+            }
+            std::vector<c128> Uc(size_t(d) * d), Udc(size_t(d) * d);
+            for (size_t q = 0; q < Uc.size(); ++q) {
+                Uc[q] = cconj(U[q]);
+                Udc[q] = cconj(Ud[q]);
+            }
+            st.fwd = push(p->h_pool, U, size_t(d) * d);
+            st.ifwd = classify(U, d);
+            st.fwd_cols = push(p->h_pool, Uc.data(), Uc.size());
+            st.ifwd_cols = classify(Uc.data(), d);
+            st.rev = push(p->h_pool, Ud.data(), Ud.size());
+            st.irev = classify(Ud.data(), d);
+            st.rev_cols = push(p->h_pool, Udc.data(), Udc.size());
+            st.irev_cols = classify(Udc.data(), d);
+        }
+        p->steps.push_back(st);
+    }
+    Ctx *c = p->ctx;
+    int s = dev_alloc(c, std::max<size_t>(p->h_pool.size(), 1) * sizeof(c128), (void **)&p->d_pool);
+    if (s != MOSSQ_OK) {
+        delete p;
+        return s;
+    }
+    if (!p->h_pool.empty())
+        MOSSQ_HIP(hipMemcpyAsync(p->d_pool, p->h_pool.data(), p->h_pool.size() * sizeof(c128),
+                                 hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    if (fuse) {
+        s = fused_plan_build(p);
+        if (s != MOSSQ_OK) {
+            mossq_program_destroy(reinterpret_cast<mossq_program *>(p));
+            return s;
+        }
+    }
+    *out = reinterpret_cast<mossq_program *>(p);
+    return MOSSQ_OK;
+}
+
+int mossq_program_destroy(mossq_program *prog) {
+    if (!prog) return MOSSQ_OK;
+    Program *p = reinterpret_cast<Program *>(prog);
+    (void)hipSetDevice(p->ctx->device);
+    fused_plan_free(p);
+    int s = dev_free(p->ctx, p->d_pool);
+    delete p;
+    return s;
+}
+
+int mossq_program_sweeps(const mossq_program *prog, int *sweeps) {
+    MOSSQ_CHECK(prog && sweeps, MOSSQ_EINVAL, "null argument");
+    const Program *p = reinterpret_cast<const Program *>(prog);
+    *sweeps = p->fused ? fused_plan_sweeps(p) : int(p->steps.size());
+    return MOSSQ_OK;
+}
+
+int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(prog && state, MOSSQ_EINVAL, "null argument");
+    const Program *p = reinterpret_cast<const Program *>(prog);
+    Ctx *c = C(ctx);
+    MOSSQ_CHECK(p->ctx == c, MOSSQ_EINVAL, "program belongs to another context");
+    if (p->fused) return fused_run(c, p, (c128 *)state);
+    return run_steps(c, p, (c128 *)state, 0, int(p->steps.size()));
+}
+
+int mossq_adjoint(mossq_ctx *ctx, const mossq_program *prog, const mossq_obs *obs, const void *in,
+                  int in_is_density, void *out_state, double *value, double *grads) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(prog && obs, MOSSQ_EINVAL, "null argument");
+    const Program *p = reinterpret_cast<const Program *>(prog);
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    Ctx *c = C(ctx);
+    MOSSQ_CHECK(p->ctx == c && o->ctx == c, MOSSQ_EINVAL, "program/observable belong to another context");
+    MOSSQ_CHECK(o->n == p->n, MOSSQ_EINVAL, "observable and program qubit counts differ");
+    if (!p->is_density) {
+        MOSSQ_CHECK(!in_is_density, MOSSQ_EINVAL, "a state-vector program needs a state-vector input");
+        return adjoint_pure(c, p, o, (const c128 *)in, (c128 *)out_state, value, grads);
+    }
+    return adjoint_density(c, p, o, (const c128 *)in, in_is_density, (c128 *)out_state, value, grads);
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+"""Gradient protocol and adjoint objectives on the HIP backend.
+
+Mirrors reference src/api/gradient_methods.jl:14-138 (AdjointGradient,
+UnsupportedGradient, evaluate, supports_gradient, value_and_gradient,
+value_and_pullback, differentiable_value), src/api/objectives.jl:15-249
+(ExpectationObjective, BatchExpectationObjective) and the AdjointGradient
+drivers of src/differentiation/objectives.jl:183-295.  The reverse pass runs in
+libmossq_hip (mossq_adjoint / mossq_batch_adjoint).
+
+`differentiable_value` is the hook the reference's ChainRulesCore `rrule`
+wraps (ext/MOSSQuantumChainRulesCoreExt.jl:7-28); here `TorchDifferentiableValue`
+plays that role for torch.autograd.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+from .backend import DeviceObservable, HIPBackend, Program, backend
+from .circuit import QCircuit
+from .states import BatchDensityMatrix, BatchStateVector, DensityMatrix, StateVector
+
+
+class AbstractGradientMethod:
+    pass
+
+
+class AdjointGradient(AbstractGradientMethod):
+    """Adjoint differentiation (reference gradient_methods.jl:14-18)."""
+
+
+class UnsupportedGradient(Exception):
+    def __init__(self, method, objective):
+        super().__init__(f"{type(method).__name__} does not support parameter gradients for "
+                         f"{type(objective).__name__}")
+
+
+class ParameterMap:
+    """Slot -> independent-parameter identity, 1-based (reference parameter_map.jl:24-116)."""
+
+    def __init__(self, indices):
+        idx = [int(i) for i in indices]
+        count = max(idx, default=0)
+        if any(i < 1 or i > len(idx) for i in idx) or len(set(idx)) != count:
+            raise ValueError("parameter indices must use every integer in 1:count")
+        self.indices = tuple(idx)
+        self.count = count
+
+    @classmethod
+    def identity(cls, nslots):
+        return cls(range(1, nslots + 1))
+
+    def __len__(self):
+        return len(self.indices)
+
+    def expand(self, values):
+        values = np.asarray(values, dtype=np.float64)
+        if values.size != self.count:
+            raise ValueError(f"expected {self.count} independent parameters, received {values.size}")
+        if not np.all(np.isfinite(values)):
+            raise ValueError("parameter values must be finite")
+        return values[np.asarray(self.indices, dtype=np.int64) - 1]
+
+    def accumulate(self, slot_grads):
+        out = np.zeros(self.count)
+        np.add.at(out, np.asarray(self.indices, dtype=np.int64) - 1, slot_grads)
+        return out
+
+
+class ExpectationObjective:
+    """real(<observable>) after running the bound template on a copy of initial_state."""
+
+    def __init__(self, template: QCircuit, observable, initial_state, parameter_map=None, backend_=None):
+        self.template = template
+        self.observable = observable
+        self.initial_state = initial_state.copy()
+        self.parameter_map = parameter_map or ParameterMap.identity(template.nparameters())
+        if len(self.parameter_map) != template.nparameters():
+            raise ValueError("parameter map does not match the template's active slots")
+        self.backend = backend_ or backend(initial_state)
+
+
+class BatchExpectationObjective:
+    """Shared circuit/parameters/observable over a batch; reduction :mean/:sum/:weighted_sum."""
+
+    def __init__(self, template, observable, initial_state, parameter_map=None, backend_=None, reduction="mean",
+                 weights=None):
+        if not isinstance(initial_state, (BatchStateVector, BatchDensityMatrix)):
+            raise TypeError("a batch objective needs a batch state")
+        if reduction not in ("mean", "sum", "weighted_sum"):
+            raise ValueError("batch reduction must be mean, sum or weighted_sum")
+        if reduction == "weighted_sum":
+            if weights is None or len(weights) != initial_state.batch_size():
+                raise ValueError("weighted_sum requires one weight per sample")
+        elif weights is not None:
+            raise ValueError("weights are only valid with weighted_sum")
+        self.template = template
+        self.observable = observable
+        self.initial_state = initial_state.copy()
+        self.parameter_map = parameter_map or ParameterMap.identity(template.nparameters())
+        self.backend = backend_ or backend(initial_state)
+        self.reduction = reduction
+        self.weights = None if weights is None else np.asarray(weights, dtype=np.float64)
+
+    def sample_weights(self):
+        B = self.initial_state.batch_size()
+        if self.reduction == "mean":
+            return np.full(B, 1.0 / B)
+        if self.reduction == "sum":
+            return np.ones(B)
+        return self.weights
+
+
+def _adjoint_one(b: HIPBackend, circuit: QCircuit, observable, state):
+    """(value, slot gradients) of real(<obs>) for one bound circuit on one state."""
+    ctx = state.storage.ctx
+    is_dm = isinstance(state, DensityMatrix) or circuit.has_channel()
+    prog = Program(circuit, state.n, is_dm, ctx, fuse=b.fuse)
+    dob = DeviceObservable(observable, state.n, ctx)
+    nd = prog.lowered.nderiv
+    grads = np.zeros(max(nd, 1))
+    value = C.c_double(0)
+    check(ctx.lib.mossq_adjoint(ctx.h, prog.h, dob.h, state.storage.ptr, int(isinstance(state, DensityMatrix)),
+                                None, C.byref(value), _lib.dptr(grads)))
+    return value.value, prog.lowered.slot_gradients(grads[:nd])
+
+
+def supports_gradient(method, objective, values=None) -> bool:
+    if not isinstance(method, AdjointGradient):
+        return False
+    if not isinstance(objective, (ExpectationObjective, BatchExpectationObjective)):
+        return False
+    return isinstance(objective.backend, HIPBackend)
+
+
+def evaluate(objective, values):
+    """Forward execution only (reference evaluate, api/objectives.jl:38-42)."""
+    from .backend import apply_, expectation
+    if callable(objective) and not isinstance(objective, (ExpectationObjective, BatchExpectationObjective)):
+        return objective(values)
+    bound = objective.template.bind(objective.parameter_map.expand(values))
+    if isinstance(objective, BatchExpectationObjective):
+        vals = np.array([expectation(objective.observable, apply_(bound, c.copy(), objective.backend)).real
+                         for c in objective.initial_state.columns])
+        return float(np.dot(objective.sample_weights(), vals))
+    state = apply_(bound, objective.initial_state.copy(), objective.backend)
+    return float(expectation(objective.observable, state, objective.backend).real)
+
+
+def value_and_gradient(method, objective, values):
+    if not supports_gradient(method, objective):
+        raise UnsupportedGradient(method, objective)
+    pm = objective.parameter_map
+    bound = objective.template.bind(pm.expand(values))
+    if isinstance(objective, BatchExpectationObjective):
+        w = objective.sample_weights()
+        total, grad = 0.0, np.zeros(pm.count)
+        for wi, col in zip(w, objective.initial_state.columns):
+            v, g = _adjoint_one(objective.backend, bound, objective.observable, col)
+            total += wi * v
+            grad += wi * pm.accumulate(g)
+        return total, grad
+    v, g = _adjoint_one(objective.backend, bound, objective.observable, objective.initial_state)
+    return v, pm.accumulate(g)
+
+
+def value_and_pullback(method, objective, values):
+    v, g = value_and_gradient(method, objective, values)
+    saved = np.array(g, copy=True)
+    return v, (lambda seed: seed * saved)
+
+
+def differentiable_value(method, objective, values):
+    return value_and_gradient(method, objective, values)[0]
+
+
+def batched_adjoint(circuits, observables, nqubits, ctx: _lib.Context | None = None):
+    """Independent pure-state programs (own qubit count, parameters, observable),
+    all from |0...0>, differentiated in one library call (mossq_batch_adjoint).
+    Returns (values[B], [slot gradient vector per item])."""
+    ctx = ctx or _lib.default_context()
+    B = len(circuits)
+    progs = [Program(c, n, False, ctx, fuse=False) for c, n in zip(circuits, nqubits)]
+    obs = [DeviceObservable(o, n, ctx) for o, n in zip(observables, nqubits)]
+    offs = np.zeros(B, dtype=np.int64)
+    tot = 0
+    for i, p in enumerate(progs):
+        offs[i] = tot
+        tot += p.lowered.nderiv
+    vals = np.zeros(B)
+    grads = np.zeros(max(tot, 1))
+    ph = (C.c_void_p * B)(*[p.h.value for p in progs])
+    oh = (C.c_void_p * B)(*[o.h.value for o in obs])
+    check(ctx.lib.mossq_batch_adjoint(ctx.h, B, ph, oh, offs.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      _lib.dptr(vals), _lib.dptr(grads)))
+    out = [p.lowered.slot_gradients(grads[o:o + p.lowered.nderiv]) for p, o in zip(progs, offs)]
+    return vals, out
+
+
+_torch_rule = None
+
+
+def torch_differentiable_value(method, objective, values):
+    """AD adapter (the analogue of the reference's ChainRulesCore rrule): a
+    torch.autograd rule whose forward is the objective value and whose backward
+    is seed * gradient from value_and_pullback.  `values` is a real torch tensor."""
+    global _torch_rule
+    if _torch_rule is None:
+        import torch
+
+        class _Rule(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, vals, m, obj):
+                v, back = value_and_pullback(m, obj, vals.detach().cpu().numpy())
+                ctx.back = back
+                return vals.new_tensor(v)
+
+            @staticmethod
+            def backward(ctx, seed):
+                return seed.new_tensor(ctx.back(float(seed))), None, None
+
+        _torch_rule = _Rule
+    return _torch_rule.apply(values, method, objective)

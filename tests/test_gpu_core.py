@@ -1,0 +1,275 @@
+"""HIP backend parity against the CPU oracle (GPU tests, call through the C ABI).
+
+Tolerances (the plain complex128 method): amplitudes 1e-12, expectations
+1e-10, gradients 1e-9.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import dense
+import mossquantum_amd as mq
+from mossquantum_amd import lower
+from mossquantum_amd.gates import haar_unitary
+from mossquantum_amd.workloads import hea_circuit, random_circuit, haar_layers, tfim, xxz
+
+pytestmark = pytest.mark.gpu
+GOLD = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+AMP, EXP, GRAD = 1e-12, 1e-10, 1e-9
+
+
+def rand_state(n, rng):
+    v = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    return v / np.linalg.norm(v)
+
+
+def rand_rho(n, rng):
+    A = rng.standard_normal((1 << n, 1 << n)) + 1j * rng.standard_normal((1 << n, 1 << n))
+    r = A @ A.conj().T
+    return r / np.trace(r)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_gate_positions(ctx, k):
+    rng = np.random.default_rng(10 + k)
+    n = 7
+    for pos in itertools.combinations(range(1, n + 1), k):
+        U = haar_unitary(rng, 1 << k)
+        psi = rand_state(n, rng)
+        sv = mq.StateVector.from_host(psi)
+        mq.apply_(mq.QuantumGate(pos, U), sv)
+        want = psi.copy()
+        oracle.apply_gate(want, n, [p - 1 for p in pos], U)
+        assert np.max(np.abs(sv.to_host() - want)) < AMP, pos
+
+
+def test_large_qubit_gates(ctx):
+    rng = np.random.default_rng(3)
+    n = 20
+    psi = rand_state(n, rng)
+    sv = mq.StateVector.from_host(psi)
+    want = psi.copy()
+    for pos in [(1,), (2,), (5,), (13,), (20,), (1, 20), (19, 3), (4, 11, 17)]:
+        U = haar_unitary(rng, 1 << len(pos))
+        g = mq.QuantumGate(pos, U)
+        mq.apply_(g, sv)
+        oracle.apply_gate(want, n, [p - 1 for p in g.ordered_positions()], g.ordered_mat())
+    assert np.max(np.abs(sv.to_host() - want)) < AMP
+
+
+def test_monomial_gates(ctx):
+    """CZ, CNOT (both orientations), SWAP, iSWAP, X, S, T, TOFFOLI, FREDKIN, PHASE, Rz."""
+    rng = np.random.default_rng(5)
+    n = 9
+    gates = [mq.CZGate(2, 7), mq.CNOTGate(3, 1), mq.CNOTGate(1, 8), mq.SWAPGate(4, 9), mq.iSWAPGate(5, 2),
+             mq.XGate(6), mq.SGate(1), mq.TGate(9), mq.TOFFOLIGate(2, 5, 3), mq.FREDKINGate(8, 1, 4),
+             mq.PHASEGate(3, 0.77), mq.RzGate(7, 1.3), mq.CPHASEGate((4, 6), 0.4)]
+    psi = rand_state(n, rng)
+    sv = mq.StateVector.from_host(psi)
+    want = psi.copy()
+    for g in gates:
+        mq.apply_(g, sv)
+        oracle.apply_gate(want, n, [p - 1 for p in g.ordered_positions()], g.ordered_mat())
+    assert np.max(np.abs(sv.to_host() - want)) < AMP
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_dm_gates(ctx, k):
+    rng = np.random.default_rng(20 + k)
+    n = 5
+    rho = rand_rho(n, rng)
+    dm = mq.DensityMatrix.from_host(rho)
+    want = rho.reshape(-1, order="F").copy()
+    for pos in list(itertools.combinations(range(1, n + 1), k))[:5]:
+        U = haar_unitary(rng, 1 << k)
+        mq.apply_(mq.QuantumGate(pos, U), dm)
+        oracle.dm_apply_gate(want, n, [p - 1 for p in pos], U)
+    assert np.max(np.abs(dm.to_host().reshape(-1, order="F") - want)) < AMP
+
+
+def test_channels_and_promotion(ctx):
+    rng = np.random.default_rng(8)
+    n = 5
+    psi = rand_state(n, rng)
+    sv = mq.StateVector.from_host(psi)
+    st = mq.apply_(mq.AmplitudeDamping(3, 0.3), sv)  # SV -> DM promotion
+    assert isinstance(st, mq.DensityMatrix)
+    want = np.outer(psi, psi.conj()).reshape(-1, order="F")
+    oracle.dm_apply_channel(want, n, [2], mq.AmplitudeDamping(3, 0.3).ordered_supermat())
+    for ch in [mq.Depolarizing(1, 0.2), mq.Depolarizing((2, 5), 0.1), mq.PhaseDamping(4, 0.36)]:
+        st = mq.apply_(ch, st)
+        oracle.dm_apply_channel(want, n, [p - 1 for p in ch.ordered_positions()], ch.ordered_supermat())
+    assert np.max(np.abs(st.to_host().reshape(-1, order="F") - want)) < AMP
+    assert abs(mq.trace(st) - 1) < 1e-12
+
+
+def test_channel_dagger(ctx):
+    rng = np.random.default_rng(9)
+    n = 4
+    X = rand_rho(n, rng)
+    ch = mq.Depolarizing((1, 3), 0.2)
+    dm = mq.DensityMatrix.from_host(X)
+    lib = ctx.lib
+    pos = np.array([0, 2], dtype=np.int32)
+    S = np.ascontiguousarray(ch.ordered_supermat().reshape(-1, order="F"))
+    mq._lib.check(lib.mossq_dm_apply_channel(ctx.h, dm.storage.ptr, n, 2, mq._lib.iptr(pos),
+                                             S.view(np.float64).ctypes.data_as(mq._lib._DP), 1))
+    want = X.reshape(-1, order="F").copy()
+    oracle.dm_apply_channel(want, n, [0, 2], ch.ordered_supermat(), dagger=True)
+    assert np.max(np.abs(dm.to_host().reshape(-1, order="F") - want)) < AMP
+
+
+def test_expectations(ctx):
+    rng = np.random.default_rng(11)
+    for n in (4, 9, 16):
+        psi = rand_state(n, rng)
+        sv = mq.StateVector.from_host(psi)
+        for H in (tfim(n), xxz(n)):
+            g = H.groups()
+            assert abs(mq.expectation(H, sv) - oracle.expect(psi, n, g)) < EXP
+            phi = rand_state(n, rng)
+            assert abs(mq.cross_expectation(mq.StateVector.from_host(phi), H, sv)
+                       - oracle.cross_expect(phi, psi, n, g)) < EXP
+            out = mq.apply(H, sv).to_host()
+            assert np.max(np.abs(out - oracle.apply_op(psi, n, g))) < AMP
+    n = 6
+    rho = rand_rho(n, rng)
+    dm = mq.DensityMatrix.from_host(rho)
+    H = tfim(n)
+    assert abs(mq.expectation(H, dm) - oracle.dm_expect(rho.reshape(-1, order="F"), n, H.groups())) < EXP
+    hr = mq.apply(H, dm).to_host()
+    assert np.max(np.abs(hr.reshape(-1, order="F")
+                         - oracle.apply_op(rho.reshape(-1, order="F"), 2 * n, H.groups()))) < AMP
+    assert abs(mq.purity(dm) - np.real(np.vdot(rho, rho))) < EXP
+
+
+def test_probabilities(ctx):
+    rng = np.random.default_rng(12)
+    n = 10
+    psi = rand_state(n, rng)
+    sv = mq.StateVector.from_host(psi)
+    for qs in ([1], [3, 1], [10, 2, 5], list(range(1, 11))):
+        raw = oracle.probabilities(psi, n, False, qs)
+        assert np.max(np.abs(mq.probabilities(sv, qs) - raw / raw.sum())) < 1e-14
+    rho = rand_rho(5, rng)
+    raw = oracle.probabilities(rho.reshape(-1, order="F"), 5, True, [2, 4])
+    assert np.max(np.abs(mq.probabilities(mq.DensityMatrix.from_host(rho), [2, 4]) - raw / raw.sum())) < 1e-14
+
+
+def test_c0_golden(ctx):
+    f = np.load(GOLD / "c0_hea12.npz")
+    n, L = 12, 10
+    c = hea_circuit(n, L, f["angles"])
+    for fuse in (False, True):
+        b = mq.HIPBackend(ctx, fuse=fuse)
+        sv = mq.apply_(c, mq.StateVector.zeros(n), b)
+        assert np.max(np.abs(sv.to_host() - f["amplitudes"])) < AMP, fuse
+        assert abs(mq.expectation(tfim(n), sv).real - float(f["energy"])) < EXP
+
+
+@pytest.mark.parametrize("n", [12, 14, 17])
+def test_fused_random_circuit(ctx, n):
+    U = haar_layers(n, 6, 77 + n)
+    c = random_circuit(n, 6, U)
+    psi0 = rand_state(n, np.random.default_rng(n))
+    want = psi0.copy()
+    oracle.run_program(want, lower(c, n, False))
+    for fuse in (False, True):
+        sv = mq.StateVector.from_host(psi0)
+        mq.apply_(c, sv, mq.HIPBackend(ctx, fuse=fuse))
+        assert np.max(np.abs(sv.to_host() - want)) < AMP, fuse
+    p = mq.Program(c, n, False, ctx, fuse=True)
+    assert p.sweeps() < len(list(c.flat()))
+
+
+def test_fused_density_program(ctx):
+    n = 6  # 12 bits: one tile
+    rng = np.random.default_rng(13)
+    c = hea_circuit(n, 2, rng.uniform(0, 2 * np.pi, 2 * n * 2), noise=(0.01, 0.02))
+    want = np.zeros(1 << (2 * n), dtype=np.complex128)
+    want[0] = 1
+    oracle.run_program(want, lower(c, n, True))
+    for fuse in (False, True):
+        dm = mq.apply_(c, mq.DensityMatrix.zeros(n), mq.HIPBackend(ctx, fuse=fuse))
+        assert np.max(np.abs(dm.to_host().reshape(-1, order="F") - want)) < AMP, fuse
+    n = 7  # 14 bits
+    c = hea_circuit(n, 1, rng.uniform(0, 2 * np.pi, 2 * n), noise=(0.01, 0.02))
+    want = np.zeros(1 << (2 * n), dtype=np.complex128)
+    want[0] = 1
+    oracle.run_program(want, lower(c, n, True))
+    dm = mq.apply_(c, mq.DensityMatrix.zeros(n), mq.HIPBackend(ctx, fuse=True))
+    assert np.max(np.abs(dm.to_host().reshape(-1, order="F") - want)) < AMP
+
+
+def test_adjoint_c0(ctx):
+    f = np.load(GOLD / "c0_hea12.npz")
+    n, L = 12, 10
+    tpl = hea_circuit(n, L)
+    obj = mq.ExpectationObjective(tpl, tfim(n), mq.StateVector.zeros(n))
+    v, g = mq.value_and_gradient(mq.AdjointGradient(), obj, f["angles"])
+    assert abs(v - float(f["energy"])) < EXP
+    assert np.max(np.abs(g - f["grads"])) < GRAD
+
+
+def test_adjoint_density_golden(ctx):
+    f = np.load(GOLD / "c2_noisy4.npz")
+    n, L = 4, 2
+    tpl = hea_circuit(n, L, noise=(0.01, 0.02))
+    obj = mq.ExpectationObjective(tpl, tfim(n), mq.DensityMatrix.zeros(n))
+    v, g = mq.value_and_gradient(mq.AdjointGradient(), obj, f["angles"])
+    assert abs(v - float(f["energy"])) < EXP
+    assert np.max(np.abs(g - f["grads"])) < GRAD
+    # pure input promoted at the first channel gives the same gradient
+    obj2 = mq.ExpectationObjective(tpl, tfim(n), mq.StateVector.zeros(n))
+    v2, g2 = mq.value_and_gradient(mq.AdjointGradient(), obj2, f["angles"])
+    assert abs(v2 - v) < EXP and np.max(np.abs(g2 - g)) < GRAD
+
+
+def test_adjoint_closed_forms(ctx):
+    Z = mq.QubitsOperator([mq.QubitsTerm({1: "Z"})])
+    for th in (0.3, 2.9):
+        obj = mq.ExpectationObjective(mq.QCircuit([mq.RyGate(1, 0.0)]), Z, mq.StateVector.zeros(1))
+        v, g = mq.value_and_gradient(mq.AdjointGradient(), obj, [th])
+        assert abs(v - np.cos(th)) < 1e-14 and abs(g[0] + np.sin(th)) < 1e-14
+        for gam in (0.25, 1.0):
+            obj = mq.ExpectationObjective(mq.QCircuit([mq.RyGate(1, 0.0), mq.AmplitudeDamping(1, gam)]), Z,
+                                          mq.DensityMatrix.zeros(1))
+            v, g = mq.value_and_gradient(mq.AdjointGradient(), obj, [th])
+            assert abs(v - (gam + (1 - gam) * np.cos(th))) < 1e-12
+            assert abs(g[0] + (1 - gam) * np.sin(th)) < 1e-12
+
+
+def test_batched_adjoint_vs_single(ctx):
+    rng = np.random.default_rng(14)
+    sizes = [10, 13, 11, 15, 10]
+    circs, obs, want_v, want_g = [], [], [], []
+    for n in sizes:
+        ang = rng.uniform(0, 2 * np.pi, 2 * n * 2)
+        c = hea_circuit(n, 2, ang)
+        circs.append(c)
+        obs.append(xxz(n))
+        v, g = oracle.adjoint(lower(c, n, False), xxz(n).groups())
+        want_v.append(v)
+        want_g.append(g)
+    vals, grads = mq.batched_adjoint(circs, obs, sizes, ctx)
+    for i in range(len(sizes)):
+        assert abs(vals[i] - want_v[i]) < EXP
+        assert np.max(np.abs(grads[i] - want_g[i])) < GRAD
+
+
+def test_unsupported_before_payload(ctx):
+    sv = mq.StateVector.zeros(3)
+    big = mq.QubitsOperator([mq.QubitsTerm({1: "X", 2: "X", 3: "X"})])
+    assert mq.supports(mq.backend(sv), big, sv, "expectation")
+    wide = mq.QuantumGate((1, 2, 3), np.eye(8))
+    assert mq.supports(mq.backend(sv), wide, sv, "apply")
+    with pytest.raises(mq.UnsupportedOperation):
+        mq.apply_(mq.QuantumGate((1, 4), np.eye(4)), sv)  # position outside the register
+    # invalid arguments come back as status codes, not faults
+    bad = np.array([2, 1], dtype=np.int32)
+    eye = np.eye(4, dtype=np.complex128).reshape(-1)
+    st = ctx.lib.mossq_sv_apply_gate(ctx.h, sv.storage.ptr, 3, 2, mq._lib.iptr(bad),
+                                     eye.view(np.float64).ctypes.data_as(mq._lib._DP))
+    assert st == mq._lib.MOSSQ_EINVAL
