@@ -162,6 +162,20 @@ class Program:
             pass
 
 
+def plan_emulate(circuit: QCircuit, n: int, is_density: bool, state: np.ndarray):
+    """Run the fused plan of `circuit` on a host vector with the tile kernel's
+    index arithmetic (no device).  Returns (new state, stats dict)."""
+    lw = lower(circuit, n, is_density)
+    recs, pool = lw.arrays()
+    st = np.ascontiguousarray(np.array(state, dtype=np.complex128).reshape(-1, order="F"))
+    stats = (C.c_int * 8)()
+    check(_lib.load().mossq_plan_emulate(n, int(is_density), len(lw.records), recs,
+                                         pool.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), pool.size,
+                                         st.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), stats))
+    keys = ("sweeps", "unfused", "segments", "transitions", "tile_ops", "dense1", "dense2", "diag")
+    return st, dict(zip(keys, list(stats)))
+
+
 # ------------------------------------------------------------------- apply! ----
 def _mat_ptr(m):
     a = np.ascontiguousarray(np.asarray(m, dtype=np.complex128).reshape(-1, order="F"))

@@ -40,6 +40,10 @@ struct Program {
     FusedPlan *fused = nullptr;
 };
 
+// Host side of mossq_program_create: validate records, build steps and pool.
+int program_build_host(Program *p, int n, int is_density, int nops, const mossq_op *ops, const double *pool,
+                       uint64_t pool_len);
+
 // Unfused forward application of steps [lo, hi).
 int run_steps(Ctx *c, const Program *p, c128 *state, int lo, int hi);
 

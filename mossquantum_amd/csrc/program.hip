@@ -249,28 +249,17 @@ static int check_mat_range(u64 off, u64 count, u64 pool_len) {
     return MOSSQ_OK;
 }
 
-}  // namespace mossq
-
-using namespace mossq;
-
-extern "C" {
-
-int mossq_program_create(mossq_ctx *ctx, int n, int is_density, int nops, const mossq_op *ops, const double *pool,
-                         uint64_t pool_len, int fuse, mossq_program **out) {
-    MOSSQ_TRY(ctx_check(ctx));
-    MOSSQ_CHECK(out && (ops || nops == 0) && (pool || pool_len == 0) && nops >= 0, MOSSQ_EINVAL,
-                "bad program arguments");
+// Validate the instruction records and build the host side of a program
+// (steps and matrix pool); no device work.
+int program_build_host(Program *p, int n, int is_density, int nops, const mossq_op *ops, const double *pool,
+                       uint64_t pool_len) {
+    MOSSQ_CHECK((ops || nops == 0) && (pool || pool_len == 0) && nops >= 0, MOSSQ_EINVAL, "bad program arguments");
     MOSSQ_CHECK(n >= 1 && n <= (is_density ? 20 : 40), MOSSQ_EINVAL, "qubit count out of range");
     const c128 *P = (const c128 *)pool;
-    Program *p = new Program();
-    p->ctx = C(ctx);
     p->n = n;
     p->is_density = is_density ? 1 : 0;
     p->nbits = is_density ? 2 * n : n;
-    auto bad = [&](int code, const std::string &m) {
-        delete p;
-        return fail(code, m);
-    };
+    auto bad = [&](int code, const std::string &m) { return fail(code, m); };
     for (int i = 0; i < nops; ++i) {
         const mossq_op &op = ops[i];
         const int k = op.k;
@@ -369,8 +358,28 @@ int mossq_program_create(mossq_ctx *ctx, int n, int is_density, int nops, const 
         }
         p->steps.push_back(st);
     }
+    return MOSSQ_OK;
+}
+
+}  // namespace mossq
+
+using namespace mossq;
+
+extern "C" {
+
+int mossq_program_create(mossq_ctx *ctx, int n, int is_density, int nops, const mossq_op *ops, const double *pool,
+                         uint64_t pool_len, int fuse, mossq_program **out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(out, MOSSQ_EINVAL, "null output");
+    Program *p = new Program();
+    p->ctx = C(ctx);
+    int s = program_build_host(p, n, is_density, nops, ops, pool, pool_len);
+    if (s != MOSSQ_OK) {
+        delete p;
+        return s;
+    }
     Ctx *c = p->ctx;
-    int s = dev_alloc(c, std::max<size_t>(p->h_pool.size(), 1) * sizeof(c128), (void **)&p->d_pool);
+    s = dev_alloc(c, std::max<size_t>(p->h_pool.size(), 1) * sizeof(c128), (void **)&p->d_pool);
     if (s != MOSSQ_OK) {
         delete p;
         return s;

@@ -9,16 +9,22 @@
 // register bits are changed by a transposition through LDS (padded one slot per
 // 16 to spread ds_read_b128 across banks).  Diagonal gates (CZ, Rz, phases,
 // their density-matrix forms) are applied from the amplitude index and may use
-// any global bit, inside or outside the tile.
+// any global bit, inside or outside the tile: sign-only diagonals flip signs,
+// diagonals on <= 2 bits select among four uniform values, wider ones gather.
 //
 // The planner (host) takes the program's instructions in order, moves
 // instructions past earlier ones they commute with (disjoint supports, or both
-// diagonal), and packs them greedily into sweeps of at most 12 tile bits, then
-// orders each sweep's instructions into register-bit segments to minimise LDS
-// transpositions, and multiplies adjacent instructions on the same bits into
-// one matrix.
+// diagonal), and packs them greedily into sweeps of at most 12 tile bits; it
+// then orders each sweep's instructions into register-bit segments, multiplies
+// adjacent instructions on the same bits into one matrix, and absorbs
+// diagonals on register bits into the dense gates before them.  A sweep loads
+// and stores in the layout of its first and last segment when that layout
+// keeps the three lowest tile bits on lanes (coalescing), saving transpositions.
+//
+// emulate_plan() executes a plan on the host with the same index arithmetic,
+// so the planner is tested on a CPU against the oracle.
 #include <algorithm>
-#include <set>
+#include <cstring>
 
 #include "mossq_program.h"
 
@@ -30,12 +36,20 @@ constexpr int TT = 256;  // threads per tile
 constexpr int TN = 16;   // amplitudes per thread
 constexpr int kCoalesceBits = 3;
 
+enum TOpType { T_DENSE1 = 0, T_DENSE2 = 1, T_SIGN = 2, T_SELECT = 3, T_GATHER = 4 };
+
+// Diagonal index of register slot j: idx = f | rp(j), where f collects the op
+// bits that are not register bits from the global index g0 of the thread's
+// slot 0 (qbit[i] >= 0), and rp(j) (4 bits per slot, precomputed by the planner
+// for the segment's register layout) collects the register bits.
 struct TOp {
-    int type;  // 0 dense 1-bit, 1 dense 2-bit, 2 diagonal
-    int a, b;  // dense: register-bit indices (a < b)
-    int k;     // diagonal: number of bits
-    int dsrc[4];  // diagonal bit sources: 0..11 tile-local bit, >= 100: global bit (src - 100)
-    int moff;     // offset in the plan's device pool
+    int type;
+    int a, b;      // dense: register-bit indices (a < b)
+    int k;         // diagonal: number of bits
+    int qbit[4];   // diagonal: global bit of op bit i if it is not a register bit, else -1
+    unsigned rp_lo, rp_hi;  // diagonal: rp(j) for j < 8 (lo) and j >= 8 (hi), 4 bits each
+    int negmask;   // T_SIGN: bit idx set -> negate
+    int moff;      // offset in the plan's device pool (matrix / diagonal values)
 };
 
 struct TSeg {
@@ -50,7 +64,7 @@ struct TSweepArgs {
     int seg0, seg1;
 };
 
-__device__ __forceinline__ int lds_pad(int l) { return l + (l >> 4); }
+__host__ __device__ __forceinline__ int lds_pad(int l) { return l + (l >> 4); }
 
 template <int A>
 __device__ __forceinline__ void reg_apply1(c128 *v, c128 m00, c128 m10, c128 m01, c128 m11) {
@@ -63,40 +77,36 @@ __device__ __forceinline__ void reg_apply1(c128 *v, c128 m00, c128 m10, c128 m01
     }
 }
 
+// Row by row, so only one matrix row (4 entries) is live in scalar registers.
 template <int A, int B>
 __device__ __forceinline__ void reg_apply2(c128 *v, const c128 *__restrict__ M) {
-    c128 m[16];
+    c128 o[TN];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) m[i] = M[i];
+    for (int r = 0; r < 4; ++r) {
+        const c128 m0 = M[r], m1 = M[r + 4], m2 = M[r + 8], m3 = M[r + 12];
+        const int orow = ((r & 1) << A) | ((r >> 1) << B);
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-        if (j & ((1 << A) | (1 << B))) continue;
-        const int i0 = j, i1 = j | (1 << A), i2 = j | (1 << B), i3 = j | (1 << A) | (1 << B);
-        c128 in0 = v[i0], in1 = v[i1], in2 = v[i2], in3 = v[i3];
-        c128 o[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            c128 acc = cmul(m[r], in0);
-            acc = cfma(m[r + 4], in1, acc);
-            acc = cfma(m[r + 8], in2, acc);
-            acc = cfma(m[r + 12], in3, acc);
-            o[r] = acc;
+        for (int j = 0; j < TN; ++j) {
+            if (j & ((1 << A) | (1 << B))) continue;
+            c128 acc = cmul(m0, v[j]);
+            acc = cfma(m1, v[j | (1 << A)], acc);
+            acc = cfma(m2, v[j | (1 << B)], acc);
+            acc = cfma(m3, v[j | (1 << A) | (1 << B)], acc);
+            o[j | orow] = acc;
         }
-        v[i0] = o[0];
-        v[i1] = o[1];
-        v[i2] = o[2];
-        v[i3] = o[3];
     }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) v[j] = o[j];
 }
 
-__device__ __forceinline__ int deposit4(int j, const int *reg) {
+__host__ __device__ __forceinline__ int deposit4(int j, const int *reg) {
     return (((j >> 0) & 1) << reg[0]) | (((j >> 1) & 1) << reg[1]) | (((j >> 2) & 1) << reg[2]) |
            (((j >> 3) & 1) << reg[3]);
 }
 
 // Local index contribution of thread t when `reg` are the register bits: the
 // 8 thread bits are the tile bits not in reg, in increasing order.
-__device__ __forceinline__ int thread_deposit(int t, const int *reg) {
+__host__ __device__ __forceinline__ int thread_deposit(int t, const int *reg) {
     int out = 0, k = 0;
 #pragma unroll
     for (int b = 0; b < TM; ++b) {
@@ -109,6 +119,32 @@ __device__ __forceinline__ int thread_deposit(int t, const int *reg) {
     return out;
 }
 
+// Global offset of a tile-local index.
+__host__ __device__ __forceinline__ u64 local_to_global(int l, const int *tbits) {
+    u64 g = 0;
+#pragma unroll
+    for (int b = 0; b < TM; ++b)
+        if ((l >> b) & 1) g |= u64(1) << tbits[b];
+    return g;
+}
+
+// Non-register part of a diagonal index, from the thread's slot-0 global index.
+__host__ __device__ __forceinline__ int diag_f(const TOp &op, u64 g0) {
+    int f = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (op.qbit[i] >= 0) f |= int((g0 >> op.qbit[i]) & 1) << i;
+    return f;
+}
+
+__host__ __device__ __forceinline__ int diag_rp(const TOp &op, int j) {
+    return int(((j < 8 ? op.rp_lo : op.rp_hi) >> (4 * (j & 7))) & 15u);
+}
+
+__device__ __forceinline__ double flip_sign(double x, unsigned flip) {
+    return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(flip) << 63));
+}
+
 __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweepArgs sw,
                                                   const TSeg *__restrict__ segs, const TOp *__restrict__ ops,
                                                   const c128 *__restrict__ pool) {
@@ -118,25 +154,25 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
     u64 base = 0;
     for (int i = 0; i < sw.ncbits; ++i)
         if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
-    u64 gt = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-        if ((t >> i) & 1) gt |= u64(1) << sw.tbits[i];
-    u64 gj[TN];
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-        u64 g = 0;
-#pragma unroll
-        for (int i = 0; i < TRB; ++i)
-            if ((j >> i) & 1) g |= u64(1) << sw.tbits[8 + i];
-        gj[j] = base | gt | g;
-    }
-    c128 v[TN];
-#pragma unroll
-    for (int j = 0; j < TN; ++j) v[j] = state[gj[j]];
 
-    int cur[TRB] = {8, 9, 10, 11};
-    int tdep = t;  // thread bits are tile bits 0..7 in the load layout
+    int cur[TRB];
+    {
+        const TSeg s0 = segs[sw.seg0];
+#pragma unroll
+        for (int i = 0; i < TRB; ++i) cur[i] = s0.reg[i];
+    }
+    int tdep = thread_deposit(t, cur);
+    u64 g0 = base | local_to_global(tdep, sw.tbits);  // global index of register slot 0
+    c128 v[TN];
+    {
+        u64 G[TRB];
+#pragma unroll
+        for (int i = 0; i < TRB; ++i) G[i] = u64(1) << sw.tbits[cur[i]];
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+            v[j] = state[g0 | ((j & 1) ? G[0] : 0) | ((j & 2) ? G[1] : 0) | ((j & 4) ? G[2] : 0) |
+                         ((j & 8) ? G[3] : 0)];
+    }
     for (int s = sw.seg0; s < sw.seg1; ++s) {
         const TSeg sg = segs[s];
         if (sg.reg[0] != cur[0] || sg.reg[1] != cur[1] || sg.reg[2] != cur[2] || sg.reg[3] != cur[3]) {
@@ -146,6 +182,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
 #pragma unroll
             for (int i = 0; i < TRB; ++i) cur[i] = sg.reg[i];
             tdep = thread_deposit(t, cur);
+            g0 = base | local_to_global(tdep, sw.tbits);
 #pragma unroll
             for (int j = 0; j < TN; ++j) v[j] = lds[lds_pad(deposit4(j, cur) | tdep)];
             __syncthreads();
@@ -153,51 +190,67 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
         for (int o = sg.op0; o < sg.op1; ++o) {
             const TOp op = ops[o];
             const c128 *M = pool + op.moff;
-            if (op.type == 0) {
-                c128 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
-                switch (op.a) {
-                    case 0: reg_apply1<0>(v, m00, m10, m01, m11); break;
-                    case 1: reg_apply1<1>(v, m00, m10, m01, m11); break;
-                    case 2: reg_apply1<2>(v, m00, m10, m01, m11); break;
-                    default: reg_apply1<3>(v, m00, m10, m01, m11); break;
-                }
-            } else if (op.type == 1) {
-                switch (op.a * 4 + op.b) {
-                    case 1: reg_apply2<0, 1>(v, M); break;
-                    case 2: reg_apply2<0, 2>(v, M); break;
-                    case 3: reg_apply2<0, 3>(v, M); break;
-                    case 6: reg_apply2<1, 2>(v, M); break;
-                    case 7: reg_apply2<1, 3>(v, M); break;
-                    default: reg_apply2<2, 3>(v, M); break;
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < TN; ++j) {
-                    const int l = deposit4(j, cur) | tdep;
-                    int idx = 0;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        if (i < op.k) {
-                            const int sbit = op.dsrc[i];
-                            const int bit = sbit < 100 ? ((l >> sbit) & 1) : int((base >> (sbit - 100)) & 1);
-                            idx |= bit << i;
-                        }
+            switch (op.type) {
+                case T_DENSE1: {
+                    c128 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
+                    switch (op.a) {
+                        case 0: reg_apply1<0>(v, m00, m10, m01, m11); break;
+                        case 1: reg_apply1<1>(v, m00, m10, m01, m11); break;
+                        case 2: reg_apply1<2>(v, m00, m10, m01, m11); break;
+                        default: reg_apply1<3>(v, m00, m10, m01, m11); break;
                     }
-                    v[j] = cmul(M[idx], v[j]);
+                    break;
+                }
+                case T_DENSE2:
+                    switch (op.a * 4 + op.b) {
+                        case 1: reg_apply2<0, 1>(v, M); break;
+                        case 2: reg_apply2<0, 2>(v, M); break;
+                        case 3: reg_apply2<0, 3>(v, M); break;
+                        case 6: reg_apply2<1, 2>(v, M); break;
+                        case 7: reg_apply2<1, 3>(v, M); break;
+                        default: reg_apply2<2, 3>(v, M); break;
+                    }
+                    break;
+                case T_SIGN: {
+                    const int f = diag_f(op, g0);
+                    const unsigned neg = unsigned(op.negmask);
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) {
+                        const unsigned flip = (neg >> (f | diag_rp(op, j))) & 1u;
+                        v[j].re = flip_sign(v[j].re, flip);
+                        v[j].im = flip_sign(v[j].im, flip);
+                    }
+                    break;
+                }
+                case T_SELECT: {
+                    const int f = diag_f(op, g0);
+                    const c128 d0 = M[0], d1 = M[1], d2 = M[2], d3 = M[3];
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) {
+                        const int idx = f | diag_rp(op, j);
+                        const c128 lo = (idx & 1) ? d1 : d0, hi = (idx & 1) ? d3 : d2;
+                        v[j] = cmul((idx & 2) ? hi : lo, v[j]);
+                    }
+                    break;
+                }
+                default: {
+                    const int f = diag_f(op, g0);
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) v[j] = cmul(M[f | diag_rp(op, j)], v[j]);
+                    break;
                 }
             }
         }
     }
-    if (cur[0] != 8 || cur[1] != 9 || cur[2] != 10 || cur[3] != 11) {
+    {
+        u64 G[TRB];
 #pragma unroll
-        for (int j = 0; j < TN; ++j) lds[lds_pad(deposit4(j, cur) | tdep)] = v[j];
-        __syncthreads();
-        // load layout: registers on tile bits 8..11, thread bits 0..7
+        for (int i = 0; i < TRB; ++i) G[i] = u64(1) << sw.tbits[cur[i]];
 #pragma unroll
-        for (int j = 0; j < TN; ++j) v[j] = lds[lds_pad((j << 8) | t)];
+        for (int j = 0; j < TN; ++j)
+            state[g0 | ((j & 1) ? G[0] : 0) | ((j & 2) ? G[1] : 0) | ((j & 4) ? G[2] : 0) | ((j & 8) ? G[3] : 0)] =
+                v[j];
     }
-#pragma unroll
-    for (int j = 0; j < TN; ++j) state[gj[j]] = v[j];
 }
 
 // ------------------------------------------------------------------ planner ----
@@ -216,7 +269,7 @@ struct POp {
 
 struct PlanItem {
     bool sweep = false;
-    int step = -1;   // unfused step index (split-half flag in `half`)
+    int step = -1;  // unfused step index
     TSweepArgs args{};
 };
 
@@ -229,6 +282,7 @@ struct FusedPlan {
     TOp *d_ops = nullptr;
     c128 *d_pool = nullptr;
     int nsweeps = 0;
+    int ntransitions = 0;
 };
 
 static bool is_diag(const MatInfo &info) {
@@ -253,15 +307,7 @@ static void lower_steps(const Program *p, std::vector<POp> &out) {
     for (size_t si = 0; si < p->steps.size(); ++si) {
         const Step &st = p->steps[si];
         const int dim = 1 << st.kb;
-        if (st.split) {
-            POp u;
-            u.kind = 2;
-            u.step = int(si);
-            u.nb = 0;
-            out.push_back(u);
-            continue;
-        }
-        if (is_diag(st.ifwd) && st.kb <= 4) {
+        if (!st.split && is_diag(st.ifwd) && st.kb <= 4) {
             if (st.ifwd.nchanged == 0) continue;  // identity
             POp d;
             d.kind = 1;
@@ -272,7 +318,7 @@ static void lower_steps(const Program *p, std::vector<POp> &out) {
             out.push_back(d);
             continue;
         }
-        if (st.kb <= 2) {
+        if (!st.split && st.kb <= 2) {
             POp d;
             d.kind = 0;
             d.nb = st.kb;
@@ -282,7 +328,7 @@ static void lower_steps(const Program *p, std::vector<POp> &out) {
             out.push_back(d);
             continue;
         }
-        if (p->is_density && !st.channel && st.k == 2 && st.u_src >= 0) {
+        if (p->is_density && !st.split && !st.channel && st.k == 2 && st.u_src >= 0) {
             // U(x)conj(U) = (U on the row bits) then (conj(U) on the column bits)
             std::vector<c128> U = mat_of(p, st.u_src, 4), Uc(16);
             for (int q = 0; q < 16; ++q) Uc[q] = cconj(U[q]);
@@ -307,8 +353,8 @@ static void lower_steps(const Program *p, std::vector<POp> &out) {
     }
 }
 
+// out = A * B (column-major d x d)
 static void product(const std::vector<c128> &A, const std::vector<c128> &B, int d, std::vector<c128> &out) {
-    // out = A * B (column-major d x d)
     out.assign(size_t(d) * d, mk(0, 0));
     for (int r = 0; r < d; ++r)
         for (int c = 0; c < d; ++c) {
@@ -318,26 +364,134 @@ static void product(const std::vector<c128> &A, const std::vector<c128> &B, int 
         }
 }
 
+// kron(Bm, Am) with Am on the low bit: 4x4 on (a, b), a < b.
+static std::vector<c128> kron2(const std::vector<c128> &Am, const std::vector<c128> &Bm) {
+    std::vector<c128> out(16);
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[r + 4 * c] = cmul(Am[(r & 1) + 2 * (c & 1)], Bm[(r >> 1) + 2 * (c >> 1)]);
+    return out;
+}
+
 static bool commutes(const POp &a, const POp &b) {
-    if (a.kind == 2 || b.kind == 2) {
-        // unfusable steps: conservative (they commute only with disjoint supports;
-        // their support is unknown here, so never)
-        return false;
-    }
+    if (a.kind == 2 || b.kind == 2) return false;  // unfusable steps never move
     if ((a.mask() & b.mask()) == 0) return true;
     return a.kind == 1 && b.kind == 1;
 }
 
 static int popc(u64 x) { return __builtin_popcountll(x); }
 
-int fused_plan_build(Program *p) {
-    if (p->nbits < TM) return MOSSQ_OK;  // too small to tile: run unfused
+// diag(D) * M for a diagonal D on bits Dbits and dense M on Mbits (Dbits subset of Mbits).
+static void absorb_diag(POp &M, const POp &D) {
+    const int d = 1 << M.nb;
+    for (int r = 0; r < d; ++r) {
+        int di = 0;
+        for (int i = 0; i < D.nb; ++i)
+            for (int q = 0; q < M.nb; ++q)
+                if (D.bits[i] == M.bits[q] && ((r >> q) & 1)) di |= 1 << i;
+        for (int c = 0; c < d; ++c) M.m[r + d * c] = cmul(D.m[di], M.m[r + d * c]);
+    }
+}
+
+// Peephole on one segment's op list (in execution order): merge dense ops on
+// identical bits, absorb register-bit diagonals into the dense ops before them
+// (pairing two one-bit gates into a two-bit gate when the diagonal spans both).
+static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg) {
+    auto in_reg = [&](int gb) {
+        for (int i = 0; i < nreg; ++i)
+            if (reg_global[i] == gb) return true;
+        return false;
+    };
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        for (size_t i = 0; i < ops.size() && !changed; ++i) {
+            POp &D = ops[i];
+            if (D.kind != 1 || D.nb > 2) continue;
+            bool all_reg = true;
+            for (int q = 0; q < D.nb; ++q) all_reg &= in_reg(D.bits[q]);
+            if (!all_reg) continue;
+            // last op before i touching each diagonal bit
+            int last[2] = {-1, -1};
+            for (int q = 0; q < D.nb; ++q)
+                for (int j = int(i) - 1; j >= 0; --j)
+                    if (ops[j].mask() & (u64(1) << D.bits[q])) {
+                        last[q] = j;
+                        break;
+                    }
+            if (D.nb == 1 || last[0] == last[1]) {
+                int j = last[0];
+                if (j >= 0 && ops[j].kind == 0 && (ops[j].mask() & D.mask()) == D.mask()) {
+                    absorb_diag(ops[j], D);
+                    ops.erase(ops.begin() + i);
+                    changed = true;
+                }
+                continue;
+            }
+            int ja = last[0], jb = last[1];
+            if (ja < 0 || jb < 0) continue;
+            POp &A = ops[ja], &B = ops[jb];
+            if (A.kind != 0 || B.kind != 0 || A.nb != 1 || B.nb != 1) continue;
+            // both one-bit gates are the last to touch their bit before D: fuse at max(ja, jb)
+            POp F;
+            F.kind = 0;
+            F.nb = 2;
+            F.bits[0] = D.bits[0];
+            F.bits[1] = D.bits[1];
+            F.m = kron2(A.bits[0] == D.bits[0] ? A.m : B.m, A.bits[0] == D.bits[0] ? B.m : A.m);
+            absorb_diag(F, D);
+            int keep = std::max(ja, jb), drop = std::min(ja, jb);
+            ops[keep] = F;
+            ops.erase(ops.begin() + i);
+            ops.erase(ops.begin() + drop);
+            changed = true;
+        }
+        // merge adjacent dense ops on identical bits, adjacent diagonals on identical bits
+        for (size_t i = 1; i < ops.size(); ++i) {
+            POp &a = ops[i - 1], &b = ops[i];
+            if (a.kind == b.kind && a.kind != 2 && a.mask() == b.mask() && a.nb == b.nb) {
+                if (a.kind == 0) {
+                    std::vector<c128> pr;
+                    product(b.m, a.m, 1 << a.nb, pr);
+                    a.m.swap(pr);
+                } else {
+                    for (size_t q = 0; q < a.m.size(); ++q) a.m[q] = cmul(a.m[q], b.m[q]);
+                }
+                ops.erase(ops.begin() + i);
+                changed = true;
+                break;
+            }
+        }
+    }
+}
+
+static bool sign_only(const std::vector<c128> &d, int *negmask) {
+    int m = 0;
+    for (size_t i = 0; i < d.size(); ++i) {
+        if (d[i].im != 0.0 || (d[i].re != 1.0 && d[i].re != -1.0)) return false;
+        if (d[i].re == -1.0) m |= 1 << i;
+    }
+    *negmask = m;
+    return true;
+}
+
+static bool safe_layout(const int *reg) {
+    for (int i = 0; i < TRB; ++i)
+        if (reg[i] < kCoalesceBits) return false;
+    return true;
+}
+
+FusedPlan *fused_plan_build_host(const Program *p, std::string *err) {
     std::vector<POp> ops;
     lower_steps(p, ops);
     FusedPlan *fp = new FusedPlan();
     const u64 coal = (u64(1) << kCoalesceBits) - 1;
     std::vector<char> done(ops.size(), 0);
     size_t first = 0;
+    auto failp = [&](const char *m) {
+        *err = m;
+        delete fp;
+        return (FusedPlan *)nullptr;
+    };
     while (true) {
         while (first < ops.size() && done[first]) ++first;
         if (first >= ops.size()) break;
@@ -351,8 +505,7 @@ int fused_plan_build(Program *p) {
         }
         // Greedy sweep with commutation lookahead.
         u64 T = coal;
-        std::vector<int> chosen;
-        std::vector<int> deferred;
+        std::vector<int> chosen, deferred;
         for (size_t i = first; i < ops.size(); ++i) {
             if (done[i]) continue;
             const POp &o = ops[i];
@@ -374,7 +527,6 @@ int fused_plan_build(Program *p) {
             if (deferred.size() > 64) break;  // bounded lookahead
         }
         for (int i : chosen) done[i] = 1;
-        // fill the tile with the lowest unused bits
         for (int b = 0; b < p->nbits && popc(T) < TM; ++b) T |= u64(1) << b;
         PlanItem it;
         it.sweep = true;
@@ -392,15 +544,13 @@ int fused_plan_build(Program *p) {
         it.args.ncbits = nc;
         // Schedule the chosen ops into register segments.
         std::vector<char> sched(chosen.size(), 0);
-        int reg[TRB] = {8, 9, 10, 11};
-        it.args.seg0 = int(fp->h_segs.size());
         size_t nsched = 0;
         auto ready = [&](size_t ci) {
             for (size_t pj = 0; pj < ci; ++pj)
                 if (!sched[pj] && !commutes(ops[chosen[pj]], ops[chosen[ci]])) return false;
             return true;
         };
-        auto fits = [&](size_t ci) {
+        auto fits = [&](size_t ci, const int *reg) {
             const POp &o = ops[chosen[ci]];
             if (o.kind == 1) return true;
             for (int q = 0; q < o.nb; ++q) {
@@ -409,77 +559,7 @@ int fused_plan_build(Program *p) {
             }
             return true;
         };
-        bool first_seg = true;
-        while (nsched < chosen.size()) {
-            TSeg sg;
-            for (int i = 0; i < TRB; ++i) sg.reg[i] = reg[i];
-            sg.op0 = int(fp->h_ops.size());
-            // emit every ready op that fits, repeatedly
-            std::vector<int> seg_ops;
-            bool progress = true;
-            while (progress) {
-                progress = false;
-                for (size_t ci = 0; ci < chosen.size(); ++ci) {
-                    if (sched[ci] || !fits(ci) || !ready(ci)) continue;
-                    sched[ci] = 1;
-                    ++nsched;
-                    seg_ops.push_back(chosen[ci]);
-                    progress = true;
-                }
-            }
-            // merge adjacent dense ops on identical bits, lower to TOps
-            std::vector<POp> merged;
-            for (int oi : seg_ops) {
-                const POp &o = ops[oi];
-                if (!merged.empty() && o.kind == 0 && merged.back().kind == 0 && merged.back().nb == o.nb &&
-                    merged.back().mask() == o.mask()) {
-                    std::vector<c128> pr;
-                    product(o.m, merged.back().m, 1 << o.nb, pr);
-                    merged.back().m.swap(pr);
-                    continue;
-                }
-                if (!merged.empty() && o.kind == 1 && merged.back().kind == 1 && merged.back().nb == o.nb &&
-                    merged.back().mask() == o.mask()) {
-                    for (size_t q = 0; q < o.m.size(); ++q) merged.back().m[q] = cmul(merged.back().m[q], o.m[q]);
-                    continue;
-                }
-                merged.push_back(o);
-            }
-            for (const POp &o : merged) {
-                TOp t{};
-                t.moff = int(fp->h_pool.size());
-                fp->h_pool.insert(fp->h_pool.end(), o.m.begin(), o.m.end());
-                if (o.kind == 1) {
-                    t.type = 2;
-                    t.k = o.nb;
-                    for (int q = 0; q < o.nb; ++q) {
-                        int l = local_of[o.bits[q]];
-                        t.dsrc[q] = l >= 0 ? l : 100 + o.bits[q];
-                    }
-                } else {
-                    auto ridx = [&](int gb) {
-                        int l = local_of[gb];
-                        for (int i = 0; i < TRB; ++i)
-                            if (reg[i] == l) return i;
-                        return -1;
-                    };
-                    if (o.nb == 1) {
-                        t.type = 0;
-                        t.a = ridx(o.bits[0]);
-                    } else {
-                        t.type = 1;
-                        t.a = ridx(o.bits[0]);
-                        t.b = ridx(o.bits[1]);
-                    }
-                }
-                fp->h_ops.push_back(t);
-            }
-            sg.op1 = int(fp->h_ops.size());
-            if (sg.op1 > sg.op0 || first_seg) fp->h_segs.push_back(sg);
-            first_seg = false;
-            if (nsched >= chosen.size()) break;
-            // next register set: bits of the first ready unscheduled dense op, then
-            // bits of further ready dense ops, then fill
+        auto next_reg = [&](int *reg) {
             std::vector<int> want;
             for (size_t ci = 0; ci < chosen.size() && want.size() < TRB; ++ci) {
                 if (sched[ci] || !ready(ci)) continue;
@@ -492,57 +572,268 @@ int fused_plan_build(Program *p) {
                 }
                 if (want.size() + add.size() <= TRB) want.insert(want.end(), add.begin(), add.end());
             }
-            if (want.empty()) {
-                // only blocked ops remain: cannot happen (a ready op always exists)
-                delete fp;
-                return fail(MOSSQ_EDEVICE, "fusion planner made no progress");
-            }
             for (int l = TM - 1; l >= 0 && want.size() < TRB; --l)
                 if (std::find(want.begin(), want.end(), l) == want.end()) want.push_back(l);
             std::sort(want.begin(), want.end());
             if (want.size() != TRB || want[0] < 0 || want[TRB - 1] >= TM ||
-                std::adjacent_find(want.begin(), want.end()) != want.end()) {
-                delete fp;
-                return fail(MOSSQ_EDEVICE, "fusion planner produced an invalid register set");
-            }
+                std::adjacent_find(want.begin(), want.end()) != want.end())
+                return false;
             for (int i = 0; i < TRB; ++i) reg[i] = want[i];
+            return true;
+        };
+        int reg[TRB] = {8, 9, 10, 11};
+        if (!next_reg(reg)) return failp("fusion planner produced an invalid register set");
+        it.args.seg0 = int(fp->h_segs.size());
+        if (!safe_layout(reg)) {
+            TSeg s0{{8, 9, 10, 11}, int(fp->h_ops.size()), int(fp->h_ops.size())};
+            fp->h_segs.push_back(s0);
+        }
+        while (nsched < chosen.size()) {
+            TSeg sg;
+            for (int i = 0; i < TRB; ++i) sg.reg[i] = reg[i];
+            std::vector<POp> seg_ops;
+            bool progress = true;
+            while (progress) {
+                progress = false;
+                for (size_t ci = 0; ci < chosen.size(); ++ci) {
+                    if (sched[ci] || !fits(ci, reg) || !ready(ci)) continue;
+                    sched[ci] = 1;
+                    ++nsched;
+                    seg_ops.push_back(ops[chosen[ci]]);
+                    progress = true;
+                }
+            }
+            if (seg_ops.empty()) return failp("fusion planner made no progress");
+            int reg_global[TRB];
+            for (int i = 0; i < TRB; ++i) reg_global[i] = it.args.tbits[reg[i]];
+            peephole(seg_ops, reg_global, TRB);
+            sg.op0 = int(fp->h_ops.size());
+            for (const POp &o : seg_ops) {
+                TOp t{};
+                t.moff = int(fp->h_pool.size());
+                if (o.kind == 1) {
+                    t.k = o.nb;
+                    int ridx[4] = {-1, -1, -1, -1};
+                    for (int q = 0; q < 4; ++q) t.qbit[q] = -1;
+                    for (int q = 0; q < o.nb; ++q) {
+                        int l = local_of[o.bits[q]];
+                        for (int i = 0; i < TRB; ++i)
+                            if (reg[i] == l && l >= 0) ridx[q] = i;
+                        if (ridx[q] < 0) t.qbit[q] = o.bits[q];
+                    }
+                    t.rp_lo = t.rp_hi = 0;
+                    for (int j = 0; j < TN; ++j) {
+                        unsigned rp = 0;
+                        for (int q = 0; q < o.nb; ++q)
+                            if (ridx[q] >= 0 && ((j >> ridx[q]) & 1)) rp |= 1u << q;
+                        if (j < 8)
+                            t.rp_lo |= rp << (4 * j);
+                        else
+                            t.rp_hi |= rp << (4 * (j - 8));
+                    }
+                    int neg = 0;
+                    if (sign_only(o.m, &neg)) {
+                        t.type = T_SIGN;
+                        t.negmask = neg;
+                        t.moff = 0;
+                    } else if (o.nb <= 2) {
+                        t.type = T_SELECT;
+                        for (int q = 0; q < 4; ++q) fp->h_pool.push_back(q < int(o.m.size()) ? o.m[q] : mk(1, 0));
+                    } else {
+                        t.type = T_GATHER;
+                        fp->h_pool.insert(fp->h_pool.end(), o.m.begin(), o.m.end());
+                    }
+                } else {
+                    fp->h_pool.insert(fp->h_pool.end(), o.m.begin(), o.m.end());
+                    auto ridx = [&](int gb) {
+                        int l = local_of[gb];
+                        for (int i = 0; i < TRB; ++i)
+                            if (reg[i] == l) return i;
+                        return -1;
+                    };
+                    if (o.nb == 1) {
+                        t.type = T_DENSE1;
+                        t.a = ridx(o.bits[0]);
+                        if (t.a < 0) return failp("dense op outside the register bits");
+                    } else {
+                        t.type = T_DENSE2;
+                        t.a = ridx(o.bits[0]);
+                        t.b = ridx(o.bits[1]);
+                        if (t.a < 0 || t.b <= t.a) return failp("dense op outside the register bits");
+                    }
+                }
+                fp->h_ops.push_back(t);
+            }
+            sg.op1 = int(fp->h_ops.size());
+            fp->h_segs.push_back(sg);
+            if (nsched >= chosen.size()) break;
+            if (!next_reg(reg)) return failp("fusion planner produced an invalid register set");
+        }
+        if (!safe_layout(fp->h_segs.back().reg)) {
+            TSeg se{{8, 9, 10, 11}, int(fp->h_ops.size()), int(fp->h_ops.size())};
+            fp->h_segs.push_back(se);
         }
         it.args.seg1 = int(fp->h_segs.size());
+        for (int q = it.args.seg0 + 1; q < it.args.seg1; ++q)
+            if (memcmp(fp->h_segs[q].reg, fp->h_segs[q - 1].reg, sizeof(int) * TRB) != 0) fp->ntransitions++;
         fp->items.push_back(it);
         fp->nsweeps++;
     }
-    // upload
-    Ctx *c = p->ctx;
-    int s = MOSSQ_OK;
-    s = dev_alloc(c, std::max<size_t>(fp->h_segs.size(), 1) * sizeof(TSeg), (void **)&fp->d_segs);
-    if (s == MOSSQ_OK) s = dev_alloc(c, std::max<size_t>(fp->h_ops.size(), 1) * sizeof(TOp), (void **)&fp->d_ops);
-    if (s == MOSSQ_OK) s = dev_alloc(c, std::max<size_t>(fp->h_pool.size(), 1) * sizeof(c128), (void **)&fp->d_pool);
-    if (s != MOSSQ_OK) {
-        if (fp->d_segs) dev_free(c, fp->d_segs);
-        if (fp->d_ops) dev_free(c, fp->d_ops);
-        delete fp;
-        return s;
+    if (fp->h_pool.empty()) fp->h_pool.push_back(mk(0, 0));
+    return fp;
+}
+
+// Host execution of a plan with the kernel's index arithmetic (tiles, layouts,
+// LDS transpositions, bit sources); unfused items use the oracle-free generic
+// block update.
+static void host_block(c128 *v, int nbits, int k, const int *bits, const c128 *m) {
+    const int d = 1 << k;
+    const u64 nb = u64(1) << (nbits - k);
+    std::vector<c128> in(d);
+    for (u64 t = 0; t < nb; ++t) {
+        u64 base = insert0s_dyn(t, bits, k);
+        for (int c = 0; c < d; ++c) {
+            u64 o = 0;
+            for (int j = 0; j < k; ++j)
+                if (c & (1 << j)) o |= u64(1) << bits[j];
+            in[c] = v[base + o];
+        }
+        for (int r = 0; r < d; ++r) {
+            c128 acc = mk(0, 0);
+            for (int c = 0; c < d; ++c) acc = cfma(m[r + d * c], in[c], acc);
+            u64 o = 0;
+            for (int j = 0; j < k; ++j)
+                if (r & (1 << j)) o |= u64(1) << bits[j];
+            v[base + o] = acc;
+        }
     }
+}
+
+static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp, u64 tile) {
+    std::vector<c128> lds((1 << TM) + (1 << TM) / 16);
+    std::vector<c128> V(size_t(TT) * TN);
+    u64 base = 0;
+    for (int i = 0; i < sw.ncbits; ++i)
+        if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
+    int cur[TRB];
+    for (int i = 0; i < TRB; ++i) cur[i] = fp->h_segs[sw.seg0].reg[i];
+    std::vector<int> tdep(TT);
+    for (int t = 0; t < TT; ++t) {
+        tdep[t] = thread_deposit(t, cur);
+        u64 gth = base | local_to_global(tdep[t], sw.tbits);
+        for (int j = 0; j < TN; ++j) V[t * TN + j] = state[gth | local_to_global(deposit4(j, cur), sw.tbits)];
+    }
+    for (int s = sw.seg0; s < sw.seg1; ++s) {
+        const TSeg &sg = fp->h_segs[s];
+        if (memcmp(sg.reg, cur, sizeof(cur)) != 0) {
+            for (int t = 0; t < TT; ++t)
+                for (int j = 0; j < TN; ++j) lds[lds_pad(deposit4(j, cur) | tdep[t])] = V[t * TN + j];
+            memcpy(cur, sg.reg, sizeof(cur));
+            for (int t = 0; t < TT; ++t) {
+                tdep[t] = thread_deposit(t, cur);
+                for (int j = 0; j < TN; ++j) V[t * TN + j] = lds[lds_pad(deposit4(j, cur) | tdep[t])];
+            }
+        }
+        for (int o = sg.op0; o < sg.op1; ++o) {
+            const TOp &op = fp->h_ops[o];
+            const c128 *M = fp->h_pool.data() + op.moff;
+            for (int t = 0; t < TT; ++t) {
+                c128 *v = &V[t * TN];
+                if (op.type == T_DENSE1) {
+                    for (int j = 0; j < TN; ++j) {
+                        if (j & (1 << op.a)) continue;
+                        c128 x = v[j], y = v[j | (1 << op.a)];
+                        v[j] = cfma(M[2], y, cmul(M[0], x));
+                        v[j | (1 << op.a)] = cfma(M[3], y, cmul(M[1], x));
+                    }
+                } else if (op.type == T_DENSE2) {
+                    for (int j = 0; j < TN; ++j) {
+                        if (j & ((1 << op.a) | (1 << op.b))) continue;
+                        int ii[4] = {j, j | (1 << op.a), j | (1 << op.b), j | (1 << op.a) | (1 << op.b)};
+                        c128 in[4] = {v[ii[0]], v[ii[1]], v[ii[2]], v[ii[3]]};
+                        for (int r = 0; r < 4; ++r) {
+                            c128 acc = cmul(M[r], in[0]);
+                            for (int c = 1; c < 4; ++c) acc = cfma(M[r + 4 * c], in[c], acc);
+                            v[ii[r]] = acc;
+                        }
+                    }
+                } else {
+                    const u64 g0 = base | local_to_global(tdep[t], sw.tbits);
+                    for (int j = 0; j < TN; ++j) {
+                        int idx = diag_f(op, g0) | diag_rp(op, j);
+                        if (op.type == T_SIGN) {
+                            if ((op.negmask >> idx) & 1) v[j] = mk(-v[j].re, -v[j].im);
+                        } else {
+                            v[j] = cmul(M[idx], v[j]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    for (int t = 0; t < TT; ++t) {
+        u64 gth = base | local_to_global(tdep[t], sw.tbits);
+        for (int j = 0; j < TN; ++j) state[gth | local_to_global(deposit4(j, cur), sw.tbits)] = V[t * TN + j];
+    }
+}
+
+void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state) {
+    for (const PlanItem &it : fp->items) {
+        if (!it.sweep) {
+            const Step &s = p->steps[it.step];
+            const c128 *P = p->h_pool.data();
+            if (s.split) {
+                host_block(state, p->nbits, s.k, s.bits, P + s.fwd);
+                host_block(state, p->nbits, s.k, s.bits + s.k, P + s.fwd_cols);
+            } else {
+                host_block(state, p->nbits, s.kb, s.bits, P + s.fwd);
+            }
+            continue;
+        }
+        const u64 ntiles = u64(1) << (p->nbits - TM);
+        for (u64 t = 0; t < ntiles; ++t) emulate_tile(state, it.args, fp, t);
+    }
+}
+
+int fused_plan_upload(Ctx *c, FusedPlan *fp) {
+    int s = dev_alloc(c, std::max<size_t>(fp->h_segs.size(), 1) * sizeof(TSeg), (void **)&fp->d_segs);
+    if (s == MOSSQ_OK) s = dev_alloc(c, std::max<size_t>(fp->h_ops.size(), 1) * sizeof(TOp), (void **)&fp->d_ops);
+    if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_pool.size() * sizeof(c128), (void **)&fp->d_pool);
+    if (s != MOSSQ_OK) return s;
     if (!fp->h_segs.empty())
         MOSSQ_HIP(hipMemcpyAsync(fp->d_segs, fp->h_segs.data(), fp->h_segs.size() * sizeof(TSeg),
                                  hipMemcpyHostToDevice, c->stream));
     if (!fp->h_ops.empty())
         MOSSQ_HIP(hipMemcpyAsync(fp->d_ops, fp->h_ops.data(), fp->h_ops.size() * sizeof(TOp), hipMemcpyHostToDevice,
                                  c->stream));
-    if (!fp->h_pool.empty())
-        MOSSQ_HIP(hipMemcpyAsync(fp->d_pool, fp->h_pool.data(), fp->h_pool.size() * sizeof(c128),
-                                 hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(fp->d_pool, fp->h_pool.data(), fp->h_pool.size() * sizeof(c128), hipMemcpyHostToDevice,
+                             c->stream));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    return MOSSQ_OK;
+}
+
+int fused_plan_build(Program *p) {
+    if (p->nbits < TM) return MOSSQ_OK;  // too small to tile: run unfused
+    std::string err;
+    FusedPlan *fp = fused_plan_build_host(p, &err);
+    if (!fp) return fail(MOSSQ_EDEVICE, err);
     p->fused = fp;
+    int s = fused_plan_upload(p->ctx, fp);
+    if (s != MOSSQ_OK) {
+        fused_plan_free(p);
+        return s;
+    }
     return MOSSQ_OK;
 }
 
 void fused_plan_free(Program *p) {
     if (!p->fused) return;
     FusedPlan *fp = p->fused;
-    dev_free(p->ctx, fp->d_segs);
-    dev_free(p->ctx, fp->d_ops);
-    dev_free(p->ctx, fp->d_pool);
+    if (p->ctx) {
+        if (fp->d_segs) dev_free(p->ctx, fp->d_segs);
+        if (fp->d_ops) dev_free(p->ctx, fp->d_ops);
+        if (fp->d_pool) dev_free(p->ctx, fp->d_pool);
+    }
     delete fp;
     p->fused = nullptr;
 }
@@ -572,3 +863,33 @@ int fused_run(Ctx *c, const Program *p, c128 *state) {
 }
 
 }  // namespace mossq
+
+using namespace mossq;
+
+extern "C" {
+
+int mossq_plan_emulate(int n, int is_density, int nops, const mossq_op *ops, const double *pool, uint64_t pool_len,
+                       double *state, int *stats) {
+    MOSSQ_CHECK(stats, MOSSQ_EINVAL, "null argument");
+    Program p;
+    MOSSQ_TRY(program_build_host(&p, n, is_density, nops, ops, pool, pool_len));
+    MOSSQ_CHECK(p.nbits >= TM, MOSSQ_EINVAL, "plans tile registers of at least 12 bits");
+    std::string err;
+    FusedPlan *fp = fused_plan_build_host(&p, &err);
+    MOSSQ_CHECK(fp, MOSSQ_EINVAL, err);
+    if (state) emulate_plan(&p, fp, (c128 *)state);
+    int unfused = 0;
+    for (auto &it : fp->items)
+        if (!it.sweep) ++unfused;
+    stats[0] = fp->nsweeps;
+    stats[1] = unfused;
+    stats[2] = int(fp->h_segs.size());
+    stats[3] = fp->ntransitions;
+    stats[4] = int(fp->h_ops.size());
+    stats[5] = stats[6] = stats[7] = 0;
+    for (const TOp &o : fp->h_ops) stats[o.type == T_DENSE1 ? 5 : o.type == T_DENSE2 ? 6 : 7]++;
+    delete fp;
+    return MOSSQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+"""The fused-sweep planner, executed on the host with the tile kernel's index
+arithmetic (mossq_plan_emulate), against the CPU oracle's one-sweep-per-gate
+execution.  No device needed."""
+import numpy as np
+import pytest
+
+import oracle
+import mossquantum_amd as mq
+from mossquantum_amd import lower
+from mossquantum_amd.backend import plan_emulate
+from mossquantum_amd.gates import haar_unitary
+from mossquantum_amd.workloads import haar_layers, hea_circuit, random_circuit
+
+AMP = 1e-12
+
+
+def rand_state(n, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    return v / np.linalg.norm(v)
+
+
+def check(circ, n, is_density, psi0):
+    want = np.array(psi0, dtype=np.complex128).copy()
+    oracle.run_program(want, lower(circ, n, is_density))
+    got, stats = plan_emulate(circ, n, is_density, psi0)
+    assert np.max(np.abs(got - want)) < AMP
+    return stats
+
+
+@pytest.mark.parametrize("n,depth", [(12, 3), (13, 4), (15, 6)])
+def test_random_circuit_plans(n, depth):
+    c = random_circuit(n, depth, haar_layers(n, depth, 100 + n))
+    st = check(c, n, False, rand_state(n, n))
+    assert st["sweeps"] < len(list(c.flat()))
+    assert st["unfused"] == 0
+
+
+def test_depth20_plan_uses_few_sweeps():
+    n, d = 14, 20
+    c = random_circuit(n, d, haar_layers(n, d, 7))
+    st = check(c, n, False, rand_state(n, 1))
+    assert st["sweeps"] <= 2 * d
+
+
+def test_ansatz_with_cnot_and_rotations():
+    n = 13
+    rng = np.random.default_rng(3)
+    c = hea_circuit(n, 3, rng.uniform(0, 2 * np.pi, 2 * n * 3))
+    check(c, n, False, rand_state(n, 2))
+
+
+def test_mixed_instruction_kinds():
+    """Diagonals on tile/thread/global bits, non-±1 diagonals, 3-qubit gates
+    (run unfused), permutations and a wide diagonal."""
+    n = 14
+    rng = np.random.default_rng(4)
+    ops = [mq.QuantumGate((1,), haar_unitary(rng)), mq.CZGate(1, 14), mq.RzGate(13, 0.3),
+           mq.CPHASEGate((2, 12), 0.7), mq.QuantumGate((3, 9, 11), haar_unitary(rng, 8)), mq.CNOTGate(14, 2),
+           mq.SWAPGate(5, 13), mq.QuantumGate((6, 14), haar_unitary(rng, 4)), mq.TOFFOLIGate(1, 7, 12),
+           mq.QuantumGate((4,), np.diag([1j, -1])), mq.CZGate(8, 9), mq.QuantumGate((8,), haar_unitary(rng)),
+           mq.QuantumGate((9,), haar_unitary(rng)), mq.CZGate(8, 9), mq.ZGate(3)]
+    c = mq.QCircuit(ops)
+    st = check(c, n, False, rand_state(n, 3))
+    assert st["unfused"] >= 1
+
+
+@pytest.mark.parametrize("n", [6, 7])
+def test_density_programs(n):
+    rng = np.random.default_rng(n)
+    c = hea_circuit(n, 2, rng.uniform(0, 2 * np.pi, 4 * n), noise=(0.01, 0.02))
+    c.append(mq.Depolarizing((1, 3), 0.05))  # two-qubit channel: unfused
+    c.append(mq.CZGate(2, n))
+    A = rng.standard_normal((1 << n, 1 << n)) + 1j * rng.standard_normal((1 << n, 1 << n))
+    rho = A @ A.conj().T
+    rho /= np.trace(rho)
+    check(c, n, True, rho.reshape(-1, order="F"))
+
+
+def test_small_registers_rejected():
+    with pytest.raises(mq.MossqError):
+        plan_emulate(mq.QCircuit([mq.XGate(1)]), 5, False, rand_state(5, 0))
