@@ -79,6 +79,90 @@ def cpu_baseline(circ, n, budget_s=12.0):
     return done, dt, oracle.num_threads()
 
 
+def batched_bytes(n, layers=6):
+    """Algorithmic bytes of one pure-state adjoint (one sweep per gate): forward
+    (3n-1) gates per layer, reverse two vectors per gate, plus the observe pass."""
+    gates = layers * (3 * n - 1)
+    return 32 * 2 ** n * (3 * gates + 1.5 * (n - 1))
+
+
+def bench_batched(ctx, B=512, layers=6, seed=20260300, cpu=True):
+    """512 independent states, n ~ U{10..20}, 6-layer RY/RZ + CNOT-ladder ansatz
+    with own seeded angles, XXZ loss; all gradients in one engine run."""
+    import ctypes as C
+    import mossquantum_amd as mq
+    from mossquantum_amd import _lib
+    from mossquantum_amd.workloads import batch_sizes, hea_circuit, xxz
+    sizes = [int(x) for x in batch_sizes(B, 10, 20, seed)]
+    rng = np.random.default_rng(seed + 1)
+    circs = [hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers)) for n in sizes]
+    obs = [xxz(n) for n in sizes]
+    t0 = time.perf_counter()
+    progs = [mq.Program(c, n, False, ctx, fuse=False) for c, n in zip(circs, sizes)]
+    dobs = [mq.DeviceObservable(o, n, ctx) for o, n in zip(obs, sizes)]
+    log(f"batched: {B} programs lowered in {time.perf_counter() - t0:.1f} s")
+    offs = np.zeros(B, dtype=np.int64)
+    tot = 0
+    for i, p in enumerate(progs):
+        offs[i] = tot
+        tot += p.lowered.nderiv
+    vals = np.zeros(B)
+    grads = np.zeros(tot)
+    ph = (C.c_void_p * B)(*[p.h.value for p in progs])
+    oh = (C.c_void_p * B)(*[o.h.value for o in dobs])
+
+    def run_engine():
+        _lib.check(ctx.lib.mossq_batch_adjoint(ctx.h, B, ph, oh, offs.ctypes.data_as(C.POINTER(C.c_int64)),
+                                               _lib.dptr(vals), _lib.dptr(grads)))
+
+    run_engine()  # warm-up (allocations, code objects)
+    reps = 3
+    ctx.sync()
+    w0 = time.perf_counter()
+    ctx.record(6)
+    for _ in range(reps):
+        run_engine()
+    ctx.record(7)
+    wall = (time.perf_counter() - w0) / reps
+    dev_ms = ctx.elapsed_ms(6, 7) / reps
+    log(f"batched engine: {wall * 1e3:.1f} ms wall, {dev_ms:.1f} ms device per batch")
+    # plain per-circuit path: one mossq_adjoint per item
+    pv = np.zeros(B)
+    pg = np.zeros(tot)
+    w0 = time.perf_counter()
+    for i, (p, o) in enumerate(zip(progs, dobs)):
+        v = C.c_double(0)
+        g = np.zeros(max(p.lowered.nderiv, 1))
+        _lib.check(ctx.lib.mossq_adjoint(ctx.h, p.h, o.h, None, 0, None, C.byref(v), _lib.dptr(g)))
+        pv[i] = v.value
+        pg[offs[i]:offs[i] + p.lowered.nderiv] = g[:p.lowered.nderiv]
+    plain = time.perf_counter() - w0
+    log(f"per-circuit path: {plain * 1e3:.1f} ms")
+    out = {"B": B, "layers": layers, "n_range": [10, 20], "ms_per_batch": wall * 1e3, "device_ms": dev_ms,
+           "grads_per_s": B / wall, "param_derivs_per_s": tot / wall,
+           "per_circuit_ms": plain * 1e3, "per_circuit_grads_per_s": B / plain, "speedup_vs_per_circuit": plain / wall,
+           "max_abs_grad_diff_vs_per_circuit": float(np.max(np.abs(grads - pg))),
+           "max_abs_value_diff_vs_per_circuit": float(np.max(np.abs(vals - pv))),
+           "effective_gbps": sum(batched_bytes(n, layers) for n in sizes) / wall / 1e9}
+    if cpu:
+        import oracle
+        from mossquantum_amd import lower
+        # one state of every size, timed with the oracle; the batch estimate sums per-size times
+        per_size = {}
+        for n in range(10, 21):
+            c = hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers))
+            lw = lower(c, n, False)
+            t1 = time.perf_counter()
+            oracle.adjoint(lw, xxz(n).groups())
+            per_size[n] = time.perf_counter() - t1
+        est = sum(per_size[n] for n in sizes)
+        out["cpu_estimate_s"] = est
+        out["cpu_grads_per_s"] = B / est
+        out["cpu_sample"] = "one adjoint per size n=10..20 with cpu_ref (OpenMP), summed over the batch's sizes"
+        log(f"cpu estimate: {est:.1f} s for the batch")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +170,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-unfused", action="store_true", help="skip the unfused leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-adjoint leg")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -145,6 +230,9 @@ def main():
         del sv_u
     norm = mq.dot(sv, sv).real
     extra["final_norm_minus_one"] = norm - 1.0
+    del sv
+    if not args.no_batched:
+        extra["batched_adjoint"] = bench_batched(ctx, cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:

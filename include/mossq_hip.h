@@ -196,6 +196,34 @@ int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs,
                         mossq_obs *const *obs, const int64_t *grad_offset, double *values,
                         double *grads);
 
+/* ---- request server (continuous batching) ----------------------------------
+ * New capability (the reference evaluates one objective at a time,
+ * src/differentiation/objectives.jl:183-218).  A request is an energy +
+ * adjoint-gradient evaluation of a lowered circuit (same records and pool as
+ * mossq_program_create) started from |0...0> (or |0><0| when is_density),
+ * for an observable created on the server's context.  Requests are admitted
+ * first-in first-out while their device memory fits the budget; state-vector
+ * requests join the batched engine at the next iteration and stay resident on
+ * the device until they finish; density-matrix requests run on a worker stream
+ * concurrently.  step/poll/completed are called from one thread; the
+ * observable must outlive its requests. */
+typedef struct mossq_server mossq_server;
+int mossq_server_create(mossq_ctx *ctx, uint64_t budget_bytes, mossq_server **out);
+int mossq_server_destroy(mossq_server *srv);
+/* Queue a request; MOSSQ_EOOM if it can never fit the budget. */
+int mossq_server_submit(mossq_server *srv, int n, int is_density, int nops, const mossq_op *ops,
+                        const double *pool, uint64_t pool_len, const mossq_obs *obs, int64_t *ticket);
+/* One scheduling iteration: collect finished results, retire, admit, enqueue
+ * one engine iteration (at most 8 iterations are queued ahead of the device).
+ * active = admitted unfinished requests, queued = waiting for admission. */
+int mossq_server_step(mossq_server *srv, int *active, int *queued);
+/* Tickets finished since the last call (up to max), count in *count. */
+int mossq_server_completed(mossq_server *srv, int64_t *tickets, int max, int *count);
+/* done=1 when finished: value and gradients (in the program's derivative
+ * order) are copied out and the ticket is released. */
+int mossq_server_poll(mossq_server *srv, int64_t ticket, int *done, double *value, double *grads,
+                      int max_grads);
+
 /* ---- timing ----------------------------------------------------------------
  * HIP events recorded on the context's stream (the stream every kernel of the
  * context is launched on). */

@@ -96,6 +96,12 @@ SIGNATURES = [
     ("mossq_plan_emulate", _I, [_I, _I, _I, C.POINTER(OpRecord), _DP, _U64, _DP, C.POINTER(C.c_int)]),
     ("mossq_adjoint", _I, [_P, _P, _P, _P, _I, _P, _DP, _DP]),
     ("mossq_batch_adjoint", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _DP, _DP]),
+    ("mossq_server_create", _I, [_P, _U64, C.POINTER(_P)]),
+    ("mossq_server_destroy", _I, [_P]),
+    ("mossq_server_submit", _I, [_P, _I, _I, _I, C.POINTER(OpRecord), _DP, _U64, _P, C.POINTER(C.c_int64)]),
+    ("mossq_server_step", _I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("mossq_server_completed", _I, [_P, C.POINTER(C.c_int64), _I, C.POINTER(C.c_int)]),
+    ("mossq_server_poll", _I, [_P, C.c_int64, C.POINTER(C.c_int), _DP, _DP, _I]),
     ("mossq_event_record", _I, [_P, _I]),
     ("mossq_event_elapsed_ms", _I, [_P, _I, _I, C.POINTER(C.c_float)]),
 ]

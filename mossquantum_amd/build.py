@@ -18,7 +18,7 @@ BUILD = HERE / "_build"
 LIB = HERE / "libmossq_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["ctx.hip", "gates.hip", "expect.hip", "program.hip", "tile.hip", "batch.hip"]
+SOURCES = ["ctx.hip", "gates.hip", "expect.hip", "program.hip", "tile.hip", "batch.hip", "serve.hip"]
 
 # Packed-FP32 instructions are disabled for all device code.
 HIP_FLAGS = [

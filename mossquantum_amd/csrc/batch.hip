@@ -1,67 +1,294 @@
-// Batched adjoint gradients over independent programs with uneven qubit counts.
+// Batched adjoint gradients over independent programs with uneven qubit counts
+// (the engine of mossq_batch_adjoint and of the request server).
 #include <algorithm>
 
-#include "mossq_program.h"
+#include "mossq_batch.h"
 
 namespace mossq {
 
-// Plain per-circuit path: every item's adjoint is enqueued back to back on the
-// context stream into device result slots; one host synchronisation at the end.
-static int batch_adjoint_sequential(Ctx *c, int B, const Program *const *progs, const Obs *const *obs,
-                                    const int64_t *grad_offset, double *values, double *grads) {
-    int nmax = 0, Gmax = 1, Dmax = 1;
-    i64 Gtot = 0, Dtot = 0;
-    for (int b = 0; b < B; ++b) {
-        nmax = std::max(nmax, progs[b]->n);
-        Gmax = std::max(Gmax, obs[b]->G);
-        Dmax = std::max(Dmax, progs[b]->nderiv);
-        Gtot += obs[b]->G;
-        Dtot += progs[b]->nderiv;
+constexpr int kBT = 256;
+
+__device__ __forceinline__ c128 bwave_sum(c128 a) {
+    for (int o = 32; o > 0; o >>= 1) {
+        a.re += __shfl_xor(a.re, o, 64);
+        a.im += __shfl_xor(a.im, o, 64);
     }
-    const u64 len = u64(1) << nmax;
-    const int P = pair_parts(nmax);
-    u64 nparts = std::max<u64>(u64(Gmax) * 512, u64(Dmax) * P);
-    c128 *a = nullptr, *bb = nullptr, *part = nullptr, *vals = nullptr, *grd = nullptr;
-    int st = dev_alloc(c, len * sizeof(c128), (void **)&a);
-    if (st == MOSSQ_OK) st = dev_alloc(c, len * sizeof(c128), (void **)&bb);
-    if (st == MOSSQ_OK) st = dev_alloc(c, nparts * sizeof(c128), (void **)&part);
-    if (st == MOSSQ_OK) st = dev_alloc(c, std::max<i64>(Gtot, 1) * sizeof(c128), (void **)&vals);
-    if (st == MOSSQ_OK) st = dev_alloc(c, std::max<i64>(Dtot, 1) * sizeof(c128), (void **)&grd);
-    i64 go = 0, dofs = 0;
-    for (int b = 0; st == MOSSQ_OK && b < B; ++b) {
-        AdjointScratch s;
-        s.a = a;
-        s.b = bb;
-        s.part = part;
-        s.value_dev = vals + go;
-        s.grads_dev = grd + dofs;
-        st = adjoint_pure_enqueue(c, progs[b], obs[b], nullptr, nullptr, s);
-        go += obs[b]->G;
-        dofs += progs[b]->nderiv;
+    return a;
+}
+
+// Deterministic block sum; valid in thread 0.
+__device__ __forceinline__ c128 bblock_sum(c128 a, c128 *red) {
+    a = bwave_sum(a);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    c128 s = mk(0, 0);
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kBT / 64; ++w) s = cadd(s, red[w]);
+    return s;
+}
+
+// One chunk of a gate micro-op: forward (psi <- U psi) or reverse
+// (psi, lambda <- U^dagger, gradient partial Re <psi|D|lambda>).
+template <int K>
+__device__ __forceinline__ void bgate_chunk(const BItemDev &it, const BOp &op, bool fwd, int c, c128 *sm, c128 *red,
+                                            c128 *__restrict__ part) {
+    constexpr int D = 1 << K;
+    u64 off[D];
+    off[0] = 0;
+    off[1] = u64(1) << op.b0;
+    if (K == 2) {
+        off[2] = u64(1) << op.b1;
+        off[3] = off[1] | off[2];
     }
-    if (st == MOSSQ_OK) {
-        std::vector<c128> hv(std::max<i64>(Gtot, 1)), hg(std::max<i64>(Dtot, 1));
-        hipError_t e = hipMemcpyAsync(hv.data(), vals, Gtot * sizeof(c128), hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess && Dtot)
-            e = hipMemcpyAsync(hg.data(), grd, Dtot * sizeof(c128), hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) {
-            st = fail(MOSSQ_EDEVICE, std::string("batch adjoint copy: ") + hipGetErrorString(e));
-        } else {
-            go = dofs = 0;
-            for (int b = 0; b < B; ++b) {
-                double v = 0;
-                for (int g = 0; g < obs[b]->G; ++g) v += hv[go + g].re;
-                values[b] = v;
-                for (int j = 0; j < progs[b]->nderiv; ++j) grads[grad_offset[b] + j] = hg[dofs + j].re;
-                go += obs[b]->G;
-                dofs += progs[b]->nderiv;
+    const bool grad = !fwd && op.grad >= 0;
+    if (threadIdx.x < D * D) {
+        sm[threadIdx.x] = it.pool[(fwd ? op.fwd : op.rev) + threadIdx.x];
+        if (grad) sm[16 + threadIdx.x] = it.pool[op.grad + threadIdx.x];
+    }
+    __syncthreads();
+    c128 *psi = it.psi, *lam = it.lam;
+    const u64 nblk = u64(1) << (it.n - K), per = (nblk + it.nchunk - 1) / u64(it.nchunk);
+    const u64 b_lo = u64(c) * per, b_hi = min(nblk, b_lo + per);
+    c128 acc = mk(0, 0);
+    for (u64 b = b_lo + threadIdx.x; b < b_hi; b += kBT) {
+        u64 base = insert0(b, op.b0);
+        if (K == 2) base = insert0(base, op.b1);
+        c128 x[D], y[D];
+#pragma unroll
+        for (int q = 0; q < D; ++q) x[q] = psi[base + off[q]];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            c128 a = mk(0, 0);
+#pragma unroll
+            for (int q = 0; q < D; ++q) a = cfma(sm[r + D * q], x[q], a);
+            y[r] = a;
+        }
+#pragma unroll
+        for (int r = 0; r < D; ++r) psi[base + off[r]] = y[r];
+        if (!fwd) {
+            c128 l[D], lo2[D];
+#pragma unroll
+            for (int q = 0; q < D; ++q) l[q] = lam[base + off[q]];
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                c128 a = mk(0, 0);
+#pragma unroll
+                for (int q = 0; q < D; ++q) a = cfma(sm[r + D * q], l[q], a);
+                lo2[r] = a;
+            }
+#pragma unroll
+            for (int r = 0; r < D; ++r) lam[base + off[r]] = lo2[r];
+            if (grad) {
+                // g = Re <psi|D|lambda> on the updated vectors (differentiation/circuit.jl:136-144)
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    c128 dl = mk(0, 0);
+#pragma unroll
+                    for (int q = 0; q < D; ++q) dl = cfma(sm[16 + r + D * q], lo2[q], dl);
+                    acc = cfmaconj(y[r], dl, acc);
+                }
             }
         }
     }
-    for (c128 *p : {a, bb, part, vals, grd})
-        if (p) dev_free(c, p);
-    return st;
+    if (grad) {
+        c128 s = bblock_sum(acc, red);
+        if (threadIdx.x == 0) part[it.chunk0 + c] = s;
+    }
+}
+
+// One engine iteration for every active item: a workgroup owns one chunk of
+// one item and performs that item's current micro-op on it.
+__global__ __launch_bounds__(kBT, 2) void k_biter(const BItemDev *__restrict__ items, int nitems, int t,
+                                               c128 *__restrict__ part) {
+    __shared__ c128 red[kBT / 64];
+    __shared__ c128 sm[32];
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (items[mid].chunk0 <= int(blockIdx.x))
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BItemDev &it = items[lo];
+    const int n = it.n, nfwd = it.nfwd;
+    const int c = int(blockIdx.x) - it.chunk0;
+    const int pc = t - it.t0;
+    if (pc < 0 || pc > 2 * nfwd) return;
+    c128 *psi = it.psi, *lam = it.lam;
+    if (pc == nfwd) {
+        // observe: E partial = sum_i conj(psi_i) (H psi)_i ; lambda = (H + H^dagger) psi
+        const u64 len = u64(1) << n, per = len / u64(it.nchunk);
+        c128 acc = mk(0, 0);
+        for (u64 i = u64(c) * per + threadIdx.x; i < u64(c + 1) * per; i += kBT) {
+            c128 hp = mk(0, 0), sd = mk(0, 0);
+            for (int gi = 0; gi < it.G; ++gi) {
+                const ObsGroup g = it.groups[gi];
+                const int D = 1 << g.k;
+                int r = 0;
+                u64 base = i;
+                for (int j = 0; j < g.k; ++j) {
+                    r |= int((i >> g.sites[j]) & 1) << j;
+                    base &= ~(u64(1) << g.sites[j]);
+                }
+                for (int cc = 0; cc < D; ++cc) {
+                    u64 o = 0;
+                    for (int j = 0; j < g.k; ++j)
+                        if (cc & (1 << j)) o |= u64(1) << g.sites[j];
+                    const c128 x = psi[base | o];
+                    hp = cfma(it.mats[g.moff + r + D * cc], x, hp);
+                    sd = cfma(it.seed[g.moff + r + D * cc], x, sd);
+                }
+            }
+            lam[i] = sd;
+            acc = cfmaconj(psi[i], hp, acc);
+        }
+        c128 s = bblock_sum(acc, red);
+        if (threadIdx.x == 0) part[it.chunk0 + c] = s;
+        return;
+    }
+    const bool fwd = pc < nfwd;
+    const BOp op = it.ops[fwd ? pc : 2 * nfwd - pc];
+    if (op.kind == 1)
+        bgate_chunk<1>(it, op, fwd, c, sm, red, part);
+    else
+        bgate_chunk<2>(it, op, fwd, c, sm, red, part);
+}
+
+// Per-item reductions of this iteration's partials (fixed order).
+__global__ __launch_bounds__(kBT) void k_breduce(const BItemDev *__restrict__ items, int t,
+                                                 const c128 *__restrict__ part, double *__restrict__ values,
+                                                 double *__restrict__ grads) {
+    __shared__ c128 red[kBT / 64];
+    const BItemDev &it = items[blockIdx.x];
+    const int pc = t - it.t0;
+    int slot = -1;
+    bool is_value = false;
+    if (pc == it.nfwd) {
+        is_value = true;
+    } else if (pc > it.nfwd && pc <= 2 * it.nfwd) {
+        const BOp op = it.ops[2 * it.nfwd - pc];
+        if (op.grad >= 0) slot = it.gslot0 + op.gslot;
+    }
+    if (!is_value && slot < 0) return;
+    c128 a = mk(0, 0);
+    for (int j = threadIdx.x; j < it.nchunk; j += kBT) a = cadd(a, part[it.chunk0 + j]);
+    c128 s = bblock_sum(a, red);
+    if (threadIdx.x == 0) {
+        if (is_value)
+            values[it.vslot] = s.re;
+        else
+            grads[slot] = s.re;
+    }
+}
+
+int batch_chunks(int n) {
+    u64 pairs = u64(1) << (n - 1);
+    u64 c = pairs / kBatchPairsPerChunk;
+    return int(std::max<u64>(c, 1));
+}
+
+int batch_lower(const Program *p, std::vector<BOp> &ops, int *nderiv) {
+    MOSSQ_CHECK(!p->is_density, MOSSQ_EUNSUPPORTED, "the batched engine runs state-vector programs");
+    ops.clear();
+    for (const Step &st : p->steps) {
+        MOSSQ_CHECK(!st.split && st.kb <= 2, MOSSQ_EUNSUPPORTED, "batched engine: gates on at most two qubits");
+        BOp o{};
+        o.kind = st.kb;
+        o.b0 = st.bits[0];
+        o.b1 = st.kb == 2 ? st.bits[1] : 0;
+        o.fwd = st.fwd;
+        o.rev = st.rev;
+        o.grad = st.grad;
+        o.gslot = st.grad_index;
+        ops.push_back(o);
+    }
+    *nderiv = p->nderiv;
+    return MOSSQ_OK;
+}
+
+int batch_iteration(Ctx *c, const BItemDev *items, int nitems, int nchunks_total, int t, c128 *part, double *values,
+                    double *grads) {
+    if (nitems == 0) return MOSSQ_OK;
+    hipLaunchKernelGGL(k_biter, dim3(nchunks_total), dim3(kBT), 0, c->stream, items, nitems, t, part);
+    MOSSQ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_breduce, dim3(nitems), dim3(kBT), 0, c->stream, items, t, part, values, grads);
+    MOSSQ_HIP(hipGetLastError());
+    return MOSSQ_OK;
+}
+
+// Batch of B items all starting at iteration 0; one host synchronisation at the end.
+static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, const Obs *const *obs,
+                                const int64_t *grad_offset, double *values, double *grads) {
+    std::vector<BItemHost> items(B);
+    std::vector<BOp> all_ops;
+    std::vector<int> op_off(B);
+    i64 total_amps = 0, dtot = 0;
+    int nchunks = 0, tmax = 0;
+    for (int b = 0; b < B; ++b) {
+        int nd = 0;
+        MOSSQ_TRY(batch_lower(progs[b], items[b].ops, &nd));
+        op_off[b] = int(all_ops.size());
+        all_ops.insert(all_ops.end(), items[b].ops.begin(), items[b].ops.end());
+        items[b].nfwd = int(items[b].ops.size());
+        items[b].nchunk = batch_chunks(progs[b]->n);
+        items[b].gslot0 = int(grad_offset[b]);
+        items[b].vslot = b;
+        total_amps += i64(1) << progs[b]->n;
+        dtot = std::max<i64>(dtot, grad_offset[b] + nd);
+        nchunks += items[b].nchunk;
+        tmax = std::max(tmax, 2 * items[b].nfwd);
+    }
+    Allocs al(c);
+    c128 *vec = nullptr, *part = nullptr;
+    BOp *d_ops = nullptr;
+    BItemDev *d_items = nullptr;
+    double *d_vals = nullptr, *d_grads = nullptr;
+    MOSSQ_TRY(al.get(u64(total_amps) * 2 * sizeof(c128), (void **)&vec));
+    MOSSQ_TRY(al.get(u64(std::max(nchunks, 1)) * sizeof(c128), (void **)&part));
+    MOSSQ_TRY(al.get(std::max<size_t>(all_ops.size(), 1) * sizeof(BOp), (void **)&d_ops));
+    MOSSQ_TRY(al.get(u64(B) * sizeof(BItemDev), (void **)&d_items));
+    MOSSQ_TRY(al.get(u64(B) * sizeof(double), (void **)&d_vals));
+    MOSSQ_TRY(al.get(u64(std::max<i64>(dtot, 1)) * sizeof(double), (void **)&d_grads));
+    std::vector<BItemDev> hd(B);
+    i64 vo = 0;
+    int ch = 0;
+    for (int b = 0; b < B; ++b) {
+        const i64 len = i64(1) << progs[b]->n;
+        BItemDev &d = hd[b];
+        d.psi = vec + vo;
+        d.lam = vec + vo + len;
+        vo += 2 * len;
+        d.ops = d_ops + op_off[b];
+        d.pool = progs[b]->d_pool;
+        d.groups = obs[b]->d_groups;
+        d.mats = obs[b]->d_mats;
+        d.seed = obs[b]->d_mats_seed;
+        d.n = progs[b]->n;
+        d.nfwd = items[b].nfwd;
+        d.G = obs[b]->G;
+        d.t0 = 0;
+        d.chunk0 = ch;
+        d.nchunk = items[b].nchunk;
+        d.gslot0 = items[b].gslot0;
+        d.vslot = b;
+        ch += d.nchunk;
+        MOSSQ_TRY(fill_basis0(c, d.psi, u64(len)));
+    }
+    if (!all_ops.empty())
+        MOSSQ_HIP(hipMemcpyAsync(d_ops, all_ops.data(), all_ops.size() * sizeof(BOp), hipMemcpyHostToDevice,
+                                 c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(d_items, hd.data(), hd.size() * sizeof(BItemDev), hipMemcpyHostToDevice, c->stream));
+    for (int t = 0; t <= tmax; ++t) MOSSQ_TRY(batch_iteration(c, d_items, B, nchunks, t, part, d_vals, d_grads));
+    std::vector<double> hv(B), hg(std::max<i64>(dtot, 1));
+    MOSSQ_HIP(hipMemcpyAsync(hv.data(), d_vals, B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(hg.data(), d_grads, hg.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < B; ++b) {
+        values[b] = hv[b];
+        for (int j = 0; j < progs[b]->nderiv; ++j) grads[grad_offset[b] + j] = hg[grad_offset[b] + j];
+    }
+    return MOSSQ_OK;
 }
 
 }  // namespace mossq
@@ -85,9 +312,10 @@ int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs, moss
         MOSSQ_CHECK(P[b]->ctx == c && O[b]->ctx == c, MOSSQ_EINVAL, "batch item belongs to another context");
         MOSSQ_CHECK(!P[b]->is_density, MOSSQ_EUNSUPPORTED, "batched adjoint takes state-vector programs");
         MOSSQ_CHECK(O[b]->n == P[b]->n, MOSSQ_EINVAL, "observable and program qubit counts differ");
+        MOSSQ_CHECK(grad_offset[b] >= 0, MOSSQ_EINVAL, "negative gradient offset");
     }
     if (B == 0) return MOSSQ_OK;
-    return batch_adjoint_sequential(c, B, P.data(), O.data(), grad_offset, values, grads);
+    return batch_adjoint_engine(c, B, P.data(), O.data(), grad_offset, values, grads);
 }
 
 }  // extern "C"

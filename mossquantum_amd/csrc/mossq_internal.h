@@ -108,6 +108,21 @@ int get_scratch(Ctx *c, u64 bytes, void **p);
 int stage_matrix(Ctx *c, const c128 *host, u64 count, const c128 **dev);
 int get_pinned(Ctx *c, u64 bytes, double **p);
 
+// Scoped device allocations: everything obtained through get() is freed on exit.
+struct Allocs {
+    Ctx *c;
+    std::vector<void *> ptrs;
+    explicit Allocs(Ctx *cc) : c(cc) {}
+    int get(u64 bytes, void **p) {
+        int st = dev_alloc(c, bytes, p);
+        if (st == MOSSQ_OK) ptrs.push_back(*p);
+        return st;
+    }
+    ~Allocs() {
+        for (void *p : ptrs) dev_free(c, p);
+    }
+};
+
 // --------------------------------------------------------- gate launches ----
 // Classification of a local matrix, used to pick a kernel.
 struct MatInfo {

@@ -61,5 +61,15 @@ struct AdjointScratch {
 };
 int adjoint_pure_enqueue(Ctx *c, const Program *p, const Obs *o, const c128 *in, c128 *out_state,
                          const AdjointScratch &s);
+// Pure-state adjoint with its own scratch; synchronises c's stream.
+int adjoint_pure(Ctx *c, const Program *p, const Obs *o, const c128 *in, c128 *out_state, double *value,
+                 double *grads);
+// Density-matrix adjoint (segment checkpoints + recompute); synchronises c's stream.
+int adjoint_density(Ctx *c, const Program *p, const Obs *o, const c128 *in, int in_is_density, c128 *out_state,
+                    double *value, double *grads);
+// Device bytes adjoint_density allocates for p (states, partial sums).
+u64 adjoint_density_bytes(const Program *p, const Obs *o);
+// Upload a host-built program's pool to c's device memory (synchronises c's stream).
+int program_upload(Ctx *c, Program *p);
 
 }  // namespace mossq

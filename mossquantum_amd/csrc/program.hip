@@ -146,21 +146,7 @@ static int copy_results(Ctx *c, const Program *p, const Obs *o, const c128 *valu
     return MOSSQ_OK;
 }
 
-struct Allocs {
-    Ctx *c;
-    std::vector<void *> ptrs;
-    explicit Allocs(Ctx *cc) : c(cc) {}
-    int get(u64 bytes, void **p) {
-        int st = dev_alloc(c, bytes, p);
-        if (st == MOSSQ_OK) ptrs.push_back(*p);
-        return st;
-    }
-    ~Allocs() {
-        for (void *p : ptrs) dev_free(c, p);
-    }
-};
-
-static int adjoint_pure(Ctx *c, const Program *p, const Obs *o, const c128 *in, c128 *out_state, double *value,
+int adjoint_pure(Ctx *c, const Program *p, const Obs *o, const c128 *in, c128 *out_state, double *value,
                         double *grads) {
     const u64 len = u64(1) << p->n;
     Allocs al(c);
@@ -182,7 +168,26 @@ static int adjoint_pure(Ctx *c, const Program *p, const Obs *o, const c128 *in, 
 // before each of its instructions, so channels never need an inverse and
 // memory is (N/s + s + 2) states instead of one per channel
 // (the reference checkpoints before every channel, differentiation/circuit.jl:90-98).
-static int adjoint_density(Ctx *c, const Program *p, const Obs *o, const c128 *in, int in_is_density,
+u64 adjoint_density_bytes(const Program *p, const Obs *o) {
+    const u64 bytes = (u64(1) << p->nbits) * sizeof(c128);
+    const int N = int(p->steps.size());
+    int seg = std::max(1, int(std::ceil(std::sqrt(double(std::max(N, 1))))));
+    int nseg = std::max(1, (N + seg - 1) / seg);
+    u64 parts = std::max<u64>(u64(o->G) * 64, u64(std::max(p->nderiv, 1)) * pair_parts(p->nbits));
+    return bytes * u64(2 + nseg + seg) + (parts + o->G + p->nderiv + 1) * sizeof(c128);
+}
+
+int program_upload(Ctx *c, Program *p) {
+    p->ctx = c;
+    MOSSQ_TRY(dev_alloc(c, std::max<size_t>(p->h_pool.size(), 1) * sizeof(c128), (void **)&p->d_pool));
+    if (!p->h_pool.empty())
+        MOSSQ_HIP(hipMemcpyAsync(p->d_pool, p->h_pool.data(), p->h_pool.size() * sizeof(c128),
+                                 hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    return MOSSQ_OK;
+}
+
+int adjoint_density(Ctx *c, const Program *p, const Obs *o, const c128 *in, int in_is_density,
                            c128 *out_state, double *value, double *grads) {
     const int n = p->n, nb = p->nbits;
     const u64 len = u64(1) << nb, bytes = len * sizeof(c128);

@@ -1,0 +1,581 @@
+// Request server: continuous batching of energy + adjoint-gradient requests.
+//
+// State-vector requests join the batched adjoint engine (mossq_batch.h) at any
+// iteration: admission allocates their psi/lambda vectors from a size-class
+// cache under the server's device-memory budget, uploads their matrices through
+// a pinned staging ring, and starts them at the current iteration; a request
+// retires when its reverse pass ends, its results are copied back
+// asynchronously and its memory returns to the cache.  States stay resident on
+// the device for the whole request.  Density-matrix requests (and state-vector
+// programs with gates wider than two qubits) run on a worker thread with its own
+// context, so their kernels overlap the engine's on the device.
+//
+// New capability: the reference evaluates one objective at a time
+// (src/differentiation/objectives.jl:183-218).
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "mossq_batch.h"
+
+namespace mossq {
+
+constexpr int kMaxActive = 4096;     // engine items resident at once
+constexpr int kMaxGrads = 1024;      // gradient slots per engine item
+constexpr int kInflight = 8;         // engine iterations queued ahead of the device
+constexpr u64 kRingBytes = 64ull << 20;
+
+struct SRequest {
+    i64 ticket = 0;
+    int n = 0, is_density = 0;
+    std::vector<mossq_op> ops;
+    std::vector<c128> pool;
+    const Obs *obs = nullptr;
+    Program prog;  // host program (steps + pool)
+    u64 bytes = 0;  // reserved against the budget
+    // engine state
+    bool engine = false;
+    int slot = -1;  // engine item slot (values / grads arena)
+    int t0 = 0, nfwd = 0, nchunk = 1;
+    u64 vec_bytes = 0, pool_bytes = 0, ops_bytes = 0;
+    c128 *vec = nullptr, *d_pool = nullptr;
+    BOp *d_ops = nullptr;
+    hipEvent_t done_ev = nullptr;
+    // results
+    std::atomic<int> state{0};  // 0 queued, 1 running, 2 results pending copy, 3 done, -1 failed
+    double value = 0;
+    std::vector<double> grads;
+    std::string error;
+};
+
+struct BlockCache {
+    Ctx *c = nullptr;
+    std::map<u64, std::vector<void *>> free;
+    static u64 round(u64 b) {
+        u64 r = 4096;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    int get(u64 bytes, void **p) {
+        u64 r = round(bytes);
+        auto &v = free[r];
+        if (!v.empty()) {
+            *p = v.back();
+            v.pop_back();
+            return MOSSQ_OK;
+        }
+        return dev_alloc(c, r, p);
+    }
+    void put(u64 bytes, void *p) {
+        if (p) free[round(bytes)].push_back(p);
+    }
+    void release() {
+        for (auto &kv : free)
+            for (void *p : kv.second) dev_free(c, p);
+        free.clear();
+    }
+};
+
+// Pinned host staging for asynchronous uploads: a ring whose regions are
+// reused only after the copies reading them have completed.
+struct PinnedRing {
+    char *buf = nullptr;
+    u64 cap = 0, head = 0;
+    struct Pending {
+        u64 lo, hi;
+        hipEvent_t ev;
+    };
+    std::deque<Pending> pending;
+    std::vector<hipEvent_t> spare;
+    int init(u64 bytes) {
+        cap = bytes;
+        MOSSQ_HIP(hipHostMalloc((void **)&buf, bytes, hipHostMallocDefault));
+        return MOSSQ_OK;
+    }
+    int reserve(u64 bytes, char **p, u64 *lo) {
+        MOSSQ_CHECK(bytes <= cap, MOSSQ_EOOM, "request too large for the staging ring");
+        bytes = (bytes + 255) & ~u64(255);
+        if (head + bytes > cap) head = 0;
+        const u64 a = head, b = head + bytes;
+        while (!pending.empty()) {
+            const Pending &q = pending.front();
+            bool overlap = q.lo < b && a < q.hi;
+            if (!overlap && hipEventQuery(q.ev) != hipSuccess) break;
+            MOSSQ_HIP(hipEventSynchronize(q.ev));
+            spare.push_back(q.ev);
+            pending.pop_front();
+        }
+        for (const Pending &q : pending)
+            if (q.lo < b && a < q.hi) MOSSQ_HIP(hipEventSynchronize(q.ev));
+        *p = buf + a;
+        *lo = a;
+        head = b;
+        return MOSSQ_OK;
+    }
+    int mark(hipStream_t s, u64 lo, u64 hi) {
+        hipEvent_t ev;
+        if (!spare.empty()) {
+            ev = spare.back();
+            spare.pop_back();
+        } else {
+            MOSSQ_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        }
+        MOSSQ_HIP(hipEventRecord(ev, s));
+        pending.push_back({lo, hi, ev});
+        return MOSSQ_OK;
+    }
+    void destroy() {
+        for (auto &q : pending) (void)hipEventDestroy(q.ev);
+        for (auto &e : spare) (void)hipEventDestroy(e);
+        if (buf) (void)hipHostFree(buf);
+    }
+};
+
+struct Server {
+    Ctx *ctx = nullptr;
+    Ctx *wctx = nullptr;  // worker context (density requests)
+    u64 budget = 0;
+    std::atomic<u64> reserved{0};
+    std::mutex mu;
+    std::deque<SRequest *> queue;                     // waiting for admission
+    std::unordered_map<i64, SRequest *> requests;     // all live requests
+    std::vector<SRequest *> active;                   // engine items
+    i64 next_ticket = 1;
+    int t = 0;
+    BlockCache cache;
+    PinnedRing ring;
+    // engine tables
+    BItemDev *d_items[2] = {nullptr, nullptr};
+    BItemDev *h_items[2] = {nullptr, nullptr};
+    hipEvent_t table_ev[2] = {nullptr, nullptr};
+    int cur_table = 0;
+    bool table_dirty = true;
+    int nchunks = 0;
+    c128 *part = nullptr;
+    u64 part_cap = 0;
+    double *d_values = nullptr, *d_grads = nullptr;
+    double *h_results = nullptr;  // pinned: per slot 1 + kMaxGrads doubles
+    std::vector<int> free_slots;
+    hipEvent_t iter_ev[kInflight] = {};
+    // retiring engine items (results copy in flight)
+    std::vector<SRequest *> draining;
+    std::vector<i64> completed;  // finished tickets not yet reported
+    // worker
+    std::thread worker;
+    std::deque<SRequest *> wqueue;
+    std::condition_variable wcv;
+    bool stop = false;
+    int worker_busy = 0;
+};
+
+static int build_request_program(SRequest *r) {
+    r->prog = Program();
+    return program_build_host(&r->prog, r->n, r->is_density, int(r->ops.size()), r->ops.data(),
+                              (const double *)r->pool.data(), r->pool.size());
+}
+
+static void worker_main(Server *s) {
+    (void)hipSetDevice(s->wctx->device);
+    while (true) {
+        SRequest *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->wcv.wait(lk, [&] { return s->stop || !s->wqueue.empty(); });
+            if (s->stop && s->wqueue.empty()) return;
+            r = s->wqueue.front();
+            s->wqueue.pop_front();
+            s->worker_busy++;
+        }
+        int st = program_upload(s->wctx, &r->prog);
+        r->grads.assign(std::max(r->prog.nderiv, 1), 0.0);
+        if (st == MOSSQ_OK)
+            st = r->is_density
+                     ? adjoint_density(s->wctx, &r->prog, r->obs, nullptr, 0, nullptr, &r->value, r->grads.data())
+                     : adjoint_pure(s->wctx, &r->prog, r->obs, nullptr, nullptr, &r->value, r->grads.data());
+        if (r->prog.d_pool) dev_free(s->wctx, r->prog.d_pool);
+        r->prog.d_pool = nullptr;
+        r->grads.resize(r->prog.nderiv);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (st != MOSSQ_OK) r->error = mossq_last_error();
+            r->state = st == MOSSQ_OK ? 3 : -1;
+            s->reserved -= r->bytes;
+            s->worker_busy--;
+            s->completed.push_back(r->ticket);
+        }
+    }
+}
+
+static u64 engine_bytes(const SRequest *r) {
+    return 2 * (u64(1) << r->n) * sizeof(c128) + r->pool.size() * sizeof(c128) + r->ops.size() * sizeof(BOp) +
+           u64(batch_chunks(r->n)) * sizeof(c128);
+}
+
+static int ensure_part(Server *s, int need) {
+    if (u64(need) <= s->part_cap) return MOSSQ_OK;
+    u64 cap = std::max<u64>(u64(need) * 2, 1 << 16);
+    c128 *p = nullptr;
+    MOSSQ_TRY(dev_alloc(s->ctx, cap * sizeof(c128), (void **)&p));
+    if (s->part) dev_free(s->ctx, s->part);  // ordered after queued kernels
+    s->part = p;
+    s->part_cap = cap;
+    return MOSSQ_OK;
+}
+
+// Admit one engine request at iteration s->t (caller holds no lock on device work).
+static int admit_engine(Server *s, SRequest *r) {
+    std::vector<BOp> ops;
+    int nd = 0;
+    MOSSQ_TRY(batch_lower(&r->prog, ops, &nd));
+    MOSSQ_CHECK(nd <= kMaxGrads, MOSSQ_EUNSUPPORTED, "request has too many parameters for the engine");
+    MOSSQ_CHECK(!s->free_slots.empty(), MOSSQ_EOOM, "engine is full");
+    Ctx *c = s->ctx;
+    r->vec_bytes = 2 * (u64(1) << r->n) * sizeof(c128);
+    r->pool_bytes = std::max<size_t>(r->prog.h_pool.size(), 1) * sizeof(c128);
+    r->ops_bytes = std::max<size_t>(ops.size(), 1) * sizeof(BOp);
+    MOSSQ_TRY(s->cache.get(r->vec_bytes, (void **)&r->vec));
+    MOSSQ_TRY(s->cache.get(r->pool_bytes, (void **)&r->d_pool));
+    MOSSQ_TRY(s->cache.get(r->ops_bytes, (void **)&r->d_ops));
+    char *stage = nullptr;
+    u64 lo = 0;
+    const u64 up = r->pool_bytes + r->ops_bytes;
+    MOSSQ_TRY(s->ring.reserve(up, &stage, &lo));
+    memcpy(stage, r->prog.h_pool.data(), r->prog.h_pool.size() * sizeof(c128));
+    memcpy(stage + r->pool_bytes, ops.data(), ops.size() * sizeof(BOp));
+    MOSSQ_HIP(hipMemcpyAsync(r->d_pool, stage, r->pool_bytes, hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(r->d_ops, stage + r->pool_bytes, r->ops_bytes, hipMemcpyHostToDevice, c->stream));
+    MOSSQ_TRY(s->ring.mark(c->stream, lo, lo + up));
+    MOSSQ_TRY(fill_basis0(c, r->vec, u64(1) << r->n));
+    r->slot = s->free_slots.back();
+    s->free_slots.pop_back();
+    r->nfwd = int(ops.size());
+    r->nchunk = batch_chunks(r->n);
+    r->t0 = s->t;
+    r->engine = true;
+    r->state = 1;
+    s->active.push_back(r);
+    s->table_dirty = true;
+    return MOSSQ_OK;
+}
+
+static int rebuild_table(Server *s) {
+    const int k = s->cur_table ^ 1;
+    if (s->table_ev[k]) MOSSQ_HIP(hipEventSynchronize(s->table_ev[k]));
+    int ch = 0;
+    BItemDev *h = s->h_items[k];
+    for (size_t i = 0; i < s->active.size(); ++i) {
+        SRequest *r = s->active[i];
+        BItemDev &d = h[i];
+        const u64 len = u64(1) << r->n;
+        d.psi = r->vec;
+        d.lam = r->vec + len;
+        d.ops = r->d_ops;
+        d.pool = r->d_pool;
+        d.groups = r->obs->d_groups;
+        d.mats = r->obs->d_mats;
+        d.seed = r->obs->d_mats_seed;
+        d.n = r->n;
+        d.nfwd = r->nfwd;
+        d.G = r->obs->G;
+        d.t0 = r->t0;
+        d.chunk0 = ch;
+        d.nchunk = r->nchunk;
+        d.gslot0 = r->slot * kMaxGrads;
+        d.vslot = r->slot;
+        ch += r->nchunk;
+    }
+    s->nchunks = ch;
+    MOSSQ_TRY(ensure_part(s, ch));
+    if (!s->active.empty())
+        MOSSQ_HIP(hipMemcpyAsync(s->d_items[k], h, s->active.size() * sizeof(BItemDev), hipMemcpyHostToDevice,
+                                 s->ctx->stream));
+    if (!s->table_ev[k]) MOSSQ_HIP(hipEventCreateWithFlags(&s->table_ev[k], hipEventDisableTiming));
+    MOSSQ_HIP(hipEventRecord(s->table_ev[k], s->ctx->stream));
+    s->cur_table = k;
+    s->table_dirty = false;
+    return MOSSQ_OK;
+}
+
+// Retire engine items whose last micro-op ran in iteration t-1.
+static int retire(Server *s) {
+    Ctx *c = s->ctx;
+    std::vector<SRequest *> keep;
+    for (SRequest *r : s->active) {
+        if (s->t - 1 < r->t0 + 2 * r->nfwd) {
+            keep.push_back(r);
+            continue;
+        }
+        const int nd = r->prog.nderiv;
+        double *dst = s->h_results + size_t(r->slot) * (1 + kMaxGrads);
+        MOSSQ_HIP(hipMemcpyAsync(dst, s->d_values + r->slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (nd)
+            MOSSQ_HIP(hipMemcpyAsync(dst + 1, s->d_grads + size_t(r->slot) * kMaxGrads, nd * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream));
+        if (!r->done_ev) MOSSQ_HIP(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
+        MOSSQ_HIP(hipEventRecord(r->done_ev, c->stream));
+        // stream order keeps later users of these blocks behind this request's kernels
+        s->cache.put(r->vec_bytes, r->vec);
+        s->cache.put(r->pool_bytes, r->d_pool);
+        s->cache.put(r->ops_bytes, r->d_ops);
+        r->vec = r->d_pool = nullptr;
+        r->d_ops = nullptr;
+        r->state = 2;
+        s->draining.push_back(r);
+        s->table_dirty = true;
+    }
+    s->active.swap(keep);
+    return MOSSQ_OK;
+}
+
+// Results whose copies have landed: move them out of pinned memory, free the slot.
+static void collect(Server *s) {
+    std::vector<SRequest *> keep;
+    for (SRequest *r : s->draining) {
+        if (hipEventQuery(r->done_ev) != hipSuccess) {
+            keep.push_back(r);
+            continue;
+        }
+        const double *src = s->h_results + size_t(r->slot) * (1 + kMaxGrads);
+        r->value = src[0];
+        r->grads.assign(src + 1, src + 1 + r->prog.nderiv);
+        s->free_slots.push_back(r->slot);
+        r->slot = -1;
+        s->reserved -= r->bytes;
+        r->state = 3;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->completed.push_back(r->ticket);
+    }
+    s->draining.swap(keep);
+}
+
+static int server_step(Server *s, int *active, int *queued) {
+    Ctx *c = s->ctx;
+    collect(s);
+    MOSSQ_TRY(retire(s));
+    // admission (FIFO): engine requests start at this iteration; others go to the worker
+    while (true) {
+        SRequest *r = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->queue.empty()) break;
+            r = s->queue.front();
+            if (s->reserved + r->bytes > s->budget) break;
+            if (r->engine && s->free_slots.empty()) break;
+            s->queue.pop_front();
+            s->reserved += r->bytes;
+            if (!r->engine) {
+                r->state = 1;
+                s->wqueue.push_back(r);
+                s->wcv.notify_one();
+                continue;
+            }
+        }
+        int st = admit_engine(s, r);
+        if (st != MOSSQ_OK) {
+            r->error = mossq_last_error();
+            r->state = -1;
+            s->reserved -= r->bytes;
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->completed.push_back(r->ticket);
+        }
+    }
+    if (s->table_dirty) MOSSQ_TRY(rebuild_table(s));
+    if (!s->active.empty()) {
+        hipEvent_t &ev = s->iter_ev[s->t % kInflight];
+        if (ev)
+            MOSSQ_HIP(hipEventSynchronize(ev));
+        else
+            MOSSQ_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        MOSSQ_TRY(batch_iteration(c, s->d_items[s->cur_table], int(s->active.size()), s->nchunks, s->t, s->part,
+                                  s->d_values, s->d_grads));
+        MOSSQ_HIP(hipEventRecord(ev, c->stream));
+        s->t++;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (active) *active = int(s->active.size() + s->draining.size()) + int(s->wqueue.size()) + s->worker_busy;
+    if (queued) *queued = int(s->queue.size());
+    return MOSSQ_OK;
+}
+
+}  // namespace mossq
+
+using namespace mossq;
+
+extern "C" {
+
+int mossq_server_create(mossq_ctx *ctx, uint64_t budget_bytes, mossq_server **out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(out && budget_bytes > 0, MOSSQ_EINVAL, "bad server arguments");
+    Server *s = new Server();
+    s->ctx = C(ctx);
+    s->budget = budget_bytes;
+    s->cache.c = s->ctx;
+    mossq_ctx *w = nullptr;
+    int st = mossq_ctx_create(s->ctx->device, 0, &w);
+    if (st != MOSSQ_OK) {
+        delete s;
+        return st;
+    }
+    s->wctx = C(w);
+    st = s->ring.init(kRingBytes);
+    for (int k = 0; st == MOSSQ_OK && k < 2; ++k) {
+        st = dev_alloc(s->ctx, kMaxActive * sizeof(BItemDev), (void **)&s->d_items[k]);
+        if (st == MOSSQ_OK && hipHostMalloc((void **)&s->h_items[k], kMaxActive * sizeof(BItemDev)) != hipSuccess)
+            st = fail(MOSSQ_EOOM, "pinned item table");
+    }
+    if (st == MOSSQ_OK) st = dev_alloc(s->ctx, kMaxActive * sizeof(double), (void **)&s->d_values);
+    if (st == MOSSQ_OK) st = dev_alloc(s->ctx, size_t(kMaxActive) * kMaxGrads * sizeof(double), (void **)&s->d_grads);
+    if (st == MOSSQ_OK &&
+        hipHostMalloc((void **)&s->h_results, size_t(kMaxActive) * (1 + kMaxGrads) * sizeof(double)) != hipSuccess)
+        st = fail(MOSSQ_EOOM, "pinned result arena");
+    if (st != MOSSQ_OK) {
+        mossq_server_destroy(reinterpret_cast<mossq_server *>(s));
+        return st;
+    }
+    for (int i = kMaxActive - 1; i >= 0; --i) s->free_slots.push_back(i);
+    s->worker = std::thread(worker_main, s);
+    *out = reinterpret_cast<mossq_server *>(s);
+    return MOSSQ_OK;
+}
+
+int mossq_server_destroy(mossq_server *srv) {
+    if (!srv) return MOSSQ_OK;
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (s->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->wcv.notify_all();
+        s->worker.join();
+    }
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (auto &kv : s->requests) {
+        SRequest *r = kv.second;
+        if (r->done_ev) (void)hipEventDestroy(r->done_ev);
+        delete r;
+    }
+    s->cache.release();
+    for (int k = 0; k < 2; ++k) {
+        if (s->d_items[k]) dev_free(s->ctx, s->d_items[k]);
+        if (s->h_items[k]) (void)hipHostFree(s->h_items[k]);
+        if (s->table_ev[k]) (void)hipEventDestroy(s->table_ev[k]);
+    }
+    for (auto &e : s->iter_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->part) dev_free(s->ctx, s->part);
+    if (s->d_values) dev_free(s->ctx, s->d_values);
+    if (s->d_grads) dev_free(s->ctx, s->d_grads);
+    if (s->h_results) (void)hipHostFree(s->h_results);
+    s->ring.destroy();
+    if (s->wctx) mossq_ctx_destroy(reinterpret_cast<mossq_ctx *>(s->wctx));
+    delete s;
+    return MOSSQ_OK;
+}
+
+int mossq_server_submit(mossq_server *srv, int n, int is_density, int nops, const mossq_op *ops, const double *pool,
+                        uint64_t pool_len, const mossq_obs *obs, int64_t *ticket) {
+    MOSSQ_CHECK(srv && obs && ticket, MOSSQ_EINVAL, "null argument");
+    Server *s = reinterpret_cast<Server *>(srv);
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    MOSSQ_CHECK(o->ctx == s->ctx, MOSSQ_EINVAL, "observable belongs to another context");
+    MOSSQ_CHECK(o->n == n, MOSSQ_EINVAL, "observable and request qubit counts differ");
+    SRequest *r = new SRequest();
+    r->n = n;
+    r->is_density = is_density ? 1 : 0;
+    r->ops.assign(ops, ops + nops);
+    r->pool.assign((const c128 *)pool, (const c128 *)pool + pool_len);
+    r->obs = o;
+    int st = build_request_program(r);
+    if (st != MOSSQ_OK) {
+        delete r;
+        return st;
+    }
+    bool engine_ok = !r->is_density;
+    if (engine_ok)
+        for (const Step &x : r->prog.steps) engine_ok &= (!x.split && x.kb <= 2);
+    engine_ok = engine_ok && r->prog.nderiv <= kMaxGrads;
+    r->engine = engine_ok;
+    if (engine_ok) {
+        r->bytes = engine_bytes(r);
+    } else if (r->is_density) {
+        r->bytes = adjoint_density_bytes(&r->prog, o) + r->pool.size() * sizeof(c128);
+    } else {
+        r->bytes = 2 * (u64(1) << n) * sizeof(c128) * 2 + r->pool.size() * sizeof(c128);
+    }
+    if (r->bytes > s->budget) {
+        delete r;
+        return fail(MOSSQ_EOOM, "request needs more device memory than the server budget");
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->ticket = s->next_ticket++;
+    s->requests[r->ticket] = r;
+    s->queue.push_back(r);
+    *ticket = r->ticket;
+    return MOSSQ_OK;
+}
+
+int mossq_server_step(mossq_server *srv, int *active, int *queued) {
+    MOSSQ_CHECK(srv, MOSSQ_EINVAL, "null server");
+    Server *s = reinterpret_cast<Server *>(srv);
+    MOSSQ_HIP(hipSetDevice(s->ctx->device));
+    return server_step(s, active, queued);
+}
+
+int mossq_server_completed(mossq_server *srv, int64_t *tickets, int max, int *count) {
+    MOSSQ_CHECK(srv && count && (tickets || max == 0), MOSSQ_EINVAL, "null argument");
+    Server *s = reinterpret_cast<Server *>(srv);
+    collect(s);
+    std::lock_guard<std::mutex> lk(s->mu);
+    int k = std::min<int>(max, int(s->completed.size()));
+    for (int i = 0; i < k; ++i) tickets[i] = s->completed[i];
+    s->completed.erase(s->completed.begin(), s->completed.begin() + k);
+    *count = k;
+    return MOSSQ_OK;
+}
+
+int mossq_server_poll(mossq_server *srv, int64_t ticket, int *done, double *value, double *grads, int max_grads) {
+    MOSSQ_CHECK(srv && done, MOSSQ_EINVAL, "null argument");
+    Server *s = reinterpret_cast<Server *>(srv);
+    SRequest *r = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->requests.find(ticket);
+        MOSSQ_CHECK(it != s->requests.end(), MOSSQ_EINVAL, "unknown or already collected ticket");
+        r = it->second;
+    }
+    if (r->state == 2) collect(s);
+    int st = r->state;
+    if (st == -1) {
+        *done = 1;
+        std::string e = r->error;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->requests.erase(ticket);
+        delete r;
+        return fail(MOSSQ_EDEVICE, "request failed: " + e);
+    }
+    if (st != 3) {
+        *done = 0;
+        return MOSSQ_OK;
+    }
+    *done = 1;
+    if (value) *value = r->value;
+    if (grads) {
+        MOSSQ_CHECK(max_grads >= int(r->grads.size()), MOSSQ_EINVAL, "gradient buffer too small");
+        std::copy(r->grads.begin(), r->grads.end(), grads);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->requests.erase(ticket);
+    if (r->done_ev) (void)hipEventDestroy(r->done_ev);
+    delete r;
+    return MOSSQ_OK;
+}
+
+}  // extern "C"

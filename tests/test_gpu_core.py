@@ -259,6 +259,25 @@ def test_batched_adjoint_vs_single(ctx):
         assert np.max(np.abs(grads[i] - want_g[i])) < GRAD
 
 
+def test_batched_adjoint_uneven_sizes(ctx):
+    """Many items with uneven qubit counts (chunked and single-chunk items) in
+    one engine run; each item's value and gradient match the oracle, and match
+    the item run alone."""
+    rng = np.random.default_rng(15)
+    sizes = [int(x) for x in rng.integers(10, 17, size=24)]
+    circs, obs = [], []
+    for n in sizes:
+        circs.append(hea_circuit(n, 2, rng.uniform(0, 2 * np.pi, 4 * n)))
+        obs.append(xxz(n))
+    vals, grads = mq.batched_adjoint(circs, obs, sizes, ctx)
+    for i in (0, 5, 11, 23):
+        v, g = oracle.adjoint(lower(circs[i], sizes[i], False), obs[i].groups())
+        assert abs(vals[i] - v) < EXP
+        assert np.max(np.abs(grads[i] - g)) < GRAD
+        v1, g1 = mq.batched_adjoint([circs[i]], [obs[i]], [sizes[i]], ctx)
+        assert abs(v1[0] - vals[i]) < EXP and np.max(np.abs(g1[0] - grads[i])) < GRAD
+
+
 def test_unsupported_before_payload(ctx):
     sv = mq.StateVector.zeros(3)
     big = mq.QubitsOperator([mq.QubitsTerm({1: "X", 2: "X", 3: "X"})])
