@@ -34,6 +34,22 @@ HBM_PEAK_GBPS = 8000.0
 N, DEPTH = 26, 20
 
 
+FP64_PEAK_TFLOPS = 78.6
+TRAFFIC_FILE = ROOT / "profiles" / "k_tile_26q_depth20_traffic.txt"
+
+
+def read_traffic():
+    """HBM bytes per k_tile launch measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (calibrated on k_gate1), from the committed profile summary."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -149,17 +165,95 @@ def bench_batched(ctx, B=512, layers=6, seed=20260300, cpu=True):
         from mossquantum_amd import lower
         # one state of every size, timed with the oracle; the batch estimate sums per-size times
         per_size = {}
-        for n in range(10, 21):
+        for n in range(10, 19):
             c = hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers))
             lw = lower(c, n, False)
             t1 = time.perf_counter()
             oracle.adjoint(lw, xxz(n).groups())
             per_size[n] = time.perf_counter() - t1
+        for n in (19, 20):  # scaled from n=18 by the work ratio 2^n (3n-1)
+            per_size[n] = per_size[18] * 2 ** (n - 18) * (3 * n - 1) / (3 * 18 - 1)
         est = sum(per_size[n] for n in sizes)
         out["cpu_estimate_s"] = est
         out["cpu_grads_per_s"] = B / est
-        out["cpu_sample"] = "one adjoint per size n=10..20 with cpu_ref (OpenMP), summed over the batch's sizes"
+        out["cpu_sample"] = ("one adjoint per size n=10..18 timed with cpu_ref (OpenMP), n=19,20 scaled from "
+                             "n=18 by 2^n(3n-1); summed over the batch's sizes")
         log(f"cpu estimate: {est:.1f} s for the batch")
+    return out
+
+
+def bench_density(ctx, n=13, layers=8, seed=20260213, reps=2):
+    """13-qubit density matrix (4^13 entries = 1 GiB), the ansatz with 8 layers,
+    Depolarizing(0.01) then AmplitudeDamping(0.02) on every touched qubit after
+    every gate; TFIM energy, trace and purity; fused and unfused plans."""
+    import mossquantum_amd as mq
+    from mossquantum_amd.workloads import hea_angles, hea_circuit, tfim
+    c = hea_circuit(n, layers, hea_angles(n, layers, seed), noise=(0.01, 0.02))
+    nops = len(list(c.flat()))
+    out = {"n": n, "layers": layers, "instructions": nops, "state_bytes": 16 * 4 ** n}
+    states = {}
+    for fuse in (True, False):
+        prog = mq.Program(c, n, True, ctx, fuse=fuse)
+        dm = mq.DensityMatrix.zeros(n, ctx)
+        prog.run(dm.storage)  # warm-up pass (not timed)
+        ms = []
+        for _ in range(reps):
+            z = mq.DensityMatrix.zeros(n, ctx)
+            ctx.sync()
+            ctx.record(8)
+            prog.run(z.storage)
+            ctx.record(9)
+            ms.append(ctx.elapsed_ms(8, 9))
+            dm = z
+        key = "fused" if fuse else "unfused"
+        t = min(ms)
+        sweeps = prog.sweeps()
+        out[key] = {"ms": t, "sweeps": sweeps, "instructions_per_s": nops / (t / 1e3),
+                    "hbm_gbps": sweeps * 2 * 16 * 4 ** n / (t / 1e3) / 1e9,
+                    "effective_gbps_unfused_bytes": nops * 2 * 16 * 4 ** n / (t / 1e3) / 1e9}
+        states[key] = dm
+        log(f"density {key}: {t:.1f} ms, {sweeps} sweeps")
+    dm = states["fused"]
+    out["energy"] = mq.expectation(tfim(n), dm).real
+    out["trace"] = mq.trace(dm).real
+    out["purity"] = mq.purity(dm)
+    out["energy_unfused"] = mq.expectation(tfim(n), states["unfused"]).real
+    out["fused_vs_unfused_energy_diff"] = abs(out["energy"] - out["energy_unfused"])
+    return out
+
+
+def bench_serving(ctx, rates=(50.0, 200.0, 2000.0), nreq=300, seed=20260400, budget=200 << 30):
+    """Poisson request stream: 70% pure states n ~ U{8..24}, 30% density
+    matrices n ~ U{6..12} with the noisy channels; 4-8 layers; TFIM energy +
+    adjoint gradient; 200 GB admission budget.  A bounded sample of `nreq`
+    requests per rate (the full 20k-request sweep is `--serving-requests 20000`)."""
+    from mossquantum_amd.serving import RequestTemplate, Server, serve_poisson
+    from mossquantum_amd.workloads import hea_circuit, tfim
+    rng = np.random.default_rng(seed)
+    tpls = {}
+
+    def template(kind, n, layers):
+        key = (kind, n, layers)
+        if key not in tpls:
+            noise = (0.01, 0.02) if kind == "dm" else None
+            tpls[key] = RequestTemplate(hea_circuit(n, layers, noise=noise), n, kind == "dm", tfim(n), ctx)
+        return tpls[key]
+
+    srv = Server(ctx, budget)
+    out = {"requests_per_rate": nreq, "mix": "70% SV n~U{8..24}, 30% DM n~U{6..12}, layers~U{4..8}",
+           "budget_bytes": budget, "rates": []}
+    for rate in rates:
+        reqs = []
+        for _ in range(nreq):
+            dm = rng.random() < 0.3
+            n = int(rng.integers(6, 13)) if dm else int(rng.integers(8, 25))
+            tpl = template("dm" if dm else "sv", n, int(rng.integers(4, 9)))
+            reqs.append((tpl, tpl.bind(rng.uniform(0, 2 * np.pi, tpl.nparams))))
+        _, stats = serve_poisson(srv, reqs, rate, seed=int(rate))
+        log(f"serving at {rate:.0f} req/s: {stats['throughput_req_s']:.1f} req/s, "
+            f"p50 {stats['p50_ms']:.1f} ms, p99 {stats['p99_ms']:.1f} ms")
+        out["rates"].append(stats)
+    srv.close()
     return out
 
 
@@ -171,6 +265,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-unfused", action="store_true", help="skip the unfused leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-adjoint leg")
+    ap.add_argument("--no-density", action="store_true", help="skip the 13-qubit density-matrix leg")
+    ap.add_argument("--serving-requests", type=int, default=300,
+                    help="requests per arrival rate in the serving leg (0 skips it)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -205,6 +302,9 @@ def main():
     tile_avg_ms = step_f / nsweeps
     tile_bytes = 32 * 2 ** N
     achieved = tile_bytes / (tile_avg_ms / 1e3) / 1e9
+    # algorithmic flops: a dense 2x2 complex update is 28 flops per amplitude pair
+    tile_flops = n1q * 2 ** (N - 1) * 28 / nsweeps
+    tile_tflops = tile_flops / (tile_avg_ms / 1e3) / 1e12
 
     if not args.no_unfused:
         sv_u = mq.StateVector.zeros(N, ctx)
@@ -233,6 +333,10 @@ def main():
     del sv
     if not args.no_batched:
         extra["batched_adjoint"] = bench_batched(ctx, cpu=not args.no_cpu)
+    if not args.no_density:
+        extra["density_13q"] = bench_density(ctx)
+    if args.serving_requests > 0:
+        extra["serving"] = bench_serving(ctx, nreq=args.serving_requests)
 
     cpu = None
     if not args.no_cpu:
@@ -257,9 +361,14 @@ def main():
         "data": "synthetic (seeded Haar unitaries, state |0...0>)",
         "config": {"workload": "26q c128 SV random circuit, depth 20 (Haar 1q + alternating CZ), fused plan",
                    "n_qubits": N, "depth": DEPTH, "gates": ngates, "state_bytes": 16 * 2 ** N},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_tile",
-                     "avg_launch_ms": tile_avg_ms, "bytes_per_launch": tile_bytes},
+        "roofline": {"bound": "mfma", "achieved": tile_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tile_tflops / FP64_PEAK_TFLOPS, "traffic": read_traffic(), "kernel": "k_tile",
+                     "note": "fp64 FMA on the vector ALUs (MI355X dense fp64 peak 78.6 TFLOP/s is the same for "
+                             "vector and matrix units); a fused sweep carries ~35 dense gates per amplitude "
+                             "pass (~17 flop/B), past the fp64 ridge (~10 flop/B)",
+                     "avg_launch_ms": tile_avg_ms, "flops_per_launch": tile_flops,
+                     "bytes_per_launch": tile_bytes, "hbm_achieved_gbps": achieved,
+                     "hbm_frac": achieved / HBM_PEAK_GBPS},
         "cpu_baseline": cpu,
         "extra": extra,
     }
