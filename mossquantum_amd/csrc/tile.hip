@@ -462,6 +462,63 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg) {
             }
         }
     }
+    // Fold one-bit dense ops into the two-bit dense op they meet first on
+    // their bit (same flops, one op fewer), then pair the remaining one-bit
+    // dense ops on different bits into Kronecker products: a two-bit op costs
+    // the same flops as two one-bit ops but half the dispatch work.
+    auto embed1 = [](const POp &one, const POp &two) {
+        std::vector<c128> I = {mk(1, 0), mk(0, 0), mk(0, 0), mk(1, 0)};
+        return one.bits[0] == two.bits[0] ? kron2(one.m, I) : kron2(I, one.m);
+    };
+    changed = true;
+    while (changed) {
+        changed = false;
+        for (size_t i = 0; i < ops.size() && !changed; ++i) {
+            if (ops[i].kind != 0) continue;
+            size_t j = i + 1;
+            while (j < ops.size() && (ops[j].mask() & ops[i].mask()) == 0) ++j;
+            if (j >= ops.size() || ops[j].kind != 0) continue;
+            POp &A = ops[i], &B = ops[j];
+            if (A.nb == 1 && B.nb == 2) {  // B * (A on one of B's bits)
+                std::vector<c128> pr;
+                product(B.m, embed1(A, B), 4, pr);
+                B.m.swap(pr);
+                ops.erase(ops.begin() + i);
+                changed = true;
+            } else if (A.nb == 2 && B.nb == 1) {  // (B on one of A's bits) * A, B moved left
+                std::vector<c128> pr;
+                product(embed1(B, A), A.m, 4, pr);
+                A.m.swap(pr);
+                ops.erase(ops.begin() + j);
+                changed = true;
+            }
+        }
+    }
+    for (size_t i = 0; i < ops.size(); ++i) {
+        if (ops[i].kind != 0 || ops[i].nb != 1) continue;
+        const int a = ops[i].bits[0];
+        if (!in_reg(a)) continue;
+        u64 touched = 0;
+        for (size_t k = i + 1; k < ops.size(); ++k) {
+            const POp &B = ops[k];
+            if (B.kind == 0 && B.nb == 1 && B.bits[0] != a && in_reg(B.bits[0]) &&
+                ((touched | (u64(1) << a)) & B.mask()) == 0) {
+                POp F;
+                F.kind = 0;
+                F.nb = 2;
+                const POp &lo = a < B.bits[0] ? ops[i] : B, &hi = a < B.bits[0] ? B : ops[i];
+                F.bits[0] = lo.bits[0];
+                F.bits[1] = hi.bits[0];
+                F.m = kron2(lo.m, hi.m);
+                ops[k] = F;
+                ops.erase(ops.begin() + i);
+                --i;
+                break;
+            }
+            touched |= B.mask();
+            if (touched & (u64(1) << a)) break;
+        }
+    }
 }
 
 static bool sign_only(const std::vector<c128> &d, int *negmask) {
