@@ -537,7 +537,7 @@ static bool safe_layout(const int *reg) {
     return true;
 }
 
-FusedPlan *fused_plan_build_host(const Program *p, std::string *err) {
+static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *err) {
     std::vector<POp> ops;
     lower_steps(p, ops);
     FusedPlan *fp = new FusedPlan();
@@ -616,51 +616,117 @@ FusedPlan *fused_plan_build_host(const Program *p, std::string *err) {
             }
             return true;
         };
-        auto next_reg = [&](int *reg) {
-            std::vector<int> want;
-            for (size_t ci = 0; ci < chosen.size() && want.size() < TRB; ++ci) {
-                if (sched[ci] || !ready(ci)) continue;
-                const POp &o = ops[chosen[ci]];
-                if (o.kind != 0) continue;
-                std::vector<int> add;
-                for (int q = 0; q < o.nb; ++q) {
-                    int l = local_of[o.bits[q]];
-                    if (std::find(want.begin(), want.end(), l) == want.end()) add.push_back(l);
-                }
-                if (want.size() + add.size() <= TRB) want.insert(want.end(), add.begin(), add.end());
-            }
-            for (int l = TM - 1; l >= 0 && want.size() < TRB; --l)
-                if (std::find(want.begin(), want.end(), l) == want.end()) want.push_back(l);
-            std::sort(want.begin(), want.end());
-            if (want.size() != TRB || want[0] < 0 || want[TRB - 1] >= TM ||
-                std::adjacent_find(want.begin(), want.end()) != want.end())
-                return false;
-            for (int i = 0; i < TRB; ++i) reg[i] = want[i];
-            return true;
-        };
-        int reg[TRB] = {8, 9, 10, 11};
-        if (!next_reg(reg)) return failp("fusion planner produced an invalid register set");
-        it.args.seg0 = int(fp->h_segs.size());
-        if (!safe_layout(reg)) {
-            TSeg s0{{8, 9, 10, 11}, int(fp->h_ops.size()), int(fp->h_ops.size())};
-            fp->h_segs.push_back(s0);
-        }
-        while (nsched < chosen.size()) {
-            TSeg sg;
-            for (int i = 0; i < TRB; ++i) sg.reg[i] = reg[i];
-            std::vector<POp> seg_ops;
+        // Ops a segment with register set `reg` would schedule from the current state.
+        auto simulate = [&](const int *reg) {
+            std::vector<char> s2(sched);
+            int count = 0;
             bool progress = true;
             while (progress) {
                 progress = false;
                 for (size_t ci = 0; ci < chosen.size(); ++ci) {
-                    if (sched[ci] || !fits(ci, reg) || !ready(ci)) continue;
-                    sched[ci] = 1;
-                    ++nsched;
-                    seg_ops.push_back(ops[chosen[ci]]);
+                    if (s2[ci] || !fits(ci, reg)) continue;
+                    bool ok = true;
+                    for (size_t pj = 0; pj < ci && ok; ++pj)
+                        if (!s2[pj] && !commutes(ops[chosen[pj]], ops[chosen[ci]])) ok = false;
+                    if (!ok) continue;
+                    s2[ci] = 1;
+                    count += ops[chosen[ci]].kind == 0 ? 2 : 1;
                     progress = true;
                 }
             }
-            if (seg_ops.empty()) return failp("fusion planner made no progress");
+            return count;
+        };
+        // Next register set: candidates seeded by each ready dense op and grown
+        // greedily with the bits of further ready ops; the candidate that
+        // schedules the most work wins (ties: fewer bits changed).
+        auto next_reg = [&](int *reg) {
+            std::vector<int> best;
+            int best_score = -1, best_keep = -1;
+            int seeds = 0;
+            for (size_t c0 = 0; c0 < chosen.size() && seeds < 16; ++c0) {
+                if (sched[c0] || ops[chosen[c0]].kind != 0 || !ready(c0)) continue;
+                ++seeds;
+                std::vector<int> want;
+                auto try_add = [&](const POp &o) {
+                    std::vector<int> add;
+                    for (int q = 0; q < o.nb; ++q) {
+                        int l = local_of[o.bits[q]];
+                        if (std::find(want.begin(), want.end(), l) == want.end()) add.push_back(l);
+                    }
+                    if (want.size() + add.size() <= TRB) want.insert(want.end(), add.begin(), add.end());
+                };
+                try_add(ops[chosen[c0]]);
+                for (size_t ci = 0; ci < chosen.size() && want.size() < TRB; ++ci)
+                    if (!sched[ci] && ci != c0 && ops[chosen[ci]].kind == 0) try_add(ops[chosen[ci]]);
+                for (int i = 0; i < TRB && want.size() < TRB; ++i)  // keep current bits
+                    if (std::find(want.begin(), want.end(), reg[i]) == want.end()) want.push_back(reg[i]);
+                std::sort(want.begin(), want.end());
+                int score = simulate(want.data());
+                int keep = 0;
+                for (int i = 0; i < TRB; ++i) keep += std::find(want.begin(), want.end(), reg[i]) != want.end();
+                if (score > best_score || (score == best_score && keep > best_keep)) {
+                    best = want;
+                    best_score = score;
+                    best_keep = keep;
+                }
+            }
+            if (best.empty()) {  // no dense op is ready: any layout works
+                best.assign(reg, reg + TRB);
+                std::sort(best.begin(), best.end());
+            }
+            if (best.size() != TRB || best[0] < 0 || best[TRB - 1] >= TM ||
+                std::adjacent_find(best.begin(), best.end()) != best.end())
+                return false;
+            for (int i = 0; i < TRB; ++i) reg[i] = best[i];
+            return true;
+        };
+        int reg0[TRB] = {8, 9, 10, 11};
+        if (!next_reg(reg0)) return failp("fusion planner produced an invalid register set");
+        // Schedule: segments as (register set, chosen indices in execution order).
+        struct SegPlan {
+            int reg[TRB];
+            std::vector<int> idx;
+            int work;
+        };
+        std::vector<SegPlan> plan;
+        while (nsched < chosen.size()) {
+            SegPlan sp;
+            memcpy(sp.reg, reg0, sizeof(reg0));
+            sp.work = 0;
+            bool progress = true;
+            while (progress) {
+                progress = false;
+                for (size_t ci = 0; ci < chosen.size(); ++ci) {
+                    if (sched[ci] || !fits(ci, reg0) || !ready(ci)) continue;
+                    sched[ci] = 1;
+                    ++nsched;
+                    sp.idx.push_back(int(ci));
+                    sp.work += ops[chosen[ci]].kind == 0 ? 2 : 1;
+                    progress = true;
+                }
+            }
+            if (sp.idx.empty()) return failp("fusion planner made no progress");
+            plan.push_back(sp);
+            if (nsched >= chosen.size()) break;
+            if (!next_reg(reg0)) return failp("fusion planner produced an invalid register set");
+        }
+        // A final segment with little work costs a transposition (and maybe the
+        // one back to the store layout): its ops wait for the next sweep.
+        while (plan.size() > 1 && plan.back().work < min_tail) {
+            for (int ci : plan.back().idx) done[chosen[ci]] = 0;
+            plan.pop_back();
+        }
+        it.args.seg0 = int(fp->h_segs.size());
+        if (!safe_layout(plan.front().reg)) {
+            TSeg s0{{8, 9, 10, 11}, int(fp->h_ops.size()), int(fp->h_ops.size())};
+            fp->h_segs.push_back(s0);
+        }
+        for (const SegPlan &sp : plan) {
+            const int *reg = sp.reg;
+            TSeg sg;
+            for (int i = 0; i < TRB; ++i) sg.reg[i] = reg[i];
+            std::vector<POp> seg_ops;
+            for (int ci : sp.idx) seg_ops.push_back(ops[chosen[ci]]);
             int reg_global[TRB];
             for (int i = 0; i < TRB; ++i) reg_global[i] = it.args.tbits[reg[i]];
             peephole(seg_ops, reg_global, TRB);
@@ -723,14 +789,29 @@ FusedPlan *fused_plan_build_host(const Program *p, std::string *err) {
             }
             sg.op1 = int(fp->h_ops.size());
             fp->h_segs.push_back(sg);
-            if (nsched >= chosen.size()) break;
-            if (!next_reg(reg)) return failp("fusion planner produced an invalid register set");
         }
         if (!safe_layout(fp->h_segs.back().reg)) {
             TSeg se{{8, 9, 10, 11}, int(fp->h_ops.size()), int(fp->h_ops.size())};
             fp->h_segs.push_back(se);
         }
         it.args.seg1 = int(fp->h_segs.size());
+        if (getenv("MOSSQ_PLAN_DEBUG")) {
+            fprintf(stderr, "sweep %d tbits:", fp->nsweeps);
+            for (int q = 0; q < TM; ++q) fprintf(stderr, " %d", it.args.tbits[q]);
+            fprintf(stderr, "\n");
+            for (int q = it.args.seg0; q < it.args.seg1; ++q) {
+                const TSeg &g = fp->h_segs[q];
+                int changed = 0;
+                if (q > it.args.seg0)
+                    for (int i = 0; i < TRB; ++i) {
+                        bool found = false;
+                        for (int k = 0; k < TRB; ++k) found |= fp->h_segs[q - 1].reg[k] == g.reg[i];
+                        changed += !found;
+                    }
+                fprintf(stderr, "  seg reg {%d,%d,%d,%d} ops %d changed %d\n", g.reg[0], g.reg[1], g.reg[2], g.reg[3],
+                        g.op1 - g.op0, changed);
+            }
+        }
         for (int q = it.args.seg0 + 1; q < it.args.seg1; ++q)
             if (memcmp(fp->h_segs[q].reg, fp->h_segs[q - 1].reg, sizeof(int) * TRB) != 0) fp->ntransitions++;
         fp->items.push_back(it);
@@ -738,6 +819,42 @@ FusedPlan *fused_plan_build_host(const Program *p, std::string *err) {
     }
     if (fp->h_pool.empty()) fp->h_pool.push_back(mk(0, 0));
     return fp;
+}
+
+// Modelled cost of a plan, in microseconds per sweep-equivalent at 26 qubits
+// (k_tile timings on MI355X): a sweep's HBM round trip, an LDS transposition,
+// and each tile op by type.
+static double plan_cost(const FusedPlan *fp) {
+    double t = 300.0 * fp->nsweeps + 87.0 * fp->ntransitions;
+    for (const TOp &o : fp->h_ops) t += o.type == T_DENSE2 ? 27.0 : o.type == T_DENSE1 ? 20.0 : 10.0;
+    for (const PlanItem &it : fp->items)
+        if (!it.sweep) t += 300.0;
+    return t;
+}
+
+// The tail threshold trades a transposition against the ops it carries into
+// the next sweep; which side wins depends on the circuit, so each candidate is
+// planned and the cheapest plan under the cost model is kept.
+FusedPlan *fused_plan_build_host(const Program *p, std::string *err) {
+    if (const char *mt = getenv("MOSSQ_PLAN_MIN_TAIL")) return plan_with_tail(p, atoi(mt), err);
+    FusedPlan *best = nullptr;
+    double best_cost = 0;
+    for (int min_tail : {0, 3, 5, 8}) {
+        FusedPlan *fp = plan_with_tail(p, min_tail, err);
+        if (!fp) {
+            delete best;
+            return nullptr;
+        }
+        const double c = plan_cost(fp);
+        if (!best || c < best_cost) {
+            delete best;
+            best = fp;
+            best_cost = c;
+        } else {
+            delete fp;
+        }
+    }
+    return best;
 }
 
 // Host execution of a plan with the kernel's index arithmetic (tiles, layouts,
