@@ -128,6 +128,11 @@ int mossq_ctx_create(int device, uint64_t mem_budget_bytes, mossq_ctx **out) {
     Ctx *c = new Ctx();
     c->device = device;
     c->budget = mem_budget_bytes;
+    e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess || c->num_cu <= 0) {
+        delete c;
+        return fail(MOSSQ_EDEVICE, "hipDeviceGetAttribute(MultiprocessorCount) failed");
+    }
     // Default priority, non-blocking w.r.t. the legacy null stream.
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {

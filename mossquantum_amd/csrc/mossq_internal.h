@@ -83,6 +83,7 @@ constexpr int kNumEvents = 16;
 
 struct Ctx {
     int device = 0;
+    int num_cu = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     u64 budget = 0;  // 0 = unlimited
     u64 in_use = 0;

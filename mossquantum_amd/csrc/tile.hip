@@ -34,27 +34,36 @@ constexpr int TM = 12;   // tile bits
 constexpr int TRB = 4;   // register bits
 constexpr int TT = 256;  // threads per tile
 constexpr int TN = 16;   // amplitudes per thread
+constexpr int TTB = 8;   // thread-index bits (TM - TRB)
 constexpr int kCoalesceBits = 3;
 
 enum TOpType { T_DENSE1 = 0, T_DENSE2 = 1, T_SIGN = 2, T_SELECT = 3, T_GATHER = 4 };
 
-// Diagonal index of register slot j: idx = f | rp(j), where f collects the op
-// bits that are not register bits from the global index g0 of the thread's
-// slot 0 (qbit[i] >= 0), and rp(j) (4 bits per slot, precomputed by the planner
-// for the segment's register layout) collects the register bits.
+// Diagonal index of register slot j: idx = f | rp(j).  f collects the op bits
+// that are not register bits: from the thread's tile-local index (lbit[i] >= 0)
+// or, for bits outside the tile, from the tile's base index (qbit[i] >= 0).
+// rp(j) (4 bits per slot, precomputed by the planner for the segment's register
+// layout) collects the register bits.
 struct TOp {
     int type;
     int a, b;      // dense: register-bit indices (a < b)
     int k;         // diagonal: number of bits
-    int qbit[4];   // diagonal: global bit of op bit i if it is not a register bit, else -1
+    int lbit[4];   // diagonal: tile-local bit of op bit i if it is a thread bit, else -1
+    int qbit[4];   // diagonal: global bit of op bit i if it lies outside the tile, else -1
     unsigned rp_lo, rp_hi;  // diagonal: rp(j) for j < 8 (lo) and j >= 8 (hi), 4 bits each
     int negmask;   // T_SIGN: bit idx set -> negate
     int moff;      // offset in the plan's device pool (matrix / diagonal values)
 };
 
+// A register layout and its instructions.  The layout's index maps are
+// precomputed on the host: thread-index bit k sits at tile-local bit tpos[k]
+// (global bit gpos[k]); register bit i at tile-local bit reg[i] (global gbit[i]).
 struct TSeg {
     int reg[TRB];  // tile-local bits held in registers (increasing)
     int op0, op1;
+    int tpos[TTB];
+    int gpos[TTB];
+    int gbit[TRB];
 };
 
 struct TSweepArgs {
@@ -99,41 +108,42 @@ __device__ __forceinline__ void reg_apply2(c128 *v, const c128 *__restrict__ M) 
     for (int j = 0; j < TN; ++j) v[j] = o[j];
 }
 
-__host__ __device__ __forceinline__ int deposit4(int j, const int *reg) {
-    return (((j >> 0) & 1) << reg[0]) | (((j >> 1) & 1) << reg[1]) | (((j >> 2) & 1) << reg[2]) |
-           (((j >> 3) & 1) << reg[3]);
-}
-
-// Local index contribution of thread t when `reg` are the register bits: the
-// 8 thread bits are the tile bits not in reg, in increasing order.
-__host__ __device__ __forceinline__ int thread_deposit(int t, const int *reg) {
-    int out = 0, k = 0;
+// Tile-local index of thread t's register slot 0 in a layout.
+__host__ __device__ __forceinline__ int seg_tdep(int t, const TSeg &sg) {
+    int d = 0;
 #pragma unroll
-    for (int b = 0; b < TM; ++b) {
-        bool isreg = (b == reg[0]) | (b == reg[1]) | (b == reg[2]) | (b == reg[3]);
-        if (!isreg) {
-            out |= ((t >> k) & 1) << b;
-            ++k;
-        }
-    }
-    return out;
+    for (int k = 0; k < TTB; ++k) d |= ((t >> k) & 1) << sg.tpos[k];
+    return d;
 }
 
-// Global offset of a tile-local index.
-__host__ __device__ __forceinline__ u64 local_to_global(int l, const int *tbits) {
+// Global offset of thread t's register slot 0 within its tile.
+__host__ __device__ __forceinline__ u64 seg_gthread(int t, const TSeg &sg) {
     u64 g = 0;
 #pragma unroll
-    for (int b = 0; b < TM; ++b)
-        if ((l >> b) & 1) g |= u64(1) << tbits[b];
+    for (int k = 0; k < TTB; ++k) g |= u64((t >> k) & 1) << sg.gpos[k];
     return g;
 }
 
-// Non-register part of a diagonal index, from the thread's slot-0 global index.
-__host__ __device__ __forceinline__ int diag_f(const TOp &op, u64 g0) {
+// LDS byte offsets of the 16 register slots.  The padding l + (l >> 4) is
+// additive over disjoint bit sets, so slot j's offset is the thread's offset
+// plus one constant per set bit of j: 15 adds for all 16 slots.
+__host__ __device__ __forceinline__ void lds_slots(int tdep, const int *reg, int *a) {
+    int r[TRB];
+#pragma unroll
+    for (int i = 0; i < TRB; ++i) r[i] = lds_pad(1 << reg[i]) * int(sizeof(c128));
+    a[0] = lds_pad(tdep) * int(sizeof(c128));
+#pragma unroll
+    for (int j = 1; j < TN; ++j) a[j] = a[j & (j - 1)] + r[__builtin_ctz(j)];
+}
+
+// Non-register part of a diagonal index.
+__host__ __device__ __forceinline__ int diag_f(const TOp &op, int tdep, u64 base) {
     int f = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        if (op.qbit[i] >= 0) f |= int((g0 >> op.qbit[i]) & 1) << i;
+    for (int i = 0; i < 4; ++i) {
+        if (op.lbit[i] >= 0) f |= ((tdep >> op.lbit[i]) & 1) << i;
+        if (op.qbit[i] >= 0) f |= int((base >> op.qbit[i]) & 1) << i;
+    }
     return f;
 }
 
@@ -149,42 +159,44 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
                                                   const TSeg *__restrict__ segs, const TOp *__restrict__ ops,
                                                   const c128 *__restrict__ pool) {
     __shared__ c128 lds[(1 << TM) + (1 << TM) / 16];
+    char *ldsb = reinterpret_cast<char *>(lds);
     const int t = threadIdx.x;
     const u64 tile = blockIdx.x;
     u64 base = 0;
     for (int i = 0; i < sw.ncbits; ++i)
         if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
 
+    TSeg sg = segs[sw.seg0];
     int cur[TRB];
-    {
-        const TSeg s0 = segs[sw.seg0];
 #pragma unroll
-        for (int i = 0; i < TRB; ++i) cur[i] = s0.reg[i];
-    }
-    int tdep = thread_deposit(t, cur);
-    u64 g0 = base | local_to_global(tdep, sw.tbits);  // global index of register slot 0
+    for (int i = 0; i < TRB; ++i) cur[i] = sg.reg[i];
+    int tdep = seg_tdep(t, sg);
     c128 v[TN];
     {
-        u64 G[TRB];
+        const u64 g = base | seg_gthread(t, sg);
 #pragma unroll
-        for (int i = 0; i < TRB; ++i) G[i] = u64(1) << sw.tbits[cur[i]];
+        for (int j = 0; j < TN; ++j) {
+            u64 o = 0;
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-            v[j] = state[g0 | ((j & 1) ? G[0] : 0) | ((j & 2) ? G[1] : 0) | ((j & 4) ? G[2] : 0) |
-                         ((j & 8) ? G[3] : 0)];
+            for (int i = 0; i < TRB; ++i)
+                if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
+            v[j] = state[g | o];
+        }
     }
     for (int s = sw.seg0; s < sw.seg1; ++s) {
-        const TSeg sg = segs[s];
+        sg = segs[s];
         if (sg.reg[0] != cur[0] || sg.reg[1] != cur[1] || sg.reg[2] != cur[2] || sg.reg[3] != cur[3]) {
+            int a[TN];
+            lds_slots(tdep, cur, a);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) lds[lds_pad(deposit4(j, cur) | tdep)] = v[j];
+            for (int j = 0; j < TN; ++j) *reinterpret_cast<c128 *>(ldsb + a[j]) = v[j];
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < TRB; ++i) cur[i] = sg.reg[i];
-            tdep = thread_deposit(t, cur);
-            g0 = base | local_to_global(tdep, sw.tbits);
+            tdep = seg_tdep(t, sg);
+            lds_slots(tdep, cur, a);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) v[j] = lds[lds_pad(deposit4(j, cur) | tdep)];
+            for (int j = 0; j < TN; ++j) v[j] = *reinterpret_cast<const c128 *>(ldsb + a[j]);
             __syncthreads();
         }
         for (int o = sg.op0; o < sg.op1; ++o) {
@@ -212,7 +224,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
                     }
                     break;
                 case T_SIGN: {
-                    const int f = diag_f(op, g0);
+                    const int f = diag_f(op, tdep, base);
                     const unsigned neg = unsigned(op.negmask);
 #pragma unroll
                     for (int j = 0; j < TN; ++j) {
@@ -223,7 +235,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
                     break;
                 }
                 case T_SELECT: {
-                    const int f = diag_f(op, g0);
+                    const int f = diag_f(op, tdep, base);
                     const c128 d0 = M[0], d1 = M[1], d2 = M[2], d3 = M[3];
 #pragma unroll
                     for (int j = 0; j < TN; ++j) {
@@ -234,7 +246,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
                     break;
                 }
                 default: {
-                    const int f = diag_f(op, g0);
+                    const int f = diag_f(op, tdep, base);
 #pragma unroll
                     for (int j = 0; j < TN; ++j) v[j] = cmul(M[f | diag_rp(op, j)], v[j]);
                     break;
@@ -243,13 +255,16 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
         }
     }
     {
-        u64 G[TRB];
+        // sg is the last segment: its layout is the current one.
+        const u64 g = base | seg_gthread(t, sg);
 #pragma unroll
-        for (int i = 0; i < TRB; ++i) G[i] = u64(1) << sw.tbits[cur[i]];
+        for (int j = 0; j < TN; ++j) {
+            u64 o = 0;
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-            state[g0 | ((j & 1) ? G[0] : 0) | ((j & 2) ? G[1] : 0) | ((j & 4) ? G[2] : 0) | ((j & 8) ? G[3] : 0)] =
-                v[j];
+            for (int i = 0; i < TRB; ++i)
+                if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
+            state[g | o] = v[j];
+        }
     }
 }
 
@@ -531,6 +546,26 @@ static bool sign_only(const std::vector<c128> &d, int *negmask) {
     return true;
 }
 
+static TSeg make_seg(const int *reg, int op0, int op1, const int *tbits) {
+    TSeg sg{};
+    for (int i = 0; i < TRB; ++i) {
+        sg.reg[i] = reg[i];
+        sg.gbit[i] = tbits[reg[i]];
+    }
+    int k = 0;
+    for (int b = 0; b < TM; ++b) {
+        if (b == reg[0] || b == reg[1] || b == reg[2] || b == reg[3]) continue;
+        sg.tpos[k] = b;
+        sg.gpos[k] = tbits[b];
+        ++k;
+    }
+    sg.op0 = op0;
+    sg.op1 = op1;
+    return sg;
+}
+
+static const int kStoreLayout[TRB] = {8, 9, 10, 11};
+
 static bool safe_layout(const int *reg) {
     for (int i = 0; i < TRB; ++i)
         if (reg[i] < kCoalesceBits) return false;
@@ -718,31 +753,34 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
         }
         it.args.seg0 = int(fp->h_segs.size());
         if (!safe_layout(plan.front().reg)) {
-            TSeg s0{{8, 9, 10, 11}, int(fp->h_ops.size()), int(fp->h_ops.size())};
-            fp->h_segs.push_back(s0);
+            const int n0 = int(fp->h_ops.size());
+            fp->h_segs.push_back(make_seg(kStoreLayout, n0, n0, it.args.tbits));
         }
         for (const SegPlan &sp : plan) {
             const int *reg = sp.reg;
-            TSeg sg;
-            for (int i = 0; i < TRB; ++i) sg.reg[i] = reg[i];
+            const int op0 = int(fp->h_ops.size());
             std::vector<POp> seg_ops;
             for (int ci : sp.idx) seg_ops.push_back(ops[chosen[ci]]);
             int reg_global[TRB];
             for (int i = 0; i < TRB; ++i) reg_global[i] = it.args.tbits[reg[i]];
             peephole(seg_ops, reg_global, TRB);
-            sg.op0 = int(fp->h_ops.size());
             for (const POp &o : seg_ops) {
                 TOp t{};
                 t.moff = int(fp->h_pool.size());
                 if (o.kind == 1) {
                     t.k = o.nb;
                     int ridx[4] = {-1, -1, -1, -1};
-                    for (int q = 0; q < 4; ++q) t.qbit[q] = -1;
+                    for (int q = 0; q < 4; ++q) t.qbit[q] = t.lbit[q] = -1;
                     for (int q = 0; q < o.nb; ++q) {
                         int l = local_of[o.bits[q]];
                         for (int i = 0; i < TRB; ++i)
                             if (reg[i] == l && l >= 0) ridx[q] = i;
-                        if (ridx[q] < 0) t.qbit[q] = o.bits[q];
+                        if (ridx[q] < 0) {
+                            if (l >= 0)
+                                t.lbit[q] = l;
+                            else
+                                t.qbit[q] = o.bits[q];
+                        }
                     }
                     t.rp_lo = t.rp_hi = 0;
                     for (int j = 0; j < TN; ++j) {
@@ -787,12 +825,11 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
                 }
                 fp->h_ops.push_back(t);
             }
-            sg.op1 = int(fp->h_ops.size());
-            fp->h_segs.push_back(sg);
+            fp->h_segs.push_back(make_seg(reg, op0, int(fp->h_ops.size()), it.args.tbits));
         }
         if (!safe_layout(fp->h_segs.back().reg)) {
-            TSeg se{{8, 9, 10, 11}, int(fp->h_ops.size()), int(fp->h_ops.size())};
-            fp->h_segs.push_back(se);
+            const int n1 = int(fp->h_ops.size());
+            fp->h_segs.push_back(make_seg(kStoreLayout, n1, n1, it.args.tbits));
         }
         it.args.seg1 = int(fp->h_segs.size());
         if (getenv("MOSSQ_PLAN_DEBUG")) {
@@ -889,23 +926,33 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
     u64 base = 0;
     for (int i = 0; i < sw.ncbits; ++i)
         if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
-    int cur[TRB];
-    for (int i = 0; i < TRB; ++i) cur[i] = fp->h_segs[sw.seg0].reg[i];
+    // Slot j's register-bit offsets, as the kernel forms them.
+    auto slot_g = [](int j, const TSeg &sg) {
+        u64 o = 0;
+        for (int i = 0; i < TRB; ++i)
+            if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
+        return o;
+    };
+    const TSeg *lay = &fp->h_segs[sw.seg0];
     std::vector<int> tdep(TT);
     for (int t = 0; t < TT; ++t) {
-        tdep[t] = thread_deposit(t, cur);
-        u64 gth = base | local_to_global(tdep[t], sw.tbits);
-        for (int j = 0; j < TN; ++j) V[t * TN + j] = state[gth | local_to_global(deposit4(j, cur), sw.tbits)];
+        tdep[t] = seg_tdep(t, *lay);
+        const u64 gth = base | seg_gthread(t, *lay);
+        for (int j = 0; j < TN; ++j) V[t * TN + j] = state[gth | slot_g(j, *lay)];
     }
     for (int s = sw.seg0; s < sw.seg1; ++s) {
         const TSeg &sg = fp->h_segs[s];
-        if (memcmp(sg.reg, cur, sizeof(cur)) != 0) {
-            for (int t = 0; t < TT; ++t)
-                for (int j = 0; j < TN; ++j) lds[lds_pad(deposit4(j, cur) | tdep[t])] = V[t * TN + j];
-            memcpy(cur, sg.reg, sizeof(cur));
+        if (memcmp(sg.reg, lay->reg, sizeof(sg.reg)) != 0) {
+            int a[TN];
             for (int t = 0; t < TT; ++t) {
-                tdep[t] = thread_deposit(t, cur);
-                for (int j = 0; j < TN; ++j) V[t * TN + j] = lds[lds_pad(deposit4(j, cur) | tdep[t])];
+                lds_slots(tdep[t], lay->reg, a);
+                for (int j = 0; j < TN; ++j) lds[a[j] / sizeof(c128)] = V[t * TN + j];
+            }
+            lay = &sg;
+            for (int t = 0; t < TT; ++t) {
+                tdep[t] = seg_tdep(t, sg);
+                lds_slots(tdep[t], sg.reg, a);
+                for (int j = 0; j < TN; ++j) V[t * TN + j] = lds[a[j] / sizeof(c128)];
             }
         }
         for (int o = sg.op0; o < sg.op1; ++o) {
@@ -932,9 +979,8 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
                         }
                     }
                 } else {
-                    const u64 g0 = base | local_to_global(tdep[t], sw.tbits);
                     for (int j = 0; j < TN; ++j) {
-                        int idx = diag_f(op, g0) | diag_rp(op, j);
+                        int idx = diag_f(op, tdep[t], base) | diag_rp(op, j);
                         if (op.type == T_SIGN) {
                             if ((op.negmask >> idx) & 1) v[j] = mk(-v[j].re, -v[j].im);
                         } else {
@@ -945,9 +991,10 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
             }
         }
     }
+    const TSeg &last = fp->h_segs[sw.seg1 - 1];
     for (int t = 0; t < TT; ++t) {
-        u64 gth = base | local_to_global(tdep[t], sw.tbits);
-        for (int j = 0; j < TN; ++j) state[gth | local_to_global(deposit4(j, cur), sw.tbits)] = V[t * TN + j];
+        const u64 gth = base | seg_gthread(t, last);
+        for (int j = 0; j < TN; ++j) state[gth | slot_g(j, last)] = V[t * TN + j];
     }
 }
 
