@@ -55,6 +55,28 @@ struct TOp {
     int moff;      // offset in the plan's device pool (matrix / diagonal values)
 };
 
+// Device form of an op: one header per op, so the kernel reads the next
+// header while the current op runs and a dense op needs a single scalar-load
+// round trip (its matrix).  code: 0..3 dense 2x2 on register bit `code`;
+// 4..9 dense 4x4 on a register-bit pair (kPairA/kPairB); 10 sign-only diagonal,
+// 11 diagonal on <= 2 bits, 12 wider diagonal.  arg: matrix offset in the pool
+// (dense) or index of the op's TDiag (diagonal).
+struct TOpHdr {
+    int code, arg;
+};
+enum : int { C_DENSE1 = 0, C_DENSE2 = 4, C_SIGN = 10, C_SELECT = 11, C_GATHER = 12 };
+constexpr int kPairA[6] = {0, 0, 0, 1, 1, 2};
+constexpr int kPairB[6] = {1, 2, 3, 2, 3, 3};
+
+struct TDiag {
+    int lbit[4];
+    int qbit[4];
+    unsigned rp_lo, rp_hi;
+    int negmask;
+    int moff;     // C_GATHER: offset of the 2^k values in the pool
+    c128 val[4];  // C_SELECT: the four values
+};
+
 // A register layout and its instructions.  The layout's index maps are
 // precomputed on the host: thread-index bit k sits at tile-local bit tpos[k]
 // (global bit gpos[k]); register bit i at tile-local bit reg[i] (global gbit[i]).
@@ -137,7 +159,8 @@ __host__ __device__ __forceinline__ void lds_slots(int tdep, const int *reg, int
 }
 
 // Non-register part of a diagonal index.
-__host__ __device__ __forceinline__ int diag_f(const TOp &op, int tdep, u64 base) {
+template <class D>
+__host__ __device__ __forceinline__ int diag_f(const D &op, int tdep, u64 base) {
     int f = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -147,7 +170,8 @@ __host__ __device__ __forceinline__ int diag_f(const TOp &op, int tdep, u64 base
     return f;
 }
 
-__host__ __device__ __forceinline__ int diag_rp(const TOp &op, int j) {
+template <class D>
+__host__ __device__ __forceinline__ int diag_rp(const D &op, int j) {
     return int(((j < 8 ? op.rp_lo : op.rp_hi) >> (4 * (j & 7))) & 15u);
 }
 
@@ -156,8 +180,8 @@ __device__ __forceinline__ double flip_sign(double x, unsigned flip) {
 }
 
 __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweepArgs sw,
-                                                  const TSeg *__restrict__ segs, const TOp *__restrict__ ops,
-                                                  const c128 *__restrict__ pool) {
+                                                  const TSeg *__restrict__ segs, const TOpHdr *__restrict__ hdr,
+                                                  const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
     __shared__ c128 lds[(1 << TM) + (1 << TM) / 16];
     char *ldsb = reinterpret_cast<char *>(lds);
     const int t = threadIdx.x;
@@ -199,56 +223,52 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
             for (int j = 0; j < TN; ++j) v[j] = *reinterpret_cast<const c128 *>(ldsb + a[j]);
             __syncthreads();
         }
+        TOpHdr h = hdr[sg.op0];  // the header array has one spare entry past the last op
         for (int o = sg.op0; o < sg.op1; ++o) {
-            const TOp op = ops[o];
-            const c128 *M = pool + op.moff;
-            switch (op.type) {
-                case T_DENSE1: {
-                    c128 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
-                    switch (op.a) {
-                        case 0: reg_apply1<0>(v, m00, m10, m01, m11); break;
-                        case 1: reg_apply1<1>(v, m00, m10, m01, m11); break;
-                        case 2: reg_apply1<2>(v, m00, m10, m01, m11); break;
-                        default: reg_apply1<3>(v, m00, m10, m01, m11); break;
-                    }
-                    break;
-                }
-                case T_DENSE2:
-                    switch (op.a * 4 + op.b) {
-                        case 1: reg_apply2<0, 1>(v, M); break;
-                        case 2: reg_apply2<0, 2>(v, M); break;
-                        case 3: reg_apply2<0, 3>(v, M); break;
-                        case 6: reg_apply2<1, 2>(v, M); break;
-                        case 7: reg_apply2<1, 3>(v, M); break;
-                        default: reg_apply2<2, 3>(v, M); break;
-                    }
-                    break;
-                case T_SIGN: {
-                    const int f = diag_f(op, tdep, base);
-                    const unsigned neg = unsigned(op.negmask);
+            const TOpHdr cur_h = h;
+            h = hdr[o + 1];
+            const c128 *M = pool + cur_h.arg;
+            switch (cur_h.code) {
+                case C_DENSE1 + 0: reg_apply1<0>(v, M[0], M[1], M[2], M[3]); break;
+                case C_DENSE1 + 1: reg_apply1<1>(v, M[0], M[1], M[2], M[3]); break;
+                case C_DENSE1 + 2: reg_apply1<2>(v, M[0], M[1], M[2], M[3]); break;
+                case C_DENSE1 + 3: reg_apply1<3>(v, M[0], M[1], M[2], M[3]); break;
+                case C_DENSE2 + 0: reg_apply2<0, 1>(v, M); break;
+                case C_DENSE2 + 1: reg_apply2<0, 2>(v, M); break;
+                case C_DENSE2 + 2: reg_apply2<0, 3>(v, M); break;
+                case C_DENSE2 + 3: reg_apply2<1, 2>(v, M); break;
+                case C_DENSE2 + 4: reg_apply2<1, 3>(v, M); break;
+                case C_DENSE2 + 5: reg_apply2<2, 3>(v, M); break;
+                case C_SIGN: {
+                    const TDiag &d = diag[cur_h.arg];
+                    const int f = diag_f(d, tdep, base);
+                    const unsigned neg = unsigned(d.negmask);
 #pragma unroll
                     for (int j = 0; j < TN; ++j) {
-                        const unsigned flip = (neg >> (f | diag_rp(op, j))) & 1u;
+                        const unsigned flip = (neg >> (f | diag_rp(d, j))) & 1u;
                         v[j].re = flip_sign(v[j].re, flip);
                         v[j].im = flip_sign(v[j].im, flip);
                     }
                     break;
                 }
-                case T_SELECT: {
-                    const int f = diag_f(op, tdep, base);
-                    const c128 d0 = M[0], d1 = M[1], d2 = M[2], d3 = M[3];
+                case C_SELECT: {
+                    const TDiag &d = diag[cur_h.arg];
+                    const int f = diag_f(d, tdep, base);
+                    const c128 d0 = d.val[0], d1 = d.val[1], d2 = d.val[2], d3 = d.val[3];
 #pragma unroll
                     for (int j = 0; j < TN; ++j) {
-                        const int idx = f | diag_rp(op, j);
+                        const int idx = f | diag_rp(d, j);
                         const c128 lo = (idx & 1) ? d1 : d0, hi = (idx & 1) ? d3 : d2;
                         v[j] = cmul((idx & 2) ? hi : lo, v[j]);
                     }
                     break;
                 }
                 default: {
-                    const int f = diag_f(op, tdep, base);
+                    const TDiag &d = diag[cur_h.arg];
+                    const int f = diag_f(d, tdep, base);
+                    const c128 *D = pool + d.moff;
 #pragma unroll
-                    for (int j = 0; j < TN; ++j) v[j] = cmul(M[f | diag_rp(op, j)], v[j]);
+                    for (int j = 0; j < TN; ++j) v[j] = cmul(D[f | diag_rp(d, j)], v[j]);
                     break;
                 }
             }
@@ -293,8 +313,11 @@ struct FusedPlan {
     std::vector<TSeg> h_segs;
     std::vector<TOp> h_ops;
     std::vector<c128> h_pool;
+    std::vector<TOpHdr> h_hdr;  // device encoding of h_ops (+1 spare entry)
+    std::vector<TDiag> h_diag;
     TSeg *d_segs = nullptr;
-    TOp *d_ops = nullptr;
+    TOpHdr *d_hdr = nullptr;
+    TDiag *d_diag = nullptr;
     c128 *d_pool = nullptr;
     int nsweeps = 0;
     int ntransitions = 0;
@@ -571,6 +594,8 @@ static bool safe_layout(const int *reg) {
         if (reg[i] < kCoalesceBits) return false;
     return true;
 }
+
+static int encode_ops(FusedPlan *fp, std::string *err);
 
 static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *err) {
     std::vector<POp> ops;
@@ -855,7 +880,53 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
         fp->nsweeps++;
     }
     if (fp->h_pool.empty()) fp->h_pool.push_back(mk(0, 0));
+    if (encode_ops(fp, err) != MOSSQ_OK) {
+        delete fp;
+        return nullptr;
+    }
     return fp;
+}
+
+// Device encoding of the planned ops (headers and diagonal records).
+static int encode_ops(FusedPlan *fp, std::string *err) {
+    fp->h_hdr.clear();
+    fp->h_diag.clear();
+    for (const TOp &o : fp->h_ops) {
+        TOpHdr h{};
+        if (o.type == T_DENSE1) {
+            h.code = C_DENSE1 + o.a;
+            h.arg = o.moff;
+        } else if (o.type == T_DENSE2) {
+            int pi = -1;
+            for (int q = 0; q < 6; ++q)
+                if (kPairA[q] == o.a && kPairB[q] == o.b) pi = q;
+            if (pi < 0) {
+                *err = "dense op on an invalid register pair";
+                return MOSSQ_EINVAL;
+            }
+            h.code = C_DENSE2 + pi;
+            h.arg = o.moff;
+        } else {
+            TDiag d{};
+            for (int q = 0; q < 4; ++q) {
+                d.lbit[q] = o.lbit[q];
+                d.qbit[q] = o.qbit[q];
+            }
+            d.rp_lo = o.rp_lo;
+            d.rp_hi = o.rp_hi;
+            d.negmask = o.negmask;
+            d.moff = o.moff;
+            if (o.type == T_SELECT)
+                for (int q = 0; q < 4; ++q) d.val[q] = fp->h_pool[o.moff + q];
+            h.code = o.type == T_SIGN ? C_SIGN : o.type == T_SELECT ? C_SELECT : C_GATHER;
+            h.arg = int(fp->h_diag.size());
+            fp->h_diag.push_back(d);
+        }
+        fp->h_hdr.push_back(h);
+    }
+    fp->h_hdr.push_back(TOpHdr{C_SIGN, 0});  // spare: read ahead past the last op, never executed
+    if (fp->h_diag.empty()) fp->h_diag.push_back(TDiag{});
+    return MOSSQ_OK;
 }
 
 // Modelled cost of a plan, in microseconds per sweep-equivalent at 26 qubits
@@ -955,22 +1026,25 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
                 for (int j = 0; j < TN; ++j) V[t * TN + j] = lds[a[j] / sizeof(c128)];
             }
         }
+        // Executes the device encoding (headers, diagonal records), as the kernel does.
         for (int o = sg.op0; o < sg.op1; ++o) {
-            const TOp &op = fp->h_ops[o];
-            const c128 *M = fp->h_pool.data() + op.moff;
+            const TOpHdr &h = fp->h_hdr[o];
+            const c128 *M = fp->h_pool.data() + h.arg;
             for (int t = 0; t < TT; ++t) {
                 c128 *v = &V[t * TN];
-                if (op.type == T_DENSE1) {
+                if (h.code < C_DENSE2) {
+                    const int a = h.code - C_DENSE1;
                     for (int j = 0; j < TN; ++j) {
-                        if (j & (1 << op.a)) continue;
-                        c128 x = v[j], y = v[j | (1 << op.a)];
+                        if (j & (1 << a)) continue;
+                        c128 x = v[j], y = v[j | (1 << a)];
                         v[j] = cfma(M[2], y, cmul(M[0], x));
-                        v[j | (1 << op.a)] = cfma(M[3], y, cmul(M[1], x));
+                        v[j | (1 << a)] = cfma(M[3], y, cmul(M[1], x));
                     }
-                } else if (op.type == T_DENSE2) {
+                } else if (h.code < C_SIGN) {
+                    const int a = kPairA[h.code - C_DENSE2], b = kPairB[h.code - C_DENSE2];
                     for (int j = 0; j < TN; ++j) {
-                        if (j & ((1 << op.a) | (1 << op.b))) continue;
-                        int ii[4] = {j, j | (1 << op.a), j | (1 << op.b), j | (1 << op.a) | (1 << op.b)};
+                        if (j & ((1 << a) | (1 << b))) continue;
+                        int ii[4] = {j, j | (1 << a), j | (1 << b), j | (1 << a) | (1 << b)};
                         c128 in[4] = {v[ii[0]], v[ii[1]], v[ii[2]], v[ii[3]]};
                         for (int r = 0; r < 4; ++r) {
                             c128 acc = cmul(M[r], in[0]);
@@ -979,12 +1053,15 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
                         }
                     }
                 } else {
+                    const TDiag &d = fp->h_diag[h.arg];
                     for (int j = 0; j < TN; ++j) {
-                        int idx = diag_f(op, tdep[t], base) | diag_rp(op, j);
-                        if (op.type == T_SIGN) {
-                            if ((op.negmask >> idx) & 1) v[j] = mk(-v[j].re, -v[j].im);
+                        int idx = diag_f(d, tdep[t], base) | diag_rp(d, j);
+                        if (h.code == C_SIGN) {
+                            if ((d.negmask >> idx) & 1) v[j] = mk(-v[j].re, -v[j].im);
+                        } else if (h.code == C_SELECT) {
+                            v[j] = cmul(d.val[idx], v[j]);
                         } else {
-                            v[j] = cmul(M[idx], v[j]);
+                            v[j] = cmul(fp->h_pool[d.moff + idx], v[j]);
                         }
                     }
                 }
@@ -1018,15 +1095,17 @@ void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state) {
 
 int fused_plan_upload(Ctx *c, FusedPlan *fp) {
     int s = dev_alloc(c, std::max<size_t>(fp->h_segs.size(), 1) * sizeof(TSeg), (void **)&fp->d_segs);
-    if (s == MOSSQ_OK) s = dev_alloc(c, std::max<size_t>(fp->h_ops.size(), 1) * sizeof(TOp), (void **)&fp->d_ops);
+    if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_hdr.size() * sizeof(TOpHdr), (void **)&fp->d_hdr);
+    if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_diag.size() * sizeof(TDiag), (void **)&fp->d_diag);
     if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_pool.size() * sizeof(c128), (void **)&fp->d_pool);
     if (s != MOSSQ_OK) return s;
     if (!fp->h_segs.empty())
         MOSSQ_HIP(hipMemcpyAsync(fp->d_segs, fp->h_segs.data(), fp->h_segs.size() * sizeof(TSeg),
                                  hipMemcpyHostToDevice, c->stream));
-    if (!fp->h_ops.empty())
-        MOSSQ_HIP(hipMemcpyAsync(fp->d_ops, fp->h_ops.data(), fp->h_ops.size() * sizeof(TOp), hipMemcpyHostToDevice,
-                                 c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(fp->d_hdr, fp->h_hdr.data(), fp->h_hdr.size() * sizeof(TOpHdr), hipMemcpyHostToDevice,
+                             c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(fp->d_diag, fp->h_diag.data(), fp->h_diag.size() * sizeof(TDiag),
+                             hipMemcpyHostToDevice, c->stream));
     MOSSQ_HIP(hipMemcpyAsync(fp->d_pool, fp->h_pool.data(), fp->h_pool.size() * sizeof(c128), hipMemcpyHostToDevice,
                              c->stream));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));
@@ -1052,7 +1131,8 @@ void fused_plan_free(Program *p) {
     FusedPlan *fp = p->fused;
     if (p->ctx) {
         if (fp->d_segs) dev_free(p->ctx, fp->d_segs);
-        if (fp->d_ops) dev_free(p->ctx, fp->d_ops);
+        if (fp->d_hdr) dev_free(p->ctx, fp->d_hdr);
+        if (fp->d_diag) dev_free(p->ctx, fp->d_diag);
         if (fp->d_pool) dev_free(p->ctx, fp->d_pool);
     }
     delete fp;
@@ -1076,7 +1156,7 @@ int fused_run(Ctx *c, const Program *p, c128 *state) {
             MOSSQ_TRY(run_steps(c, p, state, it.step, it.step + 1));
             continue;
         }
-        hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(TT), 0, c->stream, state, it.args, fp->d_segs, fp->d_ops,
+        hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(TT), 0, c->stream, state, it.args, fp->d_segs, fp->d_hdr, fp->d_diag,
                            fp->d_pool);
         MOSSQ_HIP(hipGetLastError());
     }
