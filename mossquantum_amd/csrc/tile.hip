@@ -80,12 +80,23 @@ struct TDiag {
 // A register layout and its instructions.  The layout's index maps are
 // precomputed on the host: thread-index bit k sits at tile-local bit tpos[k]
 // (global bit gpos[k]); register bit i at tile-local bit reg[i] (global gbit[i]).
+//
+// Entering a segment whose layout differs from the previous one transposes
+// the tile through LDS.  The LDS slot of tile-local index l is the XOR of one
+// column per set bit of l; the columns are chosen per transposition (unit
+// lower-triangular, hence a bijection) so that the first 8 and the first 16
+// lanes of a wave address distinct 16-byte bank groups on both the write (old
+// layout) and the read (new layout) side.  wt/wr and rt/rr are those columns
+// at the old and new layouts' thread and register bits.
 struct TSeg {
     int reg[TRB];  // tile-local bits held in registers (increasing)
     int op0, op1;
     int tpos[TTB];
     int gpos[TTB];
     int gbit[TRB];
+    int transpose;
+    int wt[TTB], wr[TRB];
+    int rt[TTB], rr[TRB];
 };
 
 struct TSweepArgs {
@@ -95,7 +106,6 @@ struct TSweepArgs {
     int seg0, seg1;
 };
 
-__host__ __device__ __forceinline__ int lds_pad(int l) { return l + (l >> 4); }
 
 template <int A>
 __device__ __forceinline__ void reg_apply1(c128 *v, c128 m00, c128 m10, c128 m01, c128 m11) {
@@ -146,16 +156,16 @@ __host__ __device__ __forceinline__ u64 seg_gthread(int t, const TSeg &sg) {
     return g;
 }
 
-// LDS byte offsets of the 16 register slots.  The padding l + (l >> 4) is
-// additive over disjoint bit sets, so slot j's offset is the thread's offset
-// plus one constant per set bit of j: 15 adds for all 16 slots.
-__host__ __device__ __forceinline__ void lds_slots(int tdep, const int *reg, int *a) {
-    int r[TRB];
+// LDS byte offsets of thread t's 16 register slots: the XOR of the columns of
+// the set thread bits, then one column per set bit of j (15 XORs for all 16).
+__host__ __device__ __forceinline__ void lds_slots(int t, const int *tcol, const int *rcol, int *a) {
+    int x = 0;
 #pragma unroll
-    for (int i = 0; i < TRB; ++i) r[i] = lds_pad(1 << reg[i]) * int(sizeof(c128));
-    a[0] = lds_pad(tdep) * int(sizeof(c128));
+    for (int k = 0; k < TTB; ++k)
+        if ((t >> k) & 1) x ^= tcol[k];
+    a[0] = x * int(sizeof(c128));
 #pragma unroll
-    for (int j = 1; j < TN; ++j) a[j] = a[j & (j - 1)] + r[__builtin_ctz(j)];
+    for (int j = 1; j < TN; ++j) a[j] = a[j & (j - 1)] ^ (rcol[__builtin_ctz(j)] * int(sizeof(c128)));
 }
 
 // Non-register part of a diagonal index.
@@ -182,7 +192,7 @@ __device__ __forceinline__ double flip_sign(double x, unsigned flip) {
 __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweepArgs sw,
                                                   const TSeg *__restrict__ segs, const TOpHdr *__restrict__ hdr,
                                                   const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
-    __shared__ c128 lds[(1 << TM) + (1 << TM) / 16];
+    __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
     const int t = threadIdx.x;
     const u64 tile = blockIdx.x;
@@ -191,9 +201,6 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
         if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
 
     TSeg sg = segs[sw.seg0];
-    int cur[TRB];
-#pragma unroll
-    for (int i = 0; i < TRB; ++i) cur[i] = sg.reg[i];
     int tdep = seg_tdep(t, sg);
     c128 v[TN];
     {
@@ -209,16 +216,14 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
     }
     for (int s = sw.seg0; s < sw.seg1; ++s) {
         sg = segs[s];
-        if (sg.reg[0] != cur[0] || sg.reg[1] != cur[1] || sg.reg[2] != cur[2] || sg.reg[3] != cur[3]) {
+        if (sg.transpose) {
             int a[TN];
-            lds_slots(tdep, cur, a);
+            lds_slots(t, sg.wt, sg.wr, a);
 #pragma unroll
             for (int j = 0; j < TN; ++j) *reinterpret_cast<c128 *>(ldsb + a[j]) = v[j];
             __syncthreads();
-#pragma unroll
-            for (int i = 0; i < TRB; ++i) cur[i] = sg.reg[i];
             tdep = seg_tdep(t, sg);
-            lds_slots(tdep, cur, a);
+            lds_slots(t, sg.rt, sg.rr, a);
 #pragma unroll
             for (int j = 0; j < TN; ++j) v[j] = *reinterpret_cast<const c128 *>(ldsb + a[j]);
             __syncthreads();
@@ -321,6 +326,7 @@ struct FusedPlan {
     c128 *d_pool = nullptr;
     int nsweeps = 0;
     int ntransitions = 0;
+    int nconflicted = 0;  // transitions without a conflict-free LDS slot map
 };
 
 static bool is_diag(const MatInfo &info) {
@@ -588,6 +594,78 @@ static TSeg make_seg(const int *reg, int op0, int op1, const int *tbits) {
 }
 
 static const int kStoreLayout[TRB] = {8, 9, 10, 11};
+
+// Rank of up to 4 vectors over GF(2).
+static int gf2_rank(int *v, int n) {
+    int r = 0;
+    for (int bit = 3; bit >= 0 && r < n; --bit) {
+        int piv = -1;
+        for (int i = r; i < n; ++i)
+            if ((v[i] >> bit) & 1) piv = i;
+        if (piv < 0) continue;
+        std::swap(v[r], v[piv]);
+        for (int i = 0; i < n; ++i)
+            if (i != r && ((v[i] >> bit) & 1)) v[i] ^= v[r];
+        ++r;
+    }
+    return r;
+}
+
+// Bank-group constraints of one side of a transposition: the slots of lanes
+// 0..7 must differ in their low 3 bits (8 lanes x 16 B cover 32 banks), those
+// of lanes 0..15 in their low 4 bits.  Columns not yet chosen (col < 0) are
+// skipped, so the check also prunes partial assignments.
+static bool lanes_ok(const TSeg &L, const int *col) {
+    int v4[4], n4 = 0, v3[3], n3 = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int c = col[L.tpos[k]];
+        if (c < 0) continue;
+        v4[n4++] = c & 15;
+        if (k < 3) v3[n3++] = c & 7;
+    }
+    return gf2_rank(v4, n4) == n4 && gf2_rank(v3, n3) == n3;
+}
+
+// Columns col[b] = (1 << b) ^ x_b with x_b < 2^min(b, 4): a unit
+// lower-triangular map of the tile-local index, so slots stay a bijection.
+static bool swizzle_search(const TSeg &A, const TSeg &B, const std::vector<int> &vars, size_t i, int *col) {
+    if (i == vars.size()) return true;
+    const int b = vars[i];
+    const int nx = 1 << std::min(b, 4);
+    for (int x = 0; x < nx; ++x) {
+        col[b] = (1 << b) ^ x;
+        if (lanes_ok(A, col) && lanes_ok(B, col) && swizzle_search(A, B, vars, i + 1, col)) return true;
+    }
+    col[b] = -1;
+    return false;
+}
+
+// Returns false when no conflict-free columns exist (the identity map is used).
+static bool set_transpose(const TSeg &prev, TSeg &next) {
+    int col[TM];
+    std::vector<int> vars;
+    for (int b = 0; b < TM; ++b) col[b] = 1 << b;
+    for (const TSeg *L : {&prev, static_cast<const TSeg *>(&next)})
+        for (int k = 0; k < 4; ++k) {
+            const int b = L->tpos[k];
+            if (b >= 3 && std::find(vars.begin(), vars.end(), b) == vars.end()) vars.push_back(b);
+        }
+    std::sort(vars.begin(), vars.end());
+    for (int b : vars) col[b] = -1;
+    const bool found = swizzle_search(prev, next, vars, 0, col);
+    if (!found)
+        for (int b = 0; b < TM; ++b) col[b] = 1 << b;  // plain layout (bank conflicts, still correct)
+    next.transpose = 1;
+    for (int k = 0; k < TTB; ++k) {
+        next.wt[k] = col[prev.tpos[k]];
+        next.rt[k] = col[next.tpos[k]];
+    }
+    for (int i = 0; i < TRB; ++i) {
+        next.wr[i] = col[prev.reg[i]];
+        next.rr[i] = col[next.reg[i]];
+    }
+    return found;
+}
 
 static bool safe_layout(const int *reg) {
     for (int i = 0; i < TRB; ++i)
@@ -875,7 +953,10 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
             }
         }
         for (int q = it.args.seg0 + 1; q < it.args.seg1; ++q)
-            if (memcmp(fp->h_segs[q].reg, fp->h_segs[q - 1].reg, sizeof(int) * TRB) != 0) fp->ntransitions++;
+            if (memcmp(fp->h_segs[q].reg, fp->h_segs[q - 1].reg, sizeof(int) * TRB) != 0) {
+                if (!set_transpose(fp->h_segs[q - 1], fp->h_segs[q])) fp->nconflicted++;
+                fp->ntransitions++;
+            }
         fp->items.push_back(it);
         fp->nsweeps++;
     }
@@ -992,7 +1073,7 @@ static void host_block(c128 *v, int nbits, int k, const int *bits, const c128 *m
 }
 
 static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp, u64 tile) {
-    std::vector<c128> lds((1 << TM) + (1 << TM) / 16);
+    std::vector<c128> lds(1 << TM);
     std::vector<c128> V(size_t(TT) * TN);
     u64 base = 0;
     for (int i = 0; i < sw.ncbits; ++i)
@@ -1013,16 +1094,21 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
     }
     for (int s = sw.seg0; s < sw.seg1; ++s) {
         const TSeg &sg = fp->h_segs[s];
-        if (memcmp(sg.reg, lay->reg, sizeof(sg.reg)) != 0) {
+        if (sg.transpose) {
             int a[TN];
+            std::vector<char> written(lds.size(), 0);
             for (int t = 0; t < TT; ++t) {
-                lds_slots(tdep[t], lay->reg, a);
-                for (int j = 0; j < TN; ++j) lds[a[j] / sizeof(c128)] = V[t * TN + j];
+                lds_slots(t, sg.wt, sg.wr, a);
+                for (int j = 0; j < TN; ++j) {
+                    lds[a[j] / sizeof(c128)] = V[t * TN + j];
+                    written[a[j] / sizeof(c128)] = 1;
+                }
             }
-            lay = &sg;
+            if (std::count(written.begin(), written.end(), 1) != int(lds.size()))
+                fprintf(stderr, "mossq: LDS slot map is not a bijection\n");
             for (int t = 0; t < TT; ++t) {
                 tdep[t] = seg_tdep(t, sg);
-                lds_slots(tdep[t], sg.reg, a);
+                lds_slots(t, sg.rt, sg.rr, a);
                 for (int j = 0; j < TN; ++j) V[t * TN + j] = lds[a[j] / sizeof(c128)];
             }
         }
@@ -1189,6 +1275,7 @@ int mossq_plan_emulate(int n, int is_density, int nops, const mossq_op *ops, con
     stats[4] = int(fp->h_ops.size());
     stats[5] = stats[6] = stats[7] = 0;
     for (const TOp &o : fp->h_ops) stats[o.type == T_DENSE1 ? 5 : o.type == T_DENSE2 ? 6 : 7]++;
+    stats[8] = fp->nconflicted;
     delete fp;
     return MOSSQ_OK;
 }

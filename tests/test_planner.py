@@ -34,6 +34,7 @@ def test_random_circuit_plans(n, depth):
     st = check(c, n, False, rand_state(n, n))
     assert st["sweeps"] < len(list(c.flat()))
     assert st["unfused"] == 0
+    assert st["conflicted_transitions"] == 0  # every transposition got a bank-conflict-free slot map
 
 
 def test_depth20_plan_uses_few_sweeps():
