@@ -107,21 +107,22 @@ struct TSweepArgs {
 };
 
 
+// Dense ops are out of place (in -> out): the op loop alternates between two
+// register sets, so no op ends with copies back into the loop-carried set.
 template <int A>
-__device__ __forceinline__ void reg_apply1(c128 *v, c128 m00, c128 m10, c128 m01, c128 m11) {
+__device__ __forceinline__ void reg_apply1(const c128 *in, c128 *out, const c128 *__restrict__ M) {
+    const c128 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
         if (j & (1 << A)) continue;
-        c128 x = v[j], y = v[j | (1 << A)];
-        v[j] = cfma(m01, y, cmul(m00, x));
-        v[j | (1 << A)] = cfma(m11, y, cmul(m10, x));
+        const c128 x = in[j], y = in[j | (1 << A)];
+        out[j] = cfma(m01, y, cmul(m00, x));
+        out[j | (1 << A)] = cfma(m11, y, cmul(m10, x));
     }
 }
 
-// Row by row, so only one matrix row (4 entries) is live in scalar registers.
 template <int A, int B>
-__device__ __forceinline__ void reg_apply2(c128 *v, const c128 *__restrict__ M) {
-    c128 o[TN];
+__device__ __forceinline__ void reg_apply2(const c128 *in, c128 *out, const c128 *__restrict__ M) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const c128 m0 = M[r], m1 = M[r + 4], m2 = M[r + 8], m3 = M[r + 12];
@@ -129,15 +130,27 @@ __device__ __forceinline__ void reg_apply2(c128 *v, const c128 *__restrict__ M) 
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
             if (j & ((1 << A) | (1 << B))) continue;
-            c128 acc = cmul(m0, v[j]);
-            acc = cfma(m1, v[j | (1 << A)], acc);
-            acc = cfma(m2, v[j | (1 << B)], acc);
-            acc = cfma(m3, v[j | (1 << A) | (1 << B)], acc);
-            o[j | orow] = acc;
+            c128 acc = cmul(m0, in[j]);
+            acc = cfma(m1, in[j | (1 << A)], acc);
+            acc = cfma(m2, in[j | (1 << B)], acc);
+            out[j | orow] = cfma(m3, in[j | (1 << A) | (1 << B)], acc);
         }
     }
-#pragma unroll
-    for (int j = 0; j < TN; ++j) v[j] = o[j];
+}
+
+__device__ __forceinline__ void dense_op(int code, const c128 *in, c128 *out, const c128 *__restrict__ M) {
+    switch (code) {
+        case C_DENSE1 + 0: reg_apply1<0>(in, out, M); break;
+        case C_DENSE1 + 1: reg_apply1<1>(in, out, M); break;
+        case C_DENSE1 + 2: reg_apply1<2>(in, out, M); break;
+        case C_DENSE1 + 3: reg_apply1<3>(in, out, M); break;
+        case C_DENSE2 + 0: reg_apply2<0, 1>(in, out, M); break;
+        case C_DENSE2 + 1: reg_apply2<0, 2>(in, out, M); break;
+        case C_DENSE2 + 2: reg_apply2<0, 3>(in, out, M); break;
+        case C_DENSE2 + 3: reg_apply2<1, 2>(in, out, M); break;
+        case C_DENSE2 + 4: reg_apply2<1, 3>(in, out, M); break;
+        default: reg_apply2<2, 3>(in, out, M); break;
+    }
 }
 
 // Tile-local index of thread t's register slot 0 in a layout.
@@ -189,6 +202,38 @@ __device__ __forceinline__ double flip_sign(double x, unsigned flip) {
     return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(flip) << 63));
 }
 
+// One op, out of place (in -> out).
+__device__ __forceinline__ void tile_op(const TOpHdr h, const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
+                                        const c128 *in, c128 *out, int tdep, u64 base) {
+    if (h.code < C_SIGN) {
+        dense_op(h.code, in, out, pool + h.arg);
+        return;
+    }
+    const TDiag &d = diag[h.arg];
+    const int f = diag_f(d, tdep, base);
+    if (h.code == C_SIGN) {
+        const unsigned neg = unsigned(d.negmask);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const unsigned flip = (neg >> (f | diag_rp(d, j))) & 1u;
+            out[j].re = flip_sign(in[j].re, flip);
+            out[j].im = flip_sign(in[j].im, flip);
+        }
+    } else if (h.code == C_SELECT) {
+        const c128 d0 = d.val[0], d1 = d.val[1], d2 = d.val[2], d3 = d.val[3];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int idx = f | diag_rp(d, j);
+            const c128 lo = (idx & 1) ? d1 : d0, hi = (idx & 1) ? d3 : d2;
+            out[j] = cmul((idx & 2) ? hi : lo, in[j]);
+        }
+    } else {
+        const c128 *D = pool + d.moff;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) out[j] = cmul(D[f | diag_rp(d, j)], in[j]);
+    }
+}
+
 __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweepArgs sw,
                                                   const TSeg *__restrict__ segs, const TOpHdr *__restrict__ hdr,
                                                   const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
@@ -202,7 +247,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
 
     TSeg sg = segs[sw.seg0];
     int tdep = seg_tdep(t, sg);
-    c128 v[TN];
+    c128 v[TN], w[TN];
     {
         const u64 g = base | seg_gthread(t, sg);
 #pragma unroll
@@ -228,55 +273,18 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
             for (int j = 0; j < TN; ++j) v[j] = *reinterpret_cast<const c128 *>(ldsb + a[j]);
             __syncthreads();
         }
-        TOpHdr h = hdr[sg.op0];  // the header array has one spare entry past the last op
-        for (int o = sg.op0; o < sg.op1; ++o) {
-            const TOpHdr cur_h = h;
-            h = hdr[o + 1];
-            const c128 *M = pool + cur_h.arg;
-            switch (cur_h.code) {
-                case C_DENSE1 + 0: reg_apply1<0>(v, M[0], M[1], M[2], M[3]); break;
-                case C_DENSE1 + 1: reg_apply1<1>(v, M[0], M[1], M[2], M[3]); break;
-                case C_DENSE1 + 2: reg_apply1<2>(v, M[0], M[1], M[2], M[3]); break;
-                case C_DENSE1 + 3: reg_apply1<3>(v, M[0], M[1], M[2], M[3]); break;
-                case C_DENSE2 + 0: reg_apply2<0, 1>(v, M); break;
-                case C_DENSE2 + 1: reg_apply2<0, 2>(v, M); break;
-                case C_DENSE2 + 2: reg_apply2<0, 3>(v, M); break;
-                case C_DENSE2 + 3: reg_apply2<1, 2>(v, M); break;
-                case C_DENSE2 + 4: reg_apply2<1, 3>(v, M); break;
-                case C_DENSE2 + 5: reg_apply2<2, 3>(v, M); break;
-                case C_SIGN: {
-                    const TDiag &d = diag[cur_h.arg];
-                    const int f = diag_f(d, tdep, base);
-                    const unsigned neg = unsigned(d.negmask);
+        // Ops in pairs, v -> w -> v, every op out of place: the loop carries v
+        // only, so no op ends with register copies.
+        int o = sg.op0;
+        for (; o + 1 < sg.op1; o += 2) {
+            const TOpHdr ha = hdr[o], hb = hdr[o + 1];
+            tile_op(ha, diag, pool, v, w, tdep, base);
+            tile_op(hb, diag, pool, w, v, tdep, base);
+        }
+        if (o < sg.op1) {
+            tile_op(hdr[o], diag, pool, v, w, tdep, base);
 #pragma unroll
-                    for (int j = 0; j < TN; ++j) {
-                        const unsigned flip = (neg >> (f | diag_rp(d, j))) & 1u;
-                        v[j].re = flip_sign(v[j].re, flip);
-                        v[j].im = flip_sign(v[j].im, flip);
-                    }
-                    break;
-                }
-                case C_SELECT: {
-                    const TDiag &d = diag[cur_h.arg];
-                    const int f = diag_f(d, tdep, base);
-                    const c128 d0 = d.val[0], d1 = d.val[1], d2 = d.val[2], d3 = d.val[3];
-#pragma unroll
-                    for (int j = 0; j < TN; ++j) {
-                        const int idx = f | diag_rp(d, j);
-                        const c128 lo = (idx & 1) ? d1 : d0, hi = (idx & 1) ? d3 : d2;
-                        v[j] = cmul((idx & 2) ? hi : lo, v[j]);
-                    }
-                    break;
-                }
-                default: {
-                    const TDiag &d = diag[cur_h.arg];
-                    const int f = diag_f(d, tdep, base);
-                    const c128 *D = pool + d.moff;
-#pragma unroll
-                    for (int j = 0; j < TN; ++j) v[j] = cmul(D[f | diag_rp(d, j)], v[j]);
-                    break;
-                }
-            }
+            for (int j = 0; j < TN; ++j) v[j] = w[j];
         }
     }
     {

@@ -12,6 +12,7 @@ import oracle
 from oracle import dense
 import mossquantum_amd as mq
 from mossquantum_amd import lower
+from mossquantum_amd.backend import plan_emulate
 from mossquantum_amd.gates import haar_unitary
 from mossquantum_amd.workloads import hea_circuit, random_circuit, haar_layers, tfim, xxz
 
@@ -182,6 +183,22 @@ def test_fused_random_circuit(ctx, n):
         assert np.max(np.abs(sv.to_host() - want)) < AMP, fuse
     p = mq.Program(c, n, False, ctx, fuse=True)
     assert p.sweeps() < len(list(c.flat()))
+
+
+def test_fused_single_long_sweep(ctx):
+    """A 12-qubit circuit is one sweep: at depth 40 that sweep runs ~370 tile
+    ops over ~40 register layouts (long op lists, many transpositions)."""
+    n = 12
+    psi0 = rand_state(n, np.random.default_rng(5))
+    for depth in (2, 40):
+        c = random_circuit(n, depth, haar_layers(n, depth, 300 + depth))
+        _, st = plan_emulate(c, n, False, psi0)
+        assert st["sweeps"] == 1
+        want = psi0.copy()
+        oracle.run_program(want, lower(c, n, False))
+        sv = mq.StateVector.from_host(psi0)
+        mq.apply_(c, sv, mq.HIPBackend(ctx, fuse=True))
+        assert np.max(np.abs(sv.to_host() - want)) < AMP, depth
 
 
 def test_fused_density_program(ctx):
