@@ -53,28 +53,47 @@ struct TOp {
     unsigned rp_lo, rp_hi;  // diagonal: rp(j) for j < 8 (lo) and j >= 8 (hi), 4 bits each
     int negmask;   // T_SIGN: bit idx set -> negate
     int moff;      // offset in the plan's device pool (matrix / diagonal values)
+    int rmask;     // T_SIGN: register-bit mask of the op's register bits
 };
 
-// Device form of an op: one header per op, so the kernel reads the next
-// header while the current op runs and a dense op needs a single scalar-load
-// round trip (its matrix).  code: 0..3 dense 2x2 on register bit `code`;
-// 4..9 dense 4x4 on a register-bit pair (kPairA/kPairB); 10 sign-only diagonal,
-// 11 diagonal on <= 2 bits, 12 wider diagonal.  arg: matrix offset in the pool
-// (dense) or index of the op's TDiag (diagonal).
-struct TOpHdr {
-    int code, arg;
-};
-enum : int { C_DENSE1 = 0, C_DENSE2 = 4, C_SIGN = 10, C_SELECT = 11, C_GATHER = 12 };
+// Device form of a segment's ops: groups, each one out-of-place step
+// (in -> out) of the two-register-set op loop, so the loop never copies.
+//   C_DENSE1 + a, C_DENSE2 + pair: dense 2x2 on register bit a / 4x4 on a
+//     register-bit pair (kPairA/kPairB);
+//   C_FLIP + any of those: the dense op, then sign flips of its outputs;
+//   C_COPY: copy with sign flips (flips at the start of a segment);
+//   C_DIAG: a general diagonal (TDiag record x).
+// Sign flips are controlled-Z-type diagonals (-1 exactly where all their bits
+// are 1), up to kGroupCz per group: flip i negates the slots in the 16-bit slot
+// pattern pat[i] wherever its condition bits are all 1.  cond[i] packs up to
+// three condition bits, 7 bits each from bit 0: 0x7F none; 0x40 | b: global bit
+// b outside the tile; else tile-local bit b.  A segment with an odd number of
+// groups ends with a plain C_COPY, so groups always come in pairs.
+enum : int { C_DENSE1 = 0, C_DENSE2 = 4, C_FLIP = 16, C_COPY = 30, C_DIAG = 31 };
+enum : int { D_SIGN = 0, D_SELECT = 1, D_GATHER = 2 };
 constexpr int kPairA[6] = {0, 0, 0, 1, 1, 2};
 constexpr int kPairB[6] = {1, 2, 3, 2, 3, 3};
+constexpr int kGroupCz = 3;
+constexpr int kCzNone = 0x7F;
+
+struct TGroup {
+    int code;
+    int moff;  // dense: matrix offset in the pool
+    int ncz;
+    int x;     // C_DIAG: TDiag index
+    unsigned pat[kGroupCz];
+    int cond[kGroupCz];
+};
 
 struct TDiag {
+    int code;  // D_SIGN, D_SELECT, D_GATHER
     int lbit[4];
     int qbit[4];
     unsigned rp_lo, rp_hi;
     int negmask;
-    int moff;     // C_GATHER: offset of the 2^k values in the pool
-    c128 val[4];  // C_SELECT: the four values
+    int moff;  // D_GATHER: offset of the 2^k values in the pool
+    int pad;
+    c128 val[4];  // D_SELECT: the four values
 };
 
 // A register layout and its instructions.  The layout's index maps are
@@ -90,7 +109,8 @@ struct TDiag {
 // at the old and new layouts' thread and register bits.
 struct TSeg {
     int reg[TRB];  // tile-local bits held in registers (increasing)
-    int op0, op1;
+    int op0, op1;  // planner ops (TOp) of the segment
+    int g0, g1;    // device groups of the segment
     int tpos[TTB];
     int gpos[TTB];
     int gbit[TRB];
@@ -107,22 +127,30 @@ struct TSweepArgs {
 };
 
 
-// Dense ops are out of place (in -> out): the op loop alternates between two
-// register sets, so no op ends with copies back into the loop-carried set.
-template <int A>
-__device__ __forceinline__ void reg_apply1(const c128 *in, c128 *out, const c128 *__restrict__ M) {
+// Sign bit of slot j under the per-lane flip pattern F.
+__device__ __forceinline__ c128 flip_slot(c128 x, unsigned F, int j) {
+    const unsigned long long m = static_cast<unsigned long long>((F << (31 - j)) & 0x80000000u) << 32;
+    return c128{__longlong_as_double(__double_as_longlong(x.re) ^ m),
+                __longlong_as_double(__double_as_longlong(x.im) ^ m)};
+}
+
+// Dense ops are out of place (in -> out); with FLIP the outputs are then
+// sign-flipped per the lane's pattern F.
+template <int A, bool FLIP>
+__device__ __forceinline__ void reg_apply1(const c128 *in, c128 *out, const c128 *__restrict__ M, unsigned F) {
     const c128 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
         if (j & (1 << A)) continue;
         const c128 x = in[j], y = in[j | (1 << A)];
-        out[j] = cfma(m01, y, cmul(m00, x));
-        out[j | (1 << A)] = cfma(m11, y, cmul(m10, x));
+        const c128 y0 = cfma(m01, y, cmul(m00, x)), y1 = cfma(m11, y, cmul(m10, x));
+        out[j] = FLIP ? flip_slot(y0, F, j) : y0;
+        out[j | (1 << A)] = FLIP ? flip_slot(y1, F, j | (1 << A)) : y1;
     }
 }
 
-template <int A, int B>
-__device__ __forceinline__ void reg_apply2(const c128 *in, c128 *out, const c128 *__restrict__ M) {
+template <int A, int B, bool FLIP>
+__device__ __forceinline__ void reg_apply2(const c128 *in, c128 *out, const c128 *__restrict__ M, unsigned F) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const c128 m0 = M[r], m1 = M[r + 4], m2 = M[r + 8], m3 = M[r + 12];
@@ -133,23 +161,9 @@ __device__ __forceinline__ void reg_apply2(const c128 *in, c128 *out, const c128
             c128 acc = cmul(m0, in[j]);
             acc = cfma(m1, in[j | (1 << A)], acc);
             acc = cfma(m2, in[j | (1 << B)], acc);
-            out[j | orow] = cfma(m3, in[j | (1 << A) | (1 << B)], acc);
+            acc = cfma(m3, in[j | (1 << A) | (1 << B)], acc);
+            out[j | orow] = FLIP ? flip_slot(acc, F, j | orow) : acc;
         }
-    }
-}
-
-__device__ __forceinline__ void dense_op(int code, const c128 *in, c128 *out, const c128 *__restrict__ M) {
-    switch (code) {
-        case C_DENSE1 + 0: reg_apply1<0>(in, out, M); break;
-        case C_DENSE1 + 1: reg_apply1<1>(in, out, M); break;
-        case C_DENSE1 + 2: reg_apply1<2>(in, out, M); break;
-        case C_DENSE1 + 3: reg_apply1<3>(in, out, M); break;
-        case C_DENSE2 + 0: reg_apply2<0, 1>(in, out, M); break;
-        case C_DENSE2 + 1: reg_apply2<0, 2>(in, out, M); break;
-        case C_DENSE2 + 2: reg_apply2<0, 3>(in, out, M); break;
-        case C_DENSE2 + 3: reg_apply2<1, 2>(in, out, M); break;
-        case C_DENSE2 + 4: reg_apply2<1, 3>(in, out, M); break;
-        default: reg_apply2<2, 3>(in, out, M); break;
     }
 }
 
@@ -202,16 +216,11 @@ __device__ __forceinline__ double flip_sign(double x, unsigned flip) {
     return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(flip) << 63));
 }
 
-// One op, out of place (in -> out).
-__device__ __forceinline__ void tile_op(const TOpHdr h, const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
-                                        const c128 *in, c128 *out, int tdep, u64 base) {
-    if (h.code < C_SIGN) {
-        dense_op(h.code, in, out, pool + h.arg);
-        return;
-    }
-    const TDiag &d = diag[h.arg];
+// General diagonal, out of place.
+__device__ __forceinline__ void diag_op(const TDiag &d, const c128 *__restrict__ pool, const c128 *in, c128 *out,
+                                        int tdep, u64 base) {
     const int f = diag_f(d, tdep, base);
-    if (h.code == C_SIGN) {
+    if (d.code == D_SIGN) {
         const unsigned neg = unsigned(d.negmask);
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
@@ -219,7 +228,7 @@ __device__ __forceinline__ void tile_op(const TOpHdr h, const TDiag *__restrict_
             out[j].re = flip_sign(in[j].re, flip);
             out[j].im = flip_sign(in[j].im, flip);
         }
-    } else if (h.code == C_SELECT) {
+    } else if (d.code == D_SELECT) {
         const c128 d0 = d.val[0], d1 = d.val[1], d2 = d.val[2], d3 = d.val[3];
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
@@ -234,8 +243,64 @@ __device__ __forceinline__ void tile_op(const TOpHdr h, const TDiag *__restrict_
     }
 }
 
+// Condition of a packed flip for this thread (0 or 1).
+__host__ __device__ __forceinline__ unsigned flip_cond(int cw, int tdep, u64 base) {
+    unsigned c = 1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int f = (cw >> (7 * i)) & 0x7F;
+        if (f == kCzNone) continue;
+        c &= (f & 0x40) ? unsigned(base >> (f & 63)) & 1u : unsigned(tdep >> f) & 1u;
+    }
+    return c;
+}
+
+// The lane's flip pattern of a group: bit j set = slot j changes sign.
+__host__ __device__ __forceinline__ unsigned group_flips(const TGroup &g, int tdep, u64 base) {
+    unsigned F = 0;
+#pragma unroll
+    for (int i = 0; i < kGroupCz; ++i)
+        if (i < g.ncz && flip_cond(g.cond[i], tdep, base)) F ^= g.pat[i];
+    return F;
+}
+
+// One group, out of place.
+__device__ __forceinline__ void group_op(const TGroup &g, const TDiag *__restrict__ diag,
+                                         const c128 *__restrict__ pool, const c128 *in, c128 *out, int tdep,
+                                         u64 base) {
+    const c128 *M = pool + g.moff;
+    const unsigned F = group_flips(g, tdep, base);
+    switch (g.code) {
+        case C_DENSE1 + 0: reg_apply1<0, false>(in, out, M, F); break;
+        case C_DENSE1 + 1: reg_apply1<1, false>(in, out, M, F); break;
+        case C_DENSE1 + 2: reg_apply1<2, false>(in, out, M, F); break;
+        case C_DENSE1 + 3: reg_apply1<3, false>(in, out, M, F); break;
+        case C_DENSE2 + 0: reg_apply2<0, 1, false>(in, out, M, F); break;
+        case C_DENSE2 + 1: reg_apply2<0, 2, false>(in, out, M, F); break;
+        case C_DENSE2 + 2: reg_apply2<0, 3, false>(in, out, M, F); break;
+        case C_DENSE2 + 3: reg_apply2<1, 2, false>(in, out, M, F); break;
+        case C_DENSE2 + 4: reg_apply2<1, 3, false>(in, out, M, F); break;
+        case C_DENSE2 + 5: reg_apply2<2, 3, false>(in, out, M, F); break;
+        case C_FLIP + C_DENSE1 + 0: reg_apply1<0, true>(in, out, M, F); break;
+        case C_FLIP + C_DENSE1 + 1: reg_apply1<1, true>(in, out, M, F); break;
+        case C_FLIP + C_DENSE1 + 2: reg_apply1<2, true>(in, out, M, F); break;
+        case C_FLIP + C_DENSE1 + 3: reg_apply1<3, true>(in, out, M, F); break;
+        case C_FLIP + C_DENSE2 + 0: reg_apply2<0, 1, true>(in, out, M, F); break;
+        case C_FLIP + C_DENSE2 + 1: reg_apply2<0, 2, true>(in, out, M, F); break;
+        case C_FLIP + C_DENSE2 + 2: reg_apply2<0, 3, true>(in, out, M, F); break;
+        case C_FLIP + C_DENSE2 + 3: reg_apply2<1, 2, true>(in, out, M, F); break;
+        case C_FLIP + C_DENSE2 + 4: reg_apply2<1, 3, true>(in, out, M, F); break;
+        case C_FLIP + C_DENSE2 + 5: reg_apply2<2, 3, true>(in, out, M, F); break;
+        case C_COPY:
+#pragma unroll
+            for (int j = 0; j < TN; ++j) out[j] = flip_slot(in[j], F, j);
+            break;
+        default: diag_op(diag[g.x], pool, in, out, tdep, base); break;
+    }
+}
+
 __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweepArgs sw,
-                                                  const TSeg *__restrict__ segs, const TOpHdr *__restrict__ hdr,
+                                                  const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
                                                   const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
@@ -273,18 +338,12 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
             for (int j = 0; j < TN; ++j) v[j] = *reinterpret_cast<const c128 *>(ldsb + a[j]);
             __syncthreads();
         }
-        // Ops in pairs, v -> w -> v, every op out of place: the loop carries v
-        // only, so no op ends with register copies.
-        int o = sg.op0;
-        for (; o + 1 < sg.op1; o += 2) {
-            const TOpHdr ha = hdr[o], hb = hdr[o + 1];
-            tile_op(ha, diag, pool, v, w, tdep, base);
-            tile_op(hb, diag, pool, w, v, tdep, base);
-        }
-        if (o < sg.op1) {
-            tile_op(hdr[o], diag, pool, v, w, tdep, base);
-#pragma unroll
-            for (int j = 0; j < TN; ++j) v[j] = w[j];
+        // Groups in pairs, v -> w -> v (a segment holds an even number): the
+        // loop carries v only.
+        for (int g = sg.g0; g < sg.g1; g += 2) {
+            const TGroup ga = groups[g], gb = groups[g + 1];
+            group_op(ga, diag, pool, v, w, tdep, base);
+            group_op(gb, diag, pool, w, v, tdep, base);
         }
     }
     {
@@ -326,10 +385,10 @@ struct FusedPlan {
     std::vector<TSeg> h_segs;
     std::vector<TOp> h_ops;
     std::vector<c128> h_pool;
-    std::vector<TOpHdr> h_hdr;  // device encoding of h_ops (+1 spare entry)
+    std::vector<TGroup> h_groups;  // device encoding of h_ops
     std::vector<TDiag> h_diag;
     TSeg *d_segs = nullptr;
-    TOpHdr *d_hdr = nullptr;
+    TGroup *d_groups = nullptr;
     TDiag *d_diag = nullptr;
     c128 *d_pool = nullptr;
     int nsweeps = 0;
@@ -893,6 +952,9 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
                                 t.qbit[q] = o.bits[q];
                         }
                     }
+                    t.rmask = 0;
+                    for (int q = 0; q < o.nb; ++q)
+                        if (ridx[q] >= 0) t.rmask |= 1 << ridx[q];
                     t.rp_lo = t.rp_hi = 0;
                     for (int j = 0; j < TN; ++j) {
                         unsigned rp = 0;
@@ -956,8 +1018,28 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
                         for (int k = 0; k < TRB; ++k) found |= fp->h_segs[q - 1].reg[k] == g.reg[i];
                         changed += !found;
                     }
-                fprintf(stderr, "  seg reg {%d,%d,%d,%d} ops %d changed %d\n", g.reg[0], g.reg[1], g.reg[2], g.reg[3],
-                        g.op1 - g.op0, changed);
+                int nd1 = 0, nd2 = 0, ndg = 0;
+                for (int o = g.op0; o < g.op1; ++o) {
+                    const int ty = fp->h_ops[o].type;
+                    (ty == T_DENSE1 ? nd1 : ty == T_DENSE2 ? nd2 : ndg)++;
+                }
+                fprintf(stderr, "  seg reg {%d,%d,%d,%d} ops %d changed %d dense1 %d dense2 %d diag %d\n", g.reg[0],
+                        g.reg[1], g.reg[2], g.reg[3], g.op1 - g.op0, changed, nd1, nd2, ndg);
+                for (int o = g.op0; o < g.op1; ++o) {
+                    const TOp &op = fp->h_ops[o];
+                    if (op.type == T_DENSE1 || op.type == T_DENSE2) continue;
+                    int nr = 0, nl = 0, nq = 0;
+                    for (int q = 0; q < op.k; ++q) {
+                        if (op.lbit[q] >= 0)
+                            ++nl;
+                        else if (op.qbit[q] >= 0)
+                            ++nq;
+                        else
+                            ++nr;
+                    }
+                    fprintf(stderr, "    diag type %d k %d reg %d lane %d out %d neg %x\n", op.type, op.k, nr, nl, nq,
+                            op.negmask);
+                }
             }
         }
         for (int q = it.args.seg0 + 1; q < it.args.seg1; ++q)
@@ -976,27 +1058,80 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
     return fp;
 }
 
-// Device encoding of the planned ops (headers and diagonal records).
+// Packed form of a controlled-Z-type sign flip (a sign-only diagonal whose
+// only -1 is at the all-ones index): its slot pattern and condition bits.
+// False for other diagonals or more than three bits outside the registers.
+static bool pack_flip(const TOp &o, unsigned *pat, int *cond) {
+    if (o.type != T_SIGN || o.negmask != (1 << ((1 << o.k) - 1))) return false;
+    int w = 0, n = 0;
+    for (int q = 0; q < o.k; ++q) {
+        int f;
+        if (o.lbit[q] >= 0)
+            f = o.lbit[q];
+        else if (o.qbit[q] >= 0)
+            f = 0x40 | o.qbit[q];
+        else
+            continue;
+        if (n == 3) return false;
+        w |= f << (7 * n++);
+    }
+    for (; n < 3; ++n) w |= kCzNone << (7 * n);
+    unsigned pm = 0;
+    for (int j = 0; j < TN; ++j)
+        if ((j & o.rmask) == o.rmask) pm |= 1u << j;
+    *pat = pm;
+    *cond = w;
+    return true;
+}
+
+// Device encoding of the planned ops: per segment, groups (one out-of-place
+// step each).  Diagonals commute with each other, so the flips after a dense op
+// are packed into its group in any order.
 static int encode_ops(FusedPlan *fp, std::string *err) {
-    fp->h_hdr.clear();
+    fp->h_groups.clear();
     fp->h_diag.clear();
-    for (const TOp &o : fp->h_ops) {
-        TOpHdr h{};
-        if (o.type == T_DENSE1) {
-            h.code = C_DENSE1 + o.a;
-            h.arg = o.moff;
-        } else if (o.type == T_DENSE2) {
-            int pi = -1;
-            for (int q = 0; q < 6; ++q)
-                if (kPairA[q] == o.a && kPairB[q] == o.b) pi = q;
-            if (pi < 0) {
-                *err = "dense op on an invalid register pair";
-                return MOSSQ_EINVAL;
+    for (TSeg &sg : fp->h_segs) {
+        sg.g0 = int(fp->h_groups.size());
+        auto add = [&](int code, int moff) {
+            TGroup g{};
+            g.code = code;
+            g.moff = moff;
+            fp->h_groups.push_back(g);
+        };
+        for (int oi = sg.op0; oi < sg.op1; ++oi) {
+            const TOp &o = fp->h_ops[oi];
+            if (o.type == T_DENSE1) {
+                add(C_DENSE1 + o.a, o.moff);
+                continue;
             }
-            h.code = C_DENSE2 + pi;
-            h.arg = o.moff;
-        } else {
+            if (o.type == T_DENSE2) {
+                int pi = -1;
+                for (int q = 0; q < 6; ++q)
+                    if (kPairA[q] == o.a && kPairB[q] == o.b) pi = q;
+                if (pi < 0) {
+                    *err = "dense op on an invalid register pair";
+                    return MOSSQ_EINVAL;
+                }
+                add(C_DENSE2 + pi, o.moff);
+                continue;
+            }
+            unsigned pat;
+            int cond;
+            if (pack_flip(o, &pat, &cond)) {
+                const bool here = int(fp->h_groups.size()) > sg.g0;
+                TGroup *g = here ? &fp->h_groups.back() : nullptr;
+                if (!g || g->code == C_DIAG || g->ncz == kGroupCz) {
+                    add(C_COPY, 0);
+                    g = &fp->h_groups.back();
+                }
+                if (g->code < C_FLIP) g->code += C_FLIP;
+                g->pat[g->ncz] = pat;
+                g->cond[g->ncz] = cond;
+                g->ncz++;
+                continue;
+            }
             TDiag d{};
+            d.code = o.type == T_SIGN ? D_SIGN : o.type == T_SELECT ? D_SELECT : D_GATHER;
             for (int q = 0; q < 4; ++q) {
                 d.lbit[q] = o.lbit[q];
                 d.qbit[q] = o.qbit[q];
@@ -1007,13 +1142,14 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
             d.moff = o.moff;
             if (o.type == T_SELECT)
                 for (int q = 0; q < 4; ++q) d.val[q] = fp->h_pool[o.moff + q];
-            h.code = o.type == T_SIGN ? C_SIGN : o.type == T_SELECT ? C_SELECT : C_GATHER;
-            h.arg = int(fp->h_diag.size());
+            add(C_DIAG, 0);
+            fp->h_groups.back().x = int(fp->h_diag.size());
             fp->h_diag.push_back(d);
         }
-        fp->h_hdr.push_back(h);
+        if ((fp->h_groups.size() - sg.g0) & 1) add(C_COPY, 0);  // groups run in pairs
+        sg.g1 = int(fp->h_groups.size());
     }
-    fp->h_hdr.push_back(TOpHdr{C_SIGN, 0});  // spare: read ahead past the last op, never executed
+    if (fp->h_groups.empty()) fp->h_groups.push_back(TGroup{});
     if (fp->h_diag.empty()) fp->h_diag.push_back(TDiag{});
     return MOSSQ_OK;
 }
@@ -1120,22 +1256,24 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
                 for (int j = 0; j < TN; ++j) V[t * TN + j] = lds[a[j] / sizeof(c128)];
             }
         }
-        // Executes the device encoding (headers, diagonal records), as the kernel does.
-        for (int o = sg.op0; o < sg.op1; ++o) {
-            const TOpHdr &h = fp->h_hdr[o];
-            const c128 *M = fp->h_pool.data() + h.arg;
+        // Executes the device encoding (groups, packed flips, diagonal records),
+        // as the kernel does.
+        for (int gi = sg.g0; gi < sg.g1; ++gi) {
+            const TGroup &g = fp->h_groups[gi];
+            const c128 *M = fp->h_pool.data() + g.moff;
+            const int dense = g.code >= C_FLIP && g.code < C_COPY ? g.code - C_FLIP : g.code;
             for (int t = 0; t < TT; ++t) {
                 c128 *v = &V[t * TN];
-                if (h.code < C_DENSE2) {
-                    const int a = h.code - C_DENSE1;
+                if (dense < C_DENSE2) {
+                    const int a = dense - C_DENSE1;
                     for (int j = 0; j < TN; ++j) {
                         if (j & (1 << a)) continue;
                         c128 x = v[j], y = v[j | (1 << a)];
                         v[j] = cfma(M[2], y, cmul(M[0], x));
                         v[j | (1 << a)] = cfma(M[3], y, cmul(M[1], x));
                     }
-                } else if (h.code < C_SIGN) {
-                    const int a = kPairA[h.code - C_DENSE2], b = kPairB[h.code - C_DENSE2];
+                } else if (dense < C_DENSE2 + 6) {
+                    const int a = kPairA[dense - C_DENSE2], b = kPairB[dense - C_DENSE2];
                     for (int j = 0; j < TN; ++j) {
                         if (j & ((1 << a) | (1 << b))) continue;
                         int ii[4] = {j, j | (1 << a), j | (1 << b), j | (1 << a) | (1 << b)};
@@ -1146,13 +1284,19 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
                             v[ii[r]] = acc;
                         }
                     }
-                } else {
-                    const TDiag &d = fp->h_diag[h.arg];
+                }
+                if (g.code >= C_FLIP && g.code <= C_COPY) {
+                    const unsigned F = group_flips(g, tdep[t], base);
+                    for (int j = 0; j < TN; ++j)
+                        if ((F >> j) & 1) v[j] = mk(-v[j].re, -v[j].im);
+                }
+                if (g.code == C_DIAG) {
+                    const TDiag &d = fp->h_diag[g.x];
                     for (int j = 0; j < TN; ++j) {
                         int idx = diag_f(d, tdep[t], base) | diag_rp(d, j);
-                        if (h.code == C_SIGN) {
+                        if (d.code == D_SIGN) {
                             if ((d.negmask >> idx) & 1) v[j] = mk(-v[j].re, -v[j].im);
-                        } else if (h.code == C_SELECT) {
+                        } else if (d.code == D_SELECT) {
                             v[j] = cmul(d.val[idx], v[j]);
                         } else {
                             v[j] = cmul(fp->h_pool[d.moff + idx], v[j]);
@@ -1189,14 +1333,14 @@ void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state) {
 
 int fused_plan_upload(Ctx *c, FusedPlan *fp) {
     int s = dev_alloc(c, std::max<size_t>(fp->h_segs.size(), 1) * sizeof(TSeg), (void **)&fp->d_segs);
-    if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_hdr.size() * sizeof(TOpHdr), (void **)&fp->d_hdr);
+    if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_groups.size() * sizeof(TGroup), (void **)&fp->d_groups);
     if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_diag.size() * sizeof(TDiag), (void **)&fp->d_diag);
     if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_pool.size() * sizeof(c128), (void **)&fp->d_pool);
     if (s != MOSSQ_OK) return s;
     if (!fp->h_segs.empty())
         MOSSQ_HIP(hipMemcpyAsync(fp->d_segs, fp->h_segs.data(), fp->h_segs.size() * sizeof(TSeg),
                                  hipMemcpyHostToDevice, c->stream));
-    MOSSQ_HIP(hipMemcpyAsync(fp->d_hdr, fp->h_hdr.data(), fp->h_hdr.size() * sizeof(TOpHdr), hipMemcpyHostToDevice,
+    MOSSQ_HIP(hipMemcpyAsync(fp->d_groups, fp->h_groups.data(), fp->h_groups.size() * sizeof(TGroup), hipMemcpyHostToDevice,
                              c->stream));
     MOSSQ_HIP(hipMemcpyAsync(fp->d_diag, fp->h_diag.data(), fp->h_diag.size() * sizeof(TDiag),
                              hipMemcpyHostToDevice, c->stream));
@@ -1225,7 +1369,7 @@ void fused_plan_free(Program *p) {
     FusedPlan *fp = p->fused;
     if (p->ctx) {
         if (fp->d_segs) dev_free(p->ctx, fp->d_segs);
-        if (fp->d_hdr) dev_free(p->ctx, fp->d_hdr);
+        if (fp->d_groups) dev_free(p->ctx, fp->d_groups);
         if (fp->d_diag) dev_free(p->ctx, fp->d_diag);
         if (fp->d_pool) dev_free(p->ctx, fp->d_pool);
     }
@@ -1250,7 +1394,7 @@ int fused_run(Ctx *c, const Program *p, c128 *state) {
             MOSSQ_TRY(run_steps(c, p, state, it.step, it.step + 1));
             continue;
         }
-        hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(TT), 0, c->stream, state, it.args, fp->d_segs, fp->d_hdr, fp->d_diag,
+        hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(TT), 0, c->stream, state, it.args, fp->d_segs, fp->d_groups, fp->d_diag,
                            fp->d_pool);
         MOSSQ_HIP(hipGetLastError());
     }
