@@ -313,19 +313,24 @@ def main():
         ms_u = time_program(ctx, unf, sv_u, usteps, 1, 2, 3)
         step_u = ms_u / usteps
         log(f"unfused: {step_u:.3f} ms/step")
-        # one-qubit dense sweeps alone (k_gate1): the first layer's 26 gates
+        # parity of the two plans after the same number of passes
+        for _ in range(args.steps + args.warmup - (usteps + 1)):
+            unf.run(sv_u.storage)
+        ctx.sync()
+        a, b = sv.to_host(), sv_u.to_host()
+        extra["fused_vs_unfused_max_abs_diff"] = float(np.max(np.abs(a - b)))
+        del a, b
+        # one-qubit dense sweeps alone (k_gate1): the first layer's 26 gates, on their own state
         from mossquantum_amd import QCircuit
         layer = QCircuit(list(circ.flat())[:N])
         g1 = mq.Program(layer, N, False, ctx, fuse=False)
-        ms_g1 = time_program(ctx, g1, sv_u, 4, 1, 4, 5)
+        sv_g = mq.StateVector.zeros(N, ctx)
+        ms_g1 = time_program(ctx, g1, sv_g, 4, 1, 4, 5)
+        del sv_g
         g1_avg = ms_g1 / (4 * N)
         extra["unfused"] = {"ms_per_step": step_u, "gates_per_s": ngates / (step_u / 1e3),
                             "hbm_gbps": alg_bytes / (step_u / 1e3) / 1e9,
                             "gate1_avg_ms": g1_avg, "gate1_gbps": tile_bytes / (g1_avg / 1e3) / 1e9}
-        # parity of the two plans after the same number of passes (1 + steps vs 1 + usteps)
-        if usteps + 1 == args.steps + args.warmup:
-            a, b = sv.to_host(), sv_u.to_host()
-            extra["fused_vs_unfused_max_abs_diff"] = float(np.max(np.abs(a - b)))
             del a, b
         del sv_u
     norm = mq.dot(sv, sv).real
