@@ -175,6 +175,16 @@ int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state);
  * Registers of fewer than 12 bits are rejected. */
 int mossq_plan_emulate(int n, int is_density, int nops, const mossq_op *ops, const double *pool,
                        uint64_t pool_len, double *state, int *stats);
+/* Host execution of the pure-state adjoint's fused reverse plan (the steps
+ * in reverse with their inverses, each parameterized step followed by its
+ * probe Re<y|D|l>, as the reverse pass of the reference's
+ * differentiation/circuit.jl:136-144), with the adjoint tile kernel's index
+ * arithmetic; needs no device.  y and l (complex, 2^n entries, in place) are
+ * the forward output and the seed; grads[j] (nderiv entries) receives the
+ * gradient row j.  stats as mossq_plan_emulate for the reverse plan.  Used to
+ * test the planner's adjoint plans. */
+int mossq_plan_emulate_adjoint(int n, int nops, const mossq_op *ops, const double *pool, uint64_t pool_len,
+                               double *y, double *l, double *grads, int *stats);
 
 /* ---- adjoint gradients ---------------------------------------------------- */
 /* value = Re <H> at C|in>, grads[j] for the j-th derivative gate in forward

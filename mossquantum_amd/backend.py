@@ -177,6 +177,25 @@ def plan_emulate(circuit: QCircuit, n: int, is_density: bool, state: np.ndarray)
     return st, dict(zip(keys, list(stats)))
 
 
+def plan_emulate_adjoint(circuit: QCircuit, n: int, y: np.ndarray, lam: np.ndarray):
+    """Run the fused reverse plan of the pure-state adjoint of `circuit` on host
+    vectors (y = forward output, lam = seed) with the adjoint tile kernel's
+    index arithmetic (no device).  Returns (y, lam, grads per lowered
+    derivative slot, stats dict)."""
+    lw = lower(circuit, n, False)
+    recs, pool = lw.arrays()
+    ys = np.ascontiguousarray(np.array(y, dtype=np.complex128))
+    ls = np.ascontiguousarray(np.array(lam, dtype=np.complex128))
+    grads = np.zeros(max(lw.nderiv, 1))
+    stats = (C.c_int * 9)()
+    dp = lambda a: a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+    check(_lib.load().mossq_plan_emulate_adjoint(n, len(lw.records), recs, dp(pool), pool.size, dp(ys), dp(ls),
+                                                 grads.ctypes.data_as(C.POINTER(C.c_double)), stats))
+    keys = ("sweeps", "unfused", "segments", "transitions", "tile_ops", "dense1", "dense2", "diag",
+            "conflicted_transitions")
+    return ys, ls, grads[:lw.nderiv], dict(zip(keys, list(stats)))
+
+
 # ------------------------------------------------------------------- apply! ----
 def _mat_ptr(m):
     a = np.ascontiguousarray(np.asarray(m, dtype=np.complex128).reshape(-1, order="F"))

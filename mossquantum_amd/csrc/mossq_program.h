@@ -38,6 +38,7 @@ struct Program {
     std::vector<c128> h_pool;
     c128 *d_pool = nullptr;
     FusedPlan *fused = nullptr;
+    FusedPlan *fused_rev = nullptr;  // pure-state adjoint reverse pass (with gradient probes)
 };
 
 // Host side of mossq_program_create: validate records, build steps and pool.
@@ -52,6 +53,17 @@ int fused_plan_build(Program *p);
 void fused_plan_free(Program *p);
 int fused_plan_sweeps(const Program *p);
 int fused_run(Ctx *c, const Program *p, c128 *state);
+// Reverse pass of the pure-state adjoint on (y, l) with the adjoint plan;
+// gradient partials into rows of `row` entries (zeroed by the caller).
+int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, int row);
+// Partials per gradient row the adjoint plan writes (0 without one).
+int fused_adjoint_parts(const Program *p);
+// One unfused reverse step of the pure-state adjoint (program.hip).
+int adjoint_rev_step(Ctx *c, const Program *p, int i, c128 *y, c128 *l, c128 *part, int row);
+// Gradient partials per row for p's pure-state adjoint.
+int adjoint_part_row(const Program *p);
+// Zero `count` complex entries with a kernel (safe inside captured graphs).
+int zero_fill(Ctx *c, c128 *v, u64 count);
 
 // Pure-state adjoint enqueued on the stream, no host synchronisation.
 // in == nullptr starts from |0...0>.  Buffers a, b hold 2^n amplitudes.

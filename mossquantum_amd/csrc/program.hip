@@ -88,6 +88,24 @@ static int apply_step_rev(Ctx *c, const Program *p, const Step &s, c128 *v) {
     return launch_dense(c, v, p->nbits, s.kb, s.bits, p->d_pool + s.rev, s.irev);
 }
 
+// One reverse step of the pure-state adjoint on (y, l): U^dagger on both and,
+// for a parameterized step, the partials of Re<y|D|l> into row grad_index.
+int adjoint_rev_step(Ctx *c, const Program *p, int i, c128 *y, c128 *l, c128 *part, int row) {
+    const Step &st = p->steps[i];
+    if (st.grad >= 0)
+        return launch_dense_pair(c, y, l, p->n, st.kb, st.bits, p->d_pool + st.rev, p->d_pool + st.grad,
+                                 part + size_t(st.grad_index) * row, false);
+    if (st.irev.monomial || st.kb > 4) {
+        MOSSQ_TRY(apply_step_rev(c, p, st, y));
+        return apply_step_rev(c, p, st, l);
+    }
+    return launch_dense_pair(c, y, l, p->n, st.kb, st.bits, p->d_pool + st.rev, nullptr, nullptr, false);
+}
+
+int adjoint_part_row(const Program *p) {
+    return std::max(pair_parts(p->n), fused_adjoint_parts(p));
+}
+
 int run_steps(Ctx *c, const Program *p, c128 *state, int lo, int hi) {
     for (int i = lo; i < hi; ++i) MOSSQ_TRY(apply_step_fwd(c, p, p->steps[i], state));
     return MOSSQ_OK;
@@ -112,20 +130,14 @@ int adjoint_pure_enqueue(Ctx *c, const Program *p, const Obs *o, const c128 *in,
     MOSSQ_TRY(expect_groups_dev(c, o, s.a, s.a, s.value_dev, s.part));
     MOSSQ_TRY(apply_op_dev(c, o, s.a, s.b, p->n, OpMats::Seed));
     // reverse: per instruction  lambda <- U^dag lambda, y <- U^dag y,
-    // g = Re <y|D|lambda>  (differentiation/circuit.jl:136-144)
-    const int P = pair_parts(p->n);
-    for (int i = int(p->steps.size()) - 1; i >= 0; --i) {
-        const Step &st = p->steps[i];
-        if (st.grad >= 0) {
-            MOSSQ_TRY(launch_dense_pair(c, s.a, s.b, p->n, st.kb, st.bits, p->d_pool + st.rev,
-                                        p->d_pool + st.grad, s.part + size_t(st.grad_index) * P, false));
-        } else if (st.irev.monomial || st.kb > 4) {
-            MOSSQ_TRY(apply_step_rev(c, p, st, s.a));
-            MOSSQ_TRY(apply_step_rev(c, p, st, s.b));
-        } else {
-            MOSSQ_TRY(launch_dense_pair(c, s.a, s.b, p->n, st.kb, st.bits, p->d_pool + st.rev, nullptr, nullptr,
-                                        false));
-        }
+    // g = Re <y|D|lambda>  (differentiation/circuit.jl:136-144); fused into
+    // LDS-tile sweeps when the program has an adjoint plan.
+    const int P = adjoint_part_row(p);
+    if (p->fused_rev) {
+        MOSSQ_TRY(zero_fill(c, s.part, u64(std::max(p->nderiv, 1)) * P));
+        MOSSQ_TRY(fused_adjoint_run(c, p, s.a, s.b, s.part, P));
+    } else {
+        for (int i = int(p->steps.size()) - 1; i >= 0; --i) MOSSQ_TRY(adjoint_rev_step(c, p, i, s.a, s.b, s.part, P));
     }
     return reduce_rows(c, s.part, p->nderiv, P, s.grads_dev);
 }
@@ -151,7 +163,7 @@ int adjoint_pure(Ctx *c, const Program *p, const Obs *o, const c128 *in, c128 *o
     const u64 len = u64(1) << p->n;
     Allocs al(c);
     AdjointScratch s;
-    const int P = pair_parts(p->n);
+    const int P = adjoint_part_row(p);
     u64 nparts = std::max<u64>(u64(o->G) * 512, u64(std::max(p->nderiv, 1)) * P);
     MOSSQ_TRY(al.get(len * sizeof(c128), (void **)&s.a));
     MOSSQ_TRY(al.get(len * sizeof(c128), (void **)&s.b));

@@ -37,7 +37,7 @@ constexpr int TN = 16;   // amplitudes per thread
 constexpr int TTB = 8;   // thread-index bits (TM - TRB)
 constexpr int kCoalesceBits = 3;
 
-enum TOpType { T_DENSE1 = 0, T_DENSE2 = 1, T_SIGN = 2, T_SELECT = 3, T_GATHER = 4 };
+enum TOpType { T_DENSE1 = 0, T_DENSE2 = 1, T_SIGN = 2, T_SELECT = 3, T_GATHER = 4, T_PROBE1 = 5, T_PROBE2 = 6 };
 
 // Diagonal index of register slot j: idx = f | rp(j).  f collects the op bits
 // that are not register bits: from the thread's tile-local index (lbit[i] >= 0)
@@ -54,6 +54,7 @@ struct TOp {
     int negmask;   // T_SIGN: bit idx set -> negate
     int moff;      // offset in the plan's device pool (matrix / diagonal values)
     int rmask;     // T_SIGN: register-bit mask of the op's register bits
+    int probe;     // T_PROBE*: parameter (gradient row) index
 };
 
 // Device form of a segment's ops: groups, each one out-of-place step
@@ -76,6 +77,9 @@ constexpr int kPairB[6] = {1, 2, 3, 2, 3, 3};
 constexpr int kGroupCz = 3;
 constexpr int kCzNone = 0x7F;
 
+// Adjoint plans add, after a group's op and flips, at most one probe: the
+// contribution of this tile to Re<y|D|l> for parameter `probe`, D a 2x2 / 4x4
+// on register bits (pcode as a dense code) at pool offset pmoff.
 struct TGroup {
     int code;
     int moff;  // dense: matrix offset in the pool
@@ -83,6 +87,9 @@ struct TGroup {
     int x;     // C_DIAG: TDiag index
     unsigned pat[kGroupCz];
     int cond[kGroupCz];
+    int probe;  // -1: none
+    int pcode;
+    int pmoff;
 };
 
 struct TDiag {
@@ -360,13 +367,208 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
     }
 }
 
+// --------------------------------------------------------- adjoint sweep ----
+// The reverse pass of the pure-state adjoint over LDS tiles: each thread holds
+// 16 amplitudes of y and the same 16 of l; every op applies to both (in
+// place), and a probe adds this wave's share of Re<y|D|l> to the partials of
+// its parameter (one entry per wave per tile, reduced afterwards in a fixed
+// order, so gradients are reproducible).  Adjoint plans hold dense ops,
+// controlled-Z-type sign flips and probes only (no general diagonals).
+
+template <int A, int B, bool FLIP>
+__device__ __forceinline__ void ip_apply2(c128 *v, const c128 *__restrict__ M, unsigned F) {
+    c128 m[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = M[i];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        if (j & ((1 << A) | (1 << B))) continue;
+        const int idx[4] = {j, j | (1 << A), j | (1 << B), j | (1 << A) | (1 << B)};
+        const c128 x0 = v[idx[0]], x1 = v[idx[1]], x2 = v[idx[2]], x3 = v[idx[3]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            c128 acc = cmul(m[r], x0);
+            acc = cfma(m[r + 4], x1, acc);
+            acc = cfma(m[r + 8], x2, acc);
+            acc = cfma(m[r + 12], x3, acc);
+            v[idx[r]] = FLIP ? flip_slot(acc, F, idx[r]) : acc;
+        }
+    }
+}
+
+__device__ __forceinline__ void dense_ip(int code, c128 *v, const c128 *__restrict__ M, unsigned F) {
+    switch (code) {
+        case C_DENSE1 + 0: reg_apply1<0, false>(v, v, M, F); break;
+        case C_DENSE1 + 1: reg_apply1<1, false>(v, v, M, F); break;
+        case C_DENSE1 + 2: reg_apply1<2, false>(v, v, M, F); break;
+        case C_DENSE1 + 3: reg_apply1<3, false>(v, v, M, F); break;
+        case C_DENSE2 + 0: ip_apply2<0, 1, false>(v, M, F); break;
+        case C_DENSE2 + 1: ip_apply2<0, 2, false>(v, M, F); break;
+        case C_DENSE2 + 2: ip_apply2<0, 3, false>(v, M, F); break;
+        case C_DENSE2 + 3: ip_apply2<1, 2, false>(v, M, F); break;
+        case C_DENSE2 + 4: ip_apply2<1, 3, false>(v, M, F); break;
+        case C_DENSE2 + 5: ip_apply2<2, 3, false>(v, M, F); break;
+        case C_FLIP + C_DENSE1 + 0: reg_apply1<0, true>(v, v, M, F); break;
+        case C_FLIP + C_DENSE1 + 1: reg_apply1<1, true>(v, v, M, F); break;
+        case C_FLIP + C_DENSE1 + 2: reg_apply1<2, true>(v, v, M, F); break;
+        case C_FLIP + C_DENSE1 + 3: reg_apply1<3, true>(v, v, M, F); break;
+        case C_FLIP + C_DENSE2 + 0: ip_apply2<0, 1, true>(v, M, F); break;
+        case C_FLIP + C_DENSE2 + 1: ip_apply2<0, 2, true>(v, M, F); break;
+        case C_FLIP + C_DENSE2 + 2: ip_apply2<0, 3, true>(v, M, F); break;
+        case C_FLIP + C_DENSE2 + 3: ip_apply2<1, 2, true>(v, M, F); break;
+        case C_FLIP + C_DENSE2 + 4: ip_apply2<1, 3, true>(v, M, F); break;
+        case C_FLIP + C_DENSE2 + 5: ip_apply2<2, 3, true>(v, M, F); break;
+        default:  // C_COPY: flips only
+#pragma unroll
+            for (int j = 0; j < TN; ++j) v[j] = flip_slot(v[j], F, j);
+            break;
+    }
+}
+
+// sum over the thread's slots of conj(y) (D l), D on register bit A / bits A, B.
+template <int A>
+__device__ __forceinline__ c128 probe1(const c128 *y, const c128 *l, const c128 *__restrict__ D) {
+    const c128 d00 = D[0], d10 = D[1], d01 = D[2], d11 = D[3];
+    c128 acc = c128{0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        if (j & (1 << A)) continue;
+        const c128 l0 = l[j], l1 = l[j | (1 << A)];
+        acc = cfmaconj(y[j], cfma(d01, l1, cmul(d00, l0)), acc);
+        acc = cfmaconj(y[j | (1 << A)], cfma(d11, l1, cmul(d10, l0)), acc);
+    }
+    return acc;
+}
+
+template <int A, int B>
+__device__ __forceinline__ c128 probe2(const c128 *y, const c128 *l, const c128 *__restrict__ D) {
+    c128 acc = c128{0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const c128 m0 = D[r], m1 = D[r + 4], m2 = D[r + 8], m3 = D[r + 12];
+        const int orow = ((r & 1) << A) | ((r >> 1) << B);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            if (j & ((1 << A) | (1 << B))) continue;
+            c128 t = cmul(m0, l[j]);
+            t = cfma(m1, l[j | (1 << A)], t);
+            t = cfma(m2, l[j | (1 << B)], t);
+            t = cfma(m3, l[j | (1 << A) | (1 << B)], t);
+            acc = cfmaconj(y[j | orow], t, acc);
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ c128 probe_op(int code, const c128 *y, const c128 *l, const c128 *__restrict__ D) {
+    switch (code) {
+        case C_DENSE1 + 0: return probe1<0>(y, l, D);
+        case C_DENSE1 + 1: return probe1<1>(y, l, D);
+        case C_DENSE1 + 2: return probe1<2>(y, l, D);
+        case C_DENSE1 + 3: return probe1<3>(y, l, D);
+        case C_DENSE2 + 0: return probe2<0, 1>(y, l, D);
+        case C_DENSE2 + 1: return probe2<0, 2>(y, l, D);
+        case C_DENSE2 + 2: return probe2<0, 3>(y, l, D);
+        case C_DENSE2 + 3: return probe2<1, 2>(y, l, D);
+        case C_DENSE2 + 4: return probe2<1, 3>(y, l, D);
+        default: return probe2<2, 3>(y, l, D);
+    }
+}
+
+__device__ __forceinline__ c128 wave_sum(c128 a) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a.re += __shfl_xor(a.re, off);
+        a.im += __shfl_xor(a.im, off);
+    }
+    return a;
+}
+
+__device__ __forceinline__ void transpose_lds(c128 *v, char *ldsb, int t, const TSeg &sg) {
+    int a[TN];
+    lds_slots(t, sg.wt, sg.wr, a);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) *reinterpret_cast<c128 *>(ldsb + a[j]) = v[j];
+    __syncthreads();
+    lds_slots(t, sg.rt, sg.rr, a);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) v[j] = *reinterpret_cast<const c128 *>(ldsb + a[j]);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 *__restrict__ ls, TSweepArgs sw,
+                                                      const TSeg *__restrict__ segs,
+                                                      const TGroup *__restrict__ groups,
+                                                      const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
+                                                      c128 *__restrict__ part, int row) {
+    __shared__ c128 lds[1 << TM];
+    char *ldsb = reinterpret_cast<char *>(lds);
+    const int t = threadIdx.x;
+    const u64 tile = blockIdx.x;
+    u64 base = 0;
+    for (int i = 0; i < sw.ncbits; ++i)
+        if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
+
+    TSeg sg = segs[sw.seg0];
+    int tdep = seg_tdep(t, sg);
+    c128 y[TN], l[TN];
+    {
+        const u64 g = base | seg_gthread(t, sg);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            u64 o = 0;
+#pragma unroll
+            for (int i = 0; i < TRB; ++i)
+                if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
+            y[j] = ys[g | o];
+            l[j] = ls[g | o];
+        }
+    }
+    const int slot = int(tile) * 4 + (t >> 6);
+    for (int s = sw.seg0; s < sw.seg1; ++s) {
+        sg = segs[s];
+        if (sg.transpose) {
+            transpose_lds(y, ldsb, t, sg);
+            transpose_lds(l, ldsb, t, sg);
+            tdep = seg_tdep(t, sg);
+        }
+        for (int gi = sg.g0; gi < sg.g1; ++gi) {
+            const TGroup g = groups[gi];
+            if (g.code != C_COPY || g.ncz > 0) {
+                const unsigned F = group_flips(g, tdep, base);
+                dense_ip(g.code, y, pool + g.moff, F);
+                dense_ip(g.code, l, pool + g.moff, F);
+            }
+            if (g.probe >= 0) {
+                const c128 acc = wave_sum(probe_op(g.pcode, y, l, pool + g.pmoff));
+                if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = acc;
+            }
+        }
+    }
+    {
+        const u64 g = base | seg_gthread(t, sg);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            u64 o = 0;
+#pragma unroll
+            for (int i = 0; i < TRB; ++i)
+                if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
+            ys[g | o] = y[j];
+            ls[g | o] = l[j];
+        }
+    }
+}
+
 // ------------------------------------------------------------------ planner ----
+// kind 3 (adjoint plans): probe Re<y|D|l> of parameter `probe` on the two
+// states at this point; it changes neither state.
 struct POp {
-    int kind = 0;  // 0 dense, 1 diagonal, 2 unfusable (run as its own launch)
+    int kind = 0;  // 0 dense, 1 diagonal, 2 unfusable (run as its own launch), 3 probe
     int nb = 0;
     int bits[4] = {};
-    std::vector<c128> m;  // dense: 2^nb x 2^nb column-major; diagonal: 2^nb values
+    std::vector<c128> m;  // dense / probe: 2^nb x 2^nb column-major; diagonal: 2^nb values
     int step = -1;
+    int probe = -1;
     u64 mask() const {
         u64 r = 0;
         for (int i = 0; i < nb; ++i) r |= u64(1) << bits[i];
@@ -391,6 +593,7 @@ struct FusedPlan {
     TGroup *d_groups = nullptr;
     TDiag *d_diag = nullptr;
     c128 *d_pool = nullptr;
+    bool adjoint = false;  // reverse plan of the pure-state adjoint (probes, no general diagonals)
     int nsweeps = 0;
     int ntransitions = 0;
     int nconflicted = 0;  // transitions without a conflict-free LDS slot map
@@ -461,6 +664,57 @@ static void lower_steps(const Program *p, std::vector<POp> &out) {
         u.kind = 2;
         u.step = int(si);
         out.push_back(u);
+    }
+}
+
+static bool sign_only(const std::vector<c128> &d, int *negmask);
+
+// Adjoint plan of a pure-state program: the steps in reverse order with
+// their inverse matrices, each parameterized step followed by the probe of its
+// derivative matrix D (differentiation/circuit.jl:136-144: y <- U^dag y,
+// l <- U^dag l, g = Re<y|D|l>).  Wider or unfusable steps keep kind 2 and run
+// as their own paired launch.
+static void lower_steps_rev(const Program *p, std::vector<POp> &out) {
+    for (int si = int(p->steps.size()) - 1; si >= 0; --si) {
+        const Step &st = p->steps[si];
+        const int dim = 1 << st.kb;
+        // Diagonals the adjoint kernel takes as sign flips: -1 exactly at the
+        // all-ones index (CZ-type) on at most 3 bits.  Other diagonals on <= 2
+        // bits become dense matrices; everything wider runs on its own.
+        int neg = 0;
+        const bool diag = !st.split && is_diag(st.irev);
+        const bool cz_type = diag && sign_only(diag_values(st.irev, dim), &neg) && neg == (1 << (dim - 1)) &&
+                             st.kb <= 3 && st.grad < 0;
+        if (st.split || (st.kb > 2 && !cz_type)) {
+            POp u;
+            u.kind = 2;
+            u.step = si;
+            out.push_back(u);
+            continue;
+        }
+        POp d;
+        d.nb = st.kb;
+        for (int i = 0; i < st.kb; ++i) d.bits[i] = st.bits[i];
+        d.step = si;
+        if (cz_type) {
+            d.kind = 1;
+            d.m = diag_values(st.irev, dim);
+            out.push_back(d);
+        } else if (!(diag && st.irev.nchanged == 0)) {
+            d.kind = 0;
+            d.m = mat_of(p, st.rev, dim);
+            out.push_back(d);
+        }
+        if (st.grad >= 0) {
+            POp pr;
+            pr.kind = 3;
+            pr.nb = st.kb;
+            for (int i = 0; i < st.kb; ++i) pr.bits[i] = st.bits[i];
+            pr.m = mat_of(p, st.grad, dim);
+            pr.step = si;
+            pr.probe = st.grad_index;
+            out.push_back(pr);
+        }
     }
 }
 
@@ -559,7 +813,7 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg) {
         // merge adjacent dense ops on identical bits, adjacent diagonals on identical bits
         for (size_t i = 1; i < ops.size(); ++i) {
             POp &a = ops[i - 1], &b = ops[i];
-            if (a.kind == b.kind && a.kind != 2 && a.mask() == b.mask() && a.nb == b.nb) {
+            if (a.kind == b.kind && a.kind != 2 && a.kind != 3 && a.mask() == b.mask() && a.nb == b.nb) {
                 if (a.kind == 0) {
                     std::vector<c128> pr;
                     product(b.m, a.m, 1 << a.nb, pr);
@@ -742,10 +996,10 @@ static bool safe_layout(const int *reg) {
 
 static int encode_ops(FusedPlan *fp, std::string *err);
 
-static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *err) {
-    std::vector<POp> ops;
-    lower_steps(p, ops);
+static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, int min_tail, bool adjoint,
+                                 std::string *err) {
     FusedPlan *fp = new FusedPlan();
+    fp->adjoint = adjoint;
     const u64 coal = (u64(1) << kCoalesceBits) - 1;
     std::vector<char> done(ops.size(), 0);
     size_t first = 0;
@@ -778,7 +1032,7 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
                     break;
                 }
             if (!blocked && o.kind != 2) {
-                u64 need = o.kind == 0 ? o.mask() : 0;
+                u64 need = o.kind == 0 || o.kind == 3 ? o.mask() : 0;
                 if (popc(T | need) <= TM) {
                     T |= need;
                     chosen.push_back(int(i));
@@ -835,7 +1089,7 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
                         if (!s2[pj] && !commutes(ops[chosen[pj]], ops[chosen[ci]])) ok = false;
                     if (!ok) continue;
                     s2[ci] = 1;
-                    count += ops[chosen[ci]].kind == 0 ? 2 : 1;
+                    count += ops[chosen[ci]].kind == 1 ? 1 : 2;
                     progress = true;
                 }
             }
@@ -849,7 +1103,7 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
             int best_score = -1, best_keep = -1;
             int seeds = 0;
             for (size_t c0 = 0; c0 < chosen.size() && seeds < 16; ++c0) {
-                if (sched[c0] || ops[chosen[c0]].kind != 0 || !ready(c0)) continue;
+                if (sched[c0] || (ops[chosen[c0]].kind != 0 && ops[chosen[c0]].kind != 3) || !ready(c0)) continue;
                 ++seeds;
                 std::vector<int> want;
                 auto try_add = [&](const POp &o) {
@@ -862,7 +1116,7 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
                 };
                 try_add(ops[chosen[c0]]);
                 for (size_t ci = 0; ci < chosen.size() && want.size() < TRB; ++ci)
-                    if (!sched[ci] && ci != c0 && ops[chosen[ci]].kind == 0) try_add(ops[chosen[ci]]);
+                    if (!sched[ci] && ci != c0 && ops[chosen[ci]].kind != 1) try_add(ops[chosen[ci]]);
                 for (int i = 0; i < TRB && want.size() < TRB; ++i)  // keep current bits
                     if (std::find(want.begin(), want.end(), reg[i]) == want.end()) want.push_back(reg[i]);
                 std::sort(want.begin(), want.end());
@@ -906,7 +1160,7 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
                     sched[ci] = 1;
                     ++nsched;
                     sp.idx.push_back(int(ci));
-                    sp.work += ops[chosen[ci]].kind == 0 ? 2 : 1;
+                    sp.work += ops[chosen[ci]].kind == 1 ? 1 : 2;
                     progress = true;
                 }
             }
@@ -985,12 +1239,13 @@ static FusedPlan *plan_with_tail(const Program *p, int min_tail, std::string *er
                             if (reg[i] == l) return i;
                         return -1;
                     };
+                    t.probe = o.probe;
                     if (o.nb == 1) {
-                        t.type = T_DENSE1;
+                        t.type = o.kind == 3 ? T_PROBE1 : T_DENSE1;
                         t.a = ridx(o.bits[0]);
                         if (t.a < 0) return failp("dense op outside the register bits");
                     } else {
-                        t.type = T_DENSE2;
+                        t.type = o.kind == 3 ? T_PROBE2 : T_DENSE2;
                         t.a = ridx(o.bits[0]);
                         t.b = ridx(o.bits[1]);
                         if (t.a < 0 || t.b <= t.a) return failp("dense op outside the register bits");
@@ -1096,7 +1351,17 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
             TGroup g{};
             g.code = code;
             g.moff = moff;
+            g.probe = -1;
             fp->h_groups.push_back(g);
+        };
+        auto pair_code = [&](const TOp &o, int *code) {
+            for (int q = 0; q < 6; ++q)
+                if (kPairA[q] == o.a && kPairB[q] == o.b) {
+                    *code = C_DENSE2 + q;
+                    return true;
+                }
+            *err = "dense op on an invalid register pair";
+            return false;
         };
         for (int oi = sg.op0; oi < sg.op1; ++oi) {
             const TOp &o = fp->h_ops[oi];
@@ -1105,22 +1370,33 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
                 continue;
             }
             if (o.type == T_DENSE2) {
-                int pi = -1;
-                for (int q = 0; q < 6; ++q)
-                    if (kPairA[q] == o.a && kPairB[q] == o.b) pi = q;
-                if (pi < 0) {
-                    *err = "dense op on an invalid register pair";
-                    return MOSSQ_EINVAL;
-                }
-                add(C_DENSE2 + pi, o.moff);
+                int code;
+                if (!pair_code(o, &code)) return MOSSQ_EINVAL;
+                add(code, o.moff);
                 continue;
             }
+            const bool here = int(fp->h_groups.size()) > sg.g0;
+            if (o.type == T_PROBE1 || o.type == T_PROBE2) {
+                TGroup *g = here ? &fp->h_groups.back() : nullptr;
+                if (!g || g->code == C_DIAG || g->probe >= 0) {
+                    add(C_COPY, 0);
+                    g = &fp->h_groups.back();
+                }
+                if (o.type == T_PROBE1) {
+                    g->pcode = C_DENSE1 + o.a;
+                } else if (!pair_code(o, &g->pcode)) {
+                    return MOSSQ_EINVAL;
+                }
+                g->probe = o.probe;
+                g->pmoff = o.moff;
+                continue;
+            }
+            if (o.type == T_SIGN && o.negmask == 0) continue;  // identity (e.g. two equal CZs merged)
             unsigned pat;
             int cond;
             if (pack_flip(o, &pat, &cond)) {
-                const bool here = int(fp->h_groups.size()) > sg.g0;
                 TGroup *g = here ? &fp->h_groups.back() : nullptr;
-                if (!g || g->code == C_DIAG || g->ncz == kGroupCz) {
+                if (!g || g->code == C_DIAG || g->ncz == kGroupCz || g->probe >= 0) {
                     add(C_COPY, 0);
                     g = &fp->h_groups.back();
                 }
@@ -1142,7 +1418,11 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
             d.moff = o.moff;
             if (o.type == T_SELECT)
                 for (int q = 0; q < 4; ++q) d.val[q] = fp->h_pool[o.moff + q];
-            add(C_DIAG, 0);
+            if (fp->adjoint) {
+                *err = "adjoint plan holds a general diagonal";
+                return MOSSQ_EINVAL;
+            }
+            add(C_DIAG, 0);  // a group of its own: diagonals never follow a probe inside a group
             fp->h_groups.back().x = int(fp->h_diag.size());
             fp->h_diag.push_back(d);
         }
@@ -1168,12 +1448,17 @@ static double plan_cost(const FusedPlan *fp) {
 // The tail threshold trades a transposition against the ops it carries into
 // the next sweep; which side wins depends on the circuit, so each candidate is
 // planned and the cheapest plan under the cost model is kept.
-FusedPlan *fused_plan_build_host(const Program *p, std::string *err) {
-    if (const char *mt = getenv("MOSSQ_PLAN_MIN_TAIL")) return plan_with_tail(p, atoi(mt), err);
+FusedPlan *fused_plan_build_host(const Program *p, std::string *err, bool adjoint) {
+    std::vector<POp> ops;
+    if (adjoint)
+        lower_steps_rev(p, ops);
+    else
+        lower_steps(p, ops);
+    if (const char *mt = getenv("MOSSQ_PLAN_MIN_TAIL")) return plan_with_tail(p, ops, atoi(mt), adjoint, err);
     FusedPlan *best = nullptr;
     double best_cost = 0;
     for (int min_tail : {0, 3, 5, 8}) {
-        FusedPlan *fp = plan_with_tail(p, min_tail, err);
+        FusedPlan *fp = plan_with_tail(p, ops, min_tail, adjoint, err);
         if (!fp) {
             delete best;
             return nullptr;
@@ -1216,9 +1501,13 @@ static void host_block(c128 *v, int nbits, int k, const int *bits, const c128 *m
     }
 }
 
-static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp, u64 tile) {
+// One tile of a sweep for one state (forward plans) or the two states y, l of
+// an adjoint plan (probes add Re<y|D|l> contributions to grads).
+static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw, const FusedPlan *fp, u64 tile,
+                         c128 *grads) {
+    const size_t ns = states.size();
     std::vector<c128> lds(1 << TM);
-    std::vector<c128> V(size_t(TT) * TN);
+    std::vector<std::vector<c128>> V(ns, std::vector<c128>(size_t(TT) * TN));
     u64 base = 0;
     for (int i = 0; i < sw.ncbits; ++i)
         if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
@@ -1234,74 +1523,90 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
     for (int t = 0; t < TT; ++t) {
         tdep[t] = seg_tdep(t, *lay);
         const u64 gth = base | seg_gthread(t, *lay);
-        for (int j = 0; j < TN; ++j) V[t * TN + j] = state[gth | slot_g(j, *lay)];
+        for (size_t q = 0; q < ns; ++q)
+            for (int j = 0; j < TN; ++j) V[q][t * TN + j] = states[q][gth | slot_g(j, *lay)];
     }
+    auto dense_host = [](int dense, const c128 *M, c128 *v) {
+        if (dense < C_DENSE2) {
+            const int a = dense - C_DENSE1;
+            for (int j = 0; j < TN; ++j) {
+                if (j & (1 << a)) continue;
+                c128 x = v[j], y = v[j | (1 << a)];
+                v[j] = cfma(M[2], y, cmul(M[0], x));
+                v[j | (1 << a)] = cfma(M[3], y, cmul(M[1], x));
+            }
+        } else if (dense < C_DENSE2 + 6) {
+            const int a = kPairA[dense - C_DENSE2], b = kPairB[dense - C_DENSE2];
+            for (int j = 0; j < TN; ++j) {
+                if (j & ((1 << a) | (1 << b))) continue;
+                int ii[4] = {j, j | (1 << a), j | (1 << b), j | (1 << a) | (1 << b)};
+                c128 in[4] = {v[ii[0]], v[ii[1]], v[ii[2]], v[ii[3]]};
+                for (int r = 0; r < 4; ++r) {
+                    c128 acc = cmul(M[r], in[0]);
+                    for (int c = 1; c < 4; ++c) acc = cfma(M[r + 4 * c], in[c], acc);
+                    v[ii[r]] = acc;
+                }
+            }
+        }
+    };
     for (int s = sw.seg0; s < sw.seg1; ++s) {
         const TSeg &sg = fp->h_segs[s];
         if (sg.transpose) {
             int a[TN];
-            std::vector<char> written(lds.size(), 0);
-            for (int t = 0; t < TT; ++t) {
-                lds_slots(t, sg.wt, sg.wr, a);
-                for (int j = 0; j < TN; ++j) {
-                    lds[a[j] / sizeof(c128)] = V[t * TN + j];
-                    written[a[j] / sizeof(c128)] = 1;
+            for (size_t q = 0; q < ns; ++q) {
+                std::vector<char> written(lds.size(), 0);
+                for (int t = 0; t < TT; ++t) {
+                    lds_slots(t, sg.wt, sg.wr, a);
+                    for (int j = 0; j < TN; ++j) {
+                        lds[a[j] / sizeof(c128)] = V[q][t * TN + j];
+                        written[a[j] / sizeof(c128)] = 1;
+                    }
+                }
+                if (std::count(written.begin(), written.end(), 1) != int(lds.size()))
+                    fprintf(stderr, "mossq: LDS slot map is not a bijection\n");
+                for (int t = 0; t < TT; ++t) {
+                    lds_slots(t, sg.rt, sg.rr, a);
+                    for (int j = 0; j < TN; ++j) V[q][t * TN + j] = lds[a[j] / sizeof(c128)];
                 }
             }
-            if (std::count(written.begin(), written.end(), 1) != int(lds.size()))
-                fprintf(stderr, "mossq: LDS slot map is not a bijection\n");
-            for (int t = 0; t < TT; ++t) {
-                tdep[t] = seg_tdep(t, sg);
-                lds_slots(t, sg.rt, sg.rr, a);
-                for (int j = 0; j < TN; ++j) V[t * TN + j] = lds[a[j] / sizeof(c128)];
-            }
+            for (int t = 0; t < TT; ++t) tdep[t] = seg_tdep(t, sg);
         }
-        // Executes the device encoding (groups, packed flips, diagonal records),
-        // as the kernel does.
+        // Executes the device encoding (groups, packed flips, diagonal records,
+        // probes), as the kernels do.
         for (int gi = sg.g0; gi < sg.g1; ++gi) {
             const TGroup &g = fp->h_groups[gi];
             const c128 *M = fp->h_pool.data() + g.moff;
             const int dense = g.code >= C_FLIP && g.code < C_COPY ? g.code - C_FLIP : g.code;
             for (int t = 0; t < TT; ++t) {
-                c128 *v = &V[t * TN];
-                if (dense < C_DENSE2) {
-                    const int a = dense - C_DENSE1;
-                    for (int j = 0; j < TN; ++j) {
-                        if (j & (1 << a)) continue;
-                        c128 x = v[j], y = v[j | (1 << a)];
-                        v[j] = cfma(M[2], y, cmul(M[0], x));
-                        v[j | (1 << a)] = cfma(M[3], y, cmul(M[1], x));
+                for (size_t q = 0; q < ns; ++q) {
+                    c128 *v = &V[q][t * TN];
+                    dense_host(dense, M, v);
+                    if (g.code >= C_FLIP && g.code <= C_COPY) {
+                        const unsigned F = group_flips(g, tdep[t], base);
+                        for (int j = 0; j < TN; ++j)
+                            if ((F >> j) & 1) v[j] = mk(-v[j].re, -v[j].im);
                     }
-                } else if (dense < C_DENSE2 + 6) {
-                    const int a = kPairA[dense - C_DENSE2], b = kPairB[dense - C_DENSE2];
-                    for (int j = 0; j < TN; ++j) {
-                        if (j & ((1 << a) | (1 << b))) continue;
-                        int ii[4] = {j, j | (1 << a), j | (1 << b), j | (1 << a) | (1 << b)};
-                        c128 in[4] = {v[ii[0]], v[ii[1]], v[ii[2]], v[ii[3]]};
-                        for (int r = 0; r < 4; ++r) {
-                            c128 acc = cmul(M[r], in[0]);
-                            for (int c = 1; c < 4; ++c) acc = cfma(M[r + 4 * c], in[c], acc);
-                            v[ii[r]] = acc;
+                    if (g.code == C_DIAG) {
+                        const TDiag &d = fp->h_diag[g.x];
+                        for (int j = 0; j < TN; ++j) {
+                            int idx = diag_f(d, tdep[t], base) | diag_rp(d, j);
+                            if (d.code == D_SIGN) {
+                                if ((d.negmask >> idx) & 1) v[j] = mk(-v[j].re, -v[j].im);
+                            } else if (d.code == D_SELECT) {
+                                v[j] = cmul(d.val[idx], v[j]);
+                            } else {
+                                v[j] = cmul(fp->h_pool[d.moff + idx], v[j]);
+                            }
                         }
                     }
                 }
-                if (g.code >= C_FLIP && g.code <= C_COPY) {
-                    const unsigned F = group_flips(g, tdep[t], base);
-                    for (int j = 0; j < TN; ++j)
-                        if ((F >> j) & 1) v[j] = mk(-v[j].re, -v[j].im);
-                }
-                if (g.code == C_DIAG) {
-                    const TDiag &d = fp->h_diag[g.x];
-                    for (int j = 0; j < TN; ++j) {
-                        int idx = diag_f(d, tdep[t], base) | diag_rp(d, j);
-                        if (d.code == D_SIGN) {
-                            if ((d.negmask >> idx) & 1) v[j] = mk(-v[j].re, -v[j].im);
-                        } else if (d.code == D_SELECT) {
-                            v[j] = cmul(d.val[idx], v[j]);
-                        } else {
-                            v[j] = cmul(fp->h_pool[d.moff + idx], v[j]);
-                        }
-                    }
+                if (g.probe >= 0 && ns == 2 && grads) {
+                    const c128 *y = &V[0][t * TN], *D = fp->h_pool.data() + g.pmoff;
+                    std::vector<c128> dl(V[1].begin() + t * TN, V[1].begin() + (t + 1) * TN);
+                    dense_host(g.pcode, D, dl.data());
+                    c128 acc = mk(0, 0);
+                    for (int j = 0; j < TN; ++j) acc = cfmaconj(y[j], dl[j], acc);
+                    grads[g.probe] = cadd(grads[g.probe], acc);
                 }
             }
         }
@@ -1309,7 +1614,8 @@ static void emulate_tile(c128 *state, const TSweepArgs &sw, const FusedPlan *fp,
     const TSeg &last = fp->h_segs[sw.seg1 - 1];
     for (int t = 0; t < TT; ++t) {
         const u64 gth = base | seg_gthread(t, last);
-        for (int j = 0; j < TN; ++j) state[gth | slot_g(j, last)] = V[t * TN + j];
+        for (size_t q = 0; q < ns; ++q)
+            for (int j = 0; j < TN; ++j) states[q][gth | slot_g(j, last)] = V[q][t * TN + j];
     }
 }
 
@@ -1327,7 +1633,31 @@ void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state) {
             continue;
         }
         const u64 ntiles = u64(1) << (p->nbits - TM);
-        for (u64 t = 0; t < ntiles; ++t) emulate_tile(state, it.args, fp, t);
+        for (u64 t = 0; t < ntiles; ++t) emulate_tile({state}, it.args, fp, t, nullptr);
+    }
+}
+
+// The adjoint plan's reverse pass on host vectors y, l; grads[j] receives
+// the sum of Re<y|D|l> over parameter j's probes (complex sums).
+void emulate_adjoint_plan(const Program *p, const FusedPlan *fp, c128 *y, c128 *l, c128 *grads) {
+    const c128 *P = p->h_pool.data();
+    const u64 len = u64(1) << p->nbits;
+    for (const PlanItem &it : fp->items) {
+        if (!it.sweep) {
+            const Step &s = p->steps[it.step];
+            host_block(y, p->nbits, s.kb, s.bits, P + s.rev);
+            host_block(l, p->nbits, s.kb, s.bits, P + s.rev);
+            if (s.grad >= 0) {
+                std::vector<c128> dl(l, l + len);
+                host_block(dl.data(), p->nbits, s.kb, s.bits, P + s.grad);
+                c128 acc = mk(0, 0);
+                for (u64 i = 0; i < len; ++i) acc = cfmaconj(y[i], dl[i], acc);
+                grads[s.grad_index] = cadd(grads[s.grad_index], acc);
+            }
+            continue;
+        }
+        const u64 ntiles = u64(1) << (p->nbits - TM);
+        for (u64 t = 0; t < ntiles; ++t) emulate_tile({y, l}, it.args, fp, t, grads);
     }
 }
 
@@ -1350,31 +1680,42 @@ int fused_plan_upload(Ctx *c, FusedPlan *fp) {
     return MOSSQ_OK;
 }
 
+// Forward plan, and for pure states the adjoint (reverse) plan.
 int fused_plan_build(Program *p) {
     if (p->nbits < TM) return MOSSQ_OK;  // too small to tile: run unfused
-    std::string err;
-    FusedPlan *fp = fused_plan_build_host(p, &err);
-    if (!fp) return fail(MOSSQ_EDEVICE, err);
-    p->fused = fp;
-    int s = fused_plan_upload(p->ctx, fp);
-    if (s != MOSSQ_OK) {
-        fused_plan_free(p);
-        return s;
+    for (int adj = 0; adj < (p->is_density ? 1 : 2); ++adj) {
+        std::string err;
+        FusedPlan *fp = fused_plan_build_host(p, &err, adj != 0);
+        if (!fp && adj) break;  // no adjoint plan: the reverse pass runs step by step
+        if (!fp) {
+            fused_plan_free(p);
+            return fail(MOSSQ_EDEVICE, err);
+        }
+        (adj ? p->fused_rev : p->fused) = fp;
+        int s = fused_plan_upload(p->ctx, fp);
+        if (s != MOSSQ_OK) {
+            fused_plan_free(p);
+            return s;
+        }
     }
     return MOSSQ_OK;
 }
 
-void fused_plan_free(Program *p) {
-    if (!p->fused) return;
-    FusedPlan *fp = p->fused;
-    if (p->ctx) {
-        if (fp->d_segs) dev_free(p->ctx, fp->d_segs);
-        if (fp->d_groups) dev_free(p->ctx, fp->d_groups);
-        if (fp->d_diag) dev_free(p->ctx, fp->d_diag);
-        if (fp->d_pool) dev_free(p->ctx, fp->d_pool);
+static void free_plan(Ctx *c, FusedPlan *&fp) {
+    if (!fp) return;
+    if (c) {
+        if (fp->d_segs) dev_free(c, fp->d_segs);
+        if (fp->d_groups) dev_free(c, fp->d_groups);
+        if (fp->d_diag) dev_free(c, fp->d_diag);
+        if (fp->d_pool) dev_free(c, fp->d_pool);
     }
     delete fp;
-    p->fused = nullptr;
+    fp = nullptr;
+}
+
+void fused_plan_free(Program *p) {
+    free_plan(p->ctx, p->fused);
+    free_plan(p->ctx, p->fused_rev);
 }
 
 int fused_plan_sweeps(const Program *p) {
@@ -1401,9 +1742,63 @@ int fused_run(Ctx *c, const Program *p, c128 *state) {
     return MOSSQ_OK;
 }
 
+int fused_adjoint_parts(const Program *p) {
+    if (!p->fused_rev) return 0;
+    return int(u64(4) << (p->nbits - TM));  // one partial per wave per tile
+}
+
+int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, int row) {
+    const FusedPlan *fp = p->fused_rev;
+    const unsigned ntiles = unsigned(u64(1) << (p->nbits - TM));
+    for (const PlanItem &it : fp->items) {
+        if (!it.sweep) {
+            MOSSQ_TRY(adjoint_rev_step(c, p, it.step, y, l, part, row));
+            continue;
+        }
+        hipLaunchKernelGGL(k_tile_adj, dim3(ntiles), dim3(TT), 0, c->stream, y, l, it.args, fp->d_segs,
+                           fp->d_groups, fp->d_diag, fp->d_pool, part, row);
+        MOSSQ_HIP(hipGetLastError());
+    }
+    return MOSSQ_OK;
+}
+
+__global__ void k_zero(c128 *__restrict__ v, u64 count) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += u64(gridDim.x) * blockDim.x)
+        v[i] = c128{0.0, 0.0};
+}
+
+int zero_fill(Ctx *c, c128 *v, u64 count) {
+    if (count == 0) return MOSSQ_OK;
+    const u64 blocks = std::min<u64>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero, dim3(unsigned(blocks)), dim3(256), 0, c->stream, v, count);
+    MOSSQ_HIP(hipGetLastError());
+    return MOSSQ_OK;
+}
+
 }  // namespace mossq
 
 using namespace mossq;
+
+static void plan_stats(const FusedPlan *fp, int *stats) {
+    int unfused = 0;
+    for (auto &it : fp->items)
+        if (!it.sweep) ++unfused;
+    stats[0] = fp->nsweeps;
+    stats[1] = unfused;
+    stats[2] = int(fp->h_segs.size());
+    stats[3] = fp->ntransitions;
+    stats[4] = int(fp->h_ops.size());
+    stats[5] = stats[6] = stats[7] = 0;
+    for (const TOp &o : fp->h_ops) {
+        if (o.type == T_DENSE1)
+            stats[5]++;
+        else if (o.type == T_DENSE2)
+            stats[6]++;
+        else if (o.type != T_PROBE1 && o.type != T_PROBE2)
+            stats[7]++;
+    }
+    stats[8] = fp->nconflicted;
+}
 
 extern "C" {
 
@@ -1414,20 +1809,27 @@ int mossq_plan_emulate(int n, int is_density, int nops, const mossq_op *ops, con
     MOSSQ_TRY(program_build_host(&p, n, is_density, nops, ops, pool, pool_len));
     MOSSQ_CHECK(p.nbits >= TM, MOSSQ_EINVAL, "plans tile registers of at least 12 bits");
     std::string err;
-    FusedPlan *fp = fused_plan_build_host(&p, &err);
+    FusedPlan *fp = fused_plan_build_host(&p, &err, false);
     MOSSQ_CHECK(fp, MOSSQ_EINVAL, err);
     if (state) emulate_plan(&p, fp, (c128 *)state);
-    int unfused = 0;
-    for (auto &it : fp->items)
-        if (!it.sweep) ++unfused;
-    stats[0] = fp->nsweeps;
-    stats[1] = unfused;
-    stats[2] = int(fp->h_segs.size());
-    stats[3] = fp->ntransitions;
-    stats[4] = int(fp->h_ops.size());
-    stats[5] = stats[6] = stats[7] = 0;
-    for (const TOp &o : fp->h_ops) stats[o.type == T_DENSE1 ? 5 : o.type == T_DENSE2 ? 6 : 7]++;
-    stats[8] = fp->nconflicted;
+    plan_stats(fp, stats);
+    delete fp;
+    return MOSSQ_OK;
+}
+
+int mossq_plan_emulate_adjoint(int n, int nops, const mossq_op *ops, const double *pool, uint64_t pool_len,
+                               double *y, double *l, double *grads, int *stats) {
+    MOSSQ_CHECK(stats && y && l && grads, MOSSQ_EINVAL, "null argument");
+    Program p;
+    MOSSQ_TRY(program_build_host(&p, n, 0, nops, ops, pool, pool_len));
+    MOSSQ_CHECK(p.nbits >= TM, MOSSQ_EINVAL, "plans tile registers of at least 12 bits");
+    std::string err;
+    FusedPlan *fp = fused_plan_build_host(&p, &err, true);
+    MOSSQ_CHECK(fp, MOSSQ_EINVAL, err);
+    std::vector<c128> g(std::max(p.nderiv, 1), mk(0, 0));
+    emulate_adjoint_plan(&p, fp, (c128 *)y, (c128 *)l, g.data());
+    for (int j = 0; j < p.nderiv; ++j) grads[j] = g[j].re;
+    plan_stats(fp, stats);
     delete fp;
     return MOSSQ_OK;
 }

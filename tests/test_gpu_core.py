@@ -230,6 +230,41 @@ def test_adjoint_c0(ctx):
     assert np.max(np.abs(g - f["grads"])) < GRAD
 
 
+def _mixed_param_circuit(n, layers, seed):
+    """Rotations of every kind, CZ and CNOT ladders, a controlled rotation and a
+    fixed two-qubit unitary per layer."""
+    rng = np.random.default_rng(seed)
+    c = mq.QCircuit()
+    for layer in range(layers):
+        for q in range(1, n + 1):
+            c.append([mq.RxGate, mq.RyGate, mq.RzGate, mq.PHASEGate][(q + layer) % 4](q, rng.uniform(0, 6.3),
+                                                                                    isparas=True))
+        for q in range(1 + layer % 2, n, 2):
+            c.append(mq.CZGate(q, q + 1) if layer % 3 else mq.CNOTGate(q, q + 1))
+        c.append(mq.CRyGate((1, n), rng.uniform(0, 6.3), isparas=True))
+        c.append(mq.QuantumGate((2, n - 1), haar_unitary(rng, 4)))
+    return c
+
+
+@pytest.mark.parametrize("n", [13, 16])
+def test_fused_adjoint(ctx, n):
+    """The fused reverse pass (adjoint tile kernel, gradient probes inside the
+    sweep) against the step-by-step reverse pass and the oracle."""
+    rng = np.random.default_rng(40 + n)
+    H = tfim(n)
+    hea = hea_circuit(n, 3)
+    circs = [(hea, rng.uniform(0, 2 * np.pi, hea.nparameters()))]
+    mix = _mixed_param_circuit(n, 3, n)
+    circs.append((mix, np.array(mix.active_parameters(), dtype=float)))
+    for tpl, th in circs:
+        lw = lower(tpl.bind(th), n, False)
+        want = lw.slot_gradients(oracle.adjoint(lw, H.groups())[1])
+        for fuse in (True, False):
+            obj = mq.ExpectationObjective(tpl, H, mq.StateVector.zeros(n), backend_=mq.HIPBackend(ctx, fuse=fuse))
+            v, g = mq.value_and_gradient(mq.AdjointGradient(), obj, th)
+            assert np.max(np.abs(g - want)) < GRAD, (fuse, n)
+
+
 def test_adjoint_density_golden(ctx):
     f = np.load(GOLD / "c2_noisy4.npz")
     n, L = 4, 2
