@@ -1059,13 +1059,23 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
         }
         it.args.ncbits = nc;
         // Schedule the chosen ops into register segments.
+        // Op ci waits for every earlier chosen op it does not commute with:
+        // pending[ci] counts those not yet scheduled.
         std::vector<char> sched(chosen.size(), 0);
-        size_t nsched = 0;
-        auto ready = [&](size_t ci) {
+        std::vector<std::vector<int>> succ(chosen.size());
+        std::vector<int> pending(chosen.size(), 0);
+        for (size_t ci = 0; ci < chosen.size(); ++ci)
             for (size_t pj = 0; pj < ci; ++pj)
-                if (!sched[pj] && !commutes(ops[chosen[pj]], ops[chosen[ci]])) return false;
-            return true;
+                if (!commutes(ops[chosen[pj]], ops[chosen[ci]])) {
+                    succ[pj].push_back(int(ci));
+                    pending[ci]++;
+                }
+        auto mark = [&](size_t ci, std::vector<char> &s, std::vector<int> &pend) {
+            s[ci] = 1;
+            for (int x : succ[ci]) pend[x]--;
         };
+        size_t nsched = 0;
+        auto ready = [&](size_t ci) { return pending[ci] == 0; };
         auto fits = [&](size_t ci, const int *reg) {
             const POp &o = ops[chosen[ci]];
             if (o.kind == 1) return true;
@@ -1078,17 +1088,14 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
         // Ops a segment with register set `reg` would schedule from the current state.
         auto simulate = [&](const int *reg) {
             std::vector<char> s2(sched);
+            std::vector<int> p2(pending);
             int count = 0;
             bool progress = true;
             while (progress) {
                 progress = false;
                 for (size_t ci = 0; ci < chosen.size(); ++ci) {
-                    if (s2[ci] || !fits(ci, reg)) continue;
-                    bool ok = true;
-                    for (size_t pj = 0; pj < ci && ok; ++pj)
-                        if (!s2[pj] && !commutes(ops[chosen[pj]], ops[chosen[ci]])) ok = false;
-                    if (!ok) continue;
-                    s2[ci] = 1;
+                    if (s2[ci] || p2[ci] || !fits(ci, reg)) continue;
+                    mark(ci, s2, p2);
                     count += ops[chosen[ci]].kind == 1 ? 1 : 2;
                     progress = true;
                 }
@@ -1157,7 +1164,7 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                 progress = false;
                 for (size_t ci = 0; ci < chosen.size(); ++ci) {
                     if (sched[ci] || !fits(ci, reg0) || !ready(ci)) continue;
-                    sched[ci] = 1;
+                    mark(ci, sched, pending);
                     ++nsched;
                     sp.idx.push_back(int(ci));
                     sp.work += ops[chosen[ci]].kind == 1 ? 1 : 2;
