@@ -331,7 +331,6 @@ def main():
         extra["unfused"] = {"ms_per_step": step_u, "gates_per_s": ngates / (step_u / 1e3),
                             "hbm_gbps": alg_bytes / (step_u / 1e3) / 1e9,
                             "gate1_avg_ms": g1_avg, "gate1_gbps": tile_bytes / (g1_avg / 1e3) / 1e9}
-            del a, b
         del sv_u
     norm = mq.dot(sv, sv).real
     extra["final_norm_minus_one"] = norm - 1.0
