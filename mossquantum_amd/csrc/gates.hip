@@ -24,6 +24,17 @@ static inline unsigned grid_for(u64 work, int items) {
     return unsigned(g);
 }
 
+// Streaming accesses (each amplitude is read and written once per sweep):
+// nontemporal hints keep the state from displacing cached data and measured
+// ~5% faster one-qubit sweeps on MI355X.
+__device__ __forceinline__ c128 load_nt(const c128 *p) {
+    return c128{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+}
+__device__ __forceinline__ void store_nt(c128 *p, c128 x) {
+    __builtin_nontemporal_store(x.re, &p->re);
+    __builtin_nontemporal_store(x.im, &p->im);
+}
+
 // ------------------------------------------------------------ 1-bit dense ----
 struct M2 {
     c128 m00, m10, m01, m11;  // column-major 2x2
@@ -41,8 +52,8 @@ __global__ __launch_bounds__(kBlock) void k_gate1(c128 *__restrict__ v, u64 npai
         u64 p = base + u64(j) * kBlock;
         idx[j] = ((p & ~lo_mask) << 1) | (p & lo_mask);
         if (p < npairs) {
-            a[j] = v[idx[j]];
-            b[j] = v[idx[j] + stride];
+            a[j] = load_nt(v + idx[j]);
+            b[j] = load_nt(v + idx[j] + stride);
         }
     }
 #pragma unroll
@@ -51,8 +62,8 @@ __global__ __launch_bounds__(kBlock) void k_gate1(c128 *__restrict__ v, u64 npai
         if (p < npairs) {
             c128 x = cfma(m.m01, b[j], cmul(m.m00, a[j]));
             c128 y = cfma(m.m11, b[j], cmul(m.m10, a[j]));
-            v[idx[j]] = x;
-            v[idx[j] + stride] = y;
+            store_nt(v + idx[j], x);
+            store_nt(v + idx[j] + stride, y);
         }
     }
 }
@@ -90,6 +101,20 @@ __global__ __launch_bounds__(kBlock) void k_dense(c128 *__restrict__ v, u64 nblo
             for (int c = 0; c < D; ++c) acc = cfma(sm[r + D * c], in[c], acc);
             v[base + off[r]] = acc;
         }
+    }
+}
+
+// One-bit dense sweep, one amplitude pair per thread (the widest grid streams
+// HBM best on MI355X: ~6 TB/s for an in-place read+write); the matrix comes
+// through the scalar cache, no LDS staging.
+__global__ __launch_bounds__(kBlock) void k_dense1(c128 *__restrict__ v, u64 npairs, int q,
+                                                    const c128 *__restrict__ mat) {
+    const c128 m00 = mat[0], m10 = mat[1], m01 = mat[2], m11 = mat[3];
+    for (u64 p = u64(blockIdx.x) * kBlock + threadIdx.x; p < npairs; p += u64(gridDim.x) * kBlock) {
+        const u64 i0 = insert0(p, q), i1 = i0 | (u64(1) << q);
+        const c128 a = load_nt(v + i0), b = load_nt(v + i1);
+        store_nt(v + i0, cfma(m01, b, cmul(m00, a)));
+        store_nt(v + i1, cfma(m11, b, cmul(m10, a)));
     }
 }
 
@@ -135,11 +160,12 @@ __global__ __launch_bounds__(kBlock) void k_monomial(c128 *__restrict__ v, u64 n
 
 // Specialisations for the common single-row-changed case (CZ, CPHASE, CCPHASE):
 // each thread touches exactly one amplitude, with the fixed bits set.
+// Launched with one amplitude per thread (the loop runs once).
 __global__ __launch_bounds__(kBlock) void k_phase1(c128 *__restrict__ v, u64 nblocks, Bits bits, int k,
                                                     u64 set_mask, c128 val) {
     for (u64 t = u64(blockIdx.x) * kBlock + threadIdx.x; t < nblocks; t += u64(gridDim.x) * kBlock) {
-        u64 i = insert0s_dyn(t, bits.b, k) | set_mask;
-        v[i] = cmul(val, v[i]);
+        const u64 i = insert0s_dyn(t, bits.b, k) | set_mask;
+        store_nt(v + i, cmul(val, load_nt(v + i)));
     }
 }
 
@@ -186,7 +212,7 @@ static int launch_mono(Ctx *c, c128 *v, int nbits, int k, const int *bits, const
         u64 set = 0;
         for (int j = 0; j < k; ++j)
             if (info.rows[0] & (1 << j)) set |= u64(1) << bits[j];
-        hipLaunchKernelGGL(k_phase1, dim3(grid_for(nblocks, 4)), dim3(kBlock), 0, c->stream, v, nblocks, b,
+        hipLaunchKernelGGL(k_phase1, dim3(grid_for(nblocks, 1)), dim3(kBlock), 0, c->stream, v, nblocks, b,
                            k, set, info.val[0]);
     } else {
         Mono m{};
@@ -224,7 +250,8 @@ int launch_dense(Ctx *c, c128 *v, int nbits, int k, const int *bits, const c128 
     unsigned grid = grid_for(nblocks, 2);
     switch (k) {
         case 1:
-            hipLaunchKernelGGL(k_dense<1>, dim3(grid), dim3(kBlock), 0, c->stream, v, nblocks, b, dmat);
+            hipLaunchKernelGGL(k_dense1, dim3(grid_for(nblocks, 1)), dim3(kBlock), 0, c->stream, v, nblocks, bits[0],
+                               dmat);
             break;
         case 2:
             hipLaunchKernelGGL(k_dense<2>, dim3(grid), dim3(kBlock), 0, c->stream, v, nblocks, b, dmat);

@@ -328,7 +328,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
 #pragma unroll
             for (int i = 0; i < TRB; ++i)
                 if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            v[j] = state[g | o];
+            v[j] = c128{__builtin_nontemporal_load(&state[g | o].re), __builtin_nontemporal_load(&state[g | o].im)};
         }
     }
     for (int s = sw.seg0; s < sw.seg1; ++s) {
@@ -362,7 +362,8 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
 #pragma unroll
             for (int i = 0; i < TRB; ++i)
                 if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            state[g | o] = v[j];
+            __builtin_nontemporal_store(v[j].re, &state[g | o].re);
+            __builtin_nontemporal_store(v[j].im, &state[g | o].im);
         }
     }
 }
