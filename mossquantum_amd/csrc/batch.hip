@@ -55,7 +55,7 @@ __device__ __forceinline__ void bgate_chunk(const BItemDev &it, const BOp &op, b
         if (K == 2) base = insert0(base, op.b1);
         c128 x[D], y[D];
 #pragma unroll
-        for (int q = 0; q < D; ++q) x[q] = psi[base + off[q]];
+        for (int q = 0; q < D; ++q) x[q] = load_nt(psi + base + off[q]);
 #pragma unroll
         for (int r = 0; r < D; ++r) {
             c128 a = mk(0, 0);
@@ -64,11 +64,11 @@ __device__ __forceinline__ void bgate_chunk(const BItemDev &it, const BOp &op, b
             y[r] = a;
         }
 #pragma unroll
-        for (int r = 0; r < D; ++r) psi[base + off[r]] = y[r];
+        for (int r = 0; r < D; ++r) store_nt(psi + base + off[r], y[r]);
         if (!fwd) {
             c128 l[D], lo2[D];
 #pragma unroll
-            for (int q = 0; q < D; ++q) l[q] = lam[base + off[q]];
+            for (int q = 0; q < D; ++q) l[q] = load_nt(lam + base + off[q]);
 #pragma unroll
             for (int r = 0; r < D; ++r) {
                 c128 a = mk(0, 0);
@@ -77,7 +77,7 @@ __device__ __forceinline__ void bgate_chunk(const BItemDev &it, const BOp &op, b
                 lo2[r] = a;
             }
 #pragma unroll
-            for (int r = 0; r < D; ++r) lam[base + off[r]] = lo2[r];
+            for (int r = 0; r < D; ++r) store_nt(lam + base + off[r], lo2[r]);
             if (grad) {
                 // g = Re <psi|D|lambda> on the updated vectors (differentiation/circuit.jl:136-144)
 #pragma unroll

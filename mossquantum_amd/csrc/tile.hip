@@ -328,7 +328,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
 #pragma unroll
             for (int i = 0; i < TRB; ++i)
                 if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            v[j] = c128{__builtin_nontemporal_load(&state[g | o].re), __builtin_nontemporal_load(&state[g | o].im)};
+            v[j] = load_nt(state + (g | o));
         }
     }
     for (int s = sw.seg0; s < sw.seg1; ++s) {
@@ -362,8 +362,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
 #pragma unroll
             for (int i = 0; i < TRB; ++i)
                 if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            __builtin_nontemporal_store(v[j].re, &state[g | o].re);
-            __builtin_nontemporal_store(v[j].im, &state[g | o].im);
+            store_nt(state + (g | o), v[j]);
         }
     }
 }
@@ -521,8 +520,8 @@ __global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 
 #pragma unroll
             for (int i = 0; i < TRB; ++i)
                 if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            y[j] = ys[g | o];
-            l[j] = ls[g | o];
+            y[j] = load_nt(ys + (g | o));
+            l[j] = load_nt(ls + (g | o));
         }
     }
     const int slot = int(tile) * 4 + (t >> 6);
@@ -554,8 +553,8 @@ __global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 
 #pragma unroll
             for (int i = 0; i < TRB; ++i)
                 if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            ys[g | o] = y[j];
-            ls[g | o] = l[j];
+            store_nt(ys + (g | o), y[j]);
+            store_nt(ls + (g | o), l[j]);
         }
     }
 }
