@@ -6,9 +6,14 @@
 // a pinned staging ring, and starts them at the current iteration; a request
 // retires when its reverse pass ends, its results are copied back
 // asynchronously and its memory returns to the cache.  States stay resident on
-// the device for the whole request.  Density-matrix requests (and state-vector
-// programs with gates wider than two qubits) run on a worker thread with its own
-// context, so their kernels overlap the engine's on the device.
+// the device for the whole request.  Density-matrix requests, state-vector
+// programs with gates wider than two qubits, and state-vector requests of at
+// least kFusedMinQubits qubits run on worker threads, each with its own
+// context (stream), so their kernels overlap the engine's and each other's on
+// the device.  The fused state-vector requests have workers of their own, so
+// they never queue behind the longer density-matrix work.  The large state-vector requests use the fused adjoint (LDS-tile
+// sweeps forward and, with gradient probes, in reverse), whose plans the worker
+// builds off the submitting thread.
 //
 // New capability: the reference evaluates one objective at a time
 // (src/differentiation/objectives.jl:183-218).
@@ -30,6 +35,9 @@ constexpr int kMaxActive = 4096;     // engine items resident at once
 constexpr int kMaxGrads = 1024;      // gradient slots per engine item
 constexpr int kInflight = 8;         // engine iterations queued ahead of the device
 constexpr u64 kRingBytes = 64ull << 20;
+constexpr int kWorkers = 4;          // worker threads (contexts) beside the engine
+constexpr int kFusedWorkers = 2;     // of which serve the fused state-vector queue
+constexpr int kFusedMinQubits = 18;  // state-vector requests this wide take the fused adjoint
 
 struct SRequest {
     i64 ticket = 0;
@@ -41,6 +49,7 @@ struct SRequest {
     u64 bytes = 0;  // reserved against the budget
     // engine state
     bool engine = false;
+    bool fused = false;  // worker request run with the fused adjoint
     int slot = -1;  // engine item slot (values / grads arena)
     int t0 = 0, nfwd = 0, nchunk = 1;
     u64 vec_bytes = 0, pool_bytes = 0, ops_bytes = 0;
@@ -139,7 +148,7 @@ struct PinnedRing {
 
 struct Server {
     Ctx *ctx = nullptr;
-    Ctx *wctx = nullptr;  // worker context (density requests)
+    std::vector<Ctx *> wctxs;  // worker contexts
     u64 budget = 0;
     std::atomic<u64> reserved{0};
     std::mutex mu;
@@ -166,9 +175,10 @@ struct Server {
     // retiring engine items (results copy in flight)
     std::vector<SRequest *> draining;
     std::vector<i64> completed;  // finished tickets not yet reported
-    // worker
-    std::thread worker;
-    std::deque<SRequest *> wqueue;
+    // workers
+    std::vector<std::thread> workers;
+    std::deque<SRequest *> wqueue;  // density-matrix and wide-gate requests
+    std::deque<SRequest *> fqueue;  // fused state-vector requests
     std::condition_variable wcv;
     bool stop = false;
     int worker_busy = 0;
@@ -180,25 +190,29 @@ static int build_request_program(SRequest *r) {
                               (const double *)r->pool.data(), r->pool.size());
 }
 
-static void worker_main(Server *s) {
-    (void)hipSetDevice(s->wctx->device);
+static void worker_main(Server *s, int wi) {
+    Ctx *wc = s->wctxs[wi];
+    (void)hipSetDevice(wc->device);
+    std::deque<SRequest *> &q = wi < kFusedWorkers ? s->fqueue : s->wqueue;
     while (true) {
         SRequest *r = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->wcv.wait(lk, [&] { return s->stop || !s->wqueue.empty(); });
-            if (s->stop && s->wqueue.empty()) return;
-            r = s->wqueue.front();
-            s->wqueue.pop_front();
+            s->wcv.wait(lk, [&] { return s->stop || !q.empty(); });
+            if (s->stop && q.empty()) return;
+            r = q.front();
+            q.pop_front();
             s->worker_busy++;
         }
-        int st = program_upload(s->wctx, &r->prog);
+        int st = program_upload(wc, &r->prog);
+        if (st == MOSSQ_OK && r->fused) st = fused_plan_build(&r->prog);
         r->grads.assign(std::max(r->prog.nderiv, 1), 0.0);
         if (st == MOSSQ_OK)
             st = r->is_density
-                     ? adjoint_density(s->wctx, &r->prog, r->obs, nullptr, 0, nullptr, &r->value, r->grads.data())
-                     : adjoint_pure(s->wctx, &r->prog, r->obs, nullptr, nullptr, &r->value, r->grads.data());
-        if (r->prog.d_pool) dev_free(s->wctx, r->prog.d_pool);
+                     ? adjoint_density(wc, &r->prog, r->obs, nullptr, 0, nullptr, &r->value, r->grads.data())
+                     : adjoint_pure(wc, &r->prog, r->obs, nullptr, nullptr, &r->value, r->grads.data());
+        fused_plan_free(&r->prog);
+        if (r->prog.d_pool) dev_free(wc, r->prog.d_pool);
         r->prog.d_pool = nullptr;
         r->grads.resize(r->prog.nderiv);
         {
@@ -371,8 +385,8 @@ static int server_step(Server *s, int *active, int *queued) {
             s->reserved += r->bytes;
             if (!r->engine) {
                 r->state = 1;
-                s->wqueue.push_back(r);
-                s->wcv.notify_one();
+                (r->fused ? s->fqueue : s->wqueue).push_back(r);
+                s->wcv.notify_all();
                 continue;
             }
         }
@@ -398,7 +412,9 @@ static int server_step(Server *s, int *active, int *queued) {
         s->t++;
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    if (active) *active = int(s->active.size() + s->draining.size()) + int(s->wqueue.size()) + s->worker_busy;
+    if (active)
+        *active = int(s->active.size() + s->draining.size()) + int(s->wqueue.size() + s->fqueue.size()) +
+                  s->worker_busy;
     if (queued) *queued = int(s->queue.size());
     return MOSSQ_OK;
 }
@@ -416,14 +432,13 @@ int mossq_server_create(mossq_ctx *ctx, uint64_t budget_bytes, mossq_server **ou
     s->ctx = C(ctx);
     s->budget = budget_bytes;
     s->cache.c = s->ctx;
-    mossq_ctx *w = nullptr;
-    int st = mossq_ctx_create(s->ctx->device, 0, &w);
-    if (st != MOSSQ_OK) {
-        delete s;
-        return st;
+    int st = MOSSQ_OK;
+    for (int i = 0; i < kWorkers && st == MOSSQ_OK; ++i) {
+        mossq_ctx *w = nullptr;
+        st = mossq_ctx_create(s->ctx->device, 0, &w);
+        if (st == MOSSQ_OK) s->wctxs.push_back(C(w));
     }
-    s->wctx = C(w);
-    st = s->ring.init(kRingBytes);
+    if (st == MOSSQ_OK) st = s->ring.init(kRingBytes);
     for (int k = 0; st == MOSSQ_OK && k < 2; ++k) {
         st = dev_alloc(s->ctx, kMaxActive * sizeof(BItemDev), (void **)&s->d_items[k]);
         if (st == MOSSQ_OK && hipHostMalloc((void **)&s->h_items[k], kMaxActive * sizeof(BItemDev)) != hipSuccess)
@@ -439,7 +454,7 @@ int mossq_server_create(mossq_ctx *ctx, uint64_t budget_bytes, mossq_server **ou
         return st;
     }
     for (int i = kMaxActive - 1; i >= 0; --i) s->free_slots.push_back(i);
-    s->worker = std::thread(worker_main, s);
+    for (int i = 0; i < kWorkers; ++i) s->workers.push_back(std::thread(worker_main, s, i));
     *out = reinterpret_cast<mossq_server *>(s);
     return MOSSQ_OK;
 }
@@ -447,14 +462,13 @@ int mossq_server_create(mossq_ctx *ctx, uint64_t budget_bytes, mossq_server **ou
 int mossq_server_destroy(mossq_server *srv) {
     if (!srv) return MOSSQ_OK;
     Server *s = reinterpret_cast<Server *>(srv);
-    if (s->worker.joinable()) {
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->stop = true;
-        }
-        s->wcv.notify_all();
-        s->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
     }
+    s->wcv.notify_all();
+    for (auto &w : s->workers)
+        if (w.joinable()) w.join();
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     for (auto &kv : s->requests) {
@@ -475,7 +489,7 @@ int mossq_server_destroy(mossq_server *srv) {
     if (s->d_grads) dev_free(s->ctx, s->d_grads);
     if (s->h_results) (void)hipHostFree(s->h_results);
     s->ring.destroy();
-    if (s->wctx) mossq_ctx_destroy(reinterpret_cast<mossq_ctx *>(s->wctx));
+    for (Ctx *w : s->wctxs) mossq_ctx_destroy(reinterpret_cast<mossq_ctx *>(w));
     delete s;
     return MOSSQ_OK;
 }
@@ -502,9 +516,15 @@ int mossq_server_submit(mossq_server *srv, int n, int is_density, int nops, cons
     if (engine_ok)
         for (const Step &x : r->prog.steps) engine_ok &= (!x.split && x.kb <= 2);
     engine_ok = engine_ok && r->prog.nderiv <= kMaxGrads;
-    r->engine = engine_ok;
-    if (engine_ok) {
+    r->fused = !r->is_density && n >= kFusedMinQubits;
+    r->engine = engine_ok && !r->fused;
+    if (r->engine) {
         r->bytes = engine_bytes(r);
+    } else if (r->fused) {
+        // two states, gradient partials (one per wave per tile, or per pair-kernel block), plans
+        const u64 row = std::max<u64>(u64(pair_parts(n)), u64(4) << (n - 12));
+        r->bytes = 2 * (u64(1) << n) * sizeof(c128) + u64(std::max(r->prog.nderiv, 1)) * row * sizeof(c128) +
+                   u64(o->G) * 512 * sizeof(c128) + 4 * r->pool.size() * sizeof(c128) + (8ull << 20);
     } else if (r->is_density) {
         r->bytes = adjoint_density_bytes(&r->prog, o) + r->pool.size() * sizeof(c128);
     } else {

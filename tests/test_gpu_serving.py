@@ -29,7 +29,9 @@ def make_requests(ctx, specs, seed):
     return reqs
 
 
-SPECS = [("sv", 8, 2), ("sv", 13, 2), ("dm", 4, 1), ("sv", 10, 3), ("sv", 15, 1), ("dm", 5, 1), ("sv", 9, 2)]
+# 18 qubits takes the fused-adjoint worker path; the rest the engine or the density worker.
+SPECS = [("sv", 8, 2), ("sv", 13, 2), ("dm", 4, 1), ("sv", 10, 3), ("sv", 15, 1), ("dm", 5, 1), ("sv", 9, 2),
+         ("sv", 18, 2)]
 
 
 def test_server_alone_vs_mix(ctx):
