@@ -25,7 +25,8 @@ struct Step {
     MatInfo ifwd_cols, irev_cols;
 };
 
-struct FusedPlan;  // LDS-tile sweep plan (tile.hip)
+struct FusedPlan;           // LDS-tile sweep plan (tile.hip)
+struct DensityAdjointPlan;  // chunk plans of the density-matrix adjoint (tile.hip)
 
 struct Program {
     Ctx *ctx = nullptr;
@@ -39,6 +40,7 @@ struct Program {
     c128 *d_pool = nullptr;
     FusedPlan *fused = nullptr;
     FusedPlan *fused_rev = nullptr;  // pure-state adjoint reverse pass (with gradient probes)
+    DensityAdjointPlan *dm_adj = nullptr;
 };
 
 // Host side of mossq_program_create: validate records, build steps and pool.
@@ -47,6 +49,8 @@ int program_build_host(Program *p, int n, int is_density, int nops, const mossq_
 
 // Unfused forward application of steps [lo, hi).
 int run_steps(Ctx *c, const Program *p, c128 *state, int lo, int hi);
+// Step i on v alone with its forward or reverse matrices (program.hip).
+int apply_step(Ctx *c, const Program *p, int i, c128 *v, bool reverse);
 
 // Fused plans (tile.hip).
 int fused_plan_build(Program *p);
@@ -62,6 +66,20 @@ int fused_adjoint_parts(const Program *p);
 int adjoint_rev_step(Ctx *c, const Program *p, int i, c128 *y, c128 *l, c128 *part, int row);
 // Gradient partials per row for p's pure-state adjoint.
 int adjoint_part_row(const Program *p);
+// Fused density-matrix adjoint (tile.hip): chunk plans cut at the
+// parameterized steps, built by fused_plan_build for density programs with
+// parameters and at least 12 bits.  layout gives the checkpoint segment
+// length seg and count nseg (0 without a plan); forward runs the chunks on
+// work (the input on entry, the final state on exit) keeping nseg
+// checkpoints; reverse takes Delta (the seed at the end) down to the first
+// parameterized step with seg - 1 recompute buffers, writing gradient
+// partials into rows of `row` entries.
+int density_adjoint_plan_build(Program *p);
+void density_adjoint_plan_free(Program *p);
+void density_adjoint_layout(const Program *p, int *seg, int *nseg);
+int density_adjoint_forward(Ctx *c, const Program *p, c128 *work, c128 *const *ckpt);
+int density_adjoint_reverse(Ctx *c, const Program *p, c128 *delta, c128 *const *ckpt, c128 *const *stored,
+                            c128 *part, int row);
 // Zero `count` complex entries with a kernel (safe inside captured graphs).
 int zero_fill(Ctx *c, c128 *v, u64 count);
 

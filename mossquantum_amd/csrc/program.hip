@@ -88,6 +88,10 @@ static int apply_step_rev(Ctx *c, const Program *p, const Step &s, c128 *v) {
     return launch_dense(c, v, p->nbits, s.kb, s.bits, p->d_pool + s.rev, s.irev);
 }
 
+int apply_step(Ctx *c, const Program *p, int i, c128 *v, bool reverse) {
+    return reverse ? apply_step_rev(c, p, p->steps[i], v) : apply_step_fwd(c, p, p->steps[i], v);
+}
+
 // One reverse step of the pure-state adjoint on (y, l): U^dagger on both and,
 // for a parameterized step, the partials of Re<y|D|l> into row grad_index.
 int adjoint_rev_step(Ctx *c, const Program *p, int i, c128 *y, c128 *l, c128 *part, int row) {
@@ -185,6 +189,11 @@ u64 adjoint_density_bytes(const Program *p, const Obs *o) {
     const int N = int(p->steps.size());
     int seg = std::max(1, int(std::ceil(std::sqrt(double(std::max(N, 1))))));
     int nseg = std::max(1, (N + seg - 1) / seg);
+    if (p->dm_adj) {
+        // fused chunks: nseg checkpoints and seg - 1 recompute buffers
+        density_adjoint_layout(p, &seg, &nseg);
+        seg -= 1;
+    }
     u64 parts = std::max<u64>(u64(o->G) * 64, u64(std::max(p->nderiv, 1)) * pair_parts(p->nbits));
     return bytes * u64(2 + nseg + seg) + (parts + o->G + p->nderiv + 1) * sizeof(c128);
 }
@@ -199,8 +208,50 @@ int program_upload(Ctx *c, Program *p) {
     return MOSSQ_OK;
 }
 
+// The fused variant: forward and recompute run as chunk plans cut at the
+// parameterized steps, so only the states before parameterized steps are kept
+// and every chunk is one or a few LDS-tile sweeps (tile.hip).
+static int adjoint_density_fused(Ctx *c, const Program *p, const Obs *o, const c128 *in, int in_is_density,
+                                 c128 *out_state, double *value, double *grads) {
+    const int n = p->n, nb = p->nbits;
+    const u64 len = u64(1) << nb, bytes = len * sizeof(c128);
+    int seg = 0, nseg = 0;
+    density_adjoint_layout(p, &seg, &nseg);
+    Allocs al(c);
+    c128 *delta = nullptr, *work = nullptr;
+    std::vector<c128 *> ckpt(nseg), stored(std::max(seg - 1, 1));
+    MOSSQ_TRY(al.get(bytes, (void **)&delta));
+    MOSSQ_TRY(al.get(bytes, (void **)&work));
+    for (auto &q : ckpt) MOSSQ_TRY(al.get(bytes, (void **)&q));
+    for (int i = 0; i + 1 < seg; ++i) MOSSQ_TRY(al.get(bytes, (void **)&stored[i]));
+    const int P = pair_parts(nb);
+    c128 *part = nullptr, *value_dev = nullptr, *grads_dev = nullptr;
+    u64 nparts = std::max<u64>(u64(o->G) * 64, u64(std::max(p->nderiv, 1)) * P);
+    MOSSQ_TRY(al.get(nparts * sizeof(c128), (void **)&part));
+    MOSSQ_TRY(al.get(u64(o->G) * sizeof(c128), (void **)&value_dev));
+    MOSSQ_TRY(al.get(u64(std::max(p->nderiv, 1)) * sizeof(c128), (void **)&grads_dev));
+
+    if (!in)
+        MOSSQ_TRY(fill_basis0(c, work, len));
+    else if (in_is_density)
+        MOSSQ_HIP(hipMemcpyAsync(work, in, bytes, hipMemcpyDeviceToDevice, c->stream));
+    else
+        MOSSQ_TRY(outer_product(c, work, in, n));
+    MOSSQ_TRY(density_adjoint_forward(c, p, work, ckpt.data()));
+    if (out_state) MOSSQ_HIP(hipMemcpyAsync(out_state, work, bytes, hipMemcpyDeviceToDevice, c->stream));
+    MOSSQ_TRY(dm_expect_groups_dev(c, o, work, value_dev, part));
+    // seed: z * H^dagger as a dense density matrix (differentiation/expectation.jl:69-72)
+    hipLaunchKernelGGL(k_fill_identity, dim3(4096), dim3(kB), 0, c->stream, work, n);
+    MOSSQ_HIP(hipGetLastError());
+    MOSSQ_TRY(apply_op_dev(c, o, work, delta, nb, OpMats::Adjoint));
+    MOSSQ_TRY(density_adjoint_reverse(c, p, delta, ckpt.data(), stored.data(), part, P));
+    MOSSQ_TRY(reduce_rows(c, part, p->nderiv, P, grads_dev));
+    return copy_results(c, p, o, value_dev, grads_dev, value, grads);
+}
+
 int adjoint_density(Ctx *c, const Program *p, const Obs *o, const c128 *in, int in_is_density,
                            c128 *out_state, double *value, double *grads) {
+    if (p->dm_adj) return adjoint_density_fused(c, p, o, in, in_is_density, out_state, value, grads);
     const int n = p->n, nb = p->nbits;
     const u64 len = u64(1) << nb, bytes = len * sizeof(c128);
     const int N = int(p->steps.size());

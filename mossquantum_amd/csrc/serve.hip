@@ -11,9 +11,11 @@
 // least kFusedMinQubits qubits run on worker threads, each with its own
 // context (stream), so their kernels overlap the engine's and each other's on
 // the device.  The fused state-vector requests have workers of their own, so
-// they never queue behind the longer density-matrix work.  The large state-vector requests use the fused adjoint (LDS-tile
-// sweeps forward and, with gradient probes, in reverse), whose plans the worker
-// builds off the submitting thread.
+// they never queue behind the longer density-matrix work.  The large
+// state-vector requests use the fused adjoint (LDS-tile sweeps forward and,
+// with gradient probes, in reverse); density-matrix requests of 6 or more
+// qubits use the fused density adjoint (chunk plans cut at the parameterized
+// steps).  Workers build these plans off the submitting thread.
 //
 // New capability: the reference evaluates one objective at a time
 // (src/differentiation/objectives.jl:183-218).
@@ -206,6 +208,7 @@ static void worker_main(Server *s, int wi) {
         }
         int st = program_upload(wc, &r->prog);
         if (st == MOSSQ_OK && r->fused) st = fused_plan_build(&r->prog);
+        if (st == MOSSQ_OK && r->is_density) st = density_adjoint_plan_build(&r->prog);
         r->grads.assign(std::max(r->prog.nderiv, 1), 0.0);
         if (st == MOSSQ_OK)
             st = r->is_density

@@ -24,6 +24,7 @@
 // emulate_plan() executes a plan on the host with the same index arithmetic,
 // so the planner is tested on a CPU against the oracle.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "mossq_program.h"
@@ -306,7 +307,9 @@ __device__ __forceinline__ void group_op(const TGroup &g, const TDiag *__restric
     }
 }
 
-__global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweepArgs sw,
+// Reads the tile from src and writes it to dst (src == dst for an in-place
+// sweep; a workgroup stores only after it has loaded its whole tile).
+__global__ __launch_bounds__(TT, 2) void k_tile(const c128 *src, c128 *dst, TSweepArgs sw,
                                                   const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
                                                   const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
     __shared__ c128 lds[1 << TM];
@@ -328,7 +331,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
 #pragma unroll
             for (int i = 0; i < TRB; ++i)
                 if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            v[j] = load_nt(state + (g | o));
+            v[j] = load_nt(src + (g | o));
         }
     }
     for (int s = sw.seg0; s < sw.seg1; ++s) {
@@ -362,7 +365,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile(c128 *__restrict__ state, TSweep
 #pragma unroll
             for (int i = 0; i < TRB; ++i)
                 if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            store_nt(state + (g | o), v[j]);
+            store_nt(dst + (g | o), v[j]);
         }
     }
 }
@@ -594,6 +597,8 @@ struct FusedPlan {
     TDiag *d_diag = nullptr;
     c128 *d_pool = nullptr;
     bool adjoint = false;  // reverse plan of the pure-state adjoint (probes, no general diagonals)
+    bool reverse_steps = false;  // unfused items apply the steps' reverse matrices
+    bool shared_device = false;  // device arrays live in a block owned elsewhere
     int nsweeps = 0;
     int ntransitions = 0;
     int nconflicted = 0;  // transitions without a conflict-free LDS slot map
@@ -616,19 +621,23 @@ static std::vector<c128> mat_of(const Program *p, int off, int dim) {
     return std::vector<c128>(p->h_pool.begin() + off, p->h_pool.begin() + off + size_t(dim) * dim);
 }
 
-// Lower the program's forward steps to plan operations.
-static void lower_steps(const Program *p, std::vector<POp> &out) {
-    for (size_t si = 0; si < p->steps.size(); ++si) {
+// Lower steps [lo, hi) to plan operations: in program order with the forward
+// matrices, or (reverse) from hi-1 down to lo with the reverse matrices
+// (U^dagger, S^dagger), as the density adjoint's chunk plans run them.
+static void lower_range(const Program *p, int lo, int hi, bool reverse, std::vector<POp> &out) {
+    for (int q = 0; q < hi - lo; ++q) {
+        const int si = reverse ? hi - 1 - q : lo + q;
         const Step &st = p->steps[si];
+        const MatInfo &info = reverse ? st.irev : st.ifwd;
         const int dim = 1 << st.kb;
-        if (!st.split && is_diag(st.ifwd) && st.kb <= 4) {
-            if (st.ifwd.nchanged == 0) continue;  // identity
+        if (!st.split && is_diag(info) && st.kb <= 4) {
+            if (info.nchanged == 0) continue;  // identity
             POp d;
             d.kind = 1;
             d.nb = st.kb;
             for (int i = 0; i < st.kb; ++i) d.bits[i] = st.bits[i];
-            d.m = diag_values(st.ifwd, dim);
-            d.step = int(si);
+            d.m = diag_values(info, dim);
+            d.step = si;
             out.push_back(d);
             continue;
         }
@@ -637,15 +646,18 @@ static void lower_steps(const Program *p, std::vector<POp> &out) {
             d.kind = 0;
             d.nb = st.kb;
             for (int i = 0; i < st.kb; ++i) d.bits[i] = st.bits[i];
-            d.m = mat_of(p, st.fwd, dim);
-            d.step = int(si);
+            d.m = mat_of(p, reverse ? st.rev : st.fwd, dim);
+            d.step = si;
             out.push_back(d);
             continue;
         }
         if (p->is_density && !st.split && !st.channel && st.k == 2 && st.u_src >= 0) {
-            // U(x)conj(U) = (U on the row bits) then (conj(U) on the column bits)
-            std::vector<c128> U = mat_of(p, st.u_src, 4), Uc(16);
-            for (int q = 0; q < 16; ++q) Uc[q] = cconj(U[q]);
+            // U(x)conj(U) = (U on the row bits) then (conj(U) on the column bits);
+            // the reverse superoperator takes U^dagger in place of U.
+            std::vector<c128> U = mat_of(p, st.u_src, 4), Ur(16), Uc(16);
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c) Ur[r + 4 * c] = reverse ? cconj(U[c + 4 * r]) : U[r + 4 * c];
+            for (int q2 = 0; q2 < 16; ++q2) Uc[q2] = cconj(Ur[q2]);
             POp r, cl;
             r.kind = cl.kind = 0;
             r.nb = cl.nb = 2;
@@ -653,19 +665,21 @@ static void lower_steps(const Program *p, std::vector<POp> &out) {
             r.bits[1] = st.bits[1];
             cl.bits[0] = st.bits[2];
             cl.bits[1] = st.bits[3];
-            r.m = U;
+            r.m = Ur;
             cl.m = Uc;
-            r.step = cl.step = int(si);
+            r.step = cl.step = si;
             out.push_back(r);
             out.push_back(cl);
             continue;
         }
         POp u;
         u.kind = 2;
-        u.step = int(si);
+        u.step = si;
         out.push_back(u);
     }
 }
+
+static void lower_steps(const Program *p, std::vector<POp> &out) { lower_range(p, 0, int(p->steps.size()), false, out); }
 
 static bool sign_only(const std::vector<c128> &d, int *negmask);
 
@@ -1455,12 +1469,7 @@ static double plan_cost(const FusedPlan *fp) {
 // The tail threshold trades a transposition against the ops it carries into
 // the next sweep; which side wins depends on the circuit, so each candidate is
 // planned and the cheapest plan under the cost model is kept.
-FusedPlan *fused_plan_build_host(const Program *p, std::string *err, bool adjoint) {
-    std::vector<POp> ops;
-    if (adjoint)
-        lower_steps_rev(p, ops);
-    else
-        lower_steps(p, ops);
+static FusedPlan *best_plan(const Program *p, const std::vector<POp> &ops, bool adjoint, std::string *err) {
     if (const char *mt = getenv("MOSSQ_PLAN_MIN_TAIL")) return plan_with_tail(p, ops, atoi(mt), adjoint, err);
     FusedPlan *best = nullptr;
     double best_cost = 0;
@@ -1480,6 +1489,15 @@ FusedPlan *fused_plan_build_host(const Program *p, std::string *err, bool adjoin
         }
     }
     return best;
+}
+
+FusedPlan *fused_plan_build_host(const Program *p, std::string *err, bool adjoint) {
+    std::vector<POp> ops;
+    if (adjoint)
+        lower_steps_rev(p, ops);
+    else
+        lower_steps(p, ops);
+    return best_plan(p, ops, adjoint, err);
 }
 
 // Host execution of a plan with the kernel's index arithmetic (tiles, layouts,
@@ -1631,11 +1649,12 @@ void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state) {
         if (!it.sweep) {
             const Step &s = p->steps[it.step];
             const c128 *P = p->h_pool.data();
+            const bool r = fp->reverse_steps;
             if (s.split) {
-                host_block(state, p->nbits, s.k, s.bits, P + s.fwd);
-                host_block(state, p->nbits, s.k, s.bits + s.k, P + s.fwd_cols);
+                host_block(state, p->nbits, s.k, s.bits, P + (r ? s.rev : s.fwd));
+                host_block(state, p->nbits, s.k, s.bits + s.k, P + (r ? s.rev_cols : s.fwd_cols));
             } else {
-                host_block(state, p->nbits, s.kb, s.bits, P + s.fwd);
+                host_block(state, p->nbits, s.kb, s.bits, P + (r ? s.rev : s.fwd));
             }
             continue;
         }
@@ -1690,6 +1709,10 @@ int fused_plan_upload(Ctx *c, FusedPlan *fp) {
 // Forward plan, and for pure states the adjoint (reverse) plan.
 int fused_plan_build(Program *p) {
     if (p->nbits < TM) return MOSSQ_OK;  // too small to tile: run unfused
+    if (p->is_density && p->nderiv > 0) {
+        int s = density_adjoint_plan_build(p);
+        if (s != MOSSQ_OK) return s;
+    }
     for (int adj = 0; adj < (p->is_density ? 1 : 2); ++adj) {
         std::string err;
         FusedPlan *fp = fused_plan_build_host(p, &err, adj != 0);
@@ -1710,7 +1733,7 @@ int fused_plan_build(Program *p) {
 
 static void free_plan(Ctx *c, FusedPlan *&fp) {
     if (!fp) return;
-    if (c) {
+    if (c && !fp->shared_device) {
         if (fp->d_segs) dev_free(c, fp->d_segs);
         if (fp->d_groups) dev_free(c, fp->d_groups);
         if (fp->d_diag) dev_free(c, fp->d_diag);
@@ -1723,6 +1746,7 @@ static void free_plan(Ctx *c, FusedPlan *&fp) {
 void fused_plan_free(Program *p) {
     free_plan(p->ctx, p->fused);
     free_plan(p->ctx, p->fused_rev);
+    density_adjoint_plan_free(p);
 }
 
 int fused_plan_sweeps(const Program *p) {
@@ -1733,20 +1757,32 @@ int fused_plan_sweeps(const Program *p) {
     return p->fused->nsweeps + unfused;
 }
 
-int fused_run(Ctx *c, const Program *p, c128 *state) {
-    const FusedPlan *fp = p->fused;
-    if (!fp) return run_steps(c, p, state, 0, int(p->steps.size()));
+// Runs plan fp on dst; the first pass reads src (src may equal dst).  A null
+// plan stands for no steps.
+static int fused_run_plan(Ctx *c, const Program *p, const FusedPlan *fp, const c128 *src, c128 *dst) {
+    const u64 bytes = (u64(1) << p->nbits) * sizeof(c128);
     const unsigned ntiles = unsigned(u64(1) << (p->nbits - TM));
-    for (const PlanItem &it : fp->items) {
-        if (!it.sweep) {
-            MOSSQ_TRY(run_steps(c, p, state, it.step, it.step + 1));
-            continue;
+    const c128 *from = src;
+    if (fp)
+        for (const PlanItem &it : fp->items) {
+            if (!it.sweep) {
+                if (from != dst) MOSSQ_HIP(hipMemcpyAsync(dst, from, bytes, hipMemcpyDeviceToDevice, c->stream));
+                from = dst;
+                MOSSQ_TRY(apply_step(c, p, it.step, dst, fp->reverse_steps));
+                continue;
+            }
+            hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(TT), 0, c->stream, from, dst, it.args, fp->d_segs,
+                               fp->d_groups, fp->d_diag, fp->d_pool);
+            MOSSQ_HIP(hipGetLastError());
+            from = dst;
         }
-        hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(TT), 0, c->stream, state, it.args, fp->d_segs, fp->d_groups, fp->d_diag,
-                           fp->d_pool);
-        MOSSQ_HIP(hipGetLastError());
-    }
+    if (from != dst) MOSSQ_HIP(hipMemcpyAsync(dst, from, bytes, hipMemcpyDeviceToDevice, c->stream));
     return MOSSQ_OK;
+}
+
+int fused_run(Ctx *c, const Program *p, c128 *state) {
+    if (!p->fused) return run_steps(c, p, state, 0, int(p->steps.size()));
+    return fused_run_plan(c, p, p->fused, state, state);
 }
 
 int fused_adjoint_parts(const Program *p) {
@@ -1782,6 +1818,206 @@ int zero_fill(Ctx *c, c128 *v, u64 count) {
     return MOSSQ_OK;
 }
 
+
+// ----------------------------------------------- fused density adjoint ----
+// Channels have no inverse, so the density-matrix reverse pass needs the state
+// before every parameterized step.  The program is cut at its parameterized
+// steps p_1 < ... < p_m:
+//   forward chunk j (j = 0..m): steps [b_j, b_{j+1}) with b_0 = 0, b_j = p_j,
+//     b_{m+1} = N; it maps S_j (the state before p_j, S_0 the input) to S_{j+1};
+//   reverse chunk j (j = 1..m): steps (p_j, b_{j+1}) from last to first with the
+//     reverse matrices S^dagger and U^dagger (.) U.
+// Every chunk is an LDS-tile plan of its own.  The forward pass keeps S_j at
+// j = 1, 1 + seg, 1 + 2 seg, ... (seg = ceil(sqrt(m))); the reverse pass
+// recomputes one segment of S_j at a time from its checkpoint, then per
+// parameter runs the reverse chunk on Delta and the paired update
+// Delta <- V Delta with Re<V Delta | G | S_j> (differentiation/circuit.jl:90-172).
+struct DensityAdjointPlan {
+    std::vector<int> psteps;       // p_1..p_m
+    std::vector<FusedPlan *> fwd;  // m + 1 chunks (null: no steps)
+    std::vector<FusedPlan *> rev;  // 1..m (rev[0] unused)
+    int seg = 1, nseg = 1;
+    void *d_block = nullptr;  // device arrays of every chunk plan
+};
+
+static FusedPlan *plan_range(const Program *p, int lo, int hi, bool reverse, std::string *err, bool *ok) {
+    *ok = true;
+    std::vector<POp> ops;
+    lower_range(p, lo, hi, reverse, ops);
+    if (ops.empty()) return nullptr;
+    FusedPlan *fp = best_plan(p, ops, false, err);
+    if (!fp) {
+        *ok = false;
+        return nullptr;
+    }
+    fp->reverse_steps = reverse;
+    return fp;
+}
+
+static void dm_plan_delete(DensityAdjointPlan *dp) {
+    for (auto *fp : dp->fwd) delete fp;
+    for (auto *fp : dp->rev) delete fp;
+    delete dp;
+}
+
+static DensityAdjointPlan *dm_plan_build_host(const Program *p, std::string *err) {
+    auto *dp = new DensityAdjointPlan();
+    const int N = int(p->steps.size());
+    for (int i = 0; i < N; ++i)
+        if (p->steps[i].grad >= 0) dp->psteps.push_back(i);
+    const int m = int(dp->psteps.size());
+    auto b = [&](int j) { return j == 0 ? 0 : j <= m ? dp->psteps[j - 1] : N; };
+    dp->fwd.assign(m + 1, nullptr);
+    dp->rev.assign(m + 1, nullptr);
+    bool ok = true;
+    for (int j = 0; j <= m && ok; ++j) dp->fwd[j] = plan_range(p, b(j), b(j + 1), false, err, &ok);
+    for (int j = 1; j <= m && ok; ++j) dp->rev[j] = plan_range(p, dp->psteps[j - 1] + 1, b(j + 1), true, err, &ok);
+    if (!ok) {
+        dm_plan_delete(dp);
+        return nullptr;
+    }
+    dp->seg = std::max(1, int(std::ceil(std::sqrt(double(std::max(m, 1))))));
+    dp->nseg = std::max(1, (m + dp->seg - 1) / dp->seg);
+    return dp;
+}
+
+// Forward chunks: S_j lives in ckpt[(j-1)/seg] when (j-1) % seg == 0 and
+// 1 <= j <= m, else in work; S_0 is in work on entry and S_{m+1} on exit.
+template <class Run>
+static int dm_forward_sched(const DensityAdjointPlan *dp, c128 *work, c128 *const *ckpt, Run run) {
+    const int m = int(dp->psteps.size()), seg = dp->seg;
+    auto loc = [&](int j) { return j >= 1 && j <= m && (j - 1) % seg == 0 ? ckpt[(j - 1) / seg] : work; };
+    for (int j = 0; j <= m; ++j) MOSSQ_TRY(run(dp->fwd[j], loc(j), loc(j + 1)));
+    return MOSSQ_OK;
+}
+
+// Reverse pass from Delta at the program's end down to p_1; stored holds
+// seg - 1 states.
+template <class Run, class Pair>
+static int dm_reverse_sched(const DensityAdjointPlan *dp, c128 *delta, c128 *const *ckpt, c128 *const *stored,
+                            Run run, Pair pair) {
+    const int m = int(dp->psteps.size()), seg = dp->seg;
+    for (int k = dp->nseg - 1; k >= 0; --k) {
+        const int a = 1 + k * seg, e = std::min(m, (k + 1) * seg);
+        auto S = [&](int j) { return j == a ? ckpt[k] : stored[j - a - 1]; };
+        for (int j = a + 1; j <= e; ++j) MOSSQ_TRY(run(dp->fwd[j - 1], S(j - 1), S(j)));
+        for (int j = e; j >= a; --j) {
+            MOSSQ_TRY(run(dp->rev[j], delta, delta));
+            MOSSQ_TRY(pair(dp->psteps[j - 1], delta, S(j)));
+        }
+    }
+    return MOSSQ_OK;
+}
+
+// One device block for the arrays of every chunk plan (one allocation and one
+// copy per program).
+static int upload_plan_block(Ctx *c, const std::vector<FusedPlan *> &plans, void **block) {
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t total = 0;
+    for (const FusedPlan *fp : plans)
+        total += al(fp->h_segs.size() * sizeof(TSeg)) + al(fp->h_groups.size() * sizeof(TGroup)) +
+                 al(fp->h_diag.size() * sizeof(TDiag)) + al(fp->h_pool.size() * sizeof(c128));
+    if (total == 0) return MOSSQ_OK;
+    std::vector<char> h(total);
+    MOSSQ_TRY(dev_alloc(c, total, block));
+    char *d = static_cast<char *>(*block);
+    size_t off = 0;
+    auto put = [&](const void *src, size_t bytes) {
+        std::memcpy(h.data() + off, src, bytes);
+        char *at = d + off;
+        off += al(bytes);
+        return at;
+    };
+    for (FusedPlan *fp : plans) {
+        fp->d_segs = reinterpret_cast<TSeg *>(put(fp->h_segs.data(), fp->h_segs.size() * sizeof(TSeg)));
+        fp->d_groups = reinterpret_cast<TGroup *>(put(fp->h_groups.data(), fp->h_groups.size() * sizeof(TGroup)));
+        fp->d_diag = reinterpret_cast<TDiag *>(put(fp->h_diag.data(), fp->h_diag.size() * sizeof(TDiag)));
+        fp->d_pool = reinterpret_cast<c128 *>(put(fp->h_pool.data(), fp->h_pool.size() * sizeof(c128)));
+        fp->shared_device = true;
+    }
+    MOSSQ_HIP(hipMemcpyAsync(d, h.data(), total, hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    return MOSSQ_OK;
+}
+
+int density_adjoint_plan_build(Program *p) {
+    if (!p->is_density || p->nderiv == 0 || p->nbits < TM || p->dm_adj) return MOSSQ_OK;
+    std::string err;
+    DensityAdjointPlan *dp = dm_plan_build_host(p, &err);
+    if (!dp) return MOSSQ_OK;  // no chunk plans: the reverse pass runs step by step
+    std::vector<FusedPlan *> plans;
+    for (auto *fp : dp->fwd)
+        if (fp) plans.push_back(fp);
+    for (auto *fp : dp->rev)
+        if (fp) plans.push_back(fp);
+    int s = upload_plan_block(p->ctx, plans, &dp->d_block);
+    if (s != MOSSQ_OK) {
+        if (dp->d_block) dev_free(p->ctx, dp->d_block);
+        dm_plan_delete(dp);
+        return s;
+    }
+    p->dm_adj = dp;
+    return MOSSQ_OK;
+}
+
+void density_adjoint_plan_free(Program *p) {
+    if (!p->dm_adj) return;
+    if (p->ctx && p->dm_adj->d_block) dev_free(p->ctx, p->dm_adj->d_block);
+    dm_plan_delete(p->dm_adj);
+    p->dm_adj = nullptr;
+}
+
+void density_adjoint_layout(const Program *p, int *seg, int *nseg) {
+    *seg = p->dm_adj ? p->dm_adj->seg : 0;
+    *nseg = p->dm_adj ? p->dm_adj->nseg : 0;
+}
+
+int density_adjoint_forward(Ctx *c, const Program *p, c128 *work, c128 *const *ckpt) {
+    return dm_forward_sched(p->dm_adj, work, ckpt, [&](const FusedPlan *fp, c128 *src, c128 *dst) {
+        return fused_run_plan(c, p, fp, src, dst);
+    });
+}
+
+int density_adjoint_reverse(Ctx *c, const Program *p, c128 *delta, c128 *const *ckpt, c128 *const *stored,
+                            c128 *part, int row) {
+    return dm_reverse_sched(
+        p->dm_adj, delta, ckpt, stored,
+        [&](const FusedPlan *fp, c128 *src, c128 *dst) { return fused_run_plan(c, p, fp, src, dst); },
+        [&](int si, c128 *d, c128 *S) {
+            const Step &st = p->steps[si];
+            return launch_dense_pair(c, d, S, p->nbits, st.kb, st.bits, p->d_pool + st.rev, p->d_pool + st.grad,
+                                     part + size_t(st.grad_index) * row, true);
+        });
+}
+
+// Host execution of the fused density adjoint with the same schedule:
+// rho holds S_0 on entry and the final state on exit; delta holds the seed at
+// the program's end on entry; grads[j] receives parameter j's complex sum.
+static void emulate_density_adjoint(const Program *p, const DensityAdjointPlan *dp, c128 *rho, c128 *delta,
+                                    c128 *grads) {
+    const u64 len = u64(1) << p->nbits;
+    std::vector<std::vector<c128>> ck(dp->nseg, std::vector<c128>(len)), sto(dp->seg - 1, std::vector<c128>(len));
+    std::vector<c128 *> ckpt, stored;
+    for (auto &v : ck) ckpt.push_back(v.data());
+    for (auto &v : sto) stored.push_back(v.data());
+    auto run = [&](const FusedPlan *fp, c128 *src, c128 *dst) {
+        if (src != dst) std::memcpy(dst, src, len * sizeof(c128));
+        if (fp) emulate_plan(p, fp, dst);
+        return MOSSQ_OK;
+    };
+    dm_forward_sched(dp, rho, ckpt.data(), run);
+    const c128 *P = p->h_pool.data();
+    dm_reverse_sched(dp, delta, ckpt.data(), stored.data(), run, [&](int si, c128 *d, c128 *S) {
+        const Step &st = p->steps[si];
+        host_block(d, p->nbits, st.kb, st.bits, P + st.rev);
+        std::vector<c128> dy(S, S + len);
+        host_block(dy.data(), p->nbits, st.kb, st.bits, P + st.grad);
+        c128 acc = mk(0, 0);
+        for (u64 i = 0; i < len; ++i) acc = cfmaconj(d[i], dy[i], acc);
+        grads[st.grad_index] = cadd(grads[st.grad_index], acc);
+        return MOSSQ_OK;
+    });
+}
 }  // namespace mossq
 
 using namespace mossq;
@@ -1838,6 +2074,38 @@ int mossq_plan_emulate_adjoint(int n, int nops, const mossq_op *ops, const doubl
     for (int j = 0; j < p.nderiv; ++j) grads[j] = g[j].re;
     plan_stats(fp, stats);
     delete fp;
+    return MOSSQ_OK;
+}
+
+int mossq_plan_emulate_density_adjoint(int n, int nops, const mossq_op *ops, const double *pool, uint64_t pool_len,
+                                       double *rho, double *delta, double *grads, int *stats) {
+    MOSSQ_CHECK(stats && rho && delta && grads, MOSSQ_EINVAL, "null argument");
+    Program p;
+    MOSSQ_TRY(program_build_host(&p, n, 1, nops, ops, pool, pool_len));
+    MOSSQ_CHECK(p.nbits >= TM, MOSSQ_EINVAL, "plans tile registers of at least 12 bits");
+    MOSSQ_CHECK(p.nderiv > 0, MOSSQ_EINVAL, "the program has no parameterized steps");
+    std::string err;
+    DensityAdjointPlan *dp = dm_plan_build_host(&p, &err);
+    MOSSQ_CHECK(dp, MOSSQ_EINVAL, err);
+    std::vector<c128> g(p.nderiv, mk(0, 0));
+    emulate_density_adjoint(&p, dp, (c128 *)rho, (c128 *)delta, g.data());
+    for (int j = 0; j < p.nderiv; ++j) grads[j] = g[j].re;
+    for (int i = 0; i < 6; ++i) stats[i] = 0;
+    stats[0] = int(dp->psteps.size());
+    stats[1] = dp->seg;
+    for (const FusedPlan *fp : dp->fwd)
+        if (fp) {
+            stats[2] += fp->nsweeps;
+            for (auto &it : fp->items) stats[4] += !it.sweep;
+            stats[5]++;
+        }
+    for (const FusedPlan *fp : dp->rev)
+        if (fp) {
+            stats[3] += fp->nsweeps;
+            for (auto &it : fp->items) stats[4] += !it.sweep;
+            stats[5]++;
+        }
+    dm_plan_delete(dp);
     return MOSSQ_OK;
 }
 

@@ -265,6 +265,44 @@ def test_fused_adjoint(ctx, n):
             assert np.max(np.abs(g - want)) < GRAD, (fuse, n)
 
 
+def _noisy_param_circuit(n, layers, seed):
+    """Noisy HEA plus a parameterized two-qubit gate, a two-qubit channel, a
+    dense two-qubit gate and a three-qubit gate (run unfused)."""
+    rng = np.random.default_rng(seed)
+    c = hea_circuit(n, layers, noise=(0.01, 0.02))
+    c.append(mq.CRyGate((1, n), rng.uniform(0, 6.3), isparas=True))
+    c.append(mq.Depolarizing((2, 3), 0.05))
+    c.append(mq.QuantumGate((2, n - 1), haar_unitary(rng, 4)))
+    c.append(mq.RxGate(3, rng.uniform(0, 6.3), isparas=True))
+    c.append(mq.QuantumGate((1, 3, n), haar_unitary(rng, 8)))
+    c.append(mq.AmplitudeDamping(n, 0.1))
+    c.append(mq.RzGate(n, rng.uniform(0, 6.3), isparas=True))
+    return c
+
+
+@pytest.mark.parametrize("n", [6, 7])
+def test_fused_density_adjoint(ctx, n):
+    """The fused density adjoint (chunk plans cut at the parameterized gates,
+    checkpoint recompute through LDS-tile sweeps) against the step-by-step
+    density adjoint and the oracle, from |0><0| and from a pure input."""
+    rng = np.random.default_rng(60 + n)
+    H = tfim(n)
+    tpl = _noisy_param_circuit(n, 3, n)
+    th = np.array(tpl.active_parameters(), dtype=float)
+    th[: 6 * n] = rng.uniform(0, 2 * np.pi, 6 * n)
+    lw = lower(tpl.bind(th), n, True)
+    want_v, want_g = oracle.adjoint(lw, H.groups())
+    want = lw.slot_gradients(want_g)
+    for fuse in (True, False):
+        obj = mq.ExpectationObjective(tpl, H, mq.DensityMatrix.zeros(n), backend_=mq.HIPBackend(ctx, fuse=fuse))
+        v, g = mq.value_and_gradient(mq.AdjointGradient(), obj, th)
+        assert abs(v - want_v) < EXP, fuse
+        assert np.max(np.abs(g - want)) < GRAD, (fuse, n)
+    obj = mq.ExpectationObjective(tpl, H, mq.StateVector.zeros(n), backend_=mq.HIPBackend(ctx, fuse=True))
+    v2, g2 = mq.value_and_gradient(mq.AdjointGradient(), obj, th)
+    assert abs(v2 - want_v) < EXP and np.max(np.abs(g2 - want)) < GRAD
+
+
 def test_adjoint_density_golden(ctx):
     f = np.load(GOLD / "c2_noisy4.npz")
     n, L = 4, 2

@@ -29,9 +29,10 @@ def make_requests(ctx, specs, seed):
     return reqs
 
 
-# 18 qubits takes the fused-adjoint worker path; the rest the engine or the density worker.
+# 18 qubits takes the fused-adjoint worker path, a 6-qubit density request the fused
+# density adjoint; the rest the engine or the density worker.
 SPECS = [("sv", 8, 2), ("sv", 13, 2), ("dm", 4, 1), ("sv", 10, 3), ("sv", 15, 1), ("dm", 5, 1), ("sv", 9, 2),
-         ("sv", 18, 2)]
+         ("sv", 18, 2), ("dm", 6, 2)]
 
 
 def test_server_alone_vs_mix(ctx):

@@ -125,3 +125,87 @@ def test_adjoint_plan_matches_oracle(n, layers):
         e0 = np.zeros(1 << n)
         e0[0] = 1
         assert np.max(np.abs(y1 - e0)) < 1e-12
+
+
+def dense_operator(n, H):
+    """H as a dense matrix, column by column from the oracle's apply_op."""
+    dim = 1 << n
+    M = np.zeros((dim, dim), dtype=np.complex128)
+    for j in range(dim):
+        e = np.zeros(dim, dtype=np.complex128)
+        e[j] = 1
+        M[:, j] = oracle.apply_op(e, n, H.groups())
+    return M
+
+
+def noisy_param_circuit(n, layers, seed):
+    """The noisy HEA plus a parameterized two-qubit gate, a two-qubit channel
+    (unfused), a dense two-qubit gate (row/column split) and a three-qubit gate
+    (unfused row and column passes)."""
+    rng = np.random.default_rng(seed)
+    c = hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers), noise=(0.01, 0.02))
+    c.append(mq.CRyGate((1, n), rng.uniform(0, 6.3), isparas=True))
+    c.append(mq.Depolarizing((2, 3), 0.05))
+    c.append(mq.QuantumGate((2, n - 1), haar_unitary(rng, 4)))
+    c.append(mq.RxGate(3, rng.uniform(0, 6.3), isparas=True))
+    c.append(mq.QuantumGate((1, 3, n), haar_unitary(rng, 8)))
+    c.append(mq.CZGate(2, n))
+    c.append(mq.AmplitudeDamping(n, 0.1))
+    c.append(mq.RzGate(n, rng.uniform(0, 6.3), isparas=True))
+    c.append(mq.Depolarizing(1, 0.03))
+    return c
+
+
+@pytest.mark.parametrize("n,layers", [(6, 2), (7, 3)])
+def test_density_adjoint_plan_matches_oracle(n, layers):
+    """The fused density adjoint (chunk plans cut at the parameterized gates,
+    checkpoints every `segment` parameters, recompute per segment) gives the
+    oracle's gradients and final state, from |0><0| and from a mixed input."""
+    from mossquantum_amd.backend import plan_emulate_density_adjoint
+    from mossquantum_amd.workloads import tfim
+    H = tfim(n)
+    Hd = dense_operator(n, H)
+    seed = Hd.conj().T.reshape(-1, order="F")  # vec(H^dagger)
+    rng = np.random.default_rng(10 + n)
+    A = rng.standard_normal((1 << n, 1 << n)) + 1j * rng.standard_normal((1 << n, 1 << n))
+    mixed = A @ A.conj().T
+    mixed /= np.trace(mixed)
+    zero = np.zeros((1 << n, 1 << n), dtype=np.complex128)
+    zero[0, 0] = 1
+    circ = noisy_param_circuit(n, layers, n)
+    lw = lower(circ, n, True)
+    for rho0 in (zero, mixed):
+        r0 = rho0.reshape(-1, order="F")
+        _, want, want_state = oracle.adjoint(lw, H.groups(), state_in=r0, want_state=True)
+        rho, _, g, st = plan_emulate_density_adjoint(circ, n, r0, seed)
+        assert st["param_steps"] == lw.nderiv and st["chunk_plans"] >= lw.nderiv
+        assert st["unfused"] >= 2
+        assert np.max(np.abs(rho - want_state)) < AMP
+        assert np.max(np.abs(g - want)) < 1e-9
+
+
+def test_density_adjoint_plan_checkpoint_segments():
+    """Segment boundaries of the checkpoint schedule: one parameter, a
+    parameter first and last, adjacent parameters (empty reverse chunks)."""
+    from mossquantum_amd.backend import plan_emulate_density_adjoint
+    from mossquantum_amd.workloads import tfim
+    n = 6
+    H = tfim(n)
+    seed = dense_operator(n, H).conj().T.reshape(-1, order="F")
+    zero = np.zeros(1 << (2 * n), dtype=np.complex128)
+    zero[0] = 1
+    rng = np.random.default_rng(5)
+    cases = [
+        [mq.HGate(1), mq.RyGate(2, 0.4, isparas=True), mq.Depolarizing(2, 0.1), mq.CNOTGate(2, 3)],
+        [mq.RyGate(1, 0.7, isparas=True), mq.RzGate(1, 0.2, isparas=True), mq.AmplitudeDamping(1, 0.2),
+         mq.CNOTGate(1, 2), mq.RxGate(2, 1.1, isparas=True)],
+        [mq.RyGate(q, rng.uniform(0, 6.3), isparas=True) for q in range(1, n + 1)] * 3 + [mq.Depolarizing(3, 0.2)],
+    ]
+    for ops in cases:
+        circ = mq.QCircuit(ops)
+        lw = lower(circ, n, True)
+        _, want, want_state = oracle.adjoint(lw, H.groups(), state_in=zero, want_state=True)
+        rho, _, g, st = plan_emulate_density_adjoint(circ, n, zero, seed)
+        assert st["param_steps"] == lw.nderiv
+        assert np.max(np.abs(rho - want_state)) < AMP
+        assert np.max(np.abs(g - want)) < 1e-9
