@@ -150,6 +150,11 @@ int mossq_dm_trace(mossq_ctx *ctx, const void *rho, int n, double *out);
  * src/execution/probabilities.jl:42-87). is_density selects the diagonal of rho. */
 int mossq_probabilities(mossq_ctx *ctx, const void *state, int n, int is_density, int k,
                         const int32_t *qubits, double *out);
+/* Bounds of the diagonal of a density matrix of n qubits: out[0] = max |Im rho_ii|,
+ * out[1] = min Re rho_ii, out[2] = 1 when every diagonal entry is finite, else 0.
+ * The host checks them against the reference's tolerance before taking a
+ * marginal (src/execution/probabilities.jl:55-64,91-103). */
+int mossq_dm_diag_bounds(mossq_ctx *ctx, const void *rho, int n, double *out);
 
 /* ---- programs (lowered circuits) ----------------------------------------- */
 /* Lower a circuit once: matrices are uploaded and the launch plan is built.
@@ -221,6 +226,16 @@ int mossq_adjoint(mossq_ctx *ctx, const mossq_program *prog, const mossq_obs *ob
 int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs,
                         mossq_obs *const *obs, const int64_t *grad_offset, double *values,
                         double *grads);
+
+/* Values Re <H_b> of B independent state-vector programs in one engine run
+ * (forward and observe steps only), item b starting from init[b] (a device
+ * vector of 2^n_b amplitudes) or from |0...0> when init or init[b] is NULL.
+ * Serves the sampled-gradient methods, whose shifted or perturbed circuits
+ * are evaluated together (reference ParameterShift / FiniteDifference /
+ * SPSAGradient, src/differentiation/sampled_gradients.jl:323-406, which run
+ * one objective evaluation at a time). */
+int mossq_batch_expectation(mossq_ctx *ctx, int B, mossq_program *const *progs, mossq_obs *const *obs,
+                            const void *const *init, double *values);
 
 /* ---- request server (continuous batching) ----------------------------------
  * New capability (the reference evaluates one objective at a time,

@@ -88,6 +88,7 @@ SIGNATURES = [
     ("mossq_dm_apply_operator", _I, [_P, _P, _P, _P]),
     ("mossq_dot", _I, [_P, _P, _P, _U64, _DP]),
     ("mossq_dm_trace", _I, [_P, _P, _I, _DP]),
+    ("mossq_dm_diag_bounds", _I, [_P, _P, _I, _DP]),
     ("mossq_probabilities", _I, [_P, _P, _I, _I, _I, _IP, _DP]),
     ("mossq_program_create", _I, [_P, _I, _I, _I, C.POINTER(OpRecord), _DP, _U64, _I, C.POINTER(_P)]),
     ("mossq_program_destroy", _I, [_P]),
@@ -98,6 +99,7 @@ SIGNATURES = [
     ("mossq_plan_emulate_density_adjoint", _I,
      [_I, _I, C.POINTER(OpRecord), _DP, _U64, _DP, _DP, _DP, C.POINTER(C.c_int)]),
     ("mossq_adjoint", _I, [_P, _P, _P, _P, _I, _P, _DP, _DP]),
+    ("mossq_batch_expectation", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _DP]),
     ("mossq_batch_adjoint", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _DP, _DP]),
     ("mossq_server_create", _I, [_P, _U64, C.POINTER(_P)]),
     ("mossq_server_destroy", _I, [_P]),

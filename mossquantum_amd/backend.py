@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from ._lib import check
 from .channels import QuantumMap
-from .circuit import QCircuit, lower
+from .circuit import Delay, QCircuit, lower
 from .gates import Gate
 from .operators import QubitsOperator, QubitsTerm
 from .states import BatchDensityMatrix, BatchStateVector, DensityMatrix, HIPArray, StateVector
@@ -81,6 +81,8 @@ def supports(b: HIPBackend, op, state, action: str = "apply") -> bool:
             return op.nqubits() <= MAX_GATE_QUBITS and max(op.positions()) <= state.n
         if isinstance(op, QuantumMap):
             return op.nqubits() <= MAX_CHANNEL_QUBITS and max(op.positions()) <= state.n
+        if isinstance(op, Delay):
+            return op.position <= state.n
         if isinstance(op, QCircuit):
             return all(supports(b, o, state, "apply") for o in op.flat())
         if isinstance(op, (QubitsOperator, QubitsTerm)):
@@ -133,9 +135,13 @@ class DeviceObservable:
 class Program:
     """A lowered circuit on the device (mossq_program)."""
 
-    def __init__(self, circuit: QCircuit, n: int, is_density: bool, ctx: _lib.Context, fuse: bool = False):
-        self.lowered = lower(circuit, n, is_density)
-        recs, pool = self.lowered.arrays()
+    def __init__(self, circuit: QCircuit | None, n: int, is_density: bool, ctx: _lib.Context, fuse: bool = False,
+                 lowered=None, pool=None):
+        """From a circuit, or from an existing lowering with an optional
+        replacement pool (same records, e.g. rebound parameters)."""
+        self.lowered = lowered if lowered is not None else lower(circuit, n, is_density)
+        recs, base_pool = self.lowered.arrays()
+        pool = base_pool if pool is None else np.ascontiguousarray(pool, dtype=np.complex128)
         self.ctx = ctx
         self.n = n
         self.is_density = is_density
@@ -249,6 +255,8 @@ def apply_(op, state, b: HIPBackend | None = None):
         keep, mp = _mat_ptr(op.ordered_supermat())
         check(ctx.lib.mossq_dm_apply_channel(ctx.h, state.storage.ptr, state.n, len(pos), _lib.iptr(pos), mp, 0))
         return state
+    if isinstance(op, Delay):
+        return state  # idle: the state is unchanged (noise_models.jl:44)
     if isinstance(op, QCircuit):
         flat = list(op.flat())
         if isinstance(state, StateVector) and any(isinstance(o, QuantumMap) for o in flat):
@@ -337,7 +345,8 @@ def _normalize(values, tol):
         raise ValueError("probabilities must be nonnegative")
     values = np.maximum(values, 0)
     total = values.sum()
-    if not np.isclose(total, 1, atol=tol, rtol=tol):
+    # Julia isapprox(total, 1; atol, rtol): |total - 1| <= max(atol, rtol * max(|total|, 1))
+    if not abs(total - 1) <= max(tol, tol * max(abs(total), 1.0)):
         raise ValueError(f"state probabilities must sum to one; normalize the state first (sum = {total})")
     return values / total
 
@@ -354,8 +363,24 @@ def probabilities(state, qubits=None, b: HIPBackend | None = None):
     if len(set(qubits)) != len(qubits) or not all(1 <= q <= state.n for q in qubits):
         raise ValueError("readout qubits must be distinct and in range")
     ctx = state.storage.ctx
-    q = np.array([x - 1 for x in qubits], dtype=np.int32)
+    tol = _probability_tolerance()
+    is_dm = isinstance(state, DensityMatrix)
+    if is_dm:
+        # every diagonal entry is checked before any marginal sums them
+        # (reference probabilities.jl:55-64)
+        bounds = np.zeros(3)
+        check(ctx.lib.mossq_dm_diag_bounds(ctx.h, state.storage.ptr, state.n, _lib.dptr(bounds)))
+        if bounds[2] == 0:
+            raise ValueError("density-matrix diagonal must be finite")
+        if bounds[0] > tol:
+            raise ValueError("density-matrix diagonal must be real")
+        if bounds[1] < -tol:
+            raise ValueError("density-matrix probabilities must be nonnegative")
+    # an empty readout register is the total probability as one entry
+    q = np.array([x - 1 for x in (qubits or [1])], dtype=np.int32)
     out = np.zeros(1 << len(q))
-    check(ctx.lib.mossq_probabilities(ctx.h, state.storage.ptr, state.n, int(isinstance(state, DensityMatrix)),
-                                      len(q), _lib.iptr(q), _lib.dptr(out)))
-    return _normalize(out, _probability_tolerance())
+    check(ctx.lib.mossq_probabilities(ctx.h, state.storage.ptr, state.n, int(is_dm), len(q), _lib.iptr(q),
+                                      _lib.dptr(out)))
+    if not qubits:
+        out = np.array([out[0] + out[1]])
+    return _normalize(out, tol)

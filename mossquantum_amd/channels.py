@@ -145,3 +145,50 @@ def BitFlip(pos, p):
 
 def PhaseFlip(pos, p):
     return PauliChannel(pos, ["I", "Z"], [1 - _prob(p, "p"), p])
+
+
+def BitPhaseFlip(pos, p):
+    return PauliChannel(pos, ["I", "Y"], [1 - _prob(p, "p"), p])
+
+
+def GeneralizedAmplitudeDamping(pos, gamma, excited_population=0.0):
+    """Damping toward the equilibrium |1> population `excited_population`
+    (reference channels/one_qubit.jl:78-86)."""
+    g = _prob(gamma, "gamma")
+    e = _prob(excited_population, "excited_population")
+    a, b = np.sqrt(1 - e), np.sqrt(e)
+    return QuantumMap(pos, [a * np.array([[1, 0], [0, np.sqrt(1 - g)]]), a * np.array([[0, np.sqrt(g)], [0, 0]]),
+                            b * np.array([[np.sqrt(1 - g), 0], [0, 1]]), b * np.array([[0, 0], [np.sqrt(g), 0]])],
+                      check_tp=True)
+
+
+def ResetChannel(pos, state=0, p=1.0):
+    """With probability p replace the qubit by |state> (reference
+    channels/one_qubit.jl:88-96)."""
+    if state not in (0, 1):
+        raise ValueError("reset state must be 0 or 1")
+    p = _prob(p, "p")
+    a = np.zeros((2, 2))
+    b = np.zeros((2, 2))
+    a[state, 0] = np.sqrt(p)
+    b[state, 1] = np.sqrt(p)
+    return QuantumMap(pos, [np.sqrt(1 - p) * np.eye(2), a, b], check_tp=True)
+
+
+def ThermalRelaxation(pos, T1, T2, duration, excited_population=0.0):
+    """Population relaxation with T1 and dephasing with T2 over `duration`
+    (common time unit; T2 <= 2 T1; infinite lifetimes allowed): phase damping
+    after generalized amplitude damping, Kraus products b*a (reference
+    channels/one_qubit.jl:98-115)."""
+    T1, T2, duration = float(T1), float(T2), float(duration)
+    if not (T1 > 0 and T2 > 0 and T2 / 2 <= T1):
+        raise ValueError("require positive T1/T2 and T2 <= 2T1")
+    if not (np.isfinite(duration) and duration >= 0):
+        raise ValueError("duration must be finite and nonnegative")
+    e = _prob(excited_population, "excited_population")
+    gamma = -np.expm1(-duration / T1)
+    inverse_phi = max(0.0, 1 / T2 - 1 / T1 / 2)
+    phase = PhaseDamping(pos, -np.expm1(-2 * duration * inverse_phi))
+    amplitude = GeneralizedAmplitudeDamping(pos, gamma, e)
+    ks = [b @ a for a in amplitude.kraus_matrices() for b in phase.kraus_matrices()]
+    return QuantumMap(pos, ks, check_tp=True)

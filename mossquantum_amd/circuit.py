@@ -18,6 +18,40 @@ from .channels import QuantumMap
 from .gates import Gate, QuantumGate, reorder_legs
 
 
+class Delay:
+    """An explicit idle interval on one qubit, in the time unit of the noise rule
+    that matches it (reference Delay, src/execution/noise_models.jl:31-44).  It
+    leaves the state unchanged; `with_noise` replaces it by what its rules
+    generate."""
+
+    def __init__(self, position: int, duration: float):
+        if int(position) != position or position <= 0:
+            raise ValueError("position must be positive")
+        duration = float(duration)
+        if not (np.isfinite(duration) and duration >= 0):
+            raise ValueError("duration must be finite and nonnegative")
+        self.position = int(position)
+        self.duration = duration
+
+    def positions(self):
+        return (self.position,)
+
+    def ordered_positions(self):
+        return (self.position,)
+
+    def nqubits(self):
+        return 1
+
+    def nparameters(self):
+        return 0
+
+    def adjoint(self):
+        return self
+
+    def __repr__(self):
+        return f"Delay({self.position}; duration={self.duration})"
+
+
 class QCircuit:
     def __init__(self, ops=()):
         self.ops = []
@@ -25,7 +59,7 @@ class QCircuit:
             self.append(o)
 
     def append(self, op):
-        if not isinstance(op, (Gate, QuantumMap, QCircuit)):
+        if not isinstance(op, (Gate, QuantumMap, QCircuit, Delay)):
             raise TypeError(f"unsupported instruction {type(op).__name__}")
         self.ops.append(op)
         return self
@@ -147,6 +181,8 @@ class Lowered:
         self.pool_len = 0
         self.nslots = 0
         self.grad_slot: list[int] = []
+        # (pool entry of the gate matrix, gate, first slot) per parameterized gate
+        self.param_gates: list[tuple[int, Gate, int]] = []
 
     def _push(self, m):
         m = np.asarray(m, dtype=np.complex128)
@@ -174,6 +210,10 @@ class Lowered:
         r.mat = self._push(op.ordered_mat())
         ders = op.differentiate() or []
         first_slot = self.nslots
+        if ders:
+            if len(ders) != op.nparameters():
+                raise ValueError("derivative gates must match the gate's active parameters")
+            self.param_gates.append((len(self.pool) - 1, op, first_slot))
         self.nslots += len(ders)
         if len(ders) > 1:
             # extra derivative gates ride on identity records placed before the
@@ -204,6 +244,19 @@ class Lowered:
         out[np.asarray(self.grad_slot, dtype=np.int64)] = device_grads
         return out
 
+    def pool_with_slots(self, slot_values, slots=None):
+        """The pool with the parameterized gates' matrices rebuilt at
+        `slot_values` (active slots in circuit order); `slots` limits the
+        rebuild to the gates holding those slots."""
+        pool = list(self.pool)
+        for li, op, s0 in self.param_gates:
+            k = op.nparameters()
+            if slots is not None and not any(s0 <= s < s0 + k for s in slots):
+                continue
+            g = op.with_parameters([float(v) for v in slot_values[s0:s0 + k]])
+            pool[li] = np.asfortranarray(g.ordered_mat()).reshape(-1, order="F").astype(np.complex128)
+        return np.ascontiguousarray(np.concatenate(pool)) if pool else np.zeros(1, dtype=np.complex128)
+
     def arrays(self):
         recs = (OpRecord * max(len(self.records), 1))(*self.records)
         pool = np.concatenate(self.pool) if self.pool else np.zeros(1, dtype=np.complex128)
@@ -213,5 +266,9 @@ class Lowered:
 def lower(circuit: QCircuit, n: int, is_density: bool) -> Lowered:
     lw = Lowered(n, is_density)
     for op in circuit.flat():
+        if isinstance(op, Delay):
+            if op.position > n:
+                raise ValueError(f"position {op.position} outside a {n}-qubit register")
+            continue  # idle: no instruction
         lw.add(op)
     return lw

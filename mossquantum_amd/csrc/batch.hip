@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kBT, 2) void k_biter(const BItemDev *__restrict__ i
     const int n = it.n, nfwd = it.nfwd;
     const int c = int(blockIdx.x) - it.chunk0;
     const int pc = t - it.t0;
-    if (pc < 0 || pc > 2 * nfwd) return;
+    if (pc < 0 || pc > (it.value_only ? nfwd : 2 * nfwd)) return;
     c128 *psi = it.psi, *lam = it.lam;
     if (pc == nfwd) {
         // observe: E partial = sum_i conj(psi_i) (H psi)_i ; lambda = (H + H^dagger) psi
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kBT, 2) void k_biter(const BItemDev *__restrict__ i
                     sd = cfma(it.seed[g.moff + r + D * cc], x, sd);
                 }
             }
-            lam[i] = sd;
+            if (!it.value_only) lam[i] = sd;
             acc = cfmaconj(psi[i], hp, acc);
         }
         c128 s = bblock_sum(acc, red);
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kBT) void k_breduce(const BItemDev *__restrict__ it
     bool is_value = false;
     if (pc == it.nfwd) {
         is_value = true;
-    } else if (pc > it.nfwd && pc <= 2 * it.nfwd) {
+    } else if (!it.value_only && pc > it.nfwd && pc <= 2 * it.nfwd) {
         const BOp op = it.ops[2 * it.nfwd - pc];
         if (op.grad >= 0) slot = it.gslot0 + op.gslot;
     }
@@ -218,8 +218,11 @@ int batch_iteration(Ctx *c, const BItemDev *items, int nitems, int nchunks_total
 }
 
 // Batch of B items all starting at iteration 0; one host synchronisation at the end.
+// value_only: forward and observe steps only (no lambda vectors, no gradients);
+// init[b], when given, is item b's initial state (else |0...0>).
 static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, const Obs *const *obs,
-                                const int64_t *grad_offset, double *values, double *grads) {
+                                const int64_t *grad_offset, double *values, double *grads, bool value_only,
+                                const void *const *init) {
     std::vector<BItemHost> items(B);
     std::vector<BOp> all_ops;
     std::vector<int> op_off(B);
@@ -232,19 +235,19 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         all_ops.insert(all_ops.end(), items[b].ops.begin(), items[b].ops.end());
         items[b].nfwd = int(items[b].ops.size());
         items[b].nchunk = batch_chunks(progs[b]->n);
-        items[b].gslot0 = int(grad_offset[b]);
+        items[b].gslot0 = value_only ? 0 : int(grad_offset[b]);
         items[b].vslot = b;
         total_amps += i64(1) << progs[b]->n;
-        dtot = std::max<i64>(dtot, grad_offset[b] + nd);
+        if (!value_only) dtot = std::max<i64>(dtot, grad_offset[b] + nd);
         nchunks += items[b].nchunk;
-        tmax = std::max(tmax, 2 * items[b].nfwd);
+        tmax = std::max(tmax, (value_only ? 1 : 2) * items[b].nfwd);
     }
     Allocs al(c);
     c128 *vec = nullptr, *part = nullptr;
     BOp *d_ops = nullptr;
     BItemDev *d_items = nullptr;
     double *d_vals = nullptr, *d_grads = nullptr;
-    MOSSQ_TRY(al.get(u64(total_amps) * 2 * sizeof(c128), (void **)&vec));
+    MOSSQ_TRY(al.get(u64(total_amps) * (value_only ? 1 : 2) * sizeof(c128), (void **)&vec));
     MOSSQ_TRY(al.get(u64(std::max(nchunks, 1)) * sizeof(c128), (void **)&part));
     MOSSQ_TRY(al.get(std::max<size_t>(all_ops.size(), 1) * sizeof(BOp), (void **)&d_ops));
     MOSSQ_TRY(al.get(u64(B) * sizeof(BItemDev), (void **)&d_items));
@@ -257,8 +260,8 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         const i64 len = i64(1) << progs[b]->n;
         BItemDev &d = hd[b];
         d.psi = vec + vo;
-        d.lam = vec + vo + len;
-        vo += 2 * len;
+        d.lam = value_only ? nullptr : vec + vo + len;
+        vo += (value_only ? 1 : 2) * len;
         d.ops = d_ops + op_off[b];
         d.pool = progs[b]->d_pool;
         d.groups = obs[b]->d_groups;
@@ -272,8 +275,13 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         d.nchunk = items[b].nchunk;
         d.gslot0 = items[b].gslot0;
         d.vslot = b;
+        d.value_only = value_only ? 1 : 0;
+        d.pad = 0;
         ch += d.nchunk;
-        MOSSQ_TRY(fill_basis0(c, d.psi, u64(len)));
+        if (init && init[b])
+            MOSSQ_HIP(hipMemcpyAsync(d.psi, init[b], u64(len) * sizeof(c128), hipMemcpyDeviceToDevice, c->stream));
+        else
+            MOSSQ_TRY(fill_basis0(c, d.psi, u64(len)));
     }
     if (!all_ops.empty())
         MOSSQ_HIP(hipMemcpyAsync(d_ops, all_ops.data(), all_ops.size() * sizeof(BOp), hipMemcpyHostToDevice,
@@ -286,7 +294,23 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
     MOSSQ_HIP(hipStreamSynchronize(c->stream));
     for (int b = 0; b < B; ++b) {
         values[b] = hv[b];
-        for (int j = 0; j < progs[b]->nderiv; ++j) grads[grad_offset[b] + j] = hg[grad_offset[b] + j];
+        if (!value_only)
+            for (int j = 0; j < progs[b]->nderiv; ++j) grads[grad_offset[b] + j] = hg[grad_offset[b] + j];
+    }
+    return MOSSQ_OK;
+}
+
+static int batch_items_check(Ctx *c, int B, mossq_program *const *progs, mossq_obs *const *obs,
+                             std::vector<const Program *> &P, std::vector<const Obs *> &O) {
+    P.resize(B);
+    O.resize(B);
+    for (int b = 0; b < B; ++b) {
+        P[b] = reinterpret_cast<const Program *>(progs[b]);
+        O[b] = reinterpret_cast<const Obs *>(obs[b]);
+        MOSSQ_CHECK(P[b] && O[b], MOSSQ_EINVAL, "null program or observable in batch");
+        MOSSQ_CHECK(P[b]->ctx == c && O[b]->ctx == c, MOSSQ_EINVAL, "batch item belongs to another context");
+        MOSSQ_CHECK(!P[b]->is_density, MOSSQ_EUNSUPPORTED, "the batched engine takes state-vector programs");
+        MOSSQ_CHECK(O[b]->n == P[b]->n, MOSSQ_EINVAL, "observable and program qubit counts differ");
     }
     return MOSSQ_OK;
 }
@@ -303,19 +327,25 @@ int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs, moss
     MOSSQ_CHECK(B >= 0 && (B == 0 || (progs && obs && grad_offset && values && grads)), MOSSQ_EINVAL,
                 "bad batch arguments");
     Ctx *c = C(ctx);
-    std::vector<const Program *> P(B);
-    std::vector<const Obs *> O(B);
-    for (int b = 0; b < B; ++b) {
-        P[b] = reinterpret_cast<const Program *>(progs[b]);
-        O[b] = reinterpret_cast<const Obs *>(obs[b]);
-        MOSSQ_CHECK(P[b] && O[b], MOSSQ_EINVAL, "null program or observable in batch");
-        MOSSQ_CHECK(P[b]->ctx == c && O[b]->ctx == c, MOSSQ_EINVAL, "batch item belongs to another context");
-        MOSSQ_CHECK(!P[b]->is_density, MOSSQ_EUNSUPPORTED, "batched adjoint takes state-vector programs");
-        MOSSQ_CHECK(O[b]->n == P[b]->n, MOSSQ_EINVAL, "observable and program qubit counts differ");
-        MOSSQ_CHECK(grad_offset[b] >= 0, MOSSQ_EINVAL, "negative gradient offset");
-    }
+    std::vector<const Program *> P;
+    std::vector<const Obs *> O;
+    MOSSQ_TRY(batch_items_check(c, B, progs, obs, P, O));
+    for (int b = 0; b < B; ++b) MOSSQ_CHECK(grad_offset[b] >= 0, MOSSQ_EINVAL, "negative gradient offset");
     if (B == 0) return MOSSQ_OK;
-    return batch_adjoint_engine(c, B, P.data(), O.data(), grad_offset, values, grads);
+    return batch_adjoint_engine(c, B, P.data(), O.data(), grad_offset, values, grads, false, nullptr);
+}
+
+int mossq_batch_expectation(mossq_ctx *ctx, int B, mossq_program *const *progs, mossq_obs *const *obs,
+                            const void *const *init, double *values) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(B >= 0 && (B == 0 || (progs && obs && values)), MOSSQ_EINVAL, "bad batch arguments");
+    Ctx *c = C(ctx);
+    std::vector<const Program *> P;
+    std::vector<const Obs *> O;
+    MOSSQ_TRY(batch_items_check(c, B, progs, obs, P, O));
+    if (B == 0) return MOSSQ_OK;
+    std::vector<int64_t> zero(B, 0);
+    return batch_adjoint_engine(c, B, P.data(), O.data(), zero.data(), values, nullptr, true, init);
 }
 
 }  // extern "C"

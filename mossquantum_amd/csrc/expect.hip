@@ -289,6 +289,42 @@ __global__ __launch_bounds__(kB) void k_marginal_reduce(const double *__restrict
     }
 }
 
+// Per-block bounds of the diagonal of rho: part[3 b + 0] = max |Im rho_ii|,
+// part[3 b + 1] = min Re rho_ii, part[3 b + 2] = 1 if every entry seen is
+// finite (else 0).
+__global__ __launch_bounds__(kB) void k_diag_bounds(const c128 *__restrict__ rho, int n, double *__restrict__ part) {
+    __shared__ double red[3][kB / 64];
+    const u64 N = u64(1) << n;
+    double mi = 0, mr = INFINITY, fin = 1;
+    for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < N; i += u64(gridDim.x) * kB) {
+        const c128 d = rho[i + N * i];
+        if (!isfinite(d.re) || !isfinite(d.im)) fin = 0;
+        mi = fmax(mi, fabs(d.im));
+        mr = fmin(mr, d.re);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mi = fmax(mi, __shfl_xor(mi, o, 64));
+        mr = fmin(mr, __shfl_xor(mr, o, 64));
+        fin = fmin(fin, __shfl_xor(fin, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = mi;
+        red[1][threadIdx.x >> 6] = mr;
+        red[2][threadIdx.x >> 6] = fin;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kB / 64; ++w) {
+            mi = fmax(mi, red[0][w]);
+            mr = fmin(mr, red[1][w]);
+            fin = fmin(fin, red[2][w]);
+        }
+        part[3 * blockIdx.x + 0] = mi;
+        part[3 * blockIdx.x + 1] = mr;
+        part[3 * blockIdx.x + 2] = fin;
+    }
+}
+
 int dm_expect_groups_dev(Ctx *c, const Obs *o, const c128 *rho, c128 *dev_out, c128 *part) {
     unsigned nb = blocks_for(u64(1) << o->n, 1, 64);
     hipLaunchKernelGGL(k_dm_expect_groups, dim3(nb, o->G), dim3(kB), 0, c->stream, rho, o->n, o->d_groups,
@@ -497,6 +533,29 @@ int mossq_probabilities(mossq_ctx *ctx, const void *state, int n, int is_density
     MOSSQ_HIP(hipGetLastError());
     MOSSQ_HIP(hipMemcpyAsync(out, res, nbins * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    return MOSSQ_OK;
+}
+
+int mossq_dm_diag_bounds(mossq_ctx *ctx, const void *rho, int n, double *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(rho && out && n >= 1 && n <= 20, MOSSQ_EINVAL, "bad diagonal-bounds arguments");
+    Ctx *c = C(ctx);
+    const unsigned nb = blocks_for(u64(1) << n, 1, 1024);
+    void *scr = nullptr;
+    MOSSQ_TRY(get_scratch(c, u64(3) * nb * sizeof(double), &scr));
+    hipLaunchKernelGGL(k_diag_bounds, dim3(nb), dim3(kB), 0, c->stream, (const c128 *)rho, n, (double *)scr);
+    MOSSQ_HIP(hipGetLastError());
+    std::vector<double> h(size_t(3) * nb);
+    MOSSQ_HIP(hipMemcpyAsync(h.data(), scr, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    out[0] = 0;
+    out[1] = INFINITY;
+    out[2] = 1;
+    for (unsigned b = 0; b < nb; ++b) {
+        out[0] = std::max(out[0], h[3 * b]);
+        out[1] = std::min(out[1], h[3 * b + 1]);
+        out[2] = std::min(out[2], h[3 * b + 2]);
+    }
     return MOSSQ_OK;
 }
 

@@ -35,6 +35,8 @@ struct BItemDev {
     int n, nfwd, G, t0;
     int chunk0, nchunk;
     int gslot0, vslot;
+    int value_only;  // stop after the observe step (no seed, no reverse pass)
+    int pad;
 };
 
 // Host-side description of one item.

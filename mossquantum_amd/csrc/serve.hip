@@ -305,6 +305,7 @@ static int rebuild_table(Server *s) {
         d.nchunk = r->nchunk;
         d.gslot0 = r->slot * kMaxGrads;
         d.vslot = r->slot;
+        d.value_only = 0;
         ch += r->nchunk;
     }
     s->nchunks = ch;

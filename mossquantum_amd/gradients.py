@@ -128,7 +128,15 @@ def _adjoint_one(b: HIPBackend, circuit: QCircuit, observable, state):
     return value.value, prog.lowered.slot_gradients(grads[:nd])
 
 
+def _sampled(method):
+    from .sampled_gradients import FiniteDifference, ParameterShift, SPSAGradient
+    return isinstance(method, (ParameterShift, FiniteDifference, SPSAGradient))
+
+
 def supports_gradient(method, objective, values=None) -> bool:
+    if _sampled(method):
+        from .sampled_gradients import supports_sampled_gradient
+        return supports_sampled_gradient(method, objective, values)
     if not isinstance(method, AdjointGradient):
         return False
     if not isinstance(objective, (ExpectationObjective, BatchExpectationObjective)):
@@ -150,7 +158,15 @@ def evaluate(objective, values):
     return float(expectation(objective.observable, state, objective.backend).real)
 
 
-def value_and_gradient(method, objective, values):
+def value_and_gradient(method, objective, values, rng=None):
+    """(value, gradient in independent-parameter order).  AdjointGradient runs
+    the device reverse pass; the sampled methods (ParameterShift,
+    FiniteDifference, SPSAGradient) evaluate their shifted circuits together
+    (rng drives SPSA's directions)."""
+    if _sampled(method):
+        from .sampled_gradients import estimate_gradient
+        est = estimate_gradient(method, objective, values, rng)
+        return est.value, est.gradient
     if not supports_gradient(method, objective):
         raise UnsupportedGradient(method, objective)
     pm = objective.parameter_map
@@ -167,8 +183,8 @@ def value_and_gradient(method, objective, values):
     return v, pm.accumulate(g)
 
 
-def value_and_pullback(method, objective, values):
-    v, g = value_and_gradient(method, objective, values)
+def value_and_pullback(method, objective, values, rng=None):
+    v, g = value_and_gradient(method, objective, values, rng)
     saved = np.array(g, copy=True)
     return v, (lambda seed: seed * saved)
 
