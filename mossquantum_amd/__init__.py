@@ -5,8 +5,11 @@ of include/mossq_hip.h); this package is the host-side mirror of the
 reference's backend, state, observable and gradient interfaces.
 """
 from ._lib import Context, MossqError, MossqOOM, MossqUnsupported, default_context, device_count, load
-from .backend import (DeviceObservable, HIPBackend, Program, UnsupportedOperation, apply, apply_, backend,
-                      cross_expectation, dot, expectation, probabilities, purity, simulate, supports, trace)
+from .backend import (DeviceObservable, HIPBackend, Program, SimulationResult, UnsupportedOperation, apply, apply_,
+                      backend, cross_expectation, dot, expectation, probabilities, purity, simulate, simulate_circuit,
+                      supports, trace)
+from .capability import (CapabilityIssue, CapabilityStep, ExecutionCapability, execution_capability,
+                         execution_capability_backend, execution_capability_step)
 from .channels import (AmplitudeDamping, BitFlip, BitPhaseFlip, Depolarizing, GeneralizedAmplitudeDamping,
                        MixedUnitary, PauliChannel, PhaseDamping, PhaseFlip, QuantumMap, ResetChannel,
                        ThermalRelaxation)

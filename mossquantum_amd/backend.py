@@ -271,9 +271,41 @@ def apply_(op, state, b: HIPBackend | None = None):
     raise UnsupportedOperation(b, op, state, "apply")
 
 
-def simulate(circuit: QCircuit, state, b: HIPBackend | None = None):
-    """Run a circuit on a copy of `state` (reference circuit * state copies first)."""
-    return apply_(circuit, state.copy(), b)
+class SimulationResult:
+    """Final state and measurement records of one execution (reference
+    SimulationResult, src/execution/simulation.jl:15-25).  Circuits here carry
+    no mid-circuit measurements, so `measurements` stays empty."""
+
+    def __init__(self, state, measurements=None):
+        self.state = state
+        self.measurements = list(measurements or [])
+
+
+def simulate_circuit(b: HIPBackend, circuit: QCircuit, state, rng=None, records=None):
+    """Backend bulk-execution hook (reference simulate_circuit!,
+    src/execution/simulation.jl:166-175): the whole record-free circuit runs as
+    one device program (fused LDS-tile sweeps when the backend fuses) on
+    `state`, which it may replace (promotion at the first channel)."""
+    return apply_(circuit, state, b)
+
+
+def simulate(circuit: QCircuit, state, b: HIPBackend | None = None, rng=None, noise_model=None):
+    """Run a circuit on a copy of `state` and return a SimulationResult
+    (reference simulate, src/execution/simulation.jl:177-199); a noise model
+    is expanded first (with_noise)."""
+    from .noise import NoisyCircuit, with_noise
+    if isinstance(circuit, NoisyCircuit):
+        if noise_model is not None:
+            raise ValueError("noise is already expanded")
+        circuit = circuit.circuit
+    elif noise_model is not None:
+        circuit = with_noise(circuit, noise_model).circuit
+    b = b or backend(state)
+    if isinstance(state, (BatchStateVector, BatchDensityMatrix)):
+        return SimulationResult(apply_(circuit, state.copy(), b))
+    _require(b, circuit, state, "apply")
+    records: list = []
+    return SimulationResult(simulate_circuit(b, circuit, state.copy(), rng, records), records)
 
 
 # -------------------------------------------------------------- expectation ----

@@ -75,8 +75,27 @@ def test_idle_relaxation(ctx):
     timed = mq.NoiseModel(rules=[mq.NoiseRule(mq.Delay, lambda g: mq.ThermalRelaxation(g.position, 1, 2,
                                                                                         g.duration))])
     idle = mq.with_noise(mq.QCircuit([mq.XGate(1), mq.Delay(1, 2)]), timed)
-    st = mq.simulate(idle.circuit, mq.StateVector.zeros(1))
+    st = mq.simulate(idle.circuit, mq.StateVector.zeros(1)).state
     assert abs(mq.probabilities(st)[1] - np.exp(-2)) < 1e-14
     # a bare Delay leaves the state unchanged
-    st = mq.simulate(mq.QCircuit([mq.Delay(1, 2)]), mq.StateVector.zeros(1))
+    st = mq.simulate(mq.QCircuit([mq.Delay(1, 2)]), mq.StateVector.zeros(1)).state
     assert mq.probabilities(st).tolist() == [1, 0]
+
+
+def test_execution_capability_and_simulate(ctx):
+    b = mq.HIPBackend(ctx)
+    sv = mq.StateVector.zeros(3, ctx)
+    c = mq.QCircuit([mq.HGate(1), mq.CNOTGate(1, 2)])
+    rep = mq.execution_capability(b, c, sv)
+    assert rep.status == "supported" and rep.final_state_types == [mq.StateVector]
+    noisy = mq.QCircuit([mq.HGate(1), mq.AmplitudeDamping(1, 0.3)])
+    assert mq.execution_capability(b, noisy, sv).final_state_types == [mq.DensityMatrix]
+    assert mq.execution_capability(b, c, sv, mode="execute").status == "unsupported"
+    model = mq.NoiseModel(rules=[mq.NoiseRule(mq.HGate, mq.BitFlip(1, 0.1))])
+    assert mq.execution_capability(b, c, sv, noise_model=model).status == "unknown"
+    res = mq.simulate(c, sv, b, noise_model=model)
+    assert isinstance(res, mq.SimulationResult) and isinstance(res.state, mq.DensityMatrix)
+    assert res.measurements == []
+    want = mq.apply_(mq.with_noise(c, model).circuit, mq.StateVector.zeros(3, ctx), b)
+    assert np.max(np.abs(res.state.to_host() - want.to_host())) < 1e-15
+    assert np.array_equal(sv.to_host(), mq.StateVector.zeros(3, ctx).to_host())  # input untouched
