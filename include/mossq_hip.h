@@ -193,14 +193,15 @@ int mossq_plan_emulate_adjoint(int n, int nops, const mossq_op *ops, const doubl
 
 /* Host execution of the fused density-matrix adjoint (density program of n
  * qubits, 2n >= 12 bits, at least one parameterized gate): the program is cut
- * at its parameterized steps into forward and reverse chunk plans, run with
- * the device schedule (checkpoints every seg parameters, recompute per
- * segment).  rho (4^n complex) holds the input on entry and the final state on
+ * at its parameterized steps into forward and reverse chunk plans (plus one
+ * forward plan per checkpoint segment), run with the device schedule
+ * (checkpoints every seg parameters, recompute per segment).  rho (4^n complex) holds the input on entry and the final state on
  * exit; delta holds the seed at the program's end on entry (the reference's
  * z * H^dagger, src/differentiation/expectation.jl:69-72) and the adjoint
  * state at the first parameterized step on exit; grads[j] = derivative j.
- * stats[0..5] = parameterized steps, checkpoint segment length, forward-chunk
- * sweeps, reverse-chunk sweeps, unfused launches, chunk plans.  Used to test
+ * stats[0..5] = parameterized steps, checkpoint segment length, sweeps of the
+ * forward pass (chunk 0 and the segment plans), sweeps of the chunk plans the
+ * reverse pass runs (recompute and reverse), unfused launches, plans.  Used to test
  * the chunk plans and the schedule. */
 int mossq_plan_emulate_density_adjoint(int n, int nops, const mossq_op *ops, const double *pool,
                                        uint64_t pool_len, double *rho, double *delta, double *grads, int *stats);

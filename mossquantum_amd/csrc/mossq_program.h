@@ -80,6 +80,8 @@ void density_adjoint_layout(const Program *p, int *seg, int *nseg);
 int density_adjoint_forward(Ctx *c, const Program *p, c128 *work, c128 *const *ckpt);
 int density_adjoint_reverse(Ctx *c, const Program *p, c128 *delta, c128 *const *ckpt, c128 *const *stored,
                             c128 *part, int row);
+// Gradient partials per row the fused density adjoint writes (zero them first).
+int density_adjoint_parts(const Program *p);
 // Zero `count` complex entries with a kernel (safe inside captured graphs).
 int zero_fill(Ctx *c, c128 *v, u64 count);
 

@@ -194,7 +194,8 @@ u64 adjoint_density_bytes(const Program *p, const Obs *o) {
         density_adjoint_layout(p, &seg, &nseg);
         seg -= 1;
     }
-    u64 parts = std::max<u64>(u64(o->G) * 64, u64(std::max(p->nderiv, 1)) * pair_parts(p->nbits));
+    const int row = p->dm_adj ? density_adjoint_parts(p) : pair_parts(p->nbits);
+    u64 parts = std::max<u64>(u64(o->G) * 64, u64(std::max(p->nderiv, 1)) * row);
     return bytes * u64(2 + nseg + seg) + (parts + o->G + p->nderiv + 1) * sizeof(c128);
 }
 
@@ -224,7 +225,7 @@ static int adjoint_density_fused(Ctx *c, const Program *p, const Obs *o, const c
     MOSSQ_TRY(al.get(bytes, (void **)&work));
     for (auto &q : ckpt) MOSSQ_TRY(al.get(bytes, (void **)&q));
     for (int i = 0; i + 1 < seg; ++i) MOSSQ_TRY(al.get(bytes, (void **)&stored[i]));
-    const int P = pair_parts(nb);
+    const int P = density_adjoint_parts(p);
     c128 *part = nullptr, *value_dev = nullptr, *grads_dev = nullptr;
     u64 nparts = std::max<u64>(u64(o->G) * 64, u64(std::max(p->nderiv, 1)) * P);
     MOSSQ_TRY(al.get(nparts * sizeof(c128), (void **)&part));
@@ -244,6 +245,9 @@ static int adjoint_density_fused(Ctx *c, const Program *p, const Obs *o, const c
     hipLaunchKernelGGL(k_fill_identity, dim3(4096), dim3(kB), 0, c->stream, work, n);
     MOSSQ_HIP(hipGetLastError());
     MOSSQ_TRY(apply_op_dev(c, o, work, delta, nb, OpMats::Adjoint));
+    // the paired launches write pair_parts entries of their rows, the tile
+    // probes whole rows; every other entry must read as zero
+    MOSSQ_TRY(zero_fill(c, part, u64(std::max(p->nderiv, 1)) * P));
     MOSSQ_TRY(density_adjoint_reverse(c, p, delta, ckpt.data(), stored.data(), part, P));
     MOSSQ_TRY(reduce_rows(c, part, p->nderiv, P, grads_dev));
     return copy_results(c, p, o, value_dev, grads_dev, value, grads);

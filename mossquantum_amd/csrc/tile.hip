@@ -307,11 +307,37 @@ __device__ __forceinline__ void group_op(const TGroup &g, const TDiag *__restric
     }
 }
 
+__device__ __forceinline__ c128 probe_op(int code, const c128 *y, const c128 *l, const c128 *__restrict__ D);
+__device__ __forceinline__ c128 wave_sum(c128 a);
+
+// Probe of a group against a fixed state: this wave's share of
+// sum conj(cur) (D fixed) over the tile, fixed read in the current layout
+// into f (the register set the group's output left free).
+__device__ __forceinline__ void fixed_probe(const TGroup &g, const c128 *cur, c128 *f, const c128 *fixed,
+                                            const TSeg &sg, int t, u64 base, const c128 *__restrict__ pool,
+                                            c128 *part, int row, int slot) {
+    const u64 gt = base | seg_gthread(t, sg);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        u64 o = 0;
+#pragma unroll
+        for (int i = 0; i < TRB; ++i)
+            if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
+        f[j] = load_nt(fixed + (gt | o));
+    }
+    const c128 acc = wave_sum(probe_op(g.pcode, cur, f, pool + g.pmoff));
+    if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = acc;
+}
+
 // Reads the tile from src and writes it to dst (src == dst for an in-place
-// sweep; a workgroup stores only after it has loaded its whole tile).
-__global__ __launch_bounds__(TT, 2) void k_tile(const c128 *src, c128 *dst, TSweepArgs sw,
-                                                  const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
-                                                  const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
+// sweep; a workgroup stores only after it has loaded its whole tile).  With
+// FIXP, a group's probe adds Re<cur|D|fixed> partials (one per wave per tile)
+// for a state `fixed` that the sweep does not change.
+template <bool FIXP>
+__device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSweepArgs &sw,
+                                           const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
+                                           const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
+                                           const c128 *fixed, c128 *part, int row) {
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
     const int t = threadIdx.x;
@@ -353,7 +379,11 @@ __global__ __launch_bounds__(TT, 2) void k_tile(const c128 *src, c128 *dst, TSwe
         for (int g = sg.g0; g < sg.g1; g += 2) {
             const TGroup ga = groups[g], gb = groups[g + 1];
             group_op(ga, diag, pool, v, w, tdep, base);
+            if (FIXP && ga.probe >= 0)
+                fixed_probe(ga, w, v, fixed, sg, t, base, pool, part, row, int(tile) * 4 + (t >> 6));
             group_op(gb, diag, pool, w, v, tdep, base);
+            if (FIXP && gb.probe >= 0)
+                fixed_probe(gb, v, w, fixed, sg, t, base, pool, part, row, int(tile) * 4 + (t >> 6));
         }
     }
     {
@@ -368,6 +398,22 @@ __global__ __launch_bounds__(TT, 2) void k_tile(const c128 *src, c128 *dst, TSwe
             store_nt(dst + (g | o), v[j]);
         }
     }
+}
+
+__global__ __launch_bounds__(TT, 2) void k_tile(const c128 *src, c128 *dst, TSweepArgs sw,
+                                                  const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
+                                                  const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
+    tile_sweep<false>(src, dst, sw, segs, groups, diag, pool, nullptr, nullptr, 0);
+}
+
+// The density adjoint's reverse chunks: Delta in place, probes against the
+// state before the chunk's parameterized gate.
+__global__ __launch_bounds__(TT, 2) void k_tile_fixp(c128 *delta, TSweepArgs sw, const TSeg *__restrict__ segs,
+                                                       const TGroup *__restrict__ groups,
+                                                       const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
+                                                       const c128 *__restrict__ fixed, c128 *__restrict__ part,
+                                                       int row) {
+    tile_sweep<true>(delta, delta, sw, segs, groups, diag, pool, fixed, part, row);
 }
 
 // --------------------------------------------------------- adjoint sweep ----
@@ -564,7 +610,9 @@ __global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 
 
 // ------------------------------------------------------------------ planner ----
 // kind 3 (adjoint plans): probe Re<y|D|l> of parameter `probe` on the two
-// states at this point; it changes neither state.
+// states at this point; it changes neither state.  A `fixed` probe reads a
+// state the plan does not change (Re<v|D|fixed>), so it depends on every op
+// before it and commutes with none.
 struct POp {
     int kind = 0;  // 0 dense, 1 diagonal, 2 unfusable (run as its own launch), 3 probe
     int nb = 0;
@@ -572,6 +620,7 @@ struct POp {
     std::vector<c128> m;  // dense / probe: 2^nb x 2^nb column-major; diagonal: 2^nb values
     int step = -1;
     int probe = -1;
+    bool fixed = false;
     u64 mask() const {
         u64 r = 0;
         for (int i = 0; i < nb; ++i) r |= u64(1) << bits[i];
@@ -753,6 +802,7 @@ static std::vector<c128> kron2(const std::vector<c128> &Am, const std::vector<c1
 
 static bool commutes(const POp &a, const POp &b) {
     if (a.kind == 2 || b.kind == 2) return false;  // unfusable steps never move
+    if (a.fixed || b.fixed) return false;
     if ((a.mask() & b.mask()) == 0) return true;
     return a.kind == 1 && b.kind == 1;
 }
@@ -1529,7 +1579,7 @@ static void host_block(c128 *v, int nbits, int k, const int *bits, const c128 *m
 // One tile of a sweep for one state (forward plans) or the two states y, l of
 // an adjoint plan (probes add Re<y|D|l> contributions to grads).
 static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw, const FusedPlan *fp, u64 tile,
-                         c128 *grads) {
+                         c128 *grads, const c128 *fixed = nullptr) {
     const size_t ns = states.size();
     std::vector<c128> lds(1 << TM);
     std::vector<std::vector<c128>> V(ns, std::vector<c128>(size_t(TT) * TN));
@@ -1625,9 +1675,17 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
                         }
                     }
                 }
-                if (g.probe >= 0 && ns == 2 && grads) {
+                if (g.probe >= 0 && grads && (ns == 2 || fixed)) {
+                    // adjoint plans: Re<y|D|l>; fixed probes: Re<v|D|fixed>, fixed
+                    // read in the current layout
                     const c128 *y = &V[0][t * TN], *D = fp->h_pool.data() + g.pmoff;
-                    std::vector<c128> dl(V[1].begin() + t * TN, V[1].begin() + (t + 1) * TN);
+                    std::vector<c128> dl(TN);
+                    if (ns == 2) {
+                        dl.assign(V[1].begin() + t * TN, V[1].begin() + (t + 1) * TN);
+                    } else {
+                        const u64 gth = base | seg_gthread(t, sg);
+                        for (int j = 0; j < TN; ++j) dl[j] = fixed[gth | slot_g(j, sg)];
+                    }
                     dense_host(g.pcode, D, dl.data());
                     c128 acc = mk(0, 0);
                     for (int j = 0; j < TN; ++j) acc = cfmaconj(y[j], dl[j], acc);
@@ -1644,7 +1702,8 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
     }
 }
 
-void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state) {
+void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state, const c128 *fixed = nullptr,
+                  c128 *grads = nullptr) {
     for (const PlanItem &it : fp->items) {
         if (!it.sweep) {
             const Step &s = p->steps[it.step];
@@ -1659,7 +1718,7 @@ void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state) {
             continue;
         }
         const u64 ntiles = u64(1) << (p->nbits - TM);
-        for (u64 t = 0; t < ntiles; ++t) emulate_tile({state}, it.args, fp, t, nullptr);
+        for (u64 t = 0; t < ntiles; ++t) emulate_tile({state}, it.args, fp, t, grads, fixed);
     }
 }
 
@@ -1780,6 +1839,23 @@ static int fused_run_plan(Ctx *c, const Program *p, const FusedPlan *fp, const c
     return MOSSQ_OK;
 }
 
+// Runs a plan with fixed probes on delta in place; probes write partials into
+// rows of `row` entries of part.
+static int fused_run_fixp(Ctx *c, const Program *p, const FusedPlan *fp, c128 *delta, const c128 *fixed,
+                          c128 *part, int row) {
+    const unsigned ntiles = unsigned(u64(1) << (p->nbits - TM));
+    for (const PlanItem &it : fp->items) {
+        if (!it.sweep) {
+            MOSSQ_TRY(apply_step(c, p, it.step, delta, fp->reverse_steps));
+            continue;
+        }
+        hipLaunchKernelGGL(k_tile_fixp, dim3(ntiles), dim3(TT), 0, c->stream, delta, it.args, fp->d_segs,
+                           fp->d_groups, fp->d_diag, fp->d_pool, fixed, part, row);
+        MOSSQ_HIP(hipGetLastError());
+    }
+    return MOSSQ_OK;
+}
+
 int fused_run(Ctx *c, const Program *p, c128 *state) {
     if (!p->fused) return run_steps(c, p, state, 0, int(p->steps.size()));
     return fused_run_plan(c, p, p->fused, state, state);
@@ -1828,14 +1904,20 @@ int zero_fill(Ctx *c, c128 *v, u64 count) {
 //   reverse chunk j (j = 1..m): steps (p_j, b_{j+1}) from last to first with the
 //     reverse matrices S^dagger and U^dagger (.) U.
 // Every chunk is an LDS-tile plan of its own.  The forward pass keeps S_j at
-// j = 1, 1 + seg, 1 + 2 seg, ... (seg = ceil(sqrt(m))); the reverse pass
-// recomputes one segment of S_j at a time from its checkpoint, then per
-// parameter runs the reverse chunk on Delta and the paired update
-// Delta <- V Delta with Re<V Delta | G | S_j> (differentiation/circuit.jl:90-172).
+// j = 1, 1 + seg, 1 + 2 seg, ... (seg = ceil(sqrt(m))), running each segment
+// of seg chunks as one plan; the reverse pass recomputes one segment of S_j
+// at a time from its checkpoint, then per parameter takes Delta through the
+// reverse chunk and the update Delta <- V Delta with Re<V Delta | G | S_j>
+// (differentiation/circuit.jl:90-172).  For a one-qubit parameterized gate
+// the reverse chunk plan ends with V and a probe of G against S_j, so the
+// whole step is one pass over Delta plus one read of S_j; wider ones run the
+// reverse chunk, then a paired launch.
 struct DensityAdjointPlan {
     std::vector<int> psteps;       // p_1..p_m
     std::vector<FusedPlan *> fwd;  // m + 1 chunks (null: no steps)
-    std::vector<FusedPlan *> rev;  // 1..m (rev[0] unused)
+    std::vector<FusedPlan *> rev;   // 1..m (rev[0] unused): reverse chunk without p_j
+    std::vector<FusedPlan *> revp;  // 1..m: reverse chunk, V_j and the probe of G_j (null: use rev[j])
+    std::vector<FusedPlan *> segfwd;  // segment k: chunks 1 + k seg .. min(m, (k + 1) seg) as one plan
     int seg = 1, nseg = 1;
     void *d_block = nullptr;  // device arrays of every chunk plan
 };
@@ -1857,7 +1939,45 @@ static FusedPlan *plan_range(const Program *p, int lo, int hi, bool reverse, std
 static void dm_plan_delete(DensityAdjointPlan *dp) {
     for (auto *fp : dp->fwd) delete fp;
     for (auto *fp : dp->rev) delete fp;
+    for (auto *fp : dp->revp) delete fp;
+    for (auto *fp : dp->segfwd) delete fp;
     delete dp;
+}
+
+// Reverse chunk of parameterized step si through its own reverse matrix, then
+// the fixed probe of its gradient matrix; null when the step is wider than a
+// tile probe takes (two register bits).
+static FusedPlan *plan_reverse_probe(const Program *p, int si, int hi, std::string *err, bool *ok) {
+    *ok = true;
+    const Step &st = p->steps[si];
+    if (st.split || st.kb > 2) return nullptr;
+    std::vector<POp> ops;
+    lower_range(p, si, hi, true, ops);
+    POp pr;
+    pr.kind = 3;
+    pr.nb = st.kb;
+    for (int i = 0; i < st.kb; ++i) pr.bits[i] = st.bits[i];
+    pr.m = mat_of(p, st.grad, 1 << st.kb);
+    pr.step = si;
+    pr.probe = st.grad_index;
+    pr.fixed = true;
+    ops.push_back(pr);
+    FusedPlan *fp = best_plan(p, ops, false, err);
+    if (!fp) {
+        *ok = false;
+        return nullptr;
+    }
+    fp->reverse_steps = true;
+    return fp;
+}
+
+// Every plan of a density adjoint plan (null entries skipped).
+static std::vector<FusedPlan *> dm_plans(const DensityAdjointPlan *dp) {
+    std::vector<FusedPlan *> out;
+    for (const auto *v : {&dp->fwd, &dp->rev, &dp->revp, &dp->segfwd})
+        for (FusedPlan *fp : *v)
+            if (fp) out.push_back(fp);
+    return out;
 }
 
 static DensityAdjointPlan *dm_plan_build_host(const Program *p, std::string *err) {
@@ -1869,41 +1989,59 @@ static DensityAdjointPlan *dm_plan_build_host(const Program *p, std::string *err
     auto b = [&](int j) { return j == 0 ? 0 : j <= m ? dp->psteps[j - 1] : N; };
     dp->fwd.assign(m + 1, nullptr);
     dp->rev.assign(m + 1, nullptr);
+    dp->revp.assign(m + 1, nullptr);
     bool ok = true;
     for (int j = 0; j <= m && ok; ++j) dp->fwd[j] = plan_range(p, b(j), b(j + 1), false, err, &ok);
-    for (int j = 1; j <= m && ok; ++j) dp->rev[j] = plan_range(p, dp->psteps[j - 1] + 1, b(j + 1), true, err, &ok);
+    for (int j = 1; j <= m && ok; ++j) {
+        dp->revp[j] = plan_reverse_probe(p, dp->psteps[j - 1], b(j + 1), err, &ok);
+        if (ok && !dp->revp[j]) dp->rev[j] = plan_range(p, dp->psteps[j - 1] + 1, b(j + 1), true, err, &ok);
+    }
     if (!ok) {
         dm_plan_delete(dp);
         return nullptr;
     }
     dp->seg = std::max(1, int(std::ceil(std::sqrt(double(std::max(m, 1))))));
     dp->nseg = std::max(1, (m + dp->seg - 1) / dp->seg);
+    dp->segfwd.assign(dp->nseg, nullptr);
+    for (int k = 0; k < dp->nseg && ok; ++k) {
+        const int e = std::min(m, (k + 1) * dp->seg);
+        dp->segfwd[k] = plan_range(p, b(1 + k * dp->seg), b(e + 1), false, err, &ok);
+    }
+    if (!ok) {
+        dm_plan_delete(dp);
+        return nullptr;
+    }
     return dp;
 }
 
-// Forward chunks: S_j lives in ckpt[(j-1)/seg] when (j-1) % seg == 0 and
-// 1 <= j <= m, else in work; S_0 is in work on entry and S_{m+1} on exit.
+// Forward pass: chunk 0 takes S_0 (in work) to S_1 in ckpt[0]; segment k
+// takes S_{1 + k seg} in ckpt[k] to the next checkpoint, the last one to
+// S_{m+1} in work.
 template <class Run>
 static int dm_forward_sched(const DensityAdjointPlan *dp, c128 *work, c128 *const *ckpt, Run run) {
-    const int m = int(dp->psteps.size()), seg = dp->seg;
-    auto loc = [&](int j) { return j >= 1 && j <= m && (j - 1) % seg == 0 ? ckpt[(j - 1) / seg] : work; };
-    for (int j = 0; j <= m; ++j) MOSSQ_TRY(run(dp->fwd[j], loc(j), loc(j + 1)));
+    MOSSQ_TRY(run(dp->fwd[0], work, ckpt[0]));
+    for (int k = 0; k < dp->nseg; ++k)
+        MOSSQ_TRY(run(dp->segfwd[k], ckpt[k], k + 1 < dp->nseg ? ckpt[k + 1] : work));
     return MOSSQ_OK;
 }
 
 // Reverse pass from Delta at the program's end down to p_1; stored holds
 // seg - 1 states.
-template <class Run, class Pair>
+template <class Run, class Pair, class RunProbe>
 static int dm_reverse_sched(const DensityAdjointPlan *dp, c128 *delta, c128 *const *ckpt, c128 *const *stored,
-                            Run run, Pair pair) {
+                            Run run, Pair pair, RunProbe run_probe) {
     const int m = int(dp->psteps.size()), seg = dp->seg;
     for (int k = dp->nseg - 1; k >= 0; --k) {
         const int a = 1 + k * seg, e = std::min(m, (k + 1) * seg);
         auto S = [&](int j) { return j == a ? ckpt[k] : stored[j - a - 1]; };
         for (int j = a + 1; j <= e; ++j) MOSSQ_TRY(run(dp->fwd[j - 1], S(j - 1), S(j)));
         for (int j = e; j >= a; --j) {
-            MOSSQ_TRY(run(dp->rev[j], delta, delta));
-            MOSSQ_TRY(pair(dp->psteps[j - 1], delta, S(j)));
+            if (dp->revp[j]) {
+                MOSSQ_TRY(run_probe(dp->revp[j], delta, S(j)));
+            } else {
+                MOSSQ_TRY(run(dp->rev[j], delta, delta));
+                MOSSQ_TRY(pair(dp->psteps[j - 1], delta, S(j)));
+            }
         }
     }
     return MOSSQ_OK;
@@ -1945,12 +2083,7 @@ int density_adjoint_plan_build(Program *p) {
     std::string err;
     DensityAdjointPlan *dp = dm_plan_build_host(p, &err);
     if (!dp) return MOSSQ_OK;  // no chunk plans: the reverse pass runs step by step
-    std::vector<FusedPlan *> plans;
-    for (auto *fp : dp->fwd)
-        if (fp) plans.push_back(fp);
-    for (auto *fp : dp->rev)
-        if (fp) plans.push_back(fp);
-    int s = upload_plan_block(p->ctx, plans, &dp->d_block);
+    int s = upload_plan_block(p->ctx, dm_plans(dp), &dp->d_block);
     if (s != MOSSQ_OK) {
         if (dp->d_block) dev_free(p->ctx, dp->d_block);
         dm_plan_delete(dp);
@@ -1987,7 +2120,13 @@ int density_adjoint_reverse(Ctx *c, const Program *p, c128 *delta, c128 *const *
             const Step &st = p->steps[si];
             return launch_dense_pair(c, d, S, p->nbits, st.kb, st.bits, p->d_pool + st.rev, p->d_pool + st.grad,
                                      part + size_t(st.grad_index) * row, true);
-        });
+        },
+        [&](const FusedPlan *fp, c128 *d, c128 *S) { return fused_run_fixp(c, p, fp, d, S, part, row); });
+}
+
+int density_adjoint_parts(const Program *p) {
+    if (!p->dm_adj) return 0;
+    return std::max(pair_parts(p->nbits), int(u64(4) << (p->nbits - TM)));
 }
 
 // Host execution of the fused density adjoint with the same schedule:
@@ -2015,6 +2154,9 @@ static void emulate_density_adjoint(const Program *p, const DensityAdjointPlan *
         c128 acc = mk(0, 0);
         for (u64 i = 0; i < len; ++i) acc = cfmaconj(d[i], dy[i], acc);
         grads[st.grad_index] = cadd(grads[st.grad_index], acc);
+        return MOSSQ_OK;
+    }, [&](const FusedPlan *fp, c128 *d, c128 *S) {
+        emulate_plan(p, fp, d, S, grads);
         return MOSSQ_OK;
     });
 }
@@ -2093,18 +2235,18 @@ int mossq_plan_emulate_density_adjoint(int n, int nops, const mossq_op *ops, con
     for (int i = 0; i < 6; ++i) stats[i] = 0;
     stats[0] = int(dp->psteps.size());
     stats[1] = dp->seg;
-    for (const FusedPlan *fp : dp->fwd)
-        if (fp) {
-            stats[2] += fp->nsweeps;
-            for (auto &it : fp->items) stats[4] += !it.sweep;
-            stats[5]++;
-        }
-    for (const FusedPlan *fp : dp->rev)
-        if (fp) {
-            stats[3] += fp->nsweeps;
-            for (auto &it : fp->items) stats[4] += !it.sweep;
-            stats[5]++;
-        }
+    // forward: chunk 0 and the segment plans; reverse: the chunk plans the
+    // recompute and the reverse pass run
+    auto count = [&](const FusedPlan *fp, int slot) {
+        if (!fp) return;
+        stats[slot] += fp->nsweeps;
+        for (auto &it : fp->items) stats[4] += !it.sweep;
+    };
+    count(dp->fwd[0], 2);
+    for (const FusedPlan *fp : dp->segfwd) count(fp, 2);
+    for (size_t j = 1; j < dp->fwd.size(); ++j) count(dp->fwd[j], 3);
+    for (const FusedPlan *fp : dp->rev) count(fp, 3);
+    stats[5] = int(dm_plans(dp).size());
     dm_plan_delete(dp);
     return MOSSQ_OK;
 }
