@@ -201,10 +201,15 @@ int mossq_plan_emulate_adjoint(int n, int nops, const mossq_op *ops, const doubl
  * state at the first parameterized step on exit; grads[j] = derivative j.
  * stats[0..5] = parameterized steps, checkpoint segment length, sweeps of the
  * forward pass (chunk 0 and the segment plans), sweeps of the chunk plans the
- * reverse pass runs (recompute and reverse), unfused launches, plans.  Used to test
- * the chunk plans and the schedule. */
+ * reverse pass runs (recompute and reverse), unfused launches, plans.  With
+ * rho == NULL only the plans are built and counted.  whole_segments selects
+ * the reverse plans: 1 one plan per checkpoint segment (where every
+ * parameterized gate acts on one qubit), 0 one per parameter, -1 as the
+ * device path chooses for this size.  Used to test the chunk
+ * plans and the schedule. */
 int mossq_plan_emulate_density_adjoint(int n, int nops, const mossq_op *ops, const double *pool,
-                                       uint64_t pool_len, double *rho, double *delta, double *grads, int *stats);
+                                       uint64_t pool_len, int whole_segments, double *rho, double *delta,
+                                       double *grads, int *stats);
 
 /* ---- adjoint gradients ---------------------------------------------------- */
 /* value = Re <H> at C|in>, grads[j] for the j-th derivative gate in forward

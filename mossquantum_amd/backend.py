@@ -202,21 +202,30 @@ def plan_emulate_adjoint(circuit: QCircuit, n: int, y: np.ndarray, lam: np.ndarr
     return ys, ls, grads[:lw.nderiv], dict(zip(keys, list(stats)))
 
 
-def plan_emulate_density_adjoint(circuit: QCircuit, n: int, rho: np.ndarray, delta: np.ndarray):
+def plan_emulate_density_adjoint(circuit: QCircuit, n: int, rho: np.ndarray, delta: np.ndarray,
+                                 whole_segments: int = -1):
     """Run the fused density-matrix adjoint of `circuit` (chunk plans cut at the
     parameterized gates, checkpoint schedule) on host vectors: rho = vec of the
     input density matrix, delta = vec of the seed at the circuit's end.
-    Returns (final rho, delta at the first parameterized gate, grads per
-    lowered derivative slot, stats dict)."""
+    whole_segments: 1 one reverse plan per checkpoint segment, 0 one per
+    parameter, -1 as the device path chooses for this size.  rho None: build
+    and count the plans only.  Returns (final rho, delta at the first
+    parameterized gate, grads per lowered derivative slot, stats dict)."""
     lw = lower(circuit, n, True)
     recs, pool = lw.arrays()
-    r = np.ascontiguousarray(np.array(rho, dtype=np.complex128))
-    d = np.ascontiguousarray(np.array(delta, dtype=np.complex128))
     grads = np.zeros(max(lw.nderiv, 1))
     stats = (C.c_int * 6)()
     dp = lambda a: a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
-    check(_lib.load().mossq_plan_emulate_density_adjoint(n, len(lw.records), recs, dp(pool), pool.size, dp(r),
-                                                         dp(d), grads.ctypes.data_as(C.POINTER(C.c_double)), stats))
+    if rho is None:  # plans and their counts only
+        check(_lib.load().mossq_plan_emulate_density_adjoint(n, len(lw.records), recs, dp(pool), pool.size,
+                                                             whole_segments, None, None, None, stats))
+        keys = ("param_steps", "segment", "forward_sweeps", "reverse_sweeps", "unfused", "chunk_plans")
+        return None, None, None, dict(zip(keys, list(stats)))
+    r = np.ascontiguousarray(np.array(rho, dtype=np.complex128))
+    d = np.ascontiguousarray(np.array(delta, dtype=np.complex128))
+    check(_lib.load().mossq_plan_emulate_density_adjoint(n, len(lw.records), recs, dp(pool), pool.size,
+                                                         whole_segments, dp(r), dp(d),
+                                                         grads.ctypes.data_as(C.POINTER(C.c_double)), stats))
     keys = ("param_steps", "segment", "forward_sweeps", "reverse_sweeps", "unfused", "chunk_plans")
     return r, d, grads[:lw.nderiv], dict(zip(keys, list(stats)))
 

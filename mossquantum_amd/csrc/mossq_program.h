@@ -78,8 +78,12 @@ int density_adjoint_plan_build(Program *p);
 void density_adjoint_plan_free(Program *p);
 void density_adjoint_layout(const Program *p, int *seg, int *nseg);
 int density_adjoint_forward(Ctx *c, const Program *p, c128 *work, c128 *const *ckpt);
+// The probe table lists the states the reverse plans' fixed probes read
+// (nseg x seg entries); reverse takes it in device memory.
+void density_adjoint_probe_table(const Program *p, c128 *const *ckpt, c128 *const *stored,
+                                 std::vector<const c128 *> &tab);
 int density_adjoint_reverse(Ctx *c, const Program *p, c128 *delta, c128 *const *ckpt, c128 *const *stored,
-                            c128 *part, int row);
+                            const c128 *const *d_tab, c128 *part, int row);
 // Gradient partials per row the fused density adjoint writes (zero them first).
 int density_adjoint_parts(const Program *p);
 // Zero `count` complex entries with a kernel (safe inside captured graphs).

@@ -248,7 +248,13 @@ static int adjoint_density_fused(Ctx *c, const Program *p, const Obs *o, const c
     // the paired launches write pair_parts entries of their rows, the tile
     // probes whole rows; every other entry must read as zero
     MOSSQ_TRY(zero_fill(c, part, u64(std::max(p->nderiv, 1)) * P));
-    MOSSQ_TRY(density_adjoint_reverse(c, p, delta, ckpt.data(), stored.data(), part, P));
+    std::vector<const c128 *> tab;
+    density_adjoint_probe_table(p, ckpt.data(), stored.data(), tab);
+    const c128 **d_tab = nullptr;
+    MOSSQ_TRY(al.get(tab.size() * sizeof(c128 *), (void **)&d_tab));
+    MOSSQ_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(c128 *), hipMemcpyHostToDevice, c->stream));
+    MOSSQ_TRY(density_adjoint_reverse(c, p, delta, ckpt.data(), stored.data(), d_tab, part, P));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));  // tab (host) is read by the copy above
     MOSSQ_TRY(reduce_rows(c, part, p->nderiv, P, grads_dev));
     return copy_results(c, p, o, value_dev, grads_dev, value, grads);
 }

@@ -331,13 +331,13 @@ __device__ __forceinline__ void fixed_probe(const TGroup &g, const c128 *cur, c1
 
 // Reads the tile from src and writes it to dst (src == dst for an in-place
 // sweep; a workgroup stores only after it has loaded its whole tile).  With
-// FIXP, a group's probe adds Re<cur|D|fixed> partials (one per wave per tile)
-// for a state `fixed` that the sweep does not change.
+// FIXP, a group's probe adds Re<cur|D|S> partials (one per wave per tile)
+// for a state S = ftab[probe - fbase] that the sweep does not change.
 template <bool FIXP>
 __device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSweepArgs &sw,
                                            const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
                                            const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
-                                           const c128 *fixed, c128 *part, int row) {
+                                           const c128 *const *ftab, int fbase, c128 *part, int row) {
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
     const int t = threadIdx.x;
@@ -380,10 +380,10 @@ __device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSw
             const TGroup ga = groups[g], gb = groups[g + 1];
             group_op(ga, diag, pool, v, w, tdep, base);
             if (FIXP && ga.probe >= 0)
-                fixed_probe(ga, w, v, fixed, sg, t, base, pool, part, row, int(tile) * 4 + (t >> 6));
+                fixed_probe(ga, w, v, ftab[ga.probe - fbase], sg, t, base, pool, part, row, int(tile) * 4 + (t >> 6));
             group_op(gb, diag, pool, w, v, tdep, base);
             if (FIXP && gb.probe >= 0)
-                fixed_probe(gb, v, w, fixed, sg, t, base, pool, part, row, int(tile) * 4 + (t >> 6));
+                fixed_probe(gb, v, w, ftab[gb.probe - fbase], sg, t, base, pool, part, row, int(tile) * 4 + (t >> 6));
         }
     }
     {
@@ -403,17 +403,17 @@ __device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSw
 __global__ __launch_bounds__(TT, 2) void k_tile(const c128 *src, c128 *dst, TSweepArgs sw,
                                                   const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
                                                   const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
-    tile_sweep<false>(src, dst, sw, segs, groups, diag, pool, nullptr, nullptr, 0);
+    tile_sweep<false>(src, dst, sw, segs, groups, diag, pool, nullptr, 0, nullptr, 0);
 }
 
-// The density adjoint's reverse chunks: Delta in place, probes against the
-// state before the chunk's parameterized gate.
+// The density adjoint's reverse chunks: Delta in place, each probe against
+// the state before its parameterized gate (ftab[probe - fbase]).
 __global__ __launch_bounds__(TT, 2) void k_tile_fixp(c128 *delta, TSweepArgs sw, const TSeg *__restrict__ segs,
                                                        const TGroup *__restrict__ groups,
                                                        const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
-                                                       const c128 *__restrict__ fixed, c128 *__restrict__ part,
-                                                       int row) {
-    tile_sweep<true>(delta, delta, sw, segs, groups, diag, pool, fixed, part, row);
+                                                       const c128 *const *__restrict__ ftab, int fbase,
+                                                       c128 *__restrict__ part, int row) {
+    tile_sweep<true>(delta, delta, sw, segs, groups, diag, pool, ftab, fbase, part, row);
 }
 
 // --------------------------------------------------------- adjoint sweep ----
@@ -1579,7 +1579,7 @@ static void host_block(c128 *v, int nbits, int k, const int *bits, const c128 *m
 // One tile of a sweep for one state (forward plans) or the two states y, l of
 // an adjoint plan (probes add Re<y|D|l> contributions to grads).
 static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw, const FusedPlan *fp, u64 tile,
-                         c128 *grads, const c128 *fixed = nullptr) {
+                         c128 *grads, const c128 *const *ftab = nullptr, int fbase = 0) {
     const size_t ns = states.size();
     std::vector<c128> lds(1 << TM);
     std::vector<std::vector<c128>> V(ns, std::vector<c128>(size_t(TT) * TN));
@@ -1675,7 +1675,7 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
                         }
                     }
                 }
-                if (g.probe >= 0 && grads && (ns == 2 || fixed)) {
+                if (g.probe >= 0 && grads && (ns == 2 || ftab)) {
                     // adjoint plans: Re<y|D|l>; fixed probes: Re<v|D|fixed>, fixed
                     // read in the current layout
                     const c128 *y = &V[0][t * TN], *D = fp->h_pool.data() + g.pmoff;
@@ -1684,6 +1684,7 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
                         dl.assign(V[1].begin() + t * TN, V[1].begin() + (t + 1) * TN);
                     } else {
                         const u64 gth = base | seg_gthread(t, sg);
+                        const c128 *fixed = ftab[g.probe - fbase];
                         for (int j = 0; j < TN; ++j) dl[j] = fixed[gth | slot_g(j, sg)];
                     }
                     dense_host(g.pcode, D, dl.data());
@@ -1702,8 +1703,8 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
     }
 }
 
-void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state, const c128 *fixed = nullptr,
-                  c128 *grads = nullptr) {
+void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state, const c128 *const *ftab = nullptr,
+                  int fbase = 0, c128 *grads = nullptr) {
     for (const PlanItem &it : fp->items) {
         if (!it.sweep) {
             const Step &s = p->steps[it.step];
@@ -1718,7 +1719,7 @@ void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state, const c128
             continue;
         }
         const u64 ntiles = u64(1) << (p->nbits - TM);
-        for (u64 t = 0; t < ntiles; ++t) emulate_tile({state}, it.args, fp, t, grads, fixed);
+        for (u64 t = 0; t < ntiles; ++t) emulate_tile({state}, it.args, fp, t, grads, ftab, fbase);
     }
 }
 
@@ -1839,10 +1840,11 @@ static int fused_run_plan(Ctx *c, const Program *p, const FusedPlan *fp, const c
     return MOSSQ_OK;
 }
 
-// Runs a plan with fixed probes on delta in place; probes write partials into
-// rows of `row` entries of part.
-static int fused_run_fixp(Ctx *c, const Program *p, const FusedPlan *fp, c128 *delta, const c128 *fixed,
-                          c128 *part, int row) {
+// Runs a plan with fixed probes on delta in place; the probe of parameter j
+// reads ftab[j - fbase] (a device table) and writes partials into row j of
+// part (rows of `row` entries).
+static int fused_run_fixp(Ctx *c, const Program *p, const FusedPlan *fp, c128 *delta, const c128 *const *ftab,
+                          int fbase, c128 *part, int row) {
     const unsigned ntiles = unsigned(u64(1) << (p->nbits - TM));
     for (const PlanItem &it : fp->items) {
         if (!it.sweep) {
@@ -1850,7 +1852,7 @@ static int fused_run_fixp(Ctx *c, const Program *p, const FusedPlan *fp, c128 *d
             continue;
         }
         hipLaunchKernelGGL(k_tile_fixp, dim3(ntiles), dim3(TT), 0, c->stream, delta, it.args, fp->d_segs,
-                           fp->d_groups, fp->d_diag, fp->d_pool, fixed, part, row);
+                           fp->d_groups, fp->d_diag, fp->d_pool, ftab, fbase, part, row);
         MOSSQ_HIP(hipGetLastError());
     }
     return MOSSQ_OK;
@@ -1918,6 +1920,9 @@ struct DensityAdjointPlan {
     std::vector<FusedPlan *> rev;   // 1..m (rev[0] unused): reverse chunk without p_j
     std::vector<FusedPlan *> revp;  // 1..m: reverse chunk, V_j and the probe of G_j (null: use rev[j])
     std::vector<FusedPlan *> segfwd;  // segment k: chunks 1 + k seg .. min(m, (k + 1) seg) as one plan
+    // segment k's whole reverse pass (every reverse chunk, V_j and fixed probe,
+    // from the last parameter down) as one plan; null: per parameter
+    std::vector<FusedPlan *> segrev;
     int seg = 1, nseg = 1;
     void *d_block = nullptr;  // device arrays of every chunk plan
 };
@@ -1941,17 +1946,19 @@ static void dm_plan_delete(DensityAdjointPlan *dp) {
     for (auto *fp : dp->rev) delete fp;
     for (auto *fp : dp->revp) delete fp;
     for (auto *fp : dp->segfwd) delete fp;
+    for (auto *fp : dp->segrev) delete fp;
     delete dp;
 }
 
 // Reverse chunk of parameterized step si through its own reverse matrix, then
 // the fixed probe of its gradient matrix; null when the step is wider than a
 // tile probe takes (two register bits).
-static FusedPlan *plan_reverse_probe(const Program *p, int si, int hi, std::string *err, bool *ok) {
-    *ok = true;
+static bool probe_fits(const Step &st) { return !st.split && st.kb <= 2; }
+
+// Steps [si, hi) from last to first with their reverse matrices, then the
+// fixed probe of step si's gradient matrix.
+static void lower_reverse_probe(const Program *p, int si, int hi, std::vector<POp> &ops) {
     const Step &st = p->steps[si];
-    if (st.split || st.kb > 2) return nullptr;
-    std::vector<POp> ops;
     lower_range(p, si, hi, true, ops);
     POp pr;
     pr.kind = 3;
@@ -1962,6 +1969,13 @@ static FusedPlan *plan_reverse_probe(const Program *p, int si, int hi, std::stri
     pr.probe = st.grad_index;
     pr.fixed = true;
     ops.push_back(pr);
+}
+
+static FusedPlan *plan_reverse_probe(const Program *p, int si, int hi, std::string *err, bool *ok) {
+    *ok = true;
+    if (!probe_fits(p->steps[si])) return nullptr;
+    std::vector<POp> ops;
+    lower_reverse_probe(p, si, hi, ops);
     FusedPlan *fp = best_plan(p, ops, false, err);
     if (!fp) {
         *ok = false;
@@ -1974,38 +1988,60 @@ static FusedPlan *plan_reverse_probe(const Program *p, int si, int hi, std::stri
 // Every plan of a density adjoint plan (null entries skipped).
 static std::vector<FusedPlan *> dm_plans(const DensityAdjointPlan *dp) {
     std::vector<FusedPlan *> out;
-    for (const auto *v : {&dp->fwd, &dp->rev, &dp->revp, &dp->segfwd})
+    for (const auto *v : {&dp->fwd, &dp->rev, &dp->revp, &dp->segfwd, &dp->segrev})
         for (FusedPlan *fp : *v)
             if (fp) out.push_back(fp);
     return out;
 }
 
-static DensityAdjointPlan *dm_plan_build_host(const Program *p, std::string *err) {
+// whole_segments: 1 plans each segment's reverse pass as one plan where its
+// parameterized gates allow, 0 plans it per parameter, -1 picks by size.
+static DensityAdjointPlan *dm_plan_build_host(const Program *p, std::string *err, int whole_segments = -1) {
     auto *dp = new DensityAdjointPlan();
     const int N = int(p->steps.size());
     for (int i = 0; i < N; ++i)
         if (p->steps[i].grad >= 0) dp->psteps.push_back(i);
     const int m = int(dp->psteps.size());
     auto b = [&](int j) { return j == 0 ? 0 : j <= m ? dp->psteps[j - 1] : N; };
+    for (int j = 2; j <= m; ++j)
+        if (p->steps[dp->psteps[j - 1]].grad_index != p->steps[dp->psteps[j - 2]].grad_index + 1) {
+            *err = "gradient rows of a density program are not consecutive";
+            delete dp;
+            return nullptr;
+        }
     dp->fwd.assign(m + 1, nullptr);
     dp->rev.assign(m + 1, nullptr);
     dp->revp.assign(m + 1, nullptr);
-    bool ok = true;
-    for (int j = 0; j <= m && ok; ++j) dp->fwd[j] = plan_range(p, b(j), b(j + 1), false, err, &ok);
-    for (int j = 1; j <= m && ok; ++j) {
-        dp->revp[j] = plan_reverse_probe(p, dp->psteps[j - 1], b(j + 1), err, &ok);
-        if (ok && !dp->revp[j]) dp->rev[j] = plan_range(p, dp->psteps[j - 1] + 1, b(j + 1), true, err, &ok);
-    }
-    if (!ok) {
-        dm_plan_delete(dp);
-        return nullptr;
-    }
     dp->seg = std::max(1, int(std::ceil(std::sqrt(double(std::max(m, 1))))));
     dp->nseg = std::max(1, (m + dp->seg - 1) / dp->seg);
     dp->segfwd.assign(dp->nseg, nullptr);
+    dp->segrev.assign(dp->nseg, nullptr);
+    bool ok = true;
+    for (int j = 0; j <= m && ok; ++j) dp->fwd[j] = plan_range(p, b(j), b(j + 1), false, err, &ok);
     for (int k = 0; k < dp->nseg && ok; ++k) {
-        const int e = std::min(m, (k + 1) * dp->seg);
-        dp->segfwd[k] = plan_range(p, b(1 + k * dp->seg), b(e + 1), false, err, &ok);
+        const int a = 1 + k * dp->seg, e = std::min(m, (k + 1) * dp->seg);
+        dp->segfwd[k] = plan_range(p, b(a), b(e + 1), false, err, &ok);
+        // One plan for the segment's reverse pass reads Delta once per sweep
+        // instead of once per parameter, but its probes wait on their loads
+        // mid-sweep; below 2^22 entries (2^10 tiles, under 4 per CU) those
+        // waits are not hidden and the per-parameter plans measure faster.
+        bool fits = whole_segments < 0 ? p->nbits >= 22 : whole_segments > 0;
+        for (int j = a; j <= e; ++j) fits &= probe_fits(p->steps[dp->psteps[j - 1]]);
+        if (ok && fits) {
+            std::vector<POp> ops;
+            for (int j = e; j >= a; --j) lower_reverse_probe(p, dp->psteps[j - 1], b(j + 1), ops);
+            dp->segrev[k] = best_plan(p, ops, false, err);
+            if (!dp->segrev[k]) {
+                ok = false;
+                break;
+            }
+            dp->segrev[k]->reverse_steps = true;
+            continue;
+        }
+        for (int j = a; j <= e && ok; ++j) {
+            dp->revp[j] = plan_reverse_probe(p, dp->psteps[j - 1], b(j + 1), err, &ok);
+            if (ok && !dp->revp[j]) dp->rev[j] = plan_range(p, dp->psteps[j - 1] + 1, b(j + 1), true, err, &ok);
+        }
     }
     if (!ok) {
         dm_plan_delete(dp);
@@ -2027,6 +2063,9 @@ static int dm_forward_sched(const DensityAdjointPlan *dp, c128 *work, c128 *cons
 
 // Reverse pass from Delta at the program's end down to p_1; stored holds
 // seg - 1 states.
+// run_probe(fp, delta, k, j) runs a plan with fixed probes whose lowest
+// parameter is j in segment k (probe of parameter i reads S_i: ckpt[k] for
+// i = 1 + k seg, else stored[i - 2 - k seg]).
 template <class Run, class Pair, class RunProbe>
 static int dm_reverse_sched(const DensityAdjointPlan *dp, c128 *delta, c128 *const *ckpt, c128 *const *stored,
                             Run run, Pair pair, RunProbe run_probe) {
@@ -2035,9 +2074,13 @@ static int dm_reverse_sched(const DensityAdjointPlan *dp, c128 *delta, c128 *con
         const int a = 1 + k * seg, e = std::min(m, (k + 1) * seg);
         auto S = [&](int j) { return j == a ? ckpt[k] : stored[j - a - 1]; };
         for (int j = a + 1; j <= e; ++j) MOSSQ_TRY(run(dp->fwd[j - 1], S(j - 1), S(j)));
+        if (dp->segrev[k]) {
+            MOSSQ_TRY(run_probe(dp->segrev[k], delta, k, a));
+            continue;
+        }
         for (int j = e; j >= a; --j) {
             if (dp->revp[j]) {
-                MOSSQ_TRY(run_probe(dp->revp[j], delta, S(j)));
+                MOSSQ_TRY(run_probe(dp->revp[j], delta, k, j));
             } else {
                 MOSSQ_TRY(run(dp->rev[j], delta, delta));
                 MOSSQ_TRY(pair(dp->psteps[j - 1], delta, S(j)));
@@ -2111,8 +2154,18 @@ int density_adjoint_forward(Ctx *c, const Program *p, c128 *work, c128 *const *c
     });
 }
 
+// Table of the states the fixed probes read: entry k seg + i holds S_{1 + k seg + i}.
+void density_adjoint_probe_table(const Program *p, c128 *const *ckpt, c128 *const *stored,
+                                 std::vector<const c128 *> &tab) {
+    const DensityAdjointPlan *dp = p->dm_adj;
+    tab.assign(size_t(dp->nseg) * dp->seg, nullptr);
+    for (int k = 0; k < dp->nseg; ++k)
+        for (int i = 0; i < dp->seg; ++i) tab[size_t(k) * dp->seg + i] = i == 0 ? ckpt[k] : stored[i - 1];
+}
+
 int density_adjoint_reverse(Ctx *c, const Program *p, c128 *delta, c128 *const *ckpt, c128 *const *stored,
-                            c128 *part, int row) {
+                            const c128 *const *d_tab, c128 *part, int row) {
+    const DensityAdjointPlan *dp = p->dm_adj;
     return dm_reverse_sched(
         p->dm_adj, delta, ckpt, stored,
         [&](const FusedPlan *fp, c128 *src, c128 *dst) { return fused_run_plan(c, p, fp, src, dst); },
@@ -2121,7 +2174,11 @@ int density_adjoint_reverse(Ctx *c, const Program *p, c128 *delta, c128 *const *
             return launch_dense_pair(c, d, S, p->nbits, st.kb, st.bits, p->d_pool + st.rev, p->d_pool + st.grad,
                                      part + size_t(st.grad_index) * row, true);
         },
-        [&](const FusedPlan *fp, c128 *d, c128 *S) { return fused_run_fixp(c, p, fp, d, S, part, row); });
+        [&](const FusedPlan *fp, c128 *d, int k, int j) {
+            const int a = 1 + k * dp->seg;
+            return fused_run_fixp(c, p, fp, d, d_tab + size_t(k) * dp->seg + (j - a),
+                                  p->steps[dp->psteps[j - 1]].grad_index, part, row);
+        });
 }
 
 int density_adjoint_parts(const Program *p) {
@@ -2155,8 +2212,13 @@ static void emulate_density_adjoint(const Program *p, const DensityAdjointPlan *
         for (u64 i = 0; i < len; ++i) acc = cfmaconj(d[i], dy[i], acc);
         grads[st.grad_index] = cadd(grads[st.grad_index], acc);
         return MOSSQ_OK;
-    }, [&](const FusedPlan *fp, c128 *d, c128 *S) {
-        emulate_plan(p, fp, d, S, grads);
+    }, [&](const FusedPlan *fp, c128 *d, int k, int j) {
+        std::vector<const c128 *> tab(size_t(dp->nseg) * dp->seg);
+        for (int kk = 0; kk < dp->nseg; ++kk)
+            for (int i = 0; i < dp->seg; ++i) tab[size_t(kk) * dp->seg + i] = i == 0 ? ckpt[kk] : stored[i - 1];
+        const int a = 1 + k * dp->seg;
+        emulate_plan(p, fp, d, tab.data() + size_t(k) * dp->seg + (j - a), p->steps[dp->psteps[j - 1]].grad_index,
+                     grads);
         return MOSSQ_OK;
     });
 }
@@ -2220,18 +2282,20 @@ int mossq_plan_emulate_adjoint(int n, int nops, const mossq_op *ops, const doubl
 }
 
 int mossq_plan_emulate_density_adjoint(int n, int nops, const mossq_op *ops, const double *pool, uint64_t pool_len,
-                                       double *rho, double *delta, double *grads, int *stats) {
-    MOSSQ_CHECK(stats && rho && delta && grads, MOSSQ_EINVAL, "null argument");
+                                       int whole_segments, double *rho, double *delta, double *grads, int *stats) {
+    MOSSQ_CHECK(stats && (!rho || (delta && grads)), MOSSQ_EINVAL, "null argument");
     Program p;
     MOSSQ_TRY(program_build_host(&p, n, 1, nops, ops, pool, pool_len));
     MOSSQ_CHECK(p.nbits >= TM, MOSSQ_EINVAL, "plans tile registers of at least 12 bits");
     MOSSQ_CHECK(p.nderiv > 0, MOSSQ_EINVAL, "the program has no parameterized steps");
     std::string err;
-    DensityAdjointPlan *dp = dm_plan_build_host(&p, &err);
+    DensityAdjointPlan *dp = dm_plan_build_host(&p, &err, whole_segments < 0 ? -1 : whole_segments > 0);
     MOSSQ_CHECK(dp, MOSSQ_EINVAL, err);
-    std::vector<c128> g(p.nderiv, mk(0, 0));
-    emulate_density_adjoint(&p, dp, (c128 *)rho, (c128 *)delta, g.data());
-    for (int j = 0; j < p.nderiv; ++j) grads[j] = g[j].re;
+    if (rho) {
+        std::vector<c128> g(p.nderiv, mk(0, 0));
+        emulate_density_adjoint(&p, dp, (c128 *)rho, (c128 *)delta, g.data());
+        for (int j = 0; j < p.nderiv; ++j) grads[j] = g[j].re;
+    }
     for (int i = 0; i < 6; ++i) stats[i] = 0;
     stats[0] = int(dp->psteps.size());
     stats[1] = dp->seg;
@@ -2245,7 +2309,8 @@ int mossq_plan_emulate_density_adjoint(int n, int nops, const mossq_op *ops, con
     count(dp->fwd[0], 2);
     for (const FusedPlan *fp : dp->segfwd) count(fp, 2);
     for (size_t j = 1; j < dp->fwd.size(); ++j) count(dp->fwd[j], 3);
-    for (const FusedPlan *fp : dp->rev) count(fp, 3);
+    for (const auto *v : {&dp->rev, &dp->revp, &dp->segrev})
+        for (const FusedPlan *fp : *v) count(fp, 3);
     stats[5] = int(dm_plans(dp).size());
     dm_plan_delete(dp);
     return MOSSQ_OK;

@@ -303,6 +303,22 @@ def test_fused_density_adjoint(ctx, n):
     assert abs(v2 - want_v) < EXP and np.max(np.abs(g2 - want)) < GRAD
 
 
+def test_fused_density_adjoint_segment_plans(ctx):
+    """At 11 qubits (22 bits) each checkpoint segment's reverse pass runs as one
+    plan with a fixed probe per parameter; against the step-by-step density
+    adjoint, which the smaller cases pin to the oracle."""
+    n = 11
+    tpl = _noisy_param_circuit(n, 2, n)
+    th = np.array(tpl.active_parameters(), dtype=float)
+    th[: 4 * n] = np.random.default_rng(n).uniform(0, 2 * np.pi, 4 * n)
+    res = {}
+    for fuse in (True, False):
+        obj = mq.ExpectationObjective(tpl, tfim(n), mq.DensityMatrix.zeros(n), backend_=mq.HIPBackend(ctx, fuse=fuse))
+        res[fuse] = mq.value_and_gradient(mq.AdjointGradient(), obj, th)
+    assert abs(res[True][0] - res[False][0]) < EXP
+    assert np.max(np.abs(res[True][1] - res[False][1])) < GRAD
+
+
 def test_adjoint_density_golden(ctx):
     f = np.load(GOLD / "c2_noisy4.npz")
     n, L = 4, 2

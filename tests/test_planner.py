@@ -177,11 +177,14 @@ def test_density_adjoint_plan_matches_oracle(n, layers):
     for rho0 in (zero, mixed):
         r0 = rho0.reshape(-1, order="F")
         _, want, want_state = oracle.adjoint(lw, H.groups(), state_in=r0, want_state=True)
-        rho, _, g, st = plan_emulate_density_adjoint(circ, n, r0, seed)
-        assert st["param_steps"] == lw.nderiv and st["chunk_plans"] >= lw.nderiv
-        assert st["unfused"] >= 2
-        assert np.max(np.abs(rho - want_state)) < AMP
-        assert np.max(np.abs(g - want)) < 1e-9
+        # reverse plans per parameter, and one per checkpoint segment (several
+        # fixed probes per plan; the segment holding the two-qubit
+        # parameterized gate falls back to per-parameter plans)
+        for whole in (0, 1):
+            rho, _, g, st = plan_emulate_density_adjoint(circ, n, r0, seed, whole)
+            assert st["param_steps"] == lw.nderiv and st["unfused"] >= 2
+            assert np.max(np.abs(rho - want_state)) < AMP
+            assert np.max(np.abs(g - want)) < 1e-9, whole
 
 
 def test_density_adjoint_plan_checkpoint_segments():
@@ -205,7 +208,8 @@ def test_density_adjoint_plan_checkpoint_segments():
         circ = mq.QCircuit(ops)
         lw = lower(circ, n, True)
         _, want, want_state = oracle.adjoint(lw, H.groups(), state_in=zero, want_state=True)
-        rho, _, g, st = plan_emulate_density_adjoint(circ, n, zero, seed)
-        assert st["param_steps"] == lw.nderiv
-        assert np.max(np.abs(rho - want_state)) < AMP
-        assert np.max(np.abs(g - want)) < 1e-9
+        for whole in (0, 1):
+            rho, _, g, st = plan_emulate_density_adjoint(circ, n, zero, seed, whole)
+            assert st["param_steps"] == lw.nderiv
+            assert np.max(np.abs(rho - want_state)) < AMP
+            assert np.max(np.abs(g - want)) < 1e-9
