@@ -222,6 +222,56 @@ def bench_density(ctx, n=13, layers=8, seed=20260213, reps=2):
     return out
 
 
+def bench_c0(ctx, n=12, layers=10, seed=20260012, reps=50):
+    """The reference's CPU-runnable case: 12 qubits, 10 ansatz layers (350
+    gates, 240 parameters), TFIM energy.  A latency case: forward run eager and
+    as a captured HIP graph replay, the adjoint gradient, and the
+    parameter-shift gradient (481 evaluations in one batched engine run)."""
+    import mossquantum_amd as mq
+    from mossquantum_amd.workloads import hea_angles, hea_circuit, tfim
+    th = hea_angles(n, layers, seed)
+    tpl = hea_circuit(n, layers)
+    c = tpl.bind(th)
+    out = {"n": n, "layers": layers, "gates": len(list(c.flat())), "parameters": len(th)}
+    for fuse in (True, False):
+        prog = mq.Program(c, n, False, ctx, fuse=fuse)
+        sv = mq.StateVector.zeros(n, ctx)
+        graph = prog.capture(sv.storage)
+        for run in (lambda: prog.run(sv.storage), graph.launch):
+            run()
+        ctx.sync()
+        key = "fused" if fuse else "unfused"
+        res = {"sweeps": prog.sweeps()}
+        for name, run in (("eager", lambda: prog.run(sv.storage)), ("graph", graph.launch)):
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.record(8)
+            for _ in range(reps):
+                run()
+            ctx.record(9)
+            ctx.sync()
+            res[f"{name}_us_per_run"] = (time.perf_counter() - t0) / reps * 1e6
+            res[f"{name}_device_us_per_run"] = ctx.elapsed_ms(8, 9) * 1e3 / reps
+        out[key] = res
+        log(f"c0 forward {key}: eager {res['eager_us_per_run']:.1f} us, graph {res['graph_us_per_run']:.1f} us "
+            f"per run ({res['sweeps']} sweeps)")
+    obj = mq.ExpectationObjective(tpl, tfim(n), mq.StateVector.zeros(n, ctx))
+    mq.value_and_gradient(mq.AdjointGradient(), obj, th)
+    t0 = time.perf_counter()
+    v, g = mq.value_and_gradient(mq.AdjointGradient(), obj, th)
+    out["adjoint_ms"] = (time.perf_counter() - t0) * 1e3
+    out["energy"] = v
+    est = mq.estimate_gradient(mq.ParameterShift(), obj, th)
+    t0 = time.perf_counter()
+    est = mq.estimate_gradient(mq.ParameterShift(), obj, th)
+    out["parameter_shift_ms"] = (time.perf_counter() - t0) * 1e3
+    out["parameter_shift_evaluations"] = est.evaluations
+    out["parameter_shift_vs_adjoint_max_abs_diff"] = float(np.max(np.abs(est.gradient - g)))
+    log(f"c0 adjoint {out['adjoint_ms']:.2f} ms; parameter shift {out['parameter_shift_ms']:.1f} ms "
+        f"({est.evaluations} evaluations)")
+    return out
+
+
 def bench_serving(ctx, rates=(50.0, 200.0, 2000.0), nreq=300, seed=20260400, budget=200 << 30):
     """Poisson request stream: 70% pure states n ~ U{8..24}, 30% density
     matrices n ~ U{6..12} with the noisy channels; 4-8 layers; TFIM energy +
@@ -335,6 +385,7 @@ def main():
     norm = mq.dot(sv, sv).real
     extra["final_norm_minus_one"] = norm - 1.0
     del sv
+    extra["c0_hea12"] = bench_c0(ctx)
     if not args.no_batched:
         extra["batched_adjoint"] = bench_batched(ctx, cpu=not args.no_cpu)
     if not args.no_density:

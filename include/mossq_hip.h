@@ -45,6 +45,7 @@ extern "C" {
 typedef struct mossq_ctx mossq_ctx;
 typedef struct mossq_program mossq_program;
 typedef struct mossq_obs mossq_obs;
+typedef struct mossq_graph mossq_graph;
 
 /* Instruction kinds of a program (a lowered QCircuit). */
 #define MOSSQ_OP_GATE 0    /* unitary gate: mat = 2^k x 2^k ordered_mat */
@@ -169,6 +170,16 @@ int mossq_program_sweeps(const mossq_program *prog, int *sweeps);
 /* Apply the program in place (reference apply!(circuit, state),
  * src/api/backend_protocols.jl:246-254). */
 int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state);
+/* Record one run of the program on `state` as a HIP graph (stream capture of
+ * the same launches mossq_program_run makes) and instantiate it; each
+ * mossq_graph_launch replays the whole run with one launch call on the
+ * context's stream.  The program and the state buffer must outlive the graph.
+ * This is the bulk-replay form of the reference's simulate_circuit! hook
+ * (src/execution/simulation.jl:166-175) for circuits run many times on one
+ * buffer; the host launch cost of a run drops from one call per sweep to one. */
+int mossq_program_capture(mossq_ctx *ctx, const mossq_program *prog, void *state, mossq_graph **out);
+int mossq_graph_launch(mossq_ctx *ctx, mossq_graph *graph);
+int mossq_graph_destroy(mossq_graph *graph);
 /* Host execution of the fused plan the planner builds for these records, with
  * the tile kernel's index arithmetic; needs no device.  Used to test the
  * planner (the reference's fuse_gates, src/utilities/algorithms/

@@ -98,6 +98,9 @@ SIGNATURES = [
     ("mossq_plan_emulate_adjoint", _I, [_I, _I, C.POINTER(OpRecord), _DP, _U64, _DP, _DP, _DP, C.POINTER(C.c_int)]),
     ("mossq_plan_emulate_density_adjoint", _I,
      [_I, _I, C.POINTER(OpRecord), _DP, _U64, _I, _DP, _DP, _DP, C.POINTER(C.c_int)]),
+    ("mossq_program_capture", _I, [_P, _P, _P, C.POINTER(_P)]),
+    ("mossq_graph_launch", _I, [_P, _P]),
+    ("mossq_graph_destroy", _I, [_P]),
     ("mossq_adjoint", _I, [_P, _P, _P, _P, _I, _P, _DP, _DP]),
     ("mossq_batch_expectation", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _DP]),
     ("mossq_batch_adjoint", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _DP, _DP]),

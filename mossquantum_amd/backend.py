@@ -159,10 +159,35 @@ class Program:
     def run(self, storage: HIPArray):
         check(self.ctx.lib.mossq_program_run(self.ctx.h, self.h, storage.ptr))
 
+    def capture(self, storage: HIPArray) -> "ProgramGraph":
+        """A HIP graph replaying this program on `storage` (one launch call per run)."""
+        h = C.c_void_p()
+        check(self.ctx.lib.mossq_program_capture(self.ctx.h, self.h, storage.ptr, C.byref(h)))
+        return ProgramGraph(self, storage, h)
+
     def __del__(self):
         try:
             if self.h and self.ctx.h:
                 self.ctx.lib.mossq_program_destroy(self.h)
+            self.h = None
+        except Exception:
+            pass
+
+
+class ProgramGraph:
+    """A captured run of a Program on one device buffer; `launch()` replays it.
+    Holds the program and the buffer so both outlive the graph."""
+
+    def __init__(self, program: Program, storage: HIPArray, h):
+        self.program, self.storage, self.ctx, self.h = program, storage, program.ctx, h
+
+    def launch(self):
+        check(self.ctx.lib.mossq_graph_launch(self.ctx.h, self.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.mossq_graph_destroy(self.h)
             self.h = None
         except Exception:
             pass

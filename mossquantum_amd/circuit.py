@@ -181,8 +181,9 @@ class Lowered:
         self.pool_len = 0
         self.nslots = 0
         self.grad_slot: list[int] = []
-        # (pool entry of the gate matrix, gate, first slot) per parameterized gate
-        self.param_gates: list[tuple[int, Gate, int]] = []
+        # (pool entry of the gate matrix, gate, first slot, pool offset) per parameterized gate
+        self.param_gates: list[tuple[int, Gate, int, int]] = []
+        self._rebind = None
 
     def _push(self, m):
         m = np.asarray(m, dtype=np.complex128)
@@ -213,7 +214,7 @@ class Lowered:
         if ders:
             if len(ders) != op.nparameters():
                 raise ValueError("derivative gates must match the gate's active parameters")
-            self.param_gates.append((len(self.pool) - 1, op, first_slot))
+            self.param_gates.append((len(self.pool) - 1, op, first_slot, r.mat))
         self.nslots += len(ders)
         if len(ders) > 1:
             # extra derivative gates ride on identity records placed before the
@@ -244,18 +245,61 @@ class Lowered:
         out[np.asarray(self.grad_slot, dtype=np.int64)] = device_grads
         return out
 
+    def _rebind_tables(self):
+        """One-qubit rotations and phases grouped by kind (slot, pool offset),
+        rebuilt vectorised; every other parameterized gate through its object.
+        Derivative matrices (dU)^dagger U of these families do not depend on
+        the angle, so only U is rewritten."""
+        from .gates import PHASEGate, RxGate, RyGate, RzGate
+        kinds = {RxGate: "rx", RyGate: "ry", RzGate: "rz", PHASEGate: "phase"}
+        fast = {k: ([], []) for k in kinds.values()}
+        slow = []
+        for li, op, s0, off in self.param_gates:
+            kind = kinds.get(type(op))
+            if kind is not None and op.nqubits() == 1 and op.nparameters() == 1:
+                fast[kind][0].append(s0)
+                fast[kind][1].append(off)
+            else:
+                slow.append((op, s0, off))
+        base = np.ascontiguousarray(np.concatenate(self.pool)) if self.pool else np.zeros(1, dtype=np.complex128)
+        fast = {k: (np.array(a, dtype=np.int64), np.array(b, dtype=np.int64)) for k, (a, b) in fast.items() if a}
+        self._rebind = (base, fast, slow)
+        return self._rebind
+
     def pool_with_slots(self, slot_values, slots=None):
         """The pool with the parameterized gates' matrices rebuilt at
         `slot_values` (active slots in circuit order); `slots` limits the
         rebuild to the gates holding those slots."""
-        pool = list(self.pool)
-        for li, op, s0 in self.param_gates:
+        base, fast, slow = self._rebind or self._rebind_tables()
+        v = np.asarray(slot_values, dtype=np.float64)
+        pool = base.copy()
+        want = None if slots is None else np.asarray(sorted(set(int(x) for x in slots)), dtype=np.int64)
+        for kind, (sl, off) in fast.items():
+            if want is not None:
+                keep = np.isin(sl, want)
+                sl, off = sl[keep], off[keep]
+                if not sl.size:
+                    continue
+            th = v[sl]
+            c, s = np.cos(th / 2), np.sin(th / 2)
+            if kind == "rx":  # [[c, -is], [-is, c]], column-major
+                vals = (c, -1j * s, -1j * s, c)
+            elif kind == "ry":  # [[c, -s], [s, c]]
+                vals = (c, s, -s, c)
+            elif kind == "rz":  # diag(e^{-i th/2}, e^{i th/2})
+                vals = (np.exp(-0.5j * th), 0.0, 0.0, np.exp(0.5j * th))
+            else:  # diag(1, e^{i th})
+                vals = (1.0, 0.0, 0.0, np.exp(1j * th))
+            for q, x in enumerate(vals):
+                pool[off + q] = x
+        for op, s0, off in slow:
             k = op.nparameters()
-            if slots is not None and not any(s0 <= s < s0 + k for s in slots):
+            if want is not None and not np.any((want >= s0) & (want < s0 + k)):
                 continue
-            g = op.with_parameters([float(v) for v in slot_values[s0:s0 + k]])
-            pool[li] = np.asfortranarray(g.ordered_mat()).reshape(-1, order="F").astype(np.complex128)
-        return np.ascontiguousarray(np.concatenate(pool)) if pool else np.zeros(1, dtype=np.complex128)
+            g = op.with_parameters([float(x) for x in v[s0:s0 + k]])
+            m = np.asfortranarray(g.ordered_mat()).reshape(-1, order="F")
+            pool[off:off + m.size] = m
+        return pool
 
     def arrays(self):
         recs = (OpRecord * max(len(self.records), 1))(*self.records)

@@ -504,6 +504,62 @@ int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state) {
     return run_steps(c, p, (c128 *)state, 0, int(p->steps.size()));
 }
 
+struct mossq_graph {
+    Ctx *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int mossq_program_capture(mossq_ctx *ctx, const mossq_program *prog, void *state, mossq_graph **out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(prog && state && out, MOSSQ_EINVAL, "null argument");
+    const Program *p = reinterpret_cast<const Program *>(prog);
+    Ctx *c = C(ctx);
+    MOSSQ_CHECK(p->ctx == c, MOSSQ_EINVAL, "program belongs to another context");
+    // The run path launches kernels only (no allocation, copy or memset), so
+    // the captured graph holds kernel nodes alone.
+    MOSSQ_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int s = p->fused ? fused_run(c, p, (c128 *)state) : run_steps(c, p, (c128 *)state, 0, int(p->steps.size()));
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (s != MOSSQ_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (s != MOSSQ_OK) return s;
+        MOSSQ_HIP(e);
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        MOSSQ_HIP(ei);
+    }
+    auto *gr = new mossq_graph();
+    gr->ctx = c;
+    gr->graph = g;
+    gr->exec = ex;
+    *out = gr;
+    return MOSSQ_OK;
+}
+
+int mossq_graph_launch(mossq_ctx *ctx, mossq_graph *graph) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(graph, MOSSQ_EINVAL, "null argument");
+    Ctx *c = C(ctx);
+    MOSSQ_CHECK(graph->ctx == c, MOSSQ_EINVAL, "graph belongs to another context");
+    MOSSQ_HIP(hipGraphLaunch(graph->exec, c->stream));
+    return MOSSQ_OK;
+}
+
+int mossq_graph_destroy(mossq_graph *graph) {
+    if (!graph) return MOSSQ_OK;
+    (void)hipSetDevice(graph->ctx->device);
+    (void)hipStreamSynchronize(graph->ctx->stream);
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    return MOSSQ_OK;
+}
+
 int mossq_adjoint(mossq_ctx *ctx, const mossq_program *prog, const mossq_obs *obs, const void *in,
                   int in_is_density, void *out_state, double *value, double *grads) {
     MOSSQ_TRY(ctx_check(ctx));

@@ -20,7 +20,7 @@ import numpy as np
 from . import _lib
 from ._lib import check
 from .backend import DeviceObservable, HIPBackend, Program, backend
-from .circuit import QCircuit
+from .circuit import QCircuit, lower
 from .states import BatchDensityMatrix, BatchStateVector, DensityMatrix, StateVector
 
 
@@ -114,11 +114,26 @@ class BatchExpectationObjective:
         return self.weights
 
 
-def _adjoint_one(b: HIPBackend, circuit: QCircuit, observable, state):
-    """(value, slot gradients) of real(<obs>) for one bound circuit on one state."""
+def _objective_program(objective, flat, state, fuse):
+    """The device program of the objective's template bound to slot values
+    `flat`: the template is lowered once per (qubit count, representation)
+    and later bindings only rewrite the parameterized gates' matrices."""
+    is_dm = isinstance(state, DensityMatrix) or objective.template.has_channel()
+    cache = objective.__dict__.setdefault("_lowered", {})
+    key = (state.n, is_dm)
+    lw = cache.get(key)
+    if lw is None:
+        lw = lower(objective.template.bind(flat), state.n, is_dm)
+        cache[key] = lw
+        pool = None
+    else:
+        pool = lw.pool_with_slots(flat)
+    return Program(None, state.n, is_dm, state.storage.ctx, fuse=fuse, lowered=lw, pool=pool)
+
+
+def _adjoint_one(b: HIPBackend, prog: Program, observable, state):
+    """(value, slot gradients) of real(<obs>) for one program on one state."""
     ctx = state.storage.ctx
-    is_dm = isinstance(state, DensityMatrix) or circuit.has_channel()
-    prog = Program(circuit, state.n, is_dm, ctx, fuse=b.fuse)
     dob = DeviceObservable(observable, state.n, ctx)
     nd = prog.lowered.nderiv
     grads = np.zeros(max(nd, 1))
@@ -170,16 +185,18 @@ def value_and_gradient(method, objective, values, rng=None):
     if not supports_gradient(method, objective):
         raise UnsupportedGradient(method, objective)
     pm = objective.parameter_map
-    bound = objective.template.bind(pm.expand(values))
+    flat = pm.expand(values)
+    b = objective.backend
     if isinstance(objective, BatchExpectationObjective):
         w = objective.sample_weights()
         total, grad = 0.0, np.zeros(pm.count)
         for wi, col in zip(w, objective.initial_state.columns):
-            v, g = _adjoint_one(objective.backend, bound, objective.observable, col)
+            v, g = _adjoint_one(b, _objective_program(objective, flat, col, b.fuse), objective.observable, col)
             total += wi * v
             grad += wi * pm.accumulate(g)
         return total, grad
-    v, g = _adjoint_one(objective.backend, bound, objective.observable, objective.initial_state)
+    st = objective.initial_state
+    v, g = _adjoint_one(b, _objective_program(objective, flat, st, b.fuse), objective.observable, st)
     return v, pm.accumulate(g)
 
 

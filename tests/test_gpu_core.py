@@ -398,3 +398,30 @@ def test_unsupported_before_payload(ctx):
     st = ctx.lib.mossq_sv_apply_gate(ctx.h, sv.storage.ptr, 3, 2, mq._lib.iptr(bad),
                                      eye.view(np.float64).ctypes.data_as(mq._lib._DP))
     assert st == mq._lib.MOSSQ_EINVAL
+
+
+@pytest.mark.parametrize("n,fuse", [(13, True), (8, False), (8, True)])
+def test_program_graph_replay(ctx, n, fuse):
+    """A captured program run replays on its buffer: after each replay (with
+    eager work and synchronisations in between) the buffer equals the eager
+    path applied as many times, and the first replay equals the oracle."""
+    rng = np.random.default_rng(70 + n)
+    c = random_circuit(n, 6, haar_layers(n, 6, 70 + n))
+    c.append(mq.QuantumGate((1, 3, n), haar_unitary(rng, 8)))  # an unfused three-qubit step
+    prog = mq.Program(c, n, False, ctx, fuse=fuse)
+    v = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    v /= np.linalg.norm(v)
+    a = mq.StateVector.from_host(v, ctx)
+    b = mq.StateVector.from_host(v, ctx)
+    other = mq.StateVector.zeros(n, ctx)
+    graph = prog.capture(a.storage)
+    want = v.copy()
+    oracle.run_program(want, lower(c, n, False))
+    for r in range(3):
+        graph.launch()
+        prog.run(b.storage)
+        mq.apply_(mq.HGate(1), other)  # unrelated eager work between replays
+        ha = a.to_host()
+        assert np.max(np.abs(ha - b.to_host())) < AMP, r
+        if r == 0:
+            assert np.max(np.abs(ha - want)) < AMP
