@@ -195,14 +195,16 @@ class ProgramGraph:
 
 def plan_emulate(circuit: QCircuit, n: int, is_density: bool, state: np.ndarray):
     """Run the fused plan of `circuit` on a host vector with the tile kernel's
-    index arithmetic (no device).  Returns (new state, stats dict)."""
+    index arithmetic (no device).  state None: plan statistics only.
+    Returns (new state, stats dict)."""
     lw = lower(circuit, n, is_density)
     recs, pool = lw.arrays()
-    st = np.ascontiguousarray(np.array(state, dtype=np.complex128).reshape(-1, order="F"))
+    st = None if state is None else np.ascontiguousarray(np.array(state, dtype=np.complex128).reshape(-1, order="F"))
     stats = (C.c_int * 9)()
     check(_lib.load().mossq_plan_emulate(n, int(is_density), len(lw.records), recs,
                                          pool.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), pool.size,
-                                         st.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), stats))
+                                         None if st is None else st.view(np.float64).ctypes.data_as(
+                                             C.POINTER(C.c_double)), stats))
     keys = ("sweeps", "unfused", "segments", "transitions", "tile_ops", "dense1", "dense2", "diag",
             "conflicted_transitions")
     return st, dict(zip(keys, list(stats)))
