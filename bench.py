@@ -272,6 +272,42 @@ def bench_c0(ctx, n=12, layers=10, seed=20260012, reps=50):
     return out
 
 
+def bench_density_adjoint(ctx, n=12, layers=8, seed=20260412):
+    """Value and full gradient of the TFIM energy of a 12-qubit noisy density
+    matrix (the serving mix's largest density request: 8 layers, 192
+    parameters, 1016 instructions), fused chunk plans against the step-by-step
+    reverse pass; device time from HIP events around the library call."""
+    import ctypes as C
+    import mossquantum_amd as mq
+    from mossquantum_amd import _lib
+    from mossquantum_amd.backend import DeviceObservable, Program
+    from mossquantum_amd.workloads import hea_angles, hea_circuit, tfim
+    c = hea_circuit(n, layers, hea_angles(n, layers, seed), noise=(0.01, 0.02))
+    lw = mq.lower(c, n, True)
+    dob = DeviceObservable(tfim(n), n, ctx)
+    out = {"n": n, "layers": layers, "instructions": len(lw.records), "parameters": lw.nderiv}
+    grads = {}
+    for fuse in (True, False):
+        prog = Program(None, n, True, ctx, fuse=fuse, lowered=lw)
+        g = np.zeros(lw.nderiv)
+        v = C.c_double()
+        st = mq.DensityMatrix.zeros(n, ctx)
+        ms = []
+        for _ in range(2):
+            ctx.sync()
+            ctx.record(8)
+            _lib.check(ctx.lib.mossq_adjoint(ctx.h, prog.h, dob.h, st.storage.ptr, 1, None, C.byref(v), _lib.dptr(g)))
+            ctx.record(9)
+            ms.append(ctx.elapsed_ms(8, 9))
+        key = "fused" if fuse else "step_by_step"
+        out[key] = {"ms": min(ms), "value": v.value}
+        grads[key] = g.copy()
+        log(f"density adjoint {key}: {min(ms):.1f} ms")
+    out["speedup"] = out["step_by_step"]["ms"] / out["fused"]["ms"]
+    out["max_abs_grad_diff"] = float(np.max(np.abs(grads["fused"] - grads["step_by_step"])))
+    return out
+
+
 def bench_serving(ctx, rates=(50.0, 200.0, 2000.0), nreq=300, seed=20260400, budget=200 << 30):
     """Poisson request stream: 70% pure states n ~ U{8..24}, 30% density
     matrices n ~ U{6..12} with the noisy channels; 4-8 layers; TFIM energy +
@@ -318,6 +354,8 @@ def main():
     ap.add_argument("--no-density", action="store_true", help="skip the 13-qubit density-matrix leg")
     ap.add_argument("--serving-requests", type=int, default=300,
                     help="requests per arrival rate in the serving leg (0 skips it)")
+    ap.add_argument("--leg", choices=("c0", "batched", "density", "density-adjoint", "serving"),
+                    help="run only this secondary leg and print its JSON (for profiling one workload)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -327,6 +365,14 @@ def main():
 
     import mossquantum_amd as mq
     from mossquantum_amd.workloads import haar_layers, random_circuit
+
+    if args.leg:
+        ctx = mq.default_context()
+        legs = {"c0": lambda: bench_c0(ctx), "batched": lambda: bench_batched(ctx, cpu=not args.no_cpu),
+                "density": lambda: bench_density(ctx), "density-adjoint": lambda: bench_density_adjoint(ctx),
+                "serving": lambda: bench_serving(ctx, nreq=args.serving_requests)}
+        print(json.dumps({"leg": args.leg, "result": legs[args.leg]()}))
+        return
 
     gold = ROOT / "tests" / "golden" / "c1_haar26.npz"
     U = np.load(gold)["unitaries"] if gold.exists() else haar_layers(N, DEPTH, 20260126)
@@ -390,6 +436,7 @@ def main():
         extra["batched_adjoint"] = bench_batched(ctx, cpu=not args.no_cpu)
     if not args.no_density:
         extra["density_13q"] = bench_density(ctx)
+        extra["density_adjoint_12q"] = bench_density_adjoint(ctx)
     if args.serving_requests > 0:
         extra["serving"] = bench_serving(ctx, nreq=args.serving_requests)
 
