@@ -354,6 +354,7 @@ def main():
     ap.add_argument("--no-density", action="store_true", help="skip the 13-qubit density-matrix leg")
     ap.add_argument("--serving-requests", type=int, default=300,
                     help="requests per arrival rate in the serving leg (0 skips it)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the planner's cost-model plan for the headline")
     ap.add_argument("--leg", choices=("c0", "batched", "density", "density-adjoint", "serving"),
                     help="run only this secondary leg and print its JSON (for profiling one workload)")
     args = ap.parse_args()
@@ -384,6 +385,15 @@ def main():
 
     sv = mq.StateVector.zeros(N, ctx)
     fused = mq.Program(circ, N, False, ctx, fuse=True)
+    autotune = None
+    if not args.no_autotune:
+        # plan selection by measurement (outside the timed region): every
+        # planner variant timed on a scratch state, the fastest kept
+        model_ms = time_program(ctx, fused, sv, 2, 1, 0, 1) / 2
+        variants, tuned_ms = fused.autotune(reps=2)
+        autotune = {"variants_timed": variants, "model_choice_ms": model_ms, "tuned_ms": tuned_ms}
+        log(f"autotune: {variants} plan variants, model choice {model_ms:.3f} ms, kept {tuned_ms:.3f} ms")
+        sv = mq.StateVector.zeros(N, ctx)
     nsweeps = fused.sweeps()
     log(f"fused plan: {nsweeps} sweeps per step")
     ms_f = time_program(ctx, fused, sv, args.steps, args.warmup, 0, 1)
@@ -393,7 +403,8 @@ def main():
 
     extra = {"fused": {"ms_per_step": step_f, "gates_per_s": ngates / (step_f / 1e3), "sweeps": nsweeps,
                        "hbm_gbps": sweep_bytes / (step_f / 1e3) / 1e9,
-                       "effective_gbps_unfused_bytes": alg_bytes / (step_f / 1e3) / 1e9}}
+                       "effective_gbps_unfused_bytes": alg_bytes / (step_f / 1e3) / 1e9,
+                       "autotune": autotune}}
     # roofline of the dominant kernel: every fused launch is one k_tile sweep
     tile_avg_ms = step_f / nsweeps
     tile_bytes = 32 * 2 ** N

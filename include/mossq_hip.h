@@ -170,6 +170,13 @@ int mossq_program_sweeps(const mossq_program *prog, int *sweeps);
 /* Apply the program in place (reference apply!(circuit, state),
  * src/api/backend_protocols.jl:246-254). */
 int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state);
+/* Autotune a fused program: plan every forward-plan variant (tail thresholds
+ * 0/3/5/8, with and without pairing one-qubit gates into 4x4 ops), time each
+ * distinct one `reps` times on a scratch state of the program's size, and keep
+ * the fastest (the planner's cost-model choice is one candidate).  variants =
+ * distinct plans timed (0 for an unfused program), best_ms = the kept plan's
+ * time per run.  Graphs captured earlier keep the plan they recorded. */
+int mossq_program_autotune(mossq_ctx *ctx, mossq_program *prog, int reps, int *variants, double *best_ms);
 /* Record one run of the program on `state` as a HIP graph (stream capture of
  * the same launches mossq_program_run makes) and instantiate it; each
  * mossq_graph_launch replays the whole run with one launch call on the

@@ -159,6 +159,14 @@ class Program:
     def run(self, storage: HIPArray):
         check(self.ctx.lib.mossq_program_run(self.ctx.h, self.h, storage.ptr))
 
+    def autotune(self, reps: int = 3) -> tuple[int, float]:
+        """Time every forward-plan variant on the device and keep the fastest;
+        returns (variants timed, ms per run of the kept plan)."""
+        nv = C.c_int(0)
+        best = C.c_double(0)
+        check(self.ctx.lib.mossq_program_autotune(self.ctx.h, self.h, int(reps), C.byref(nv), C.byref(best)))
+        return nv.value, best.value
+
     def capture(self, storage: HIPArray) -> "ProgramGraph":
         """A HIP graph replaying this program on `storage` (one launch call per run)."""
         h = C.c_void_p()

@@ -57,6 +57,9 @@ int fused_plan_build(Program *p);
 void fused_plan_free(Program *p);
 int fused_plan_sweeps(const Program *p);
 int fused_run(Ctx *c, const Program *p, c128 *state);
+// Time every forward-plan variant on a scratch state (reps runs each) and keep
+// the fastest as p->fused; tried = distinct variants, best_ms = its run time.
+int fused_plan_autotune(Program *p, int reps, int *tried, double *best_ms);
 // Reverse pass of the pure-state adjoint on (y, l) with the adjoint plan;
 // gradient partials into rows of `row` entries (zeroed by the caller).
 int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, int row);

@@ -504,6 +504,14 @@ int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state) {
     return run_steps(c, p, (c128 *)state, 0, int(p->steps.size()));
 }
 
+int mossq_program_autotune(mossq_ctx *ctx, mossq_program *prog, int reps, int *variants, double *best_ms) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(prog && variants && best_ms && reps >= 1, MOSSQ_EINVAL, "bad autotune arguments");
+    Program *p = reinterpret_cast<Program *>(prog);
+    MOSSQ_CHECK(p->ctx == C(ctx), MOSSQ_EINVAL, "program belongs to another context");
+    return fused_plan_autotune(p, reps, variants, best_ms);
+}
+
 struct mossq_graph {
     Ctx *ctx = nullptr;
     hipGraph_t graph = nullptr;

@@ -824,7 +824,7 @@ static void absorb_diag(POp &M, const POp &D) {
 // Peephole on one segment's op list (in execution order): merge dense ops on
 // identical bits, absorb register-bit diagonals into the dense ops before them
 // (pairing two one-bit gates into a two-bit gate when the diagonal spans both).
-static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg) {
+static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, bool kron = true) {
     auto in_reg = [&](int gb) {
         for (int i = 0; i < nreg; ++i)
             if (reg_global[i] == gb) return true;
@@ -923,7 +923,7 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg) {
             }
         }
     }
-    for (size_t i = 0; i < ops.size(); ++i) {
+    for (size_t i = 0; kron && i < ops.size(); ++i) {
         if (ops[i].kind != 0 || ops[i].nb != 1) continue;
         const int a = ops[i].bits[0];
         if (!in_reg(a)) continue;
@@ -1060,7 +1060,7 @@ static bool safe_layout(const int *reg) {
 
 static int encode_ops(FusedPlan *fp, std::string *err);
 
-static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, int min_tail, bool adjoint,
+static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, int min_tail, bool adjoint, bool kron,
                                  std::string *err) {
     FusedPlan *fp = new FusedPlan();
     fp->adjoint = adjoint;
@@ -1258,7 +1258,7 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
             for (int ci : sp.idx) seg_ops.push_back(ops[chosen[ci]]);
             int reg_global[TRB];
             for (int i = 0; i < TRB; ++i) reg_global[i] = it.args.tbits[reg[i]];
-            peephole(seg_ops, reg_global, TRB);
+            peephole(seg_ops, reg_global, TRB, kron);
             for (const POp &o : seg_ops) {
                 TOp t{};
                 t.moff = int(fp->h_pool.size());
@@ -1520,11 +1520,11 @@ static double plan_cost(const FusedPlan *fp) {
 // the next sweep; which side wins depends on the circuit, so each candidate is
 // planned and the cheapest plan under the cost model is kept.
 static FusedPlan *best_plan(const Program *p, const std::vector<POp> &ops, bool adjoint, std::string *err) {
-    if (const char *mt = getenv("MOSSQ_PLAN_MIN_TAIL")) return plan_with_tail(p, ops, atoi(mt), adjoint, err);
+    if (const char *mt = getenv("MOSSQ_PLAN_MIN_TAIL")) return plan_with_tail(p, ops, atoi(mt), adjoint, true, err);
     FusedPlan *best = nullptr;
     double best_cost = 0;
     for (int min_tail : {0, 3, 5, 8}) {
-        FusedPlan *fp = plan_with_tail(p, ops, min_tail, adjoint, err);
+        FusedPlan *fp = plan_with_tail(p, ops, min_tail, adjoint, true, err);
         if (!fp) {
             delete best;
             return nullptr;
@@ -1880,6 +1880,79 @@ int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, in
                            fp->d_groups, fp->d_diag, fp->d_pool, part, row);
         MOSSQ_HIP(hipGetLastError());
     }
+    return MOSSQ_OK;
+}
+
+// ------------------------------------------------------------- autotuner ----
+// The planner's cost model picks one forward plan; the autotuner plans every
+// variant (tail thresholds 0/3/5/8, with and without pairing one-qubit gates
+// into 4x4 ops), times each distinct one on the device, and keeps the fastest.
+static bool same_plan(const FusedPlan *a, const FusedPlan *b) {
+    if (a->nsweeps != b->nsweeps || a->ntransitions != b->ntransitions || a->items.size() != b->items.size() ||
+        a->h_groups.size() != b->h_groups.size() || a->h_segs.size() != b->h_segs.size())
+        return false;
+    for (size_t i = 0; i < a->h_groups.size(); ++i)
+        if (a->h_groups[i].code != b->h_groups[i].code || a->h_groups[i].ncz != b->h_groups[i].ncz) return false;
+    return true;
+}
+
+int fused_plan_autotune(Program *p, int reps, int *tried, double *best_ms) {
+    *tried = 0;
+    *best_ms = 0;
+    if (!p->fused) return MOSSQ_OK;  // unfused program: nothing to choose
+    Ctx *c = p->ctx;
+    std::vector<POp> ops;
+    lower_steps(p, ops);
+    std::vector<FusedPlan *> cand;
+    std::string err;
+    for (bool kron : {true, false})
+        for (int tail : {0, 3, 5, 8}) {
+            FusedPlan *fp = plan_with_tail(p, ops, tail, false, kron, &err);
+            if (!fp) continue;
+            bool dup = same_plan(fp, p->fused);
+            for (const FusedPlan *q : cand) dup = dup || same_plan(q, fp);
+            if (dup)
+                delete fp;
+            else
+                cand.push_back(fp);
+        }
+    cand.insert(cand.begin(), p->fused);  // the model's choice is candidate 0
+    const u64 len = u64(1) << p->nbits;
+    c128 *scratch = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int s = dev_alloc(c, len * sizeof(c128), (void **)&scratch);
+    std::vector<double> ms(cand.size(), 1e300);
+    if (s == MOSSQ_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        s = fail(MOSSQ_EDEVICE, "hipEventCreate failed");
+    if (s == MOSSQ_OK) s = fill_basis0(c, scratch, len);
+    for (size_t i = 0; i < cand.size() && s == MOSSQ_OK; ++i) {
+        if (i > 0) s = fused_plan_upload(c, cand[i]);
+        if (s == MOSSQ_OK) s = fused_run_plan(c, p, cand[i], scratch, scratch);  // warm-up
+        if (s != MOSSQ_OK) break;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) s = fail(MOSSQ_EDEVICE, "hipEventRecord failed");
+        for (int r = 0; r < reps && s == MOSSQ_OK; ++r) s = fused_run_plan(c, p, cand[i], scratch, scratch);
+        float t = 0;
+        if (s == MOSSQ_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                              hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            s = fail(MOSSQ_EDEVICE, "timing a plan variant failed");
+        ms[i] = double(t) / std::max(reps, 1);
+    }
+    size_t best = 0;
+    if (s == MOSSQ_OK)
+        for (size_t i = 1; i < cand.size(); ++i)
+            if (ms[i] < ms[best]) best = i;
+    for (size_t i = 1; i < cand.size(); ++i)
+        if (i != best) free_plan(c, cand[i]);
+    if (s == MOSSQ_OK && best != 0) {
+        free_plan(c, p->fused);
+        p->fused = cand[best];
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (scratch) dev_free(c, scratch);
+    if (s != MOSSQ_OK) return s;
+    *tried = int(cand.size());
+    *best_ms = ms[best];
     return MOSSQ_OK;
 }
 

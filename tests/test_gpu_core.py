@@ -425,3 +425,20 @@ def test_program_graph_replay(ctx, n, fuse):
         assert np.max(np.abs(ha - b.to_host())) < AMP, r
         if r == 0:
             assert np.max(np.abs(ha - want)) < AMP
+
+
+def test_autotune_keeps_results(ctx):
+    """The autotuner times the forward-plan variants and keeps one; the program
+    still gives the oracle's amplitudes, and an unfused program has nothing to tune."""
+    n, d = 15, 8
+    c = random_circuit(n, d, haar_layers(n, d, 90))
+    prog = mq.Program(c, n, False, ctx, fuse=True)
+    variants, ms = prog.autotune(reps=2)
+    assert variants >= 1 and ms > 0
+    sv = mq.StateVector.zeros(n, ctx)
+    prog.run(sv.storage)
+    want = np.zeros(1 << n, dtype=np.complex128)
+    want[0] = 1
+    oracle.run_program(want, lower(c, n, False))
+    assert np.max(np.abs(sv.to_host() - want)) < AMP
+    assert mq.Program(c, n, False, ctx, fuse=False).autotune() == (0, 0.0)
