@@ -439,6 +439,21 @@ def main():
                             "hbm_gbps": alg_bytes / (step_u / 1e3) / 1e9,
                             "gate1_avg_ms": g1_avg, "gate1_gbps": tile_bytes / (g1_avg / 1e3) / 1e9}
         del sv_u
+        # the reference's own fusion (fuse_gates: adjacent gates merged while
+        # their support stays within 5 qubits), then one sweep per merged gate
+        fg = mq.fuse_gates(circ, 5)
+        kprog = mq.Program(fg, N, False, ctx, fuse=False)
+        sv_k = mq.StateVector.zeros(N, ctx)
+        ms_k = time_program(ctx, kprog, sv_k, 1, 0, 6, 7)
+        sv_1 = mq.StateVector.zeros(N, ctx)
+        fused.run(sv_1.storage)
+        ctx.sync()
+        a, b = sv_1.to_host(), sv_k.to_host()
+        extra["fuse_gates_k5"] = {"ms_per_step": ms_k, "gates_per_s": ngates / (ms_k / 1e3),
+                                  "merged_gates": len(list(fg.flat())),
+                                  "max_abs_diff_vs_fused": float(np.max(np.abs(a - b)))}
+        log(f"fuse_gates(5): {len(list(fg.flat()))} gates, {ms_k:.1f} ms/step")
+        del a, b, sv_k, sv_1
     norm = mq.dot(sv, sv).real
     extra["final_norm_minus_one"] = norm - 1.0
     del sv

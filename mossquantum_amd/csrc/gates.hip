@@ -62,6 +62,19 @@ struct Bits {
     int b[10];
 };
 
+// Offset of local index c of a k-bit block (bit j of c on global bit b[j]).
+template <int K>
+__device__ __forceinline__ u64 block_off(int c, const Bits &bits) {
+    u64 o = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) o |= u64((c >> j) & 1) << bits.b[j];
+    return o;
+}
+
+// One block of 2^K amplitudes per thread, the matrix in LDS (read as a
+// wave-wide broadcast).  The block's inputs stay in registers; outputs are
+// produced one row at a time (rows not unrolled), so K = 3..5 keep 4 * 2^K
+// data VGPRs and no per-slot offset table (no scratch, several waves per SIMD).
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_dense(c128 *__restrict__ v, u64 nblocks, Bits bits,
                                                    const c128 *__restrict__ mat) {
@@ -69,26 +82,17 @@ __global__ __launch_bounds__(kBlock) void k_dense(c128 *__restrict__ v, u64 nblo
     __shared__ c128 sm[D * D];
     for (int i = threadIdx.x; i < D * D; i += kBlock) sm[i] = mat[i];
     __syncthreads();
-    u64 off[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        u64 o = 0;
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-            if (c & (1 << j)) o |= u64(1) << bits.b[j];
-        off[c] = o;
-    }
     for (u64 t = u64(blockIdx.x) * kBlock + threadIdx.x; t < nblocks; t += u64(gridDim.x) * kBlock) {
         u64 base = insert0s<K>(t, bits.b);
         c128 in[D];
 #pragma unroll
-        for (int c = 0; c < D; ++c) in[c] = load_nt(v + base + off[c]);
-#pragma unroll
+        for (int c = 0; c < D; ++c) in[c] = load_nt(v + base + block_off<K>(c, bits));
+#pragma unroll(K <= 2 ? D : 1)
         for (int r = 0; r < D; ++r) {
             c128 acc = mk(0.0, 0.0);
 #pragma unroll
             for (int c = 0; c < D; ++c) acc = cfma(sm[r + D * c], in[c], acc);
-            store_nt(v + base + off[r], acc);
+            store_nt(v + base + block_off<K>(r, bits), acc);
         }
     }
 }
@@ -284,7 +288,11 @@ int apply_host_matrix(Ctx *c, c128 *v, int nbits, int k, const int *bits, const 
 // x <- V x and (unless Y_FIXED) y <- V y on the same k bits, then (if GRAD)
 // accumulate sum_blocks x_b^dagger D y_b with the updated vectors.  One
 // read+write of both vectors; the gradient reduction costs no extra sweep.
-// Per-workgroup partial sums go to part[blockIdx.x].
+// Per-workgroup partial sums go to part[blockIdx.x].  With y updated, D (V y)
+// is formed as (D V) y from the product staged in LDS, so a block's outputs
+// go out row by row (rows not unrolled for K >= 2: no per-slot offset table
+// or output arrays, no scratch, and the LDS matrices are not hoisted into
+// registers).
 template <int K, bool GRAD, bool Y_FIXED>
 __global__ __launch_bounds__(kBlock) void k_pair(c128 *__restrict__ x, c128 *__restrict__ y, u64 nblocks,
                                                   Bits bits, const c128 *__restrict__ V,
@@ -292,55 +300,51 @@ __global__ __launch_bounds__(kBlock) void k_pair(c128 *__restrict__ x, c128 *__r
     constexpr int D = 1 << K;
     __shared__ c128 sV[D * D];
     __shared__ c128 sD[GRAD ? D * D : 1];
+    __shared__ c128 sG[GRAD && !Y_FIXED ? D * D : 1];
     __shared__ c128 red[kBlock / 64];
     for (int i = threadIdx.x; i < D * D; i += kBlock) {
         sV[i] = V[i];
         if (GRAD) sD[i] = Dm[i];
     }
     __syncthreads();
-    u64 off[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        u64 o = 0;
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-            if (c & (1 << j)) o |= u64(1) << bits.b[j];
-        off[c] = o;
+    if (GRAD && !Y_FIXED) {
+        // G = D V (column-major), the gradient matrix on the input of y
+        for (int i = threadIdx.x; i < D * D; i += kBlock) {
+            const int r = i % D, c = i / D;
+            c128 a = mk(0.0, 0.0);
+            for (int q = 0; q < D; ++q) a = cfma(sD[r + D * q], sV[q + D * c], a);
+            sG[i] = a;
+        }
+        __syncthreads();
     }
+    const c128 *G = Y_FIXED ? sD : sG;
     c128 acc = mk(0.0, 0.0);
     for (u64 t = u64(blockIdx.x) * kBlock + threadIdx.x; t < nblocks; t += u64(gridDim.x) * kBlock) {
-        u64 base = insert0s<K>(t, bits.b);
-        c128 xi[D], yo[D];
+        const u64 base = insert0s<K>(t, bits.b);
+        c128 xi[D], yi[D];
 #pragma unroll
-        for (int c = 0; c < D; ++c) xi[c] = load_nt(x + base + off[c]);
-        if (Y_FIXED) {
-            if (GRAD) {
+        for (int c = 0; c < D; ++c) xi[c] = load_nt(x + base + block_off<K>(c, bits));
+        if (GRAD || !Y_FIXED) {
 #pragma unroll
-                for (int c = 0; c < D; ++c) yo[c] = load_nt(y + base + off[c]);
-            }
-        } else {
-            c128 yi[D];
-#pragma unroll
-            for (int c = 0; c < D; ++c) yi[c] = load_nt(y + base + off[c]);
-#pragma unroll
-            for (int r = 0; r < D; ++r) {
-                c128 ay = mk(0.0, 0.0);
-#pragma unroll
-                for (int c = 0; c < D; ++c) ay = cfma(sV[r + D * c], yi[c], ay);
-                yo[r] = ay;
-                store_nt(y + base + off[r], ay);
-            }
+            for (int c = 0; c < D; ++c) yi[c] = load_nt(y + base + block_off<K>(c, bits));
         }
-#pragma unroll
+#pragma unroll(K <= 1 ? D : 1)
         for (int r = 0; r < D; ++r) {
+            const u64 o = base + block_off<K>(r, bits);
             c128 ax = mk(0.0, 0.0);
 #pragma unroll
             for (int c = 0; c < D; ++c) ax = cfma(sV[r + D * c], xi[c], ax);
-            store_nt(x + base + off[r], ax);
+            store_nt(x + o, ax);
+            if (!Y_FIXED) {
+                c128 ay = mk(0.0, 0.0);
+#pragma unroll
+                for (int c = 0; c < D; ++c) ay = cfma(sV[r + D * c], yi[c], ay);
+                store_nt(y + o, ay);
+            }
             if (GRAD) {
                 c128 dy = mk(0.0, 0.0);
 #pragma unroll
-                for (int c = 0; c < D; ++c) dy = cfma(sD[r + D * c], yo[c], dy);
+                for (int c = 0; c < D; ++c) dy = cfma(G[r + D * c], yi[c], dy);
                 acc = cfmaconj(ax, dy, acc);
             }
         }
