@@ -355,9 +355,14 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=300,
                     help="requests per arrival rate in the serving leg (0 skips it)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the planner's cost-model plan for the headline")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the fused 26-qubit headline (every other leg skipped), e.g. under a profiler")
     ap.add_argument("--leg", choices=("c0", "batched", "density", "density-adjoint", "serving"),
                     help="run only this secondary leg and print its JSON (for profiling one workload)")
     args = ap.parse_args()
+    if args.headline_only:
+        args.no_cpu = args.no_unfused = args.no_batched = args.no_density = True
+        args.serving_requests = 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box has one GPU); "
@@ -457,7 +462,8 @@ def main():
     norm = mq.dot(sv, sv).real
     extra["final_norm_minus_one"] = norm - 1.0
     del sv
-    extra["c0_hea12"] = bench_c0(ctx)
+    if not args.headline_only:
+        extra["c0_hea12"] = bench_c0(ctx)
     if not args.no_batched:
         extra["batched_adjoint"] = bench_batched(ctx, cpu=not args.no_cpu)
     if not args.no_density:
