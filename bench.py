@@ -104,7 +104,9 @@ def batched_bytes(n, layers=6):
 
 def bench_batched(ctx, B=512, layers=6, seed=20260300, cpu=True):
     """512 independent states, n ~ U{10..20}, 6-layer RY/RZ + CNOT-ladder ansatz
-    with own seeded angles, XXZ loss; all gradients in one engine run."""
+    with own seeded angles, XXZ loss; all gradients in one engine run.  Items
+    of 12+ qubits run as batched tile sweeps; the gate-by-gate engine over the
+    same batch is timed beside it."""
     import ctypes as C
     import mossquantum_amd as mq
     from mossquantum_amd import _lib
@@ -113,53 +115,65 @@ def bench_batched(ctx, B=512, layers=6, seed=20260300, cpu=True):
     rng = np.random.default_rng(seed + 1)
     circs = [hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers)) for n in sizes]
     obs = [xxz(n) for n in sizes]
-    t0 = time.perf_counter()
-    progs = [mq.Program(c, n, False, ctx, fuse=False) for c, n in zip(circs, sizes)]
     dobs = [mq.DeviceObservable(o, n, ctx) for o, n in zip(obs, sizes)]
-    log(f"batched: {B} programs lowered in {time.perf_counter() - t0:.1f} s")
-    offs = np.zeros(B, dtype=np.int64)
-    tot = 0
-    for i, p in enumerate(progs):
-        offs[i] = tot
-        tot += p.lowered.nderiv
-    vals = np.zeros(B)
-    grads = np.zeros(tot)
-    ph = (C.c_void_p * B)(*[p.h.value for p in progs])
     oh = (C.c_void_p * B)(*[o.h.value for o in dobs])
+    out = {"B": B, "layers": layers, "n_range": [10, 20]}
+    results = {}
+    for fuse in (True, False):
+        t0 = time.perf_counter()
+        progs = [mq.Program(c, n, False, ctx, fuse=fuse) for c, n in zip(circs, sizes)]
+        log(f"batched: {B} programs lowered{' and planned' if fuse else ''} in {time.perf_counter() - t0:.1f} s")
+        offs = np.zeros(B, dtype=np.int64)
+        tot = 0
+        for i, p in enumerate(progs):
+            offs[i] = tot
+            tot += p.lowered.nderiv
+        vals = np.zeros(B)
+        grads = np.zeros(tot)
+        ph = (C.c_void_p * B)(*[p.h.value for p in progs])
 
-    def run_engine():
-        _lib.check(ctx.lib.mossq_batch_adjoint(ctx.h, B, ph, oh, offs.ctypes.data_as(C.POINTER(C.c_int64)),
-                                               _lib.dptr(vals), _lib.dptr(grads)))
+        def run_engine():
+            _lib.check(ctx.lib.mossq_batch_adjoint(ctx.h, B, ph, oh, offs.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                   _lib.dptr(vals), _lib.dptr(grads)))
 
-    run_engine()  # warm-up (allocations, code objects)
-    reps = 3
-    ctx.sync()
-    w0 = time.perf_counter()
-    ctx.record(6)
-    for _ in range(reps):
-        run_engine()
-    ctx.record(7)
-    wall = (time.perf_counter() - w0) / reps
-    dev_ms = ctx.elapsed_ms(6, 7) / reps
-    log(f"batched engine: {wall * 1e3:.1f} ms wall, {dev_ms:.1f} ms device per batch")
-    # plain per-circuit path: one mossq_adjoint per item
-    pv = np.zeros(B)
-    pg = np.zeros(tot)
-    w0 = time.perf_counter()
-    for i, (p, o) in enumerate(zip(progs, dobs)):
-        v = C.c_double(0)
-        g = np.zeros(max(p.lowered.nderiv, 1))
-        _lib.check(ctx.lib.mossq_adjoint(ctx.h, p.h, o.h, None, 0, None, C.byref(v), _lib.dptr(g)))
-        pv[i] = v.value
-        pg[offs[i]:offs[i] + p.lowered.nderiv] = g[:p.lowered.nderiv]
-    plain = time.perf_counter() - w0
-    log(f"per-circuit path: {plain * 1e3:.1f} ms")
-    out = {"B": B, "layers": layers, "n_range": [10, 20], "ms_per_batch": wall * 1e3, "device_ms": dev_ms,
-           "grads_per_s": B / wall, "param_derivs_per_s": tot / wall,
-           "per_circuit_ms": plain * 1e3, "per_circuit_grads_per_s": B / plain, "speedup_vs_per_circuit": plain / wall,
-           "max_abs_grad_diff_vs_per_circuit": float(np.max(np.abs(grads - pg))),
-           "max_abs_value_diff_vs_per_circuit": float(np.max(np.abs(vals - pv))),
-           "effective_gbps": sum(batched_bytes(n, layers) for n in sizes) / wall / 1e9}
+        run_engine()  # warm-up (allocations, code objects)
+        reps = 3 if fuse else 1
+        ctx.sync()
+        w0 = time.perf_counter()
+        ctx.record(6)
+        for _ in range(reps):
+            run_engine()
+        ctx.record(7)
+        wall = (time.perf_counter() - w0) / reps
+        dev_ms = ctx.elapsed_ms(6, 7) / reps
+        key = "tiled" if fuse else "stepped"
+        log(f"batched engine ({key}): {wall * 1e3:.1f} ms wall, {dev_ms:.1f} ms device per batch")
+        results[key] = (wall, dev_ms, vals.copy(), grads.copy())
+        if fuse:
+            # plain per-circuit path: one mossq_adjoint per item (fused adjoint for 12+ qubits)
+            pv = np.zeros(B)
+            pg = np.zeros(tot)
+            w0 = time.perf_counter()
+            for i, (p, o) in enumerate(zip(progs, dobs)):
+                v = C.c_double(0)
+                g = np.zeros(max(p.lowered.nderiv, 1))
+                _lib.check(ctx.lib.mossq_adjoint(ctx.h, p.h, o.h, None, 0, None, C.byref(v), _lib.dptr(g)))
+                pv[i] = v.value
+                pg[offs[i]:offs[i] + p.lowered.nderiv] = g[:p.lowered.nderiv]
+            plain = time.perf_counter() - w0
+            log(f"per-circuit path: {plain * 1e3:.1f} ms")
+            out.update({"per_circuit_ms": plain * 1e3, "per_circuit_grads_per_s": B / plain,
+                        "max_abs_grad_diff_vs_per_circuit": float(np.max(np.abs(grads - pg))),
+                        "max_abs_value_diff_vs_per_circuit": float(np.max(np.abs(vals - pv)))})
+        del progs
+    wall, dev_ms, vals, grads = results["tiled"]
+    swall, sdev, svals, sgrads = results["stepped"]
+    out.update({"ms_per_batch": wall * 1e3, "device_ms": dev_ms, "grads_per_s": B / wall,
+                "param_derivs_per_s": grads.size / wall, "speedup_vs_per_circuit": out["per_circuit_ms"] / 1e3 / wall,
+                "stepped_engine_ms": swall * 1e3, "stepped_engine_grads_per_s": B / swall,
+                "max_abs_grad_diff_tiled_vs_stepped": float(np.max(np.abs(grads - sgrads))),
+                "max_abs_value_diff_tiled_vs_stepped": float(np.max(np.abs(vals - svals))),
+                "unfused_convention_gbps": sum(batched_bytes(n, layers) for n in sizes) / wall / 1e9})
     if cpu:
         import oracle
         from mossquantum_amd import lower

@@ -217,7 +217,55 @@ int batch_iteration(Ctx *c, const BItemDev *items, int nitems, int nchunks_total
     return MOSSQ_OK;
 }
 
-// Batch of B items all starting at iteration 0; one host synchronisation at the end.
+// Initial states |0...0> for the items of a table whose psi is not preset; a
+// workgroup clears one chunk of one item.
+__global__ __launch_bounds__(kBT) void k_binit(const BItemDev *__restrict__ items, int nitems) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (items[mid].chunk0 <= int(blockIdx.x))
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BItemDev &it = items[lo];
+    if (it.preset) return;
+    const u64 len = u64(1) << it.n, per = len / u64(it.nchunk);
+    const u64 c = u64(blockIdx.x) - u64(it.chunk0);
+    for (u64 i = c * per + threadIdx.x; i < (c + 1) * per; i += kBT) it.psi[i] = mk(i == 0 ? 1.0 : 0.0, 0.0);
+}
+
+// Gradient rows of the items whose reverse pass ran as batched fused sweeps:
+// one workgroup sums one parameter's row of partials in a fixed order.
+struct BRows {
+    const c128 *part;
+    int row, count, gslot0, p0;
+};
+
+__global__ __launch_bounds__(kBT) void k_brows(const BRows *__restrict__ rows, int nrows, double *__restrict__ grads) {
+    __shared__ c128 red[kBT / 64];
+    int lo = 0, hi = nrows - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (rows[mid].p0 <= int(blockIdx.x))
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BRows r = rows[lo];
+    const int j = int(blockIdx.x) - r.p0;
+    const c128 *src = r.part + size_t(j) * r.row;
+    c128 a = mk(0, 0);
+    for (int k = threadIdx.x; k < r.row; k += kBT) a = cadd(a, src[k]);
+    const c128 s = bblock_sum(a, red);
+    if (threadIdx.x == 0) grads[r.gslot0 + j] = s.re;
+}
+
+// Batch of B items; one host synchronisation at the end.  Items whose programs
+// carry all-sweep fused plans (12 or more qubits) run their forward and
+// reverse passes as batched LDS-tile sweeps (launch s: sweep s of every such
+// item) around one engine iteration for the observe step; the others run the
+// gate-by-gate engine micro-program.
 // value_only: forward and observe steps only (no lambda vectors, no gradients);
 // init[b], when given, is item b's initial state (else |0...0>).
 static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, const Obs *const *obs,
@@ -225,40 +273,60 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
                                 const void *const *init) {
     std::vector<BItemHost> items(B);
     std::vector<BOp> all_ops;
-    std::vector<int> op_off(B);
-    i64 total_amps = 0, dtot = 0;
-    int nchunks = 0, tmax = 0;
+    std::vector<int> op_off(B), nder(B);
+    std::vector<char> tiled(B);
+    std::vector<int> order;  // tiled items first, then stepped ones
+    i64 total_amps = 0, dtot = 0, rev_parts = 0;
+    int nchunks[2] = {0, 0}, ntab[2] = {0, 0}, tmax = 0;
     for (int b = 0; b < B; ++b) {
-        int nd = 0;
-        MOSSQ_TRY(batch_lower(progs[b], items[b].ops, &nd));
+        tiled[b] = fused_plan_batchable(progs[b], !value_only);
+        if (tiled[b]) {
+            items[b].ops.clear();
+            nder[b] = progs[b]->nderiv;
+            if (!value_only) rev_parts += i64(std::max(progs[b]->nderiv, 1)) * fused_adjoint_parts(progs[b]);
+        } else {
+            MOSSQ_TRY(batch_lower(progs[b], items[b].ops, &nder[b]));
+        }
         op_off[b] = int(all_ops.size());
         all_ops.insert(all_ops.end(), items[b].ops.begin(), items[b].ops.end());
         items[b].nfwd = int(items[b].ops.size());
         items[b].nchunk = batch_chunks(progs[b]->n);
         items[b].gslot0 = value_only ? 0 : int(grad_offset[b]);
-        items[b].vslot = b;
         total_amps += i64(1) << progs[b]->n;
-        if (!value_only) dtot = std::max<i64>(dtot, grad_offset[b] + nd);
-        nchunks += items[b].nchunk;
-        tmax = std::max(tmax, (value_only ? 1 : 2) * items[b].nfwd);
+        if (!value_only) dtot = std::max<i64>(dtot, grad_offset[b] + nder[b]);
+        const int k = tiled[b] ? 0 : 1;
+        nchunks[k] += items[b].nchunk;
+        ++ntab[k];
+        if (!tiled[b]) tmax = std::max(tmax, (value_only ? 1 : 2) * items[b].nfwd);
     }
+    for (int k = 0; k < 2; ++k)
+        for (int b = 0; b < B; ++b)
+            if ((tiled[b] != 0) == (k == 0)) order.push_back(b);
     Allocs al(c);
-    c128 *vec = nullptr, *part = nullptr;
+    c128 *vec = nullptr, *part = nullptr, *rpart = nullptr;
     BOp *d_ops = nullptr;
     BItemDev *d_items = nullptr;
+    BRows *d_rows = nullptr;
     double *d_vals = nullptr, *d_grads = nullptr;
     MOSSQ_TRY(al.get(u64(total_amps) * (value_only ? 1 : 2) * sizeof(c128), (void **)&vec));
-    MOSSQ_TRY(al.get(u64(std::max(nchunks, 1)) * sizeof(c128), (void **)&part));
+    MOSSQ_TRY(al.get(u64(std::max({nchunks[0], nchunks[1], 1})) * sizeof(c128), (void **)&part));
     MOSSQ_TRY(al.get(std::max<size_t>(all_ops.size(), 1) * sizeof(BOp), (void **)&d_ops));
     MOSSQ_TRY(al.get(u64(B) * sizeof(BItemDev), (void **)&d_items));
     MOSSQ_TRY(al.get(u64(B) * sizeof(double), (void **)&d_vals));
     MOSSQ_TRY(al.get(u64(std::max<i64>(dtot, 1)) * sizeof(double), (void **)&d_grads));
+    if (rev_parts) MOSSQ_TRY(al.get(u64(rev_parts) * sizeof(c128), (void **)&rpart));
     std::vector<BItemDev> hd(B);
-    i64 vo = 0;
-    int ch = 0;
-    for (int b = 0; b < B; ++b) {
+    std::vector<const Program *> tprogs;
+    std::vector<u64> tx, tl, tpart;
+    std::vector<int> trow;
+    std::vector<BRows> rows;
+    i64 vo = 0, ro = 0;
+    int ch[2] = {0, 0}, np = 0;
+    for (int pos = 0; pos < B; ++pos) {
+        const int b = order[pos];
+        const int k = tiled[b] ? 0 : 1;
         const i64 len = i64(1) << progs[b]->n;
-        BItemDev &d = hd[b];
+        BItemDev &d = hd[pos];
         d.psi = vec + vo;
         d.lam = value_only ? nullptr : vec + vo + len;
         vo += (value_only ? 1 : 2) * len;
@@ -271,23 +339,68 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         d.nfwd = items[b].nfwd;
         d.G = obs[b]->G;
         d.t0 = 0;
-        d.chunk0 = ch;
+        d.chunk0 = ch[k];
         d.nchunk = items[b].nchunk;
         d.gslot0 = items[b].gslot0;
         d.vslot = b;
         d.value_only = value_only ? 1 : 0;
-        d.pad = 0;
-        ch += d.nchunk;
-        if (init && init[b])
+        d.preset = (init && init[b]) ? 1 : 0;
+        ch[k] += d.nchunk;
+        if (d.preset)
             MOSSQ_HIP(hipMemcpyAsync(d.psi, init[b], u64(len) * sizeof(c128), hipMemcpyDeviceToDevice, c->stream));
-        else
-            MOSSQ_TRY(fill_basis0(c, d.psi, u64(len)));
+        if (tiled[b]) {
+            tprogs.push_back(progs[b]);
+            tx.push_back(u64(d.psi - vec));
+            tl.push_back(value_only ? 0 : u64(d.lam - vec));
+            const int row = value_only ? 0 : fused_adjoint_parts(progs[b]);
+            tpart.push_back(u64(ro));
+            trow.push_back(row);
+            if (!value_only && nder[b] > 0) {
+                rows.push_back(BRows{rpart + ro, row, nder[b], d.gslot0, np});
+                np += nder[b];
+            }
+            ro += i64(std::max(nder[b], 1)) * row;
+        }
+    }
+    BatchSweeps fwd, rev;
+    const int nt = int(tprogs.size());
+    if (nt) {
+        const u64 vec_len = u64(total_amps) * (value_only ? 1 : 2);
+        MOSSQ_TRY(batch_sweeps_build(c, al, nt, tprogs.data(), vec, vec_len, tx.data(), nullptr, nullptr, 0, nullptr,
+                                     nullptr, false, &fwd));
+        if (!value_only)
+            MOSSQ_TRY(batch_sweeps_build(c, al, nt, tprogs.data(), vec, vec_len, tx.data(), tl.data(), rpart,
+                                         u64(rev_parts), tpart.data(), trow.data(), true, &rev));
+    }
+    if (!rows.empty()) {
+        MOSSQ_TRY(al.get(rows.size() * sizeof(BRows), (void **)&d_rows));
+        MOSSQ_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * sizeof(BRows), hipMemcpyHostToDevice, c->stream));
     }
     if (!all_ops.empty())
         MOSSQ_HIP(hipMemcpyAsync(d_ops, all_ops.data(), all_ops.size() * sizeof(BOp), hipMemcpyHostToDevice,
                                  c->stream));
     MOSSQ_HIP(hipMemcpyAsync(d_items, hd.data(), hd.size() * sizeof(BItemDev), hipMemcpyHostToDevice, c->stream));
-    for (int t = 0; t <= tmax; ++t) MOSSQ_TRY(batch_iteration(c, d_items, B, nchunks, t, part, d_vals, d_grads));
+    const BItemDev *tab[2] = {d_items, d_items + ntab[0]};
+    for (int k = 0; k < 2; ++k)
+        if (ntab[k]) {
+            hipLaunchKernelGGL(k_binit, dim3(nchunks[k]), dim3(kBT), 0, c->stream, tab[k], ntab[k]);
+            MOSSQ_HIP(hipGetLastError());
+        }
+    if (ntab[0]) {
+        MOSSQ_TRY(batch_sweeps_run(c, fwd, false));
+        MOSSQ_TRY(batch_iteration(c, tab[0], ntab[0], nchunks[0], 0, part, d_vals, d_grads));  // observe
+    }
+    if (ntab[1])
+        for (int t = 0; t <= tmax; ++t)
+            MOSSQ_TRY(batch_iteration(c, tab[1], ntab[1], nchunks[1], t, part, d_vals, d_grads));
+    if (ntab[0] && !value_only) {
+        MOSSQ_TRY(zero_fill(c, rpart, u64(rev_parts)));
+        MOSSQ_TRY(batch_sweeps_run(c, rev, true));
+        if (np) {
+            hipLaunchKernelGGL(k_brows, dim3(np), dim3(kBT), 0, c->stream, d_rows, int(rows.size()), d_grads);
+            MOSSQ_HIP(hipGetLastError());
+        }
+    }
     std::vector<double> hv(B), hg(std::max<i64>(dtot, 1));
     MOSSQ_HIP(hipMemcpyAsync(hv.data(), d_vals, B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     MOSSQ_HIP(hipMemcpyAsync(hg.data(), d_grads, hg.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -36,7 +36,7 @@ struct BItemDev {
     int chunk0, nchunk;
     int gslot0, vslot;
     int value_only;  // stop after the observe step (no seed, no reverse pass)
-    int pad;
+    int preset;      // psi already holds the initial state (k_binit leaves it)
 };
 
 // Host-side description of one item.

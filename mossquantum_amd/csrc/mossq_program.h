@@ -89,6 +89,26 @@ int density_adjoint_reverse(Ctx *c, const Program *p, c128 *delta, c128 *const *
                             const c128 *const *d_tab, c128 *part, int row);
 // Gradient partials per row the fused density adjoint writes (zero them first).
 int density_adjoint_parts(const Program *p);
+// Batched fused sweeps (tile.hip): the plans of many programs advanced
+// together, launch s running sweep s of every program that has one, each
+// program on its own vectors.  A program qualifies when every item of its
+// forward plan (and, for adjoint sweeps, of its adjoint plan) is a sweep.
+bool fused_plan_batchable(const Program *p, bool adjoint);
+struct BatchSweeps {
+    void *d_items = nullptr;  // device launch items (layout private to tile.hip)
+    int *d_wg = nullptr;      // workgroup -> item of its launch
+    c128 *vec = nullptr, *part = nullptr;
+    std::vector<int> item0, wg0, tiles;  // per launch
+};
+// Builds and uploads the launch tables into al's memory (synchronises c's
+// stream).  Every vector lies in the allocation vec (vec_len entries), given
+// by its offset: forward, x[b] is program b's state; adjoint, x[b] and l[b]
+// are y and lambda, and the gradient partials go to rows of row[b] entries
+// at part + part_off[b] (part holds part_len entries).
+int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c128 *vec, u64 vec_len,
+                       const u64 *x, const u64 *l, c128 *part, u64 part_len, const u64 *part_off, const int *row,
+                       bool adjoint, BatchSweeps *out);
+int batch_sweeps_run(Ctx *c, const BatchSweeps &bs, bool adjoint);
 // Zero `count` complex entries with a kernel (safe inside captured graphs).
 int zero_fill(Ctx *c, c128 *v, u64 count);
 

@@ -329,6 +329,14 @@ __device__ __forceinline__ void fixed_probe(const TGroup &g, const c128 *cur, c1
     if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = acc;
 }
 
+// Global index bits of a tile: tile-id bit i sits at global bit cbits[i].
+__device__ __forceinline__ u64 tile_base(const TSweepArgs &sw, u64 tile) {
+    u64 base = 0;
+    for (int i = 0; i < sw.ncbits; ++i)
+        if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
+    return base;
+}
+
 // Reads the tile from src and writes it to dst (src == dst for an in-place
 // sweep; a workgroup stores only after it has loaded its whole tile).  With
 // FIXP, a group's probe adds Re<cur|D|S> partials (one per wave per tile)
@@ -337,16 +345,14 @@ template <bool FIXP>
 __device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSweepArgs &sw,
                                            const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
                                            const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
-                                           const c128 *const *ftab, int fbase, c128 *part, int row) {
+                                           const c128 *const *ftab, int fbase, c128 *part, int row, u64 tile) {
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
     const int t = threadIdx.x;
-    const u64 tile = blockIdx.x;
-    u64 base = 0;
-    for (int i = 0; i < sw.ncbits; ++i)
-        if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
+    const u64 base = tile_base(sw, tile);
+    const int seg0 = sw.seg0, seg1 = sw.seg1;
 
-    TSeg sg = segs[sw.seg0];
+    TSeg sg = segs[seg0];
     int tdep = seg_tdep(t, sg);
     c128 v[TN], w[TN];
     {
@@ -360,7 +366,7 @@ __device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSw
             v[j] = load_nt(src + (g | o));
         }
     }
-    for (int s = sw.seg0; s < sw.seg1; ++s) {
+    for (int s = seg0; s < seg1; ++s) {
         sg = segs[s];
         if (sg.transpose) {
             int a[TN];
@@ -403,7 +409,7 @@ __device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSw
 __global__ __launch_bounds__(TT, 2) void k_tile(const c128 *src, c128 *dst, TSweepArgs sw,
                                                   const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
                                                   const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
-    tile_sweep<false>(src, dst, sw, segs, groups, diag, pool, nullptr, 0, nullptr, 0);
+    tile_sweep<false>(src, dst, sw, segs, groups, diag, pool, nullptr, 0, nullptr, 0, blockIdx.x);
 }
 
 // The density adjoint's reverse chunks: Delta in place, each probe against
@@ -413,7 +419,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile_fixp(c128 *delta, TSweepArgs sw,
                                                        const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
                                                        const c128 *const *__restrict__ ftab, int fbase,
                                                        c128 *__restrict__ part, int row) {
-    tile_sweep<true>(delta, delta, sw, segs, groups, diag, pool, ftab, fbase, part, row);
+    tile_sweep<true>(delta, delta, sw, segs, groups, diag, pool, ftab, fbase, part, row, blockIdx.x);
 }
 
 // --------------------------------------------------------- adjoint sweep ----
@@ -545,20 +551,17 @@ __device__ __forceinline__ void transpose_lds(c128 *v, char *ldsb, int t, const 
     __syncthreads();
 }
 
-__global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 *__restrict__ ls, TSweepArgs sw,
-                                                      const TSeg *__restrict__ segs,
-                                                      const TGroup *__restrict__ groups,
-                                                      const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
-                                                      c128 *__restrict__ part, int row) {
+__device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__restrict__ ls, const TSweepArgs &sw,
+                                               const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
+                                               const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
+                                               c128 *__restrict__ part, int row, u64 tile) {
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
     const int t = threadIdx.x;
-    const u64 tile = blockIdx.x;
-    u64 base = 0;
-    for (int i = 0; i < sw.ncbits; ++i)
-        if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
+    const u64 base = tile_base(sw, tile);
+    const int seg0 = sw.seg0, seg1 = sw.seg1;
 
-    TSeg sg = segs[sw.seg0];
+    TSeg sg = segs[seg0];
     int tdep = seg_tdep(t, sg);
     c128 y[TN], l[TN];
     {
@@ -574,7 +577,7 @@ __global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 
         }
     }
     const int slot = int(tile) * 4 + (t >> 6);
-    for (int s = sw.seg0; s < sw.seg1; ++s) {
+    for (int s = seg0; s < seg1; ++s) {
         sg = segs[s];
         if (sg.transpose) {
             transpose_lds(y, ldsb, t, sg);
@@ -606,6 +609,58 @@ __global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 
             store_nt(ls + (g | o), l[j]);
         }
     }
+}
+
+__global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 *__restrict__ ls, TSweepArgs sw,
+                                                      const TSeg *__restrict__ segs,
+                                                      const TGroup *__restrict__ groups,
+                                                      const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
+                                                      c128 *__restrict__ part, int row) {
+    tile_adj_sweep(ys, ls, sw, segs, groups, diag, pool, part, row, blockIdx.x);
+}
+
+// ------------------------------------------------------ batched sweeps ----
+// Sweeps of many programs in one launch: item i is one sweep of one program's
+// plan on that program's own vectors, over workgroups [tile0, tile0 + tiles)
+// of the launch; wg_item maps each workgroup to its item.  The vectors are
+// offsets into one allocation `vec` (partials into `part`), which the host
+// checks against that allocation's size.
+// The plan tables' addresses become constant-memory pointers in the kernel
+// (the tables are read-only while a sweep runs).  Generic pointers read from
+// memory would load through the flat path, into vector registers; constant
+// ones load uniform records as scalars, as kernel-argument tables do.
+template <class T>
+using cptr = __attribute__((address_space(4))) const T *;
+
+template <class T>
+__device__ __forceinline__ const T *table(u64 addr) {
+    return (const T *)(cptr<T>)addr;
+}
+
+struct TBItem {
+    u64 x;  // the state (forward sweeps, in place) or y (adjoint sweeps)
+    u64 l;  // lambda (adjoint sweeps)
+    u64 part;  // gradient partial rows (adjoint sweeps)
+    u64 segs, groups, diag, pool;  // plan tables (device addresses)
+    int row;
+    int tile0;
+    TSweepArgs args;
+};
+
+__global__ __launch_bounds__(TT, 2) void k_btile(const TBItem *__restrict__ items, const int *__restrict__ wg_item,
+                                                   c128 *__restrict__ vec) {
+    const TBItem &it = items[__builtin_amdgcn_readfirstlane(wg_item[blockIdx.x])];
+    tile_sweep<false>(vec + it.x, vec + it.x, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),
+                      table<TDiag>(it.diag), table<c128>(it.pool), nullptr, 0, nullptr, 0,
+                      u64(blockIdx.x - it.tile0));
+}
+
+__global__ __launch_bounds__(TT, 2) void k_btile_adj(const TBItem *__restrict__ items,
+                                                       const int *__restrict__ wg_item, c128 *__restrict__ vec,
+                                                       c128 *__restrict__ part) {
+    const TBItem &it = items[__builtin_amdgcn_readfirstlane(wg_item[blockIdx.x])];
+    tile_adj_sweep(vec + it.x, vec + it.l, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),
+                   table<TDiag>(it.diag), table<c128>(it.pool), part + it.part, it.row, u64(blockIdx.x - it.tile0));
 }
 
 // ------------------------------------------------------------------ planner ----
@@ -1878,6 +1933,90 @@ int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, in
         }
         hipLaunchKernelGGL(k_tile_adj, dim3(ntiles), dim3(TT), 0, c->stream, y, l, it.args, fp->d_segs,
                            fp->d_groups, fp->d_diag, fp->d_pool, part, row);
+        MOSSQ_HIP(hipGetLastError());
+    }
+    return MOSSQ_OK;
+}
+
+bool fused_plan_batchable(const Program *p, bool adjoint) {
+    if (!p->fused || (adjoint && !p->fused_rev)) return false;
+    for (const FusedPlan *fp : {p->fused, adjoint ? p->fused_rev : nullptr}) {
+        if (!fp) continue;
+        for (const PlanItem &it : fp->items)
+            if (!it.sweep) return false;
+    }
+    return true;
+}
+
+int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c128 *vec, u64 vec_len,
+                       const u64 *x, const u64 *l, c128 *part, u64 part_len, const u64 *part_off, const int *row,
+                       bool adjoint, BatchSweeps *out) {
+    std::vector<TBItem> items;
+    out->vec = vec;
+    out->part = part;
+    std::vector<int> wg;
+    out->item0.clear();
+    out->wg0.clear();
+    out->tiles.clear();
+    size_t nlaunch = 0;
+    for (int b = 0; b < m; ++b) {
+        MOSSQ_CHECK(fused_plan_batchable(progs[b], adjoint), MOSSQ_EINVAL, "program has no all-sweep fused plan");
+        const FusedPlan *fp = adjoint ? progs[b]->fused_rev : progs[b]->fused;
+        nlaunch = std::max(nlaunch, fp->items.size());
+    }
+    for (size_t s = 0; s < nlaunch; ++s) {
+        const size_t i0 = items.size(), w0 = wg.size();
+        u64 tiles = 0;
+        for (int b = 0; b < m; ++b) {
+            const Program *p = progs[b];
+            const FusedPlan *fp = adjoint ? p->fused_rev : p->fused;
+            if (s >= fp->items.size()) continue;
+            TBItem it{};
+            const u64 len = u64(1) << p->nbits, nt = len >> TM;
+            it.x = x[b];
+            it.l = adjoint ? l[b] : 0;
+            it.part = adjoint ? part_off[b] : 0;
+            it.segs = reinterpret_cast<u64>(fp->d_segs);
+            it.groups = reinterpret_cast<u64>(fp->d_groups);
+            it.diag = reinterpret_cast<u64>(fp->d_diag);
+            it.pool = reinterpret_cast<u64>(fp->d_pool);
+            it.row = adjoint ? row[b] : 0;
+            it.tile0 = int(tiles);
+            it.args = fp->items[s].args;
+            MOSSQ_CHECK(tiles + nt <= u64(INT32_MAX), MOSSQ_EINVAL, "batched sweep launch too large");
+            MOSSQ_CHECK(x[b] + len <= vec_len && (!adjoint || l[b] + len <= vec_len), MOSSQ_EINVAL,
+                        "batched sweep vector outside its allocation");
+            MOSSQ_CHECK(!adjoint || (row[b] >= int(4 * nt) &&
+                                     part_off[b] + u64(std::max(p->nderiv, 1)) * u64(row[b]) <= part_len),
+                        MOSSQ_EINVAL, "gradient partial rows outside their allocation");
+            wg.insert(wg.end(), size_t(nt), int(items.size() - i0));
+            tiles += nt;
+            items.push_back(it);
+        }
+        out->item0.push_back(int(i0));
+        out->wg0.push_back(int(w0));
+        out->tiles.push_back(int(tiles));
+    }
+    if (items.empty()) return MOSSQ_OK;
+    MOSSQ_TRY(al.get(items.size() * sizeof(TBItem), &out->d_items));
+    MOSSQ_TRY(al.get(wg.size() * sizeof(int), (void **)&out->d_wg));
+    MOSSQ_HIP(hipMemcpyAsync(out->d_items, items.data(), items.size() * sizeof(TBItem), hipMemcpyHostToDevice,
+                             c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(out->d_wg, wg.data(), wg.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));  // the host tables go out of scope
+    return MOSSQ_OK;
+}
+
+int batch_sweeps_run(Ctx *c, const BatchSweeps &bs, bool adjoint) {
+    const TBItem *items = static_cast<const TBItem *>(bs.d_items);
+    for (size_t s = 0; s < bs.tiles.size(); ++s) {
+        if (bs.tiles[s] == 0) continue;
+        if (adjoint)
+            hipLaunchKernelGGL(k_btile_adj, dim3(unsigned(bs.tiles[s])), dim3(TT), 0, c->stream, items + bs.item0[s],
+                               bs.d_wg + bs.wg0[s], bs.vec, bs.part);
+        else
+            hipLaunchKernelGGL(k_btile, dim3(unsigned(bs.tiles[s])), dim3(TT), 0, c->stream, items + bs.item0[s],
+                               bs.d_wg + bs.wg0[s], bs.vec);
         MOSSQ_HIP(hipGetLastError());
     }
     return MOSSQ_OK;

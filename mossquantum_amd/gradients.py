@@ -210,13 +210,15 @@ def differentiable_value(method, objective, values):
     return value_and_gradient(method, objective, values)[0]
 
 
-def batched_adjoint(circuits, observables, nqubits, ctx: _lib.Context | None = None):
+def batched_adjoint(circuits, observables, nqubits, ctx: _lib.Context | None = None, fuse: bool = True):
     """Independent pure-state programs (own qubit count, parameters, observable),
     all from |0...0>, differentiated in one library call (mossq_batch_adjoint).
+    With fuse, items of 12 or more qubits run as batched LDS-tile sweeps; the
+    others (and every item without fuse) run gate by gate.
     Returns (values[B], [slot gradient vector per item])."""
     ctx = ctx or _lib.default_context()
     B = len(circuits)
-    progs = [Program(c, n, False, ctx, fuse=False) for c, n in zip(circuits, nqubits)]
+    progs = [Program(c, n, False, ctx, fuse=fuse) for c, n in zip(circuits, nqubits)]
     obs = [DeviceObservable(o, n, ctx) for o, n in zip(observables, nqubits)]
     offs = np.zeros(B, dtype=np.int64)
     tot = 0
