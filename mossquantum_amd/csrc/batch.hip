@@ -121,27 +121,29 @@ __global__ __launch_bounds__(kBT, 2) void k_biter(const BItemDev *__restrict__ i
         const u64 len = u64(1) << n, per = len / u64(it.nchunk);
         c128 acc = mk(0, 0);
         for (u64 i = u64(c) * per + threadIdx.x; i < u64(c + 1) * per; i += kBT) {
-            c128 hp = mk(0, 0), sd = mk(0, 0);
+            // only the nonzero flip patterns of each group load; the diagonal
+            // terms of all groups share psi[i]
+            c128 hp = mk(0, 0), sd = mk(0, 0), dh = mk(0, 0), ds = mk(0, 0);
             for (int gi = 0; gi < it.G; ++gi) {
                 const ObsGroup g = it.groups[gi];
                 const int D = 1 << g.k;
-                int r = 0;
-                u64 base = i;
-                for (int j = 0; j < g.k; ++j) {
-                    r |= int((i >> g.sites[j]) & 1) << j;
-                    base &= ~(u64(1) << g.sites[j]);
+                const int r = group_row(g, i);
+                const c128 *m = it.mats + g.moff + r, *sm = it.seed + g.moff + r;
+                if (g.xmask & 1) {
+                    dh = cadd(dh, m[D * r]);
+                    ds = cadd(ds, sm[D * r]);
                 }
-                for (int cc = 0; cc < D; ++cc) {
-                    u64 o = 0;
-                    for (int j = 0; j < g.k; ++j)
-                        if (cc & (1 << j)) o |= u64(1) << g.sites[j];
-                    const c128 x = psi[base | o];
-                    hp = cfma(it.mats[g.moff + r + D * cc], x, hp);
-                    sd = cfma(it.seed[g.moff + r + D * cc], x, sd);
-                }
+                for (int x = 1; x < D; ++x)
+                    if ((g.xmask >> x) & 1) {
+                        const c128 y = psi[i ^ group_flip(g, x)];
+                        hp = cfma(m[D * (r ^ x)], y, hp);
+                        sd = cfma(sm[D * (r ^ x)], y, sd);
+                    }
             }
-            if (!it.value_only) lam[i] = sd;
-            acc = cfmaconj(psi[i], hp, acc);
+            const c128 x0 = psi[i];
+            hp = cfma(dh, x0, hp);
+            if (!it.value_only) lam[i] = cfma(ds, x0, sd);
+            acc = cfmaconj(x0, hp, acc);
         }
         c128 s = bblock_sum(acc, red);
         if (threadIdx.x == 0) part[it.chunk0 + c] = s;

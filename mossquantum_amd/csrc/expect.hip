@@ -154,31 +154,24 @@ int expect_total_host(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, do
 }
 
 // ---------------------------------------------------------- operator action ----
-// out[i] = sum_g sum_c M_g[r_g(i), c] psi[i with the sites of g set to c].
+// out[i] = sum_g sum_c M_g[r_g(i), c] psi[i with the sites of g set to c],
+// summed over the nonzero flip patterns c = r ^ x of each group.
 // nbits may exceed the observable's n (left action on density-matrix rows).
 __global__ __launch_bounds__(kB) void k_apply_op(c128 *__restrict__ out, const c128 *__restrict__ psi, u64 len,
                                                  int G, const ObsGroup *__restrict__ groups,
                                                  const c128 *__restrict__ mats) {
     for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < len; i += u64(gridDim.x) * kB) {
-        c128 acc = mk(0, 0);
+        c128 acc = mk(0, 0), dg = mk(0, 0);
         for (int gi = 0; gi < G; ++gi) {
             const ObsGroup g = groups[gi];
             const int D = 1 << g.k;
-            int r = 0;
-            u64 base = i;
-            for (int j = 0; j < g.k; ++j) {
-                r |= int((i >> g.sites[j]) & 1) << j;
-                base &= ~(u64(1) << g.sites[j]);
-            }
+            const int r = group_row(g, i);
             const c128 *m = mats + g.moff + r;
-            for (int cc = 0; cc < D; ++cc) {
-                u64 o = 0;
-                for (int j = 0; j < g.k; ++j)
-                    if (cc & (1 << j)) o |= u64(1) << g.sites[j];
-                acc = cfma(m[size_t(D) * cc], psi[base | o], acc);
-            }
+            if (g.xmask & 1) dg = cadd(dg, m[size_t(D) * r]);
+            for (int x = 1; x < D; ++x)
+                if ((g.xmask >> x) & 1) acc = cfma(m[size_t(D) * (r ^ x)], psi[i ^ group_flip(g, x)], acc);
         }
-        out[i] = acc;
+        out[i] = cfma(dg, psi[i], acc);
     }
 }
 
@@ -370,6 +363,11 @@ int mossq_obs_create(mossq_ctx *ctx, int n, int ngroups, const int32_t *group_k,
         so += gr.k;
         int D = 1 << gr.k;
         gr.moff = mo;
+        for (int r = 0; r < D; ++r)
+            for (int x = 0; x < D; ++x) {
+                const c128 v = M[mo + r + D * (r ^ x)];
+                if (v.re != 0.0 || v.im != 0.0) gr.xmask |= 1 << x;
+            }
         for (int i = 0; i < D * D; ++i) o->h_mats.push_back(M[mo + i]);
         // Adjoint seed matrix M + M^dagger, column-major: the seed of a real
         // objective is (conj(z) H + z H^dagger) psi at z = 1

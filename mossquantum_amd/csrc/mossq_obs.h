@@ -9,7 +9,25 @@ struct ObsGroup {
     int k;         // number of sites (0..3)
     int sites[3];  // increasing 0-based sites
     int moff;      // offset of the 2^k x 2^k column-major matrix
+    int xmask;     // bit x set: M[r, r ^ x] != 0 for some local row r
 };
+
+// Row-wise action of a group sum, out_i = sum_g sum_x A_g[r, r ^ x] psi[i ^ flip_g(x)]
+// with r the local index of i on g's sites: only the flip patterns x in
+// xmask load, and every group's x = 0 (diagonal) term shares the one load of
+// psi[i].
+__host__ __device__ __forceinline__ u64 group_flip(const ObsGroup &g, int x) {
+    u64 o = 0;
+    for (int j = 0; j < g.k; ++j)
+        if (x & (1 << j)) o |= u64(1) << g.sites[j];
+    return o;
+}
+
+__host__ __device__ __forceinline__ int group_row(const ObsGroup &g, u64 i) {
+    int r = 0;
+    for (int j = 0; j < g.k; ++j) r |= int((i >> g.sites[j]) & 1) << j;
+    return r;
+}
 
 struct ObsSites {
     int s[40];

@@ -91,6 +91,7 @@ struct TGroup {
     int probe;  // -1: none
     int pcode;
     int pmoff;
+    unsigned pmask;  // nonzero parts of D: bit 2e real, bit 2e + 1 imaginary part of entry e
 };
 
 struct TDiag {
@@ -307,8 +308,9 @@ __device__ __forceinline__ void group_op(const TGroup &g, const TDiag *__restric
     }
 }
 
-__device__ __forceinline__ c128 probe_op(int code, const c128 *y, const c128 *l, const c128 *__restrict__ D);
-__device__ __forceinline__ c128 wave_sum(c128 a);
+__device__ __forceinline__ double probe_op_re(int code, unsigned mask, const c128 *y, const c128 *l,
+                                              const c128 *__restrict__ D);
+__device__ __forceinline__ double wave_sum_re(double a);
 
 // Probe of a group against a fixed state: this wave's share of
 // sum conj(cur) (D fixed) over the tile, fixed read in the current layout
@@ -325,8 +327,8 @@ __device__ __forceinline__ void fixed_probe(const TGroup &g, const c128 *cur, c1
             if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
         f[j] = load_nt(fixed + (gt | o));
     }
-    const c128 acc = wave_sum(probe_op(g.pcode, cur, f, pool + g.pmoff));
-    if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = acc;
+    const double acc = wave_sum_re(probe_op_re(g.pcode, g.pmask, cur, f, pool + g.pmoff));
+    if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = c128{acc, 0.0};
 }
 
 // Global index bits of a tile: tile-id bit i sits at global bit cbits[i].
@@ -480,64 +482,61 @@ __device__ __forceinline__ void dense_ip(int code, c128 *v, const c128 *__restri
     }
 }
 
-// sum over the thread's slots of conj(y) (D l), D on register bit A / bits A, B.
-template <int A>
-__device__ __forceinline__ c128 probe1(const c128 *y, const c128 *l, const c128 *__restrict__ D) {
-    const c128 d00 = D[0], d10 = D[1], d01 = D[2], d11 = D[3];
-    c128 acc = c128{0.0, 0.0};
+// Re sum over the thread's slots of conj(y) (D l), accumulating only the
+// parts of D that are nonzero (mask bit 2e: real, 2e + 1: imaginary part of
+// entry e = r + dim * c, which pairs conj(y) at row r with l at column c):
+// Re(D conj(a) b) = Re D Re(conj(a) b) - Im D Im(conj(a) b).  Gradient probes
+// of rotations touch two of the eight parts of a 2x2 D.
+template <int NB>
+__device__ __forceinline__ double probe_part(const c128 *y, const c128 *l, int msk, int orow, int ocol, bool imag) {
+    double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-        if (j & (1 << A)) continue;
-        const c128 l0 = l[j], l1 = l[j | (1 << A)];
-        acc = cfmaconj(y[j], cfma(d01, l1, cmul(d00, l0)), acc);
-        acc = cfmaconj(y[j | (1 << A)], cfma(d11, l1, cmul(d10, l0)), acc);
+        if (j & msk) continue;
+        const c128 a = y[j | orow], b = l[j | ocol];
+        acc = imag ? fma(a.re, b.im, fma(-a.im, b.re, acc)) : fma(a.re, b.re, fma(a.im, b.im, acc));
     }
     return acc;
 }
 
 template <int A, int B>
-__device__ __forceinline__ c128 probe2(const c128 *y, const c128 *l, const c128 *__restrict__ D) {
-    c128 acc = c128{0.0, 0.0};
+__device__ __forceinline__ double probe_re(const c128 *y, const c128 *l, const c128 *__restrict__ D, unsigned mask) {
+    constexpr int dim = B < 0 ? 2 : 4;
+    constexpr int msk = B < 0 ? (1 << A) : ((1 << A) | (1 << B));
+    double acc = 0.0;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const c128 m0 = D[r], m1 = D[r + 4], m2 = D[r + 8], m3 = D[r + 12];
-        const int orow = ((r & 1) << A) | ((r >> 1) << B);
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            if (j & ((1 << A) | (1 << B))) continue;
-            c128 t = cmul(m0, l[j]);
-            t = cfma(m1, l[j | (1 << A)], t);
-            t = cfma(m2, l[j | (1 << B)], t);
-            t = cfma(m3, l[j | (1 << A) | (1 << B)], t);
-            acc = cfmaconj(y[j | orow], t, acc);
-        }
+    for (int e = 0; e < dim * dim; ++e) {
+        const int r = e % dim, cc = e / dim;
+        const int orow = B < 0 ? (r << A) : (((r & 1) << A) | ((r >> 1) << (B < 0 ? 0 : B)));
+        const int ocol = B < 0 ? (cc << A) : (((cc & 1) << A) | ((cc >> 1) << (B < 0 ? 0 : B)));
+        if ((mask >> (2 * e)) & 1u) acc = fma(D[e].re, probe_part<0>(y, l, msk, orow, ocol, false), acc);
+        if ((mask >> (2 * e + 1)) & 1u) acc = fma(-D[e].im, probe_part<0>(y, l, msk, orow, ocol, true), acc);
     }
     return acc;
 }
 
-__device__ __forceinline__ c128 probe_op(int code, const c128 *y, const c128 *l, const c128 *__restrict__ D) {
+__device__ __forceinline__ double probe_op_re(int code, unsigned mask, const c128 *y, const c128 *l,
+                                              const c128 *__restrict__ D) {
     switch (code) {
-        case C_DENSE1 + 0: return probe1<0>(y, l, D);
-        case C_DENSE1 + 1: return probe1<1>(y, l, D);
-        case C_DENSE1 + 2: return probe1<2>(y, l, D);
-        case C_DENSE1 + 3: return probe1<3>(y, l, D);
-        case C_DENSE2 + 0: return probe2<0, 1>(y, l, D);
-        case C_DENSE2 + 1: return probe2<0, 2>(y, l, D);
-        case C_DENSE2 + 2: return probe2<0, 3>(y, l, D);
-        case C_DENSE2 + 3: return probe2<1, 2>(y, l, D);
-        case C_DENSE2 + 4: return probe2<1, 3>(y, l, D);
-        default: return probe2<2, 3>(y, l, D);
+        case C_DENSE1 + 0: return probe_re<0, -1>(y, l, D, mask);
+        case C_DENSE1 + 1: return probe_re<1, -1>(y, l, D, mask);
+        case C_DENSE1 + 2: return probe_re<2, -1>(y, l, D, mask);
+        case C_DENSE1 + 3: return probe_re<3, -1>(y, l, D, mask);
+        case C_DENSE2 + 0: return probe_re<0, 1>(y, l, D, mask);
+        case C_DENSE2 + 1: return probe_re<0, 2>(y, l, D, mask);
+        case C_DENSE2 + 2: return probe_re<0, 3>(y, l, D, mask);
+        case C_DENSE2 + 3: return probe_re<1, 2>(y, l, D, mask);
+        case C_DENSE2 + 4: return probe_re<1, 3>(y, l, D, mask);
+        default: return probe_re<2, 3>(y, l, D, mask);
     }
 }
 
-__device__ __forceinline__ c128 wave_sum(c128 a) {
+__device__ __forceinline__ double wave_sum_re(double a) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        a.re += __shfl_xor(a.re, off);
-        a.im += __shfl_xor(a.im, off);
-    }
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
     return a;
 }
+
 
 __device__ __forceinline__ void transpose_lds(c128 *v, char *ldsb, int t, const TSeg &sg) {
     int a[TN];
@@ -592,8 +591,8 @@ __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__re
                 dense_ip(g.code, l, pool + g.moff, F);
             }
             if (g.probe >= 0) {
-                const c128 acc = wave_sum(probe_op(g.pcode, y, l, pool + g.pmoff));
-                if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = acc;
+                const double acc = wave_sum_re(probe_op_re(g.pcode, g.pmask, y, l, pool + g.pmoff));
+                if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = c128{acc, 0.0};
             }
         }
     }
@@ -1515,6 +1514,13 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
                 }
                 g->probe = o.probe;
                 g->pmoff = o.moff;
+                g->pmask = 0;
+                const int ne = o.type == T_PROBE1 ? 4 : 16;
+                for (int e = 0; e < ne; ++e) {
+                    const c128 d = fp->h_pool[o.moff + e];
+                    if (d.re != 0.0) g->pmask |= 1u << (2 * e);
+                    if (d.im != 0.0) g->pmask |= 1u << (2 * e + 1);
+                }
                 continue;
             }
             if (o.type == T_SIGN && o.negmask == 0) continue;  // identity (e.g. two equal CZs merged)
