@@ -191,9 +191,10 @@ int mossq_graph_destroy(mossq_graph *graph);
  * the tile kernel's index arithmetic; needs no device.  Used to test the
  * planner (the reference's fuse_gates, src/utilities/algorithms/
  * gate_fusion.jl:10-62, is its CPU counterpart).  state (complex, in place) has
- * 2^n (or 4^n) entries; stats[0..8] = sweeps, unfused launches, segments,
+ * 2^n (or 4^n) entries; stats[0..9] = sweeps, unfused launches, segments,
  * LDS transpositions, tile ops, of which one-bit dense, two-bit dense,
- * diagonal; transpositions without a bank-conflict-free LDS slot map.
+ * diagonal; transpositions without a bank-conflict-free LDS slot map; groups
+ * that run as register permutations (two-bit ops such as CNOT and SWAP).
  * state == NULL returns the statistics only.
  * Registers of fewer than 12 bits are rejected. */
 int mossq_plan_emulate(int n, int is_density, int nops, const mossq_op *ops, const double *pool,

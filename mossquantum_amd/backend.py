@@ -208,13 +208,13 @@ def plan_emulate(circuit: QCircuit, n: int, is_density: bool, state: np.ndarray)
     lw = lower(circuit, n, is_density)
     recs, pool = lw.arrays()
     st = None if state is None else np.ascontiguousarray(np.array(state, dtype=np.complex128).reshape(-1, order="F"))
-    stats = (C.c_int * 9)()
+    stats = (C.c_int * 10)()
     check(_lib.load().mossq_plan_emulate(n, int(is_density), len(lw.records), recs,
                                          pool.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), pool.size,
                                          None if st is None else st.view(np.float64).ctypes.data_as(
                                              C.POINTER(C.c_double)), stats))
     keys = ("sweeps", "unfused", "segments", "transitions", "tile_ops", "dense1", "dense2", "diag",
-            "conflicted_transitions")
+            "conflicted_transitions", "permutations")
     return st, dict(zip(keys, list(stats)))
 
 
@@ -228,12 +228,12 @@ def plan_emulate_adjoint(circuit: QCircuit, n: int, y: np.ndarray, lam: np.ndarr
     ys = np.ascontiguousarray(np.array(y, dtype=np.complex128))
     ls = np.ascontiguousarray(np.array(lam, dtype=np.complex128))
     grads = np.zeros(max(lw.nderiv, 1))
-    stats = (C.c_int * 9)()
+    stats = (C.c_int * 10)()
     dp = lambda a: a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
     check(_lib.load().mossq_plan_emulate_adjoint(n, len(lw.records), recs, dp(pool), pool.size, dp(ys), dp(ls),
                                                  grads.ctypes.data_as(C.POINTER(C.c_double)), stats))
     keys = ("sweeps", "unfused", "segments", "transitions", "tile_ops", "dense1", "dense2", "diag",
-            "conflicted_transitions")
+            "conflicted_transitions", "permutations")
     return ys, ls, grads[:lw.nderiv], dict(zip(keys, list(stats)))
 
 

@@ -96,6 +96,44 @@ __device__ __forceinline__ void bgate_chunk(const BItemDev &it, const BOp &op, b
     }
 }
 
+// Observe step on rows [i0, i1) of one item: E partial = sum_i conj(psi_i)
+// (H psi)_i and lambda = (H + H^dagger) psi.  Only the nonzero flip patterns
+// of each group load; the diagonal terms of all groups share psi[i].  The
+// group records and matrices come from LDS when they fit, else from the
+// observable's device tables.
+constexpr int kObsGroupsLds = 64;
+constexpr int kObsMatsLds = 512;
+
+__device__ __forceinline__ c128 observe_rows(const BItemDev &it, u64 i0, u64 i1, const ObsGroup *groups,
+                                             const c128 *mats, const c128 *seed) {
+    const c128 *__restrict__ psi = it.psi;
+    c128 acc = mk(0, 0);
+    for (u64 i = i0 + threadIdx.x; i < i1; i += kBT) {
+        c128 hp = mk(0, 0), sd = mk(0, 0), dh = mk(0, 0), ds = mk(0, 0);
+        for (int gi = 0; gi < it.G; ++gi) {
+            const ObsGroup g = groups[gi];
+            const int D = 1 << g.k;
+            const int r = group_row(g, i);
+            const c128 *m = mats + g.moff + r, *sm = seed + g.moff + r;
+            if (g.xmask & 1) {
+                dh = cadd(dh, m[D * r]);
+                ds = cadd(ds, sm[D * r]);
+            }
+            for (int x = 1; x < D; ++x)
+                if ((g.xmask >> x) & 1) {
+                    const c128 y = psi[i ^ group_flip(g, x)];
+                    hp = cfma(m[D * (r ^ x)], y, hp);
+                    sd = cfma(sm[D * (r ^ x)], y, sd);
+                }
+        }
+        const c128 x0 = psi[i];
+        hp = cfma(dh, x0, hp);
+        if (!it.value_only) it.lam[i] = cfma(ds, x0, sd);
+        acc = cfmaconj(x0, hp, acc);
+    }
+    return acc;
+}
+
 // One engine iteration for every active item: a workgroup owns one chunk of
 // one item and performs that item's current micro-op on it.
 __global__ __launch_bounds__(kBT, 2) void k_biter(const BItemDev *__restrict__ items, int nitems, int t,
@@ -117,33 +155,23 @@ __global__ __launch_bounds__(kBT, 2) void k_biter(const BItemDev *__restrict__ i
     if (pc < 0 || pc > (it.value_only ? nfwd : 2 * nfwd)) return;
     c128 *psi = it.psi, *lam = it.lam;
     if (pc == nfwd) {
-        // observe: E partial = sum_i conj(psi_i) (H psi)_i ; lambda = (H + H^dagger) psi
+        __shared__ ObsGroup s_groups[kObsGroupsLds];
+        __shared__ c128 s_mats[kObsMatsLds], s_seed[kObsMatsLds];
         const u64 len = u64(1) << n, per = len / u64(it.nchunk);
-        c128 acc = mk(0, 0);
-        for (u64 i = u64(c) * per + threadIdx.x; i < u64(c + 1) * per; i += kBT) {
-            // only the nonzero flip patterns of each group load; the diagonal
-            // terms of all groups share psi[i]
-            c128 hp = mk(0, 0), sd = mk(0, 0), dh = mk(0, 0), ds = mk(0, 0);
-            for (int gi = 0; gi < it.G; ++gi) {
-                const ObsGroup g = it.groups[gi];
-                const int D = 1 << g.k;
-                const int r = group_row(g, i);
-                const c128 *m = it.mats + g.moff + r, *sm = it.seed + g.moff + r;
-                if (g.xmask & 1) {
-                    dh = cadd(dh, m[D * r]);
-                    ds = cadd(ds, sm[D * r]);
-                }
-                for (int x = 1; x < D; ++x)
-                    if ((g.xmask >> x) & 1) {
-                        const c128 y = psi[i ^ group_flip(g, x)];
-                        hp = cfma(m[D * (r ^ x)], y, hp);
-                        sd = cfma(sm[D * (r ^ x)], y, sd);
-                    }
+        const ObsGroup last = it.groups[it.G - 1];
+        const int nm = last.moff + (1 << (2 * last.k));
+        c128 acc;
+        if (it.G <= kObsGroupsLds && nm <= kObsMatsLds) {
+            for (int j = threadIdx.x; j < it.G; j += kBT) s_groups[j] = it.groups[j];
+            for (int j = threadIdx.x; j < nm; j += kBT) {
+                s_mats[j] = it.mats[j];
+                s_seed[j] = it.seed[j];
             }
-            const c128 x0 = psi[i];
-            hp = cfma(dh, x0, hp);
-            if (!it.value_only) lam[i] = cfma(ds, x0, sd);
-            acc = cfmaconj(x0, hp, acc);
+            __syncthreads();
+            acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, s_groups, s_mats, s_seed);
+        } else {
+            acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, const_table(it.groups), const_table(it.mats),
+                               const_table(it.seed));
         }
         c128 s = bblock_sum(acc, red);
         if (threadIdx.x == 0) part[it.chunk0 + c] = s;

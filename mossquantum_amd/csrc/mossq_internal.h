@@ -122,6 +122,16 @@ int stage_matrix(Ctx *c, const c128 *host, u64 count, const c128 **dev);
 int get_pinned(Ctx *c, u64 bytes, double **p);
 
 // Scoped device allocations: everything obtained through get() is freed on exit.
+// A device table that a kernel only reads, as a constant-memory pointer: a
+// pointer read from memory is otherwise generic and loads through the flat
+// path into vector registers, where constant loads of uniform addresses are
+// scalar.
+template <class T>
+__device__ __forceinline__ const T *const_table(const T *p) {
+    typedef __attribute__((address_space(4))) const T cT;
+    return (const T *)(cT *)reinterpret_cast<uintptr_t>(p);
+}
+
 struct Allocs {
     Ctx *c;
     std::vector<void *> ptrs;

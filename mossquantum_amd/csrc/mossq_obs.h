@@ -16,16 +16,21 @@ struct ObsGroup {
 // with r the local index of i on g's sites: only the flip patterns x in
 // xmask load, and every group's x = 0 (diagonal) term shares the one load of
 // psi[i].
+// (Loops over the three site slots with constant indices: a dynamic index
+// into the record would put it in scratch memory.)
 __host__ __device__ __forceinline__ u64 group_flip(const ObsGroup &g, int x) {
     u64 o = 0;
-    for (int j = 0; j < g.k; ++j)
-        if (x & (1 << j)) o |= u64(1) << g.sites[j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        if (j < g.k && (x & (1 << j))) o |= u64(1) << g.sites[j];
     return o;
 }
 
 __host__ __device__ __forceinline__ int group_row(const ObsGroup &g, u64 i) {
     int r = 0;
-    for (int j = 0; j < g.k; ++j) r |= int((i >> g.sites[j]) & 1) << j;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        if (j < g.k) r |= int((i >> g.sites[j]) & 1) << j;
     return r;
 }
 

@@ -64,14 +64,18 @@ struct TOp {
 //     register-bit pair (kPairA/kPairB);
 //   C_FLIP + any of those: the dense op, then sign flips of its outputs;
 //   C_COPY: copy with sign flips (flips at the start of a segment);
-//   C_DIAG: a general diagonal (TDiag record x).
+//   C_DIAG: a general diagonal (TDiag record x);
+//   C_PERM + 6 * pair + kind: a 4x4 permutation on a register-bit pair
+//     (CNOT either way, SWAP, X on either bit or both; perm_src), which only
+//     moves slots; + kPermFlip: the same, then sign flips.
 // Sign flips are controlled-Z-type diagonals (-1 exactly where all their bits
 // are 1), up to kGroupCz per group: flip i negates the slots in the 16-bit slot
 // pattern pat[i] wherever its condition bits are all 1.  cond[i] packs up to
 // three condition bits, 7 bits each from bit 0: 0x7F none; 0x40 | b: global bit
 // b outside the tile; else tile-local bit b.  A segment with an odd number of
 // groups ends with a plain C_COPY, so groups always come in pairs.
-enum : int { C_DENSE1 = 0, C_DENSE2 = 4, C_FLIP = 16, C_COPY = 30, C_DIAG = 31 };
+enum : int { C_DENSE1 = 0, C_DENSE2 = 4, C_FLIP = 16, C_COPY = 30, C_DIAG = 31, C_PERM = 32 };
+constexpr int kPermFlip = 36;
 enum : int { D_SIGN = 0, D_SELECT = 1, D_GATHER = 2 };
 constexpr int kPairA[6] = {0, 0, 0, 1, 1, 2};
 constexpr int kPairB[6] = {1, 2, 3, 2, 3, 3};
@@ -176,6 +180,58 @@ __device__ __forceinline__ void reg_apply2(const c128 *in, c128 *out, const c128
     }
 }
 
+// Permutation ops on register bits (A, B): slot j takes slot perm_src(j).
+// With r = bit A + 2 bit B the local index, kind 0 is CNOT controlled by A
+// (r: 1 <-> 3), kind 1 CNOT controlled by B (2 <-> 3), kind 2 SWAP (1 <-> 2),
+// kinds 3, 4, 5 flip bit A, bit B, both.
+__host__ __device__ constexpr int perm_src(int j, int A, int B, int kind) {
+    const int a = (j >> A) & 1, b = (j >> B) & 1;
+    return kind == 0   ? (a ? j ^ (1 << B) : j)
+           : kind == 1 ? (b ? j ^ (1 << A) : j)
+           : kind == 2 ? (a != b ? j ^ ((1 << A) | (1 << B)) : j)
+           : kind == 3 ? j ^ (1 << A)
+           : kind == 4 ? j ^ (1 << B)
+                       : j ^ ((1 << A) | (1 << B));
+}
+
+// In place: each instantiation reads through its own local copy (with one
+// source array for all 36 cases, the compiler merges their moves into moves
+// from a selected address, which takes the register arrays to scratch).
+template <int A, int B, int K, bool FLIP>
+__device__ __forceinline__ void reg_perm(const c128 *in, c128 *out, unsigned F) {
+    c128 t[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) t[j] = in[j];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) out[j] = FLIP ? flip_slot(t[perm_src(j, A, B, K)], F, j) : t[perm_src(j, A, B, K)];
+}
+
+// Permutation groups (code >= C_PERM): pk = code - C_PERM below kPermFlip.
+#define MOSSQ_PERM6(P, STMT) \
+    STMT(P, 0)               \
+    STMT(P, 1)               \
+    STMT(P, 2)               \
+    STMT(P, 3)               \
+    STMT(P, 4)               \
+    STMT(P, 5)
+#define MOSSQ_PERM36(STMT) \
+    MOSSQ_PERM6(0, STMT)   \
+    MOSSQ_PERM6(1, STMT)   \
+    MOSSQ_PERM6(2, STMT)   \
+    MOSSQ_PERM6(3, STMT)   \
+    MOSSQ_PERM6(4, STMT)   \
+    MOSSQ_PERM6(5, STMT)
+
+template <bool FLIP>
+__device__ __forceinline__ void perm_ip(int pk, c128 *v, unsigned F) {
+#define MOSSQ_PERM_IP(P, K)                               \
+    case 6 * P + K:                                       \
+        reg_perm<kPairA[P], kPairB[P], K, FLIP>(v, v, F); \
+        break;
+    switch (pk) { MOSSQ_PERM36(MOSSQ_PERM_IP) }
+#undef MOSSQ_PERM_IP
+}
+
 // Tile-local index of thread t's register slot 0 in a layout.
 __host__ __device__ __forceinline__ int seg_tdep(int t, const TSeg &sg) {
     int d = 0;
@@ -274,11 +330,23 @@ __host__ __device__ __forceinline__ unsigned group_flips(const TGroup &g, int td
 }
 
 // One group, out of place.
+// `in` is scratch afterwards (permutations run in place on it).
 __device__ __forceinline__ void group_op(const TGroup &g, const TDiag *__restrict__ diag,
-                                         const c128 *__restrict__ pool, const c128 *in, c128 *out, int tdep,
+                                         const c128 *__restrict__ pool, c128 *in, c128 *out, int tdep,
                                          u64 base) {
     const c128 *M = pool + g.moff;
     const unsigned F = group_flips(g, tdep, base);
+    if (g.code >= C_PERM) {
+        // permuted in place, then moved: out-of-place permutation switches
+        // spill (see reg_perm)
+        if (g.code >= C_PERM + kPermFlip)
+            perm_ip<true>(g.code - C_PERM - kPermFlip, in, F);
+        else
+            perm_ip<false>(g.code - C_PERM, in, F);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) out[j] = in[j];
+        return;
+    }
     switch (g.code) {
         case C_DENSE1 + 0: reg_apply1<0, false>(in, out, M, F); break;
         case C_DENSE1 + 1: reg_apply1<1, false>(in, out, M, F); break;
@@ -454,6 +522,13 @@ __device__ __forceinline__ void ip_apply2(c128 *v, const c128 *__restrict__ M, u
 }
 
 __device__ __forceinline__ void dense_ip(int code, c128 *v, const c128 *__restrict__ M, unsigned F) {
+    if (code >= C_PERM) {
+        if (code >= C_PERM + kPermFlip)
+            perm_ip<true>(code - C_PERM - kPermFlip, v, F);
+        else
+            perm_ip<false>(code - C_PERM, v, F);
+        return;
+    }
     switch (code) {
         case C_DENSE1 + 0: reg_apply1<0, false>(v, v, M, F); break;
         case C_DENSE1 + 1: reg_apply1<1, false>(v, v, M, F); break;
@@ -854,6 +929,31 @@ static std::vector<c128> kron2(const std::vector<c128> &Am, const std::vector<c1
     return out;
 }
 
+// Kind (perm_src) of a 4x4 column-major matrix that is one of the slot
+// permutations the tile kernels move without arithmetic, else -1.
+static int perm_kind(const c128 *M) {
+    int src[4];
+    for (int r = 0; r < 4; ++r) {
+        src[r] = -1;
+        for (int c = 0; c < 4; ++c) {
+            const c128 v = M[r + 4 * c];
+            if (v.re == 1.0 && v.im == 0.0 && src[r] < 0)
+                src[r] = c;
+            else if (v.re != 0.0 || v.im != 0.0)
+                return -1;
+        }
+        if (src[r] < 0) return -1;
+    }
+    for (int kind = 0; kind < 6; ++kind) {
+        bool ok = true;
+        for (int r = 0; r < 4; ++r) ok &= perm_src(r, 0, 1, kind) == src[r];
+        if (ok) return kind;
+    }
+    return -1;
+}
+
+static bool is_perm(const POp &o) { return o.kind == 0 && o.nb == 2 && perm_kind(o.m.data()) >= 0; }
+
 static bool commutes(const POp &a, const POp &b) {
     if (a.kind == 2 || b.kind == 2) return false;  // unfusable steps never move
     if (a.fixed || b.fixed) return false;
@@ -903,7 +1003,8 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, boo
                     }
             if (D.nb == 1 || last[0] == last[1]) {
                 int j = last[0];
-                if (j >= 0 && ops[j].kind == 0 && (ops[j].mask() & D.mask()) == D.mask()) {
+                // a permutation stays arithmetic-free: the diagonal keeps its own op
+                if (j >= 0 && ops[j].kind == 0 && !is_perm(ops[j]) && (ops[j].mask() & D.mask()) == D.mask()) {
                     absorb_diag(ops[j], D);
                     ops.erase(ops.begin() + i);
                     changed = true;
@@ -946,9 +1047,10 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, boo
         }
     }
     // Fold one-bit dense ops into the two-bit dense op they meet first on
-    // their bit (same flops, one op fewer), then pair the remaining one-bit
-    // dense ops on different bits into Kronecker products: a two-bit op costs
-    // the same flops as two one-bit ops but half the dispatch work.
+    // their bit (same flops, one op fewer; not into permutations, which cost
+    // no arithmetic), then pair the remaining one-bit dense ops on different
+    // bits into Kronecker products: a two-bit op costs the same flops as two
+    // one-bit ops but half the dispatch work.
     auto embed1 = [](const POp &one, const POp &two) {
         std::vector<c128> I = {mk(1, 0), mk(0, 0), mk(0, 0), mk(1, 0)};
         return one.bits[0] == two.bits[0] ? kron2(one.m, I) : kron2(I, one.m);
@@ -962,6 +1064,7 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, boo
             while (j < ops.size() && (ops[j].mask() & ops[i].mask()) == 0) ++j;
             if (j >= ops.size() || ops[j].kind != 0) continue;
             POp &A = ops[i], &B = ops[j];
+            if (is_perm(A) || is_perm(B)) continue;
             if (A.nb == 1 && B.nb == 2) {  // B * (A on one of B's bits)
                 std::vector<c128> pr;
                 product(B.m, embed1(A, B), 4, pr);
@@ -1497,6 +1600,8 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
             if (o.type == T_DENSE2) {
                 int code;
                 if (!pair_code(o, &code)) return MOSSQ_EINVAL;
+                const int kind = perm_kind(fp->h_pool.data() + o.moff);
+                if (kind >= 0) code = C_PERM + 6 * (code - C_DENSE2) + kind;
                 add(code, o.moff);
                 continue;
             }
@@ -1532,7 +1637,10 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
                     add(C_COPY, 0);
                     g = &fp->h_groups.back();
                 }
-                if (g->code < C_FLIP) g->code += C_FLIP;
+                if (g->code < C_FLIP)
+                    g->code += C_FLIP;
+                else if (g->code >= C_PERM && g->code < C_PERM + kPermFlip)
+                    g->code += kPermFlip;
                 g->pat[g->ncz] = pat;
                 g->cond[g->ncz] = cond;
                 g->ncz++;
@@ -1571,7 +1679,10 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
 // and each tile op by type.
 static double plan_cost(const FusedPlan *fp) {
     double t = 300.0 * fp->nsweeps + 87.0 * fp->ntransitions;
-    for (const TOp &o : fp->h_ops) t += o.type == T_DENSE2 ? 27.0 : o.type == T_DENSE1 ? 20.0 : 10.0;
+    for (const TOp &o : fp->h_ops)
+        t += o.type == T_DENSE2 ? (perm_kind(fp->h_pool.data() + o.moff) >= 0 ? 6.0 : 27.0)
+             : o.type == T_DENSE1 ? 20.0
+                                  : 10.0;
     for (const PlanItem &it : fp->items)
         if (!it.sweep) t += 300.0;
     return t;
@@ -1712,12 +1823,21 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
         for (int gi = sg.g0; gi < sg.g1; ++gi) {
             const TGroup &g = fp->h_groups[gi];
             const c128 *M = fp->h_pool.data() + g.moff;
-            const int dense = g.code >= C_FLIP && g.code < C_COPY ? g.code - C_FLIP : g.code;
+            const bool perm = g.code >= C_PERM;
+            const int pk = perm ? (g.code - C_PERM) % kPermFlip : 0;
+            const int dense = perm ? -1 : g.code >= C_FLIP && g.code < C_COPY ? g.code - C_FLIP : g.code;
+            const bool flips = perm ? g.code - C_PERM >= kPermFlip : g.code >= C_FLIP && g.code <= C_COPY;
             for (int t = 0; t < TT; ++t) {
                 for (size_t q = 0; q < ns; ++q) {
                     c128 *v = &V[q][t * TN];
-                    dense_host(dense, M, v);
-                    if (g.code >= C_FLIP && g.code <= C_COPY) {
+                    if (perm) {
+                        c128 in[TN];
+                        std::copy(v, v + TN, in);
+                        for (int j = 0; j < TN; ++j) v[j] = in[perm_src(j, kPairA[pk / 6], kPairB[pk / 6], pk % 6)];
+                    } else {
+                        dense_host(dense, M, v);
+                    }
+                    if (flips) {
                         const unsigned F = group_flips(g, tdep[t], base);
                         for (int j = 0; j < TN; ++j)
                             if ((F >> j) & 1) v[j] = mk(-v[j].re, -v[j].im);
@@ -2463,6 +2583,8 @@ static void plan_stats(const FusedPlan *fp, int *stats) {
             stats[7]++;
     }
     stats[8] = fp->nconflicted;
+    stats[9] = 0;
+    for (const TGroup &g : fp->h_groups) stats[9] += g.code >= C_PERM;
 }
 
 extern "C" {

@@ -7,7 +7,7 @@ import pytest
 import oracle
 import mossquantum_amd as mq
 from mossquantum_amd import lower
-from mossquantum_amd.backend import plan_emulate
+from mossquantum_amd.backend import plan_emulate, plan_emulate_adjoint
 from mossquantum_amd.gates import haar_unitary
 from mossquantum_amd.workloads import haar_layers, hea_circuit, random_circuit
 
@@ -64,6 +64,35 @@ def test_mixed_instruction_kinds():
     c = mq.QCircuit(ops)
     st = check(c, n, False, rand_state(n, 3))
     assert st["unfused"] >= 1
+
+
+def test_permutation_groups():
+    """Two-bit permutations (CNOT either way, SWAP, X on one or both bits) run
+    as register moves, alone and followed by controlled-Z sign flips; the
+    planner keeps one-qubit gates out of them."""
+    n = 13
+    rng = np.random.default_rng(5)
+    X = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    I2 = np.eye(2, dtype=np.complex128)
+    ops = []
+    for layer in range(3):
+        for q in range(1, n + 1):
+            ops.append(mq.QuantumGate((q,), haar_unitary(rng)))
+        for q in range(1, n, 2):
+            ops += [mq.CNOTGate(q, q + 1), mq.CNOTGate(q + 1, q), mq.SWAPGate(q, q + 1)]
+        for q in range(2, n, 3):
+            ops += [mq.QuantumGate((q, q + 1), np.kron(X, I2)), mq.QuantumGate((q, q + 1), np.kron(I2, X)),
+                    mq.QuantumGate((q, q + 1), np.kron(X, X)), mq.CZGate(q, q + 1)]
+        ops += [mq.CNOTGate(1, 2), mq.CZGate(1, 2), mq.CZGate(2, 5)]
+    c = mq.QCircuit(ops)
+    st = check(c, n, False, rand_state(n, 5))
+    assert st["permutations"] > 0
+    # forward and reverse (adjoint) plans of an ansatz keep CNOTs as permutations
+    circ = hea_circuit(n, 2, rng.uniform(0, 2 * np.pi, 4 * n))
+    _, st = plan_emulate(circ, n, False, None)
+    assert st["permutations"] > 0
+    _, _, _, st = plan_emulate_adjoint(circ, n, rand_state(n, 6), rand_state(n, 7))
+    assert st["permutations"] >= n - 1
 
 
 @pytest.mark.parametrize("n", [6, 7])
