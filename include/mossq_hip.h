@@ -245,9 +245,12 @@ int mossq_adjoint(mossq_ctx *ctx, const mossq_program *prog, const mossq_obs *ob
 
 /* Batched adjoint gradients of B independent programs (uneven qubit counts,
  * own parameters, own observable), all started from |0...0>.  grads of item b
- * start at grads[grad_offset[b]].  New capability: the reference's batch
- * reverse is per column with one shared circuit
- * (src/differentiation/batch.jl:35-66). */
+ * start at grads[grad_offset[b]].  Programs created with fuse=1 of 12 or more
+ * qubits (whose fused plans are all sweeps) run their forward and reverse
+ * passes as batched LDS-tile sweeps, one launch per sweep index for the whole
+ * batch; the other items (gates on at most two qubits) run gate by gate.
+ * New capability: the reference's batch reverse is per column with one
+ * shared circuit (src/differentiation/batch.jl:35-66). */
 int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs,
                         mossq_obs *const *obs, const int64_t *grad_offset, double *values,
                         double *grads);
@@ -255,6 +258,8 @@ int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs,
 /* Values Re <H_b> of B independent state-vector programs in one engine run
  * (forward and observe steps only), item b starting from init[b] (a device
  * vector of 2^n_b amplitudes) or from |0...0> when init or init[b] is NULL.
+ * Fused programs of 12 or more qubits run their forward pass as batched
+ * tile sweeps, as in mossq_batch_adjoint.
  * Serves the sampled-gradient methods, whose shifted or perturbed circuits
  * are evaluated together (reference ParameterShift / FiniteDifference /
  * SPSAGradient, src/differentiation/sampled_gradients.jl:323-406, which run
