@@ -379,8 +379,9 @@ def main():
         args.serving_requests = 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box has one GPU); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (items and requests would shard "
+              "across GPUs without exchange, a single large state by its top qubits); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import mossquantum_amd as mq

@@ -67,14 +67,16 @@ struct TOp {
 //   C_DIAG: a general diagonal (TDiag record x);
 //   C_PERM + 6 * pair + kind: a 4x4 permutation on a register-bit pair
 //     (CNOT either way, SWAP, X on either bit or both; perm_src), which only
-//     moves slots; + kPermFlip: the same, then sign flips.
+//     moves slots; + kPermFlip: the same, then sign flips;
+//   C_DIAG1 + a (adjoint plans): a diagonal 2x2 on register bit a, one
+//     complex product per slot (no flips ride on it).
 // Sign flips are controlled-Z-type diagonals (-1 exactly where all their bits
 // are 1), up to kGroupCz per group: flip i negates the slots in the 16-bit slot
 // pattern pat[i] wherever its condition bits are all 1.  cond[i] packs up to
 // three condition bits, 7 bits each from bit 0: 0x7F none; 0x40 | b: global bit
 // b outside the tile; else tile-local bit b.  A segment with an odd number of
 // groups ends with a plain C_COPY, so groups always come in pairs.
-enum : int { C_DENSE1 = 0, C_DENSE2 = 4, C_FLIP = 16, C_COPY = 30, C_DIAG = 31, C_PERM = 32 };
+enum : int { C_DENSE1 = 0, C_DENSE2 = 4, C_FLIP = 16, C_COPY = 30, C_DIAG = 31, C_PERM = 32, C_DIAG1 = 112 };
 constexpr int kPermFlip = 36;
 enum : int { D_SIGN = 0, D_SELECT = 1, D_GATHER = 2 };
 constexpr int kPairA[6] = {0, 0, 0, 1, 1, 2};
@@ -336,7 +338,7 @@ __device__ __forceinline__ void group_op(const TGroup &g, const TDiag *__restric
                                          u64 base) {
     const c128 *M = pool + g.moff;
     const unsigned F = group_flips(g, tdep, base);
-    if (g.code >= C_PERM) {
+    if (g.code >= C_PERM) {  // C_DIAG1 groups occur in adjoint plans only (dense_ip)
         // permuted in place, then moved: out-of-place permutation switches
         // spill (see reg_perm)
         if (g.code >= C_PERM + kPermFlip)
@@ -521,7 +523,23 @@ __device__ __forceinline__ void ip_apply2(c128 *v, const c128 *__restrict__ M, u
     }
 }
 
+template <int A>
+__device__ __forceinline__ void diag1_ip(c128 *v, const c128 *__restrict__ M) {
+    const c128 d0 = M[0], d1 = M[3];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) v[j] = cmul((j >> A) & 1 ? d1 : d0, v[j]);
+}
+
 __device__ __forceinline__ void dense_ip(int code, c128 *v, const c128 *__restrict__ M, unsigned F) {
+    if (code >= C_DIAG1) {
+        switch (code - C_DIAG1) {
+            case 0: diag1_ip<0>(v, M); break;
+            case 1: diag1_ip<1>(v, M); break;
+            case 2: diag1_ip<2>(v, M); break;
+            default: diag1_ip<3>(v, M); break;
+        }
+        return;
+    }
     if (code >= C_PERM) {
         if (code >= C_PERM + kPermFlip)
             perm_ip<true>(code - C_PERM - kPermFlip, v, F);
@@ -872,7 +890,8 @@ static void lower_steps_rev(const Program *p, std::vector<POp> &out) {
         const int dim = 1 << st.kb;
         // Diagonals the adjoint kernel takes as sign flips: -1 exactly at the
         // all-ones index (CZ-type) on at most 3 bits.  Other diagonals on <= 2
-        // bits become dense matrices; everything wider runs on its own.
+        // bits become dense matrices (one-bit ones then run as diagonal groups,
+        // C_DIAG1); everything wider runs on its own.
         int neg = 0;
         const bool diag = !st.split && is_diag(st.irev);
         const bool cz_type = diag && sign_only(diag_values(st.irev, dim), &neg) && neg == (1 << (dim - 1)) &&
@@ -1080,14 +1099,17 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, boo
             }
         }
     }
+    // (A diagonal one-bit op stays alone: adjoint plans run it as one complex
+    // product per amplitude, cheaper than its share of a 4x4.)
+    auto diag1 = [](const POp &o) { return o.m[1].re == 0 && o.m[1].im == 0 && o.m[2].re == 0 && o.m[2].im == 0; };
     for (size_t i = 0; kron && i < ops.size(); ++i) {
-        if (ops[i].kind != 0 || ops[i].nb != 1) continue;
+        if (ops[i].kind != 0 || ops[i].nb != 1 || diag1(ops[i])) continue;
         const int a = ops[i].bits[0];
         if (!in_reg(a)) continue;
         u64 touched = 0;
         for (size_t k = i + 1; k < ops.size(); ++k) {
             const POp &B = ops[k];
-            if (B.kind == 0 && B.nb == 1 && B.bits[0] != a && in_reg(B.bits[0]) &&
+            if (B.kind == 0 && B.nb == 1 && !diag1(B) && B.bits[0] != a && in_reg(B.bits[0]) &&
                 ((touched | (u64(1) << a)) & B.mask()) == 0) {
                 POp F;
                 F.kind = 0;
@@ -1594,7 +1616,9 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
         for (int oi = sg.op0; oi < sg.op1; ++oi) {
             const TOp &o = fp->h_ops[oi];
             if (o.type == T_DENSE1) {
-                add(C_DENSE1 + o.a, o.moff);
+                const c128 *M = fp->h_pool.data() + o.moff;
+                const bool diag2 = M[1].re == 0.0 && M[1].im == 0.0 && M[2].re == 0.0 && M[2].im == 0.0;
+                add((fp->adjoint && diag2 ? C_DIAG1 : C_DENSE1) + o.a, o.moff);
                 continue;
             }
             if (o.type == T_DENSE2) {
@@ -1633,7 +1657,7 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
             int cond;
             if (pack_flip(o, &pat, &cond)) {
                 TGroup *g = here ? &fp->h_groups.back() : nullptr;
-                if (!g || g->code == C_DIAG || g->ncz == kGroupCz || g->probe >= 0) {
+                if (!g || g->code == C_DIAG || g->code >= C_DIAG1 || g->ncz == kGroupCz || g->probe >= 0) {
                     add(C_COPY, 0);
                     g = &fp->h_groups.back();
                 }
@@ -1823,9 +1847,12 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
         for (int gi = sg.g0; gi < sg.g1; ++gi) {
             const TGroup &g = fp->h_groups[gi];
             const c128 *M = fp->h_pool.data() + g.moff;
-            const bool perm = g.code >= C_PERM;
+            const bool perm = g.code >= C_PERM && g.code < C_DIAG1;
             const int pk = perm ? (g.code - C_PERM) % kPermFlip : 0;
-            const int dense = perm ? -1 : g.code >= C_FLIP && g.code < C_COPY ? g.code - C_FLIP : g.code;
+            const int dense = perm                  ? -1
+                              : g.code >= C_DIAG1   ? C_DENSE1 + (g.code - C_DIAG1)
+                              : g.code >= C_FLIP && g.code < C_COPY ? g.code - C_FLIP
+                                                                     : g.code;
             const bool flips = perm ? g.code - C_PERM >= kPermFlip : g.code >= C_FLIP && g.code <= C_COPY;
             for (int t = 0; t < TT; ++t) {
                 for (size_t q = 0; q < ns; ++q) {
@@ -2584,7 +2611,7 @@ static void plan_stats(const FusedPlan *fp, int *stats) {
     }
     stats[8] = fp->nconflicted;
     stats[9] = 0;
-    for (const TGroup &g : fp->h_groups) stats[9] += g.code >= C_PERM;
+    for (const TGroup &g : fp->h_groups) stats[9] += g.code >= C_PERM && g.code < C_DIAG1;
 }
 
 extern "C" {

@@ -407,6 +407,24 @@ def test_batched_adjoint_tiled_matches_stepped(ctx):
     assert np.array_equal(v2, v_t) and all(np.array_equal(a, b) for a, b in zip(g2, g_t))
 
 
+def test_batched_adjoint_tiled_uneven_to_20_qubits(ctx):
+    """A mixed batch up to 20 qubits (items of 1 to 256 tiles per sweep, and
+    items below 12 qubits on the gate-by-gate engine in the same call): the
+    tiled engine matches the gate-by-gate engine on every value and gradient,
+    and the oracle on the smaller items."""
+    rng = np.random.default_rng(18)
+    sizes = [int(x) for x in rng.integers(10, 21, size=40)] + [20, 12, 11]
+    circs = [hea_circuit(n, 2, rng.uniform(0, 2 * np.pi, 4 * n)) for n in sizes]
+    obs = [xxz(n) for n in sizes]
+    v_t, g_t = mq.batched_adjoint(circs, obs, sizes, ctx, fuse=True)
+    v_s, g_s = mq.batched_adjoint(circs, obs, sizes, ctx, fuse=False)
+    assert np.max(np.abs(v_t - v_s)) < EXP
+    assert max(np.max(np.abs(a - b)) for a, b in zip(g_t, g_s)) < GRAD
+    for i in [i for i, n in enumerate(sizes) if n <= 13][:3]:
+        v, g = oracle.adjoint(lower(circs[i], sizes[i], False), obs[i].groups())
+        assert abs(v_t[i] - v) < EXP and np.max(np.abs(g_t[i] - g)) < GRAD
+
+
 def test_batched_expectation_tiled_with_initial_states(ctx):
     """Value-only batches of fused programs from given initial states match
     one expectation per item."""
