@@ -237,14 +237,19 @@ int batch_lower(const Program *p, std::vector<BOp> &ops, int *nderiv) {
     return MOSSQ_OK;
 }
 
-int batch_iteration(Ctx *c, const BItemDev *items, int nitems, int nchunks_total, int t, c128 *part, double *values,
-                    double *grads) {
+int batch_iteration_on(hipStream_t st, const BItemDev *items, int nitems, int nchunks_total, int t, c128 *part,
+                       double *values, double *grads) {
     if (nitems == 0) return MOSSQ_OK;
-    hipLaunchKernelGGL(k_biter, dim3(nchunks_total), dim3(kBT), 0, c->stream, items, nitems, t, part);
+    hipLaunchKernelGGL(k_biter, dim3(nchunks_total), dim3(kBT), 0, st, items, nitems, t, part);
     MOSSQ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_breduce, dim3(nitems), dim3(kBT), 0, c->stream, items, t, part, values, grads);
+    hipLaunchKernelGGL(k_breduce, dim3(nitems), dim3(kBT), 0, st, items, t, part, values, grads);
     MOSSQ_HIP(hipGetLastError());
     return MOSSQ_OK;
+}
+
+int batch_iteration(Ctx *c, const BItemDev *items, int nitems, int nchunks_total, int t, c128 *part, double *values,
+                    double *grads) {
+    return batch_iteration_on(c->stream, items, nitems, nchunks_total, t, part, values, grads);
 }
 
 // Initial states |0...0> for the items of a table whose psi is not preset; a
@@ -333,13 +338,14 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         for (int b = 0; b < B; ++b)
             if ((tiled[b] != 0) == (k == 0)) order.push_back(b);
     Allocs al(c);
-    c128 *vec = nullptr, *part = nullptr, *rpart = nullptr;
+    c128 *vec = nullptr, *part = nullptr, *part2 = nullptr, *rpart = nullptr;
     BOp *d_ops = nullptr;
     BItemDev *d_items = nullptr;
     BRows *d_rows = nullptr;
     double *d_vals = nullptr, *d_grads = nullptr;
     MOSSQ_TRY(al.get(u64(total_amps) * (value_only ? 1 : 2) * sizeof(c128), (void **)&vec));
-    MOSSQ_TRY(al.get(u64(std::max({nchunks[0], nchunks[1], 1})) * sizeof(c128), (void **)&part));
+    MOSSQ_TRY(al.get(u64(std::max(nchunks[0], 1)) * sizeof(c128), (void **)&part));
+    MOSSQ_TRY(al.get(u64(std::max(nchunks[1], 1)) * sizeof(c128), (void **)&part2));
     MOSSQ_TRY(al.get(std::max<size_t>(all_ops.size(), 1) * sizeof(BOp), (void **)&d_ops));
     MOSSQ_TRY(al.get(u64(B) * sizeof(BItemDev), (void **)&d_items));
     MOSSQ_TRY(al.get(u64(B) * sizeof(double), (void **)&d_vals));
@@ -416,13 +422,24 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
             hipLaunchKernelGGL(k_binit, dim3(nchunks[k]), dim3(kBT), 0, c->stream, tab[k], ntab[k]);
             MOSSQ_HIP(hipGetLastError());
         }
+    // With both kinds of items, the gate-by-gate items (many small launches)
+    // run on the second stream beside the tiled sweeps; they share no memory
+    // but the result arrays, where their slots are disjoint.
+    const bool overlap = ntab[0] && ntab[1];
+    hipStream_t s2 = c->stream;
+    if (overlap) {
+        MOSSQ_TRY(ctx_aux(c));
+        MOSSQ_HIP(hipEventRecord(c->aux_ev[0], c->stream));
+        MOSSQ_HIP(hipStreamWaitEvent(c->aux, c->aux_ev[0], 0));
+        s2 = c->aux;
+    }
+    if (ntab[1])
+        for (int t = 0; t <= tmax; ++t)
+            MOSSQ_TRY(batch_iteration_on(s2, tab[1], ntab[1], nchunks[1], t, part2, d_vals, d_grads));
     if (ntab[0]) {
         MOSSQ_TRY(batch_sweeps_run(c, fwd, false));
         MOSSQ_TRY(batch_iteration(c, tab[0], ntab[0], nchunks[0], 0, part, d_vals, d_grads));  // observe
     }
-    if (ntab[1])
-        for (int t = 0; t <= tmax; ++t)
-            MOSSQ_TRY(batch_iteration(c, tab[1], ntab[1], nchunks[1], t, part, d_vals, d_grads));
     if (ntab[0] && !value_only) {
         MOSSQ_TRY(zero_fill(c, rpart, u64(rev_parts)));
         MOSSQ_TRY(batch_sweeps_run(c, rev, true));
@@ -430,6 +447,10 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
             hipLaunchKernelGGL(k_brows, dim3(np), dim3(kBT), 0, c->stream, d_rows, int(rows.size()), d_grads);
             MOSSQ_HIP(hipGetLastError());
         }
+    }
+    if (overlap) {
+        MOSSQ_HIP(hipEventRecord(c->aux_ev[1], c->aux));
+        MOSSQ_HIP(hipStreamWaitEvent(c->stream, c->aux_ev[1], 0));
     }
     std::vector<double> hv(B), hg(std::max<i64>(dtot, 1));
     MOSSQ_HIP(hipMemcpyAsync(hv.data(), d_vals, B * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -50,6 +50,23 @@ int dev_free(Ctx *c, void *p) {
     return MOSSQ_OK;
 }
 
+int ctx_aux(Ctx *c) {
+    if (c->aux) return MOSSQ_OK;
+    hipStream_t st = nullptr;
+    MOSSQ_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+    for (hipEvent_t &ev : c->aux_ev) {
+        const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            for (hipEvent_t &x : c->aux_ev)
+                if (x) (void)hipEventDestroy(x), x = nullptr;
+            (void)hipStreamDestroy(st);
+            return fail(MOSSQ_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
+    c->aux = st;
+    return MOSSQ_OK;
+}
+
 int get_scratch(Ctx *c, u64 bytes, void **p) {
     if (c->scratch_bytes < bytes) {
         MOSSQ_TRY(dev_free(c, c->scratch));
@@ -155,6 +172,12 @@ int mossq_ctx_destroy(mossq_ctx *ctx) {
     Ctx *c = C(ctx);
     MOSSQ_HIP(hipSetDevice(c->device));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    if (c->aux) {
+        MOSSQ_HIP(hipStreamSynchronize(c->aux));
+        for (hipEvent_t &ev : c->aux_ev)
+            if (ev) (void)hipEventDestroy(ev);
+        (void)hipStreamDestroy(c->aux);
+    }
     for (auto &a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);

@@ -60,5 +60,8 @@ int batch_chunks(int n);
 // chunk prefix `chunk0` in the table, total chunks `nchunks_total`).
 int batch_iteration(Ctx *c, const BItemDev *items, int nitems, int nchunks_total, int t, c128 *part,
                     double *values, double *grads);
+// The same on stream st.
+int batch_iteration_on(hipStream_t st, const BItemDev *items, int nitems, int nchunks_total, int t, c128 *part,
+                       double *values, double *grads);
 
 }  // namespace mossq

@@ -108,7 +108,14 @@ struct Ctx {
     u64 mat_stage_bytes = 0;
     double *host_pinned = nullptr;  // pinned host staging for results
     u64 host_pinned_bytes = 0;
+    // second in-order stream (default priority) for work that overlaps the
+    // main stream's, with two ordering events; created on first use
+    hipStream_t aux = nullptr;
+    hipEvent_t aux_ev[2] = {};
 };
+
+// The context's second stream and its events (created on first use).
+int ctx_aux(Ctx *c);
 
 int ctx_check(mossq_ctx *ctx);
 Ctx *C(mossq_ctx *ctx);
@@ -121,7 +128,6 @@ int get_scratch(Ctx *c, u64 bytes, void **p);
 int stage_matrix(Ctx *c, const c128 *host, u64 count, const c128 **dev);
 int get_pinned(Ctx *c, u64 bytes, double **p);
 
-// Scoped device allocations: everything obtained through get() is freed on exit.
 // A device table that a kernel only reads, as a constant-memory pointer: a
 // pointer read from memory is otherwise generic and loads through the flat
 // path into vector registers, where constant loads of uniform addresses are
@@ -132,6 +138,7 @@ __device__ __forceinline__ const T *const_table(const T *p) {
     return (const T *)(cT *)reinterpret_cast<uintptr_t>(p);
 }
 
+// Scoped device allocations: everything obtained through get() is freed on exit.
 struct Allocs {
     Ctx *c;
     std::vector<void *> ptrs;
