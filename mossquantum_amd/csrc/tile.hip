@@ -1717,6 +1717,9 @@ static double plan_cost(const FusedPlan *fp) {
 // planned and the cheapest plan under the cost model is kept.
 static FusedPlan *best_plan(const Program *p, const std::vector<POp> &ops, bool adjoint, std::string *err) {
     if (const char *mt = getenv("MOSSQ_PLAN_MIN_TAIL")) return plan_with_tail(p, ops, atoi(mt), adjoint, true, err);
+    // One tile holds the whole state: every op fits the first sweep, and a
+    // tail threshold could only push ops into an extra pass.
+    if (p->nbits <= TM) return plan_with_tail(p, ops, 0, adjoint, true, err);
     FusedPlan *best = nullptr;
     double best_cost = 0;
     for (int min_tail : {0, 3, 5, 8}) {
