@@ -1728,6 +1728,9 @@ static FusedPlan *best_plan(const Program *p, const std::vector<POp> &ops, bool 
             delete best;
             return nullptr;
         }
+        // a one-sweep plan (short chunks of the density adjoint, small
+        // circuits) cannot be beaten by moving ops into further sweeps
+        if (min_tail == 0 && fp->nsweeps <= 1) return fp;
         const double c = plan_cost(fp);
         if (!best || c < best_cost) {
             delete best;
