@@ -134,6 +134,47 @@ __device__ __forceinline__ c128 observe_rows(const BItemDev &it, u64 i0, u64 i1,
     return acc;
 }
 
+// Observe step on chunk c of item it, its partial energy into part.
+__device__ __forceinline__ void observe_chunk(const BItemDev &it, int c, c128 *red, c128 *__restrict__ part) {
+    __shared__ ObsGroup s_groups[kObsGroupsLds];
+    __shared__ c128 s_mats[kObsMatsLds], s_seed[kObsMatsLds];
+    const u64 len = u64(1) << it.n, per = len / u64(it.nchunk);
+    const ObsGroup last = it.groups[it.G - 1];
+    const int nm = last.moff + (1 << (2 * last.k));
+    c128 acc;
+    if (it.G <= kObsGroupsLds && nm <= kObsMatsLds) {
+        for (int j = threadIdx.x; j < it.G; j += kBT) s_groups[j] = it.groups[j];
+        for (int j = threadIdx.x; j < nm; j += kBT) {
+            s_mats[j] = it.mats[j];
+            s_seed[j] = it.seed[j];
+        }
+        __syncthreads();
+        acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, s_groups, s_mats, s_seed);
+    } else {
+        acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, const_table(it.groups), const_table(it.mats),
+                           const_table(it.seed));
+    }
+    c128 s = bblock_sum(acc, red);
+    if (threadIdx.x == 0) part[it.chunk0 + c] = s;
+}
+
+// The observe step alone, for a table whose items are all at it (the tiled
+// items of a batch): without the gate micro-ops' registers it runs at a
+// higher occupancy, which hides more of the partner loads.
+__global__ __launch_bounds__(kBT) void k_bobserve(const BItemDev *__restrict__ items, int nitems,
+                                                  c128 *__restrict__ part) {
+    __shared__ c128 red[kBT / 64];
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (items[mid].chunk0 <= int(blockIdx.x))
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    observe_chunk(items[lo], int(blockIdx.x) - items[lo].chunk0, red, part);
+}
+
 // One engine iteration for every active item: a workgroup owns one chunk of
 // one item and performs that item's current micro-op on it.
 __global__ __launch_bounds__(kBT, 2) void k_biter(const BItemDev *__restrict__ items, int nitems, int t,
@@ -155,26 +196,7 @@ __global__ __launch_bounds__(kBT, 2) void k_biter(const BItemDev *__restrict__ i
     if (pc < 0 || pc > (it.value_only ? nfwd : 2 * nfwd)) return;
     c128 *psi = it.psi, *lam = it.lam;
     if (pc == nfwd) {
-        __shared__ ObsGroup s_groups[kObsGroupsLds];
-        __shared__ c128 s_mats[kObsMatsLds], s_seed[kObsMatsLds];
-        const u64 len = u64(1) << n, per = len / u64(it.nchunk);
-        const ObsGroup last = it.groups[it.G - 1];
-        const int nm = last.moff + (1 << (2 * last.k));
-        c128 acc;
-        if (it.G <= kObsGroupsLds && nm <= kObsMatsLds) {
-            for (int j = threadIdx.x; j < it.G; j += kBT) s_groups[j] = it.groups[j];
-            for (int j = threadIdx.x; j < nm; j += kBT) {
-                s_mats[j] = it.mats[j];
-                s_seed[j] = it.seed[j];
-            }
-            __syncthreads();
-            acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, s_groups, s_mats, s_seed);
-        } else {
-            acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, const_table(it.groups), const_table(it.mats),
-                               const_table(it.seed));
-        }
-        c128 s = bblock_sum(acc, red);
-        if (threadIdx.x == 0) part[it.chunk0 + c] = s;
+        observe_chunk(it, c, red, part);
         return;
     }
     const bool fwd = pc < nfwd;
@@ -438,7 +460,11 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
             MOSSQ_TRY(batch_iteration_on(s2, tab[1], ntab[1], nchunks[1], t, part2, d_vals, d_grads));
     if (ntab[0]) {
         MOSSQ_TRY(batch_sweeps_run(c, fwd, false));
-        MOSSQ_TRY(batch_iteration(c, tab[0], ntab[0], nchunks[0], 0, part, d_vals, d_grads));  // observe
+        // observe (the tiled items have no engine gate steps: t = 0 is their observe step)
+        hipLaunchKernelGGL(k_bobserve, dim3(nchunks[0]), dim3(kBT), 0, c->stream, tab[0], ntab[0], part);
+        MOSSQ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_breduce, dim3(ntab[0]), dim3(kBT), 0, c->stream, tab[0], 0, part, d_vals, d_grads);
+        MOSSQ_HIP(hipGetLastError());
     }
     if (ntab[0] && !value_only) {
         MOSSQ_TRY(zero_fill(c, rpart, u64(rev_parts)));
