@@ -1,5 +1,5 @@
 """The repository's top-level scripts parse (bench.py and the build/smoke entry
-are run by the driver on the GPU machine without a prior import here)."""
+run on a GPU machine, where a syntax error would only show at run time)."""
 import ast
 import pathlib
 
