@@ -266,6 +266,50 @@ def test_fused_adjoint(ctx, n):
             assert np.max(np.abs(g - want)) < GRAD, (fuse, n)
 
 
+def _permutation_circuit(n, rng):
+    """Rotations interleaved with every two-bit permutation the tile kernels
+    move without arithmetic (CNOT either way, SWAP, X on one or both bits),
+    some followed by controlled-Z flips, plus phase rotations (one-bit
+    diagonals in adjoint plans)."""
+    X = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    I2 = np.eye(2, dtype=np.complex128)
+    ops = []
+    for layer in range(2):
+        for q in range(1, n + 1):
+            ops += [mq.RyGate(q, rng.uniform(0, 6.3), isparas=True), mq.RzGate(q, rng.uniform(0, 6.3), isparas=True)]
+        for q in range(1, n, 2):
+            ops += [mq.CNOTGate(q, q + 1), mq.CNOTGate(q + 1, q), mq.SWAPGate(q, q + 1), mq.CZGate(q, q + 1)]
+        for q in range(2, n, 3):
+            ops += [mq.QuantumGate((q, q + 1), np.kron(X, I2)), mq.QuantumGate((q, q + 1), np.kron(I2, X)),
+                    mq.QuantumGate((q, q + 1), np.kron(X, X))]
+        ops.append(mq.PHASEGate(n, rng.uniform(0, 6.3), isparas=True))
+    return mq.QCircuit(ops)
+
+
+@pytest.mark.parametrize("n", [12, 14])
+def test_fused_permutation_groups(ctx, n):
+    """Forward and adjoint tile plans with permutation groups against the
+    oracle (amplitudes, energy and gradient)."""
+    rng = np.random.default_rng(60 + n)
+    c = _permutation_circuit(n, rng)
+    _, st = plan_emulate(c, n, False, None)
+    assert st["permutations"] > 0
+    psi0 = rand_state(n, rng)
+    want = psi0.copy()
+    oracle.run_program(want, lower(c, n, False))
+    sv = mq.StateVector.from_host(psi0)
+    mq.apply_(c, sv, mq.HIPBackend(ctx, fuse=True))
+    assert np.max(np.abs(sv.to_host() - want)) < AMP
+    th = np.array(c.active_parameters(), dtype=float)
+    H = tfim(n)
+    lw = lower(c, n, False)
+    v_want, g_slots = oracle.adjoint(lw, H.groups())
+    want_g = lw.slot_gradients(g_slots)
+    obj = mq.ExpectationObjective(c, H, mq.StateVector.zeros(n), backend_=mq.HIPBackend(ctx, fuse=True))
+    v, g = mq.value_and_gradient(mq.AdjointGradient(), obj, th)
+    assert abs(v - v_want) < EXP and np.max(np.abs(g - want_g)) < GRAD
+
+
 def _noisy_param_circuit(n, layers, seed):
     """Noisy HEA plus a parameterized two-qubit gate, a two-qubit channel, a
     dense two-qubit gate and a three-qubit gate (run unfused)."""
