@@ -96,7 +96,7 @@ int density_adjoint_parts(const Program *p);
 bool fused_plan_batchable(const Program *p, bool adjoint);
 struct BatchSweeps {
     void *d_items = nullptr;  // device launch items (layout private to tile.hip)
-    int *d_wg = nullptr;      // workgroup -> item of its launch
+    int *d_wg = nullptr;      // workgroup -> item of its launch << 16 | tile
     c128 *vec = nullptr, *part = nullptr;
     std::vector<int> item0, wg0, tiles;  // per launch
 };

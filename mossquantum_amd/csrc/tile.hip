@@ -713,8 +713,9 @@ __global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 
 
 // ------------------------------------------------------ batched sweeps ----
 // Sweeps of many programs in one launch: item i is one sweep of one program's
-// plan on that program's own vectors, over workgroups [tile0, tile0 + tiles)
-// of the launch; wg_item maps each workgroup to its item.  The vectors are
+// plan on that program's own vectors, one workgroup per tile; wg_item gives
+// each workgroup its item and tile (item << 16 | tile; tile0 is the item's
+// first tile in a count of the launch's tiles, used on the host).  The vectors are
 // offsets into one allocation `vec` (partials into `part`), which the host
 // checks against that allocation's size.
 // The plan tables' addresses become constant-memory pointers in the kernel
@@ -741,18 +742,19 @@ struct TBItem {
 
 __global__ __launch_bounds__(TT, 2) void k_btile(const TBItem *__restrict__ items, const int *__restrict__ wg_item,
                                                    c128 *__restrict__ vec) {
-    const TBItem &it = items[__builtin_amdgcn_readfirstlane(wg_item[blockIdx.x])];
+    const int e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);
+    const TBItem &it = items[e >> 16];
     tile_sweep<false>(vec + it.x, vec + it.x, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),
-                      table<TDiag>(it.diag), table<c128>(it.pool), nullptr, 0, nullptr, 0,
-                      u64(blockIdx.x - it.tile0));
+                      table<TDiag>(it.diag), table<c128>(it.pool), nullptr, 0, nullptr, 0, u64(e & 0xFFFF));
 }
 
 __global__ __launch_bounds__(TT, 2) void k_btile_adj(const TBItem *__restrict__ items,
                                                        const int *__restrict__ wg_item, c128 *__restrict__ vec,
                                                        c128 *__restrict__ part) {
-    const TBItem &it = items[__builtin_amdgcn_readfirstlane(wg_item[blockIdx.x])];
+    const int e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);
+    const TBItem &it = items[e >> 16];
     tile_adj_sweep(vec + it.x, vec + it.l, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),
-                   table<TDiag>(it.diag), table<c128>(it.pool), part + it.part, it.row, u64(blockIdx.x - it.tile0));
+                   table<TDiag>(it.diag), table<c128>(it.pool), part + it.part, it.row, u64(e & 0xFFFF));
 }
 
 // ------------------------------------------------------------------ planner ----
@@ -2148,10 +2150,23 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
             MOSSQ_CHECK(!adjoint || (row[b] >= int(4 * nt) &&
                                      part_off[b] + u64(std::max(p->nderiv, 1)) * u64(row[b]) <= part_len),
                         MOSSQ_EINVAL, "gradient partial rows outside their allocation");
-            wg.insert(wg.end(), size_t(nt), int(items.size() - i0));
+            MOSSQ_CHECK(nt <= 65536, MOSSQ_EINVAL, "batched sweeps take at most 28 qubits");
             tiles += nt;
             items.push_back(it);
         }
+        // Workgroups take the launch's items round robin, one tile each
+        // (entry = item << 16 | tile): every stretch of the grid mixes the
+        // heavy tiles of small items with the light ones of large items.
+        u64 maxnt = 0;
+        for (size_t i = i0; i < items.size(); ++i) {
+            const u64 nt = i + 1 < items.size() ? u64(items[i + 1].tile0 - items[i].tile0) : tiles - items[i].tile0;
+            maxnt = std::max(maxnt, nt);
+        }
+        for (u64 r = 0; r < maxnt; ++r)
+            for (size_t i = i0; i < items.size(); ++i) {
+                const u64 nt = i + 1 < items.size() ? u64(items[i + 1].tile0 - items[i].tile0) : tiles - items[i].tile0;
+                if (r < nt) wg.push_back(int(((i - i0) << 16) | r));
+            }
         out->item0.push_back(int(i0));
         out->wg0.push_back(int(w0));
         out->tiles.push_back(int(tiles));
