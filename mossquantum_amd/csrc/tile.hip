@@ -2100,7 +2100,8 @@ int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, in
 }
 
 bool fused_plan_batchable(const Program *p, bool adjoint) {
-    if (!p->fused || (adjoint && !p->fused_rev)) return false;
+    // (the launch tables give a tile 16 bits: up to 28 qubits)
+    if (!p->fused || (adjoint && !p->fused_rev) || p->nbits - TM > 16) return false;
     for (const FusedPlan *fp : {p->fused, adjoint ? p->fused_rev : nullptr}) {
         if (!fp) continue;
         for (const PlanItem &it : fp->items)
