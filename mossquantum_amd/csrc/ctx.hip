@@ -23,12 +23,35 @@ int ctx_check(mossq_ctx *ctx) {
     return MOSSQ_OK;
 }
 
-int dev_alloc(Ctx *c, u64 bytes, void **p) {
-    if (bytes == 0) bytes = 16;
+// Releases every cached block (after the context's queued work).
+static int cache_flush(Ctx *c) {
+    if (!c->cached) return MOSSQ_OK;
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    for (auto &kv : c->cache)
+        for (void *p : kv.second) {
+            (void)hipFree(p);
+            auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
+            if (it != c->allocs.end()) {
+                c->in_use -= it->second;
+                c->allocs.erase(it);
+            }
+        }
+    c->cache.clear();
+    c->cached = 0;
+    return MOSSQ_OK;
+}
+
+static int raw_alloc(Ctx *c, u64 bytes, void **p) {
+    if (c->budget && c->in_use + bytes > c->budget) MOSSQ_TRY(cache_flush(c));
     if (c->budget && c->in_use + bytes > c->budget)
         return fail(MOSSQ_EOOM, "device memory budget exceeded: " + std::to_string(c->in_use) + " + " +
                                     std::to_string(bytes) + " > " + std::to_string(c->budget));
     hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && c->cached) {
+        (void)hipGetLastError();
+        MOSSQ_TRY(cache_flush(c));
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(MOSSQ_EOOM, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -38,8 +61,45 @@ int dev_alloc(Ctx *c, u64 bytes, void **p) {
     return MOSSQ_OK;
 }
 
+int cached_alloc(Ctx *c, u64 bytes, void **p) {
+    u64 r = 4096;
+    while (r < bytes) r <<= 1;
+    auto it = c->cache.find(r);
+    if (it != c->cache.end() && !it->second.empty()) {
+        *p = it->second.back();
+        it->second.pop_back();
+        c->cached -= r;
+        return MOSSQ_OK;
+    }
+    return raw_alloc(c, r, p);
+}
+
+int cached_free(Ctx *c, void *p) {
+    if (!p) return MOSSQ_OK;
+    auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
+    MOSSQ_CHECK(it != c->allocs.end(), MOSSQ_EINVAL, "pointer was not allocated by this context");
+    if (c->cached + it->second <= kAllocCacheCap) {
+        // kept for reuse on this stream, which orders it after the queued work
+        c->cache[it->second].push_back(p);
+        c->cached += it->second;
+        return MOSSQ_OK;
+    }
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    MOSSQ_HIP(hipFree(p));
+    c->in_use -= it->second;
+    c->allocs.erase(it);
+    return MOSSQ_OK;
+}
+
+int dev_alloc(Ctx *c, u64 bytes, void **p) {
+    if (bytes == 0) bytes = 16;
+    if (c->cache_allocs) return cached_alloc(c, bytes, p);
+    return raw_alloc(c, bytes, p);
+}
+
 int dev_free(Ctx *c, void *p) {
     if (!p) return MOSSQ_OK;
+    if (c->cache_allocs) return cached_free(c, p);
     auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
     MOSSQ_CHECK(it != c->allocs.end(), MOSSQ_EINVAL, "pointer was not allocated by this context");
     // Memory may still be read by queued kernels: order the free after them.

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/mossq_hip.h"
@@ -112,7 +113,17 @@ struct Ctx {
     // main stream's, with two ordering events; created on first use
     hipStream_t aux = nullptr;
     hipEvent_t aux_ev[2] = {};
+    // Block cache: cached_alloc rounds sizes up to powers of two and
+    // cached_free keeps blocks (up to kAllocCacheCap bytes) for reuse instead
+    // of hipFree, which waits for the whole device.  Blocks are reused on this
+    // context's stream only, after the work queued on it; a budget or hipMalloc
+    // failure releases them first.  The library's per-call scratch (Allocs)
+    // always goes through it; with cache_allocs every dev_alloc does.
+    bool cache_allocs = false;
+    std::map<u64, std::vector<void *>> cache;
+    u64 cached = 0;
 };
+constexpr u64 kAllocCacheCap = u64(16) << 30;
 
 // The context's second stream and its events (created on first use).
 int ctx_aux(Ctx *c);
@@ -138,18 +149,22 @@ __device__ __forceinline__ const T *const_table(const T *p) {
     return (const T *)(cT *)reinterpret_cast<uintptr_t>(p);
 }
 
-// Scoped device allocations: everything obtained through get() is freed on exit.
+int cached_alloc(Ctx *c, u64 bytes, void **p);
+int cached_free(Ctx *c, void *p);
+
+// Scoped per-call scratch: everything obtained through get() goes back to the
+// context's block cache on exit.
 struct Allocs {
     Ctx *c;
     std::vector<void *> ptrs;
     explicit Allocs(Ctx *cc) : c(cc) {}
     int get(u64 bytes, void **p) {
-        int st = dev_alloc(c, bytes, p);
+        int st = cached_alloc(c, bytes == 0 ? 16 : bytes, p);
         if (st == MOSSQ_OK) ptrs.push_back(*p);
         return st;
     }
     ~Allocs() {
-        for (void *p : ptrs) dev_free(c, p);
+        for (void *p : ptrs) cached_free(c, p);
     }
 };
 

@@ -440,7 +440,10 @@ int mossq_server_create(mossq_ctx *ctx, uint64_t budget_bytes, mossq_server **ou
     for (int i = 0; i < kWorkers && st == MOSSQ_OK; ++i) {
         mossq_ctx *w = nullptr;
         st = mossq_ctx_create(s->ctx->device, 0, &w);
-        if (st == MOSSQ_OK) s->wctxs.push_back(C(w));
+        if (st == MOSSQ_OK) {
+            C(w)->cache_allocs = true;  // no device-wide waits when a request's buffers are released
+            s->wctxs.push_back(C(w));
+        }
     }
     if (st == MOSSQ_OK) st = s->ring.init(kRingBytes);
     for (int k = 0; st == MOSSQ_OK && k < 2; ++k) {
