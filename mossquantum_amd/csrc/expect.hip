@@ -381,10 +381,10 @@ int mossq_obs_create(mossq_ctx *ctx, int n, int ngroups, const int32_t *group_k,
         o->h_groups.push_back(gr);
     }
     Ctx *c = o->ctx;
-    int st = dev_alloc(c, sizeof(ObsGroup) * ngroups, (void **)&o->d_groups);
-    if (st == MOSSQ_OK) st = dev_alloc(c, sizeof(c128) * o->h_mats.size(), (void **)&o->d_mats);
-    if (st == MOSSQ_OK) st = dev_alloc(c, sizeof(c128) * o->h_seed.size(), (void **)&o->d_mats_seed);
-    if (st == MOSSQ_OK) st = dev_alloc(c, sizeof(c128) * o->h_adj.size(), (void **)&o->d_mats_adj);
+    int st = cached_alloc(c, sizeof(ObsGroup) * ngroups, (void **)&o->d_groups);
+    if (st == MOSSQ_OK) st = cached_alloc(c, sizeof(c128) * o->h_mats.size(), (void **)&o->d_mats);
+    if (st == MOSSQ_OK) st = cached_alloc(c, sizeof(c128) * o->h_seed.size(), (void **)&o->d_mats_seed);
+    if (st == MOSSQ_OK) st = cached_alloc(c, sizeof(c128) * o->h_adj.size(), (void **)&o->d_mats_adj);
     if (st != MOSSQ_OK) {
         mossq_obs_destroy(reinterpret_cast<mossq_obs *>(o));
         return st;
@@ -408,10 +408,10 @@ int mossq_obs_destroy(mossq_obs *obs) {
     Ctx *c = o->ctx;
     (void)hipSetDevice(c->device);
     int st = MOSSQ_OK;
-    if (o->d_groups) st |= dev_free(c, o->d_groups);
-    if (o->d_mats) st |= dev_free(c, o->d_mats);
-    if (o->d_mats_seed) st |= dev_free(c, o->d_mats_seed);
-    if (o->d_mats_adj) st |= dev_free(c, o->d_mats_adj);
+    if (o->d_groups) st |= cached_free(c, o->d_groups);
+    if (o->d_mats) st |= cached_free(c, o->d_mats);
+    if (o->d_mats_seed) st |= cached_free(c, o->d_mats_seed);
+    if (o->d_mats_adj) st |= cached_free(c, o->d_mats_adj);
     delete o;
     return st ? MOSSQ_EDEVICE : MOSSQ_OK;
 }

@@ -201,7 +201,7 @@ u64 adjoint_density_bytes(const Program *p, const Obs *o) {
 
 int program_upload(Ctx *c, Program *p) {
     p->ctx = c;
-    MOSSQ_TRY(dev_alloc(c, std::max<size_t>(p->h_pool.size(), 1) * sizeof(c128), (void **)&p->d_pool));
+    MOSSQ_TRY(cached_alloc(c, std::max<size_t>(p->h_pool.size(), 1) * sizeof(c128), (void **)&p->d_pool));
     if (!p->h_pool.empty())
         MOSSQ_HIP(hipMemcpyAsync(p->d_pool, p->h_pool.data(), p->h_pool.size() * sizeof(c128),
                                  hipMemcpyHostToDevice, c->stream));
@@ -457,7 +457,7 @@ int mossq_program_create(mossq_ctx *ctx, int n, int is_density, int nops, const 
         return s;
     }
     Ctx *c = p->ctx;
-    s = dev_alloc(c, std::max<size_t>(p->h_pool.size(), 1) * sizeof(c128), (void **)&p->d_pool);
+    s = cached_alloc(c, std::max<size_t>(p->h_pool.size(), 1) * sizeof(c128), (void **)&p->d_pool);
     if (s != MOSSQ_OK) {
         delete p;
         return s;
@@ -482,7 +482,7 @@ int mossq_program_destroy(mossq_program *prog) {
     Program *p = reinterpret_cast<Program *>(prog);
     (void)hipSetDevice(p->ctx->device);
     fused_plan_free(p);
-    int s = dev_free(p->ctx, p->d_pool);
+    int s = cached_free(p->ctx, p->d_pool);
     delete p;
     return s;
 }

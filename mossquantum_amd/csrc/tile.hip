@@ -1964,10 +1964,10 @@ void emulate_adjoint_plan(const Program *p, const FusedPlan *fp, c128 *y, c128 *
 }
 
 int fused_plan_upload(Ctx *c, FusedPlan *fp) {
-    int s = dev_alloc(c, std::max<size_t>(fp->h_segs.size(), 1) * sizeof(TSeg), (void **)&fp->d_segs);
-    if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_groups.size() * sizeof(TGroup), (void **)&fp->d_groups);
-    if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_diag.size() * sizeof(TDiag), (void **)&fp->d_diag);
-    if (s == MOSSQ_OK) s = dev_alloc(c, fp->h_pool.size() * sizeof(c128), (void **)&fp->d_pool);
+    int s = cached_alloc(c, std::max<size_t>(fp->h_segs.size(), 1) * sizeof(TSeg), (void **)&fp->d_segs);
+    if (s == MOSSQ_OK) s = cached_alloc(c, fp->h_groups.size() * sizeof(TGroup), (void **)&fp->d_groups);
+    if (s == MOSSQ_OK) s = cached_alloc(c, std::max<size_t>(fp->h_diag.size(), 1) * sizeof(TDiag), (void **)&fp->d_diag);
+    if (s == MOSSQ_OK) s = cached_alloc(c, fp->h_pool.size() * sizeof(c128), (void **)&fp->d_pool);
     if (s != MOSSQ_OK) return s;
     if (!fp->h_segs.empty())
         MOSSQ_HIP(hipMemcpyAsync(fp->d_segs, fp->h_segs.data(), fp->h_segs.size() * sizeof(TSeg),
@@ -2010,10 +2010,10 @@ int fused_plan_build(Program *p) {
 static void free_plan(Ctx *c, FusedPlan *&fp) {
     if (!fp) return;
     if (c && !fp->shared_device) {
-        if (fp->d_segs) dev_free(c, fp->d_segs);
-        if (fp->d_groups) dev_free(c, fp->d_groups);
-        if (fp->d_diag) dev_free(c, fp->d_diag);
-        if (fp->d_pool) dev_free(c, fp->d_pool);
+        if (fp->d_segs) cached_free(c, fp->d_segs);
+        if (fp->d_groups) cached_free(c, fp->d_groups);
+        if (fp->d_diag) cached_free(c, fp->d_diag);
+        if (fp->d_pool) cached_free(c, fp->d_pool);
     }
     delete fp;
     fp = nullptr;
@@ -2234,7 +2234,7 @@ int fused_plan_autotune(Program *p, int reps, int *tried, double *best_ms) {
     const u64 len = u64(1) << p->nbits;
     c128 *scratch = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    int s = dev_alloc(c, len * sizeof(c128), (void **)&scratch);
+    int s = cached_alloc(c, len * sizeof(c128), (void **)&scratch);
     std::vector<double> ms(cand.size(), 1e300);
     if (s == MOSSQ_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         s = fail(MOSSQ_EDEVICE, "hipEventCreate failed");
@@ -2263,7 +2263,7 @@ int fused_plan_autotune(Program *p, int reps, int *tried, double *best_ms) {
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    if (scratch) dev_free(c, scratch);
+    if (scratch) cached_free(c, scratch);
     if (s != MOSSQ_OK) return s;
     *tried = int(cand.size());
     *best_ms = ms[best];
@@ -2487,7 +2487,7 @@ static int upload_plan_block(Ctx *c, const std::vector<FusedPlan *> &plans, void
                  al(fp->h_diag.size() * sizeof(TDiag)) + al(fp->h_pool.size() * sizeof(c128));
     if (total == 0) return MOSSQ_OK;
     std::vector<char> h(total);
-    MOSSQ_TRY(dev_alloc(c, total, block));
+    MOSSQ_TRY(cached_alloc(c, total, block));
     char *d = static_cast<char *>(*block);
     size_t off = 0;
     auto put = [&](const void *src, size_t bytes) {
@@ -2515,7 +2515,7 @@ int density_adjoint_plan_build(Program *p) {
     if (!dp) return MOSSQ_OK;  // no chunk plans: the reverse pass runs step by step
     int s = upload_plan_block(p->ctx, dm_plans(dp), &dp->d_block);
     if (s != MOSSQ_OK) {
-        if (dp->d_block) dev_free(p->ctx, dp->d_block);
+        if (dp->d_block) cached_free(p->ctx, dp->d_block);
         dm_plan_delete(dp);
         return s;
     }
@@ -2525,7 +2525,7 @@ int density_adjoint_plan_build(Program *p) {
 
 void density_adjoint_plan_free(Program *p) {
     if (!p->dm_adj) return;
-    if (p->ctx && p->dm_adj->d_block) dev_free(p->ctx, p->dm_adj->d_block);
+    if (p->ctx && p->dm_adj->d_block) cached_free(p->ctx, p->dm_adj->d_block);
     dm_plan_delete(p->dm_adj);
     p->dm_adj = nullptr;
 }
