@@ -75,7 +75,10 @@ int mossq_device_count(int *count);
 int mossq_ctx_create(int device, uint64_t mem_budget_bytes, mossq_ctx **out);
 int mossq_ctx_destroy(mossq_ctx *ctx);
 int mossq_sync(mossq_ctx *ctx);
-/* Bytes currently allocated through this context. */
+/* Bytes currently allocated through this context, including the blocks the
+ * library keeps for reuse (its per-call scratch and table memory is released
+ * to a per-context cache rather than to the device; a budget overrun or a
+ * failed allocation releases the cache first, and mossq_ctx_destroy frees it). */
 int mossq_mem_in_use(mossq_ctx *ctx, uint64_t *bytes);
 
 /* ---- device buffers (back the HIPArray storage family: similar/copyto!/copy,
