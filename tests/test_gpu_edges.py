@@ -96,3 +96,35 @@ def test_batches_of_one(ctx):
     vals, grads = mq.batched_adjoint([nop], [xxz(n)], [n], ctx)
     assert grads[0].shape == (0,)
     assert abs(vals[0] - oracle.adjoint(lower(nop, n, False), xxz(n).groups())[0]) < EXP
+
+
+def test_block_cache_under_a_budget():
+    """The library keeps released scratch blocks for reuse; on a context with
+    a memory budget they count as in use, and an allocation that only fits
+    without them releases them first.  Repeated calls reuse the blocks and
+    give identical results."""
+    import ctypes as C
+    n = 18
+    budget = 48 << 20
+    small = mq._lib.Context(0, budget)
+    try:
+        c = hea_circuit(n, 1, np.random.default_rng(3).uniform(0, 6.3, 2 * n))
+        want = oracle.adjoint(lower(c, n, False), xxz(n).groups())
+        vals = []
+        for _ in range(2):
+            v, g = mq.batched_adjoint([c], [xxz(n)], [n], small, fuse=False)
+            vals.append((v[0], g[0]))
+            assert abs(v[0] - want[0]) < EXP
+        assert vals[0][0] == vals[1][0] and np.array_equal(vals[0][1], vals[1][1])
+        held = small.mem_in_use()
+        assert held > 0  # the scratch blocks stay with the context
+        ptr = C.c_void_p()
+        # a buffer that only fits once the cached blocks are released
+        size = budget - held + (1 << 20)
+        mq._lib.check(small.lib.mossq_alloc(small.h, size, C.byref(ptr)))
+        assert small.mem_in_use() <= budget
+        mq._lib.check(small.lib.mossq_free(small.h, ptr))
+        with pytest.raises(mq._lib.MossqError):
+            mq._lib.check(small.lib.mossq_alloc(small.h, budget + 1, C.byref(ptr)))
+    finally:
+        small.close()
