@@ -39,7 +39,7 @@ constexpr int kInflight = 8;         // engine iterations queued ahead of the de
 constexpr u64 kRingBytes = 64ull << 20;
 constexpr int kWorkers = 4;          // worker threads (contexts) beside the engine
 constexpr int kFusedWorkers = 2;     // of which serve the fused state-vector queue
-constexpr int kFusedMinQubits = 18;  // state-vector requests this wide take the fused adjoint
+constexpr int kFusedMinQubits = 14;  // state-vector requests this wide take the fused adjoint
 
 struct SRequest {
     i64 ticket = 0;

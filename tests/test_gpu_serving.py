@@ -62,10 +62,10 @@ def test_server_alone_vs_mix(ctx):
 
 
 def test_server_budget_admission(ctx):
-    """A budget that fits one 16-qubit state-vector request at a time still
-    completes a queue of them (first-in first-out admission)."""
-    reqs = make_requests(ctx, [("sv", 16, 1)] * 4, 3)
-    one = 2 * 16 * 2 ** 16 * 1.5
+    """A budget that fits one 13-qubit state-vector request (engine path) at a
+    time still completes a queue of them (first-in first-out admission)."""
+    reqs = make_requests(ctx, [("sv", 13, 1)] * 4, 3)
+    one = 2 * 16 * 2 ** 13 * 1.5
     srv = Server(ctx, int(one))
     tickets = [srv.submit(t, p) for t, p, _ in reqs]
     active, queued = srv.step()
