@@ -110,11 +110,9 @@ def bench_batched(ctx, B=512, layers=6, seed=20260300, cpu=True):
     import ctypes as C
     import mossquantum_amd as mq
     from mossquantum_amd import _lib
-    from mossquantum_amd.workloads import batch_sizes, hea_circuit, xxz
-    sizes = [int(x) for x in batch_sizes(B, 10, 20, seed)]
-    rng = np.random.default_rng(seed + 1)
-    circs = [hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers)) for n in sizes]
-    obs = [xxz(n) for n in sizes]
+    from mossquantum_amd.workloads import c3_batch, hea_circuit, xxz
+    sizes, circs, obs = c3_batch(B, layers, seed)
+    rng = np.random.default_rng(seed + 2)
     dobs = [mq.DeviceObservable(o, n, ctx) for o, n in zip(obs, sizes)]
     oh = (C.c_void_p * B)(*[o.h.value for o in dobs])
     out = {"B": B, "layers": layers, "n_range": [10, 20]}

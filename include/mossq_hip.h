@@ -110,7 +110,7 @@ int mossq_sv_apply_gate(mossq_ctx *ctx, void *psi, int n, int k, const int32_t *
                         const double *mat);
 /* U rho U^dagger in ONE sweep of U (x) conj(U) on bits (pos, pos+n)
  * (reference _dm_apply_threaded!, src/backends/cpu/gates/generic.jl:24-34, which
- * takes two sweeps).  k <= 2. */
+ * takes two sweeps).  k <= 5. */
 int mossq_dm_apply_gate(mossq_ctx *ctx, void *rho, int n, int k, const int32_t *pos,
                         const double *mat);
 /* Kraus channel via its 4^k x 4^k supermatrix on bits (pos, pos+n); dagger=1
@@ -178,7 +178,9 @@ int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state);
  * distinct one `reps` times on a scratch state of the program's size, and keep
  * the fastest (the planner's cost-model choice is one candidate).  variants =
  * distinct plans timed (0 for an unfused program), best_ms = the kept plan's
- * time per run.  Graphs captured earlier keep the plan they recorded. */
+ * time per run.  Graphs captured earlier keep the plan they recorded: a
+ * replaced plan's device tables stay allocated until the last such graph is
+ * destroyed (or the program is). */
 int mossq_program_autotune(mossq_ctx *ctx, mossq_program *prog, int reps, int *variants, double *best_ms);
 /* Record one run of the program on `state` as a HIP graph (stream capture of
  * the same launches mossq_program_run makes) and instantiate it; each
@@ -233,6 +235,18 @@ int mossq_plan_emulate_density_adjoint(int n, int nops, const mossq_op *ops, con
                                        uint64_t pool_len, int whole_segments, double *rho, double *delta,
                                        double *grads, int *stats);
 
+/* Host only (needs no device): the workgroup tables of the batched tile
+ * sweeps (mossq_batch_adjoint, mossq_batch_expectation) for one sweep index
+ * over m items of ntiles[i] tiles each.  Items are split into launches of at
+ * most 65536; launch_items[L] = items of launch L (max_launches entries), and
+ * entries (cap entries) receives every launch's workgroup entries in order,
+ * each (item within its launch) << 16 | tile, dealt round robin over the
+ * launch's items.  *nlaunches and *nentries are set even when the arrays are
+ * too small (then MOSSQ_EINVAL).  No counterpart in the reference; it tests
+ * the packing of the launch tables. */
+int mossq_batch_launch_tables(int m, const int32_t *ntiles, int max_launches, int32_t *launch_items,
+                              uint64_t cap, uint32_t *entries, int *nlaunches, uint64_t *nentries);
+
 /* ---- adjoint gradients ---------------------------------------------------- */
 /* value = Re <H> at C|in>, grads[j] for the j-th derivative gate in forward
  * order (reference _circuit_adjoint + back_propagate,
@@ -279,8 +293,11 @@ int mossq_batch_expectation(mossq_ctx *ctx, int B, mossq_program *const *progs, 
  * first-in first-out while their device memory fits the budget; state-vector
  * requests join the batched engine at the next iteration and stay resident on
  * the device until they finish; density-matrix requests run on a worker stream
- * concurrently.  step/poll/completed are called from one thread; the
- * observable must outlive its requests. */
+ * concurrently.  The budget bounds all device memory the requests use,
+ * including the blocks the server keeps for reuse between requests (they are
+ * released before an allocation would pass the budget); the server's fixed
+ * tables (about 40 MiB) come on top.  step/poll/completed are called from one
+ * thread; the observable must outlive its requests. */
 typedef struct mossq_server mossq_server;
 int mossq_server_create(mossq_ctx *ctx, uint64_t budget_bytes, mossq_server **out);
 int mossq_server_destroy(mossq_server *srv);
@@ -293,6 +310,9 @@ int mossq_server_submit(mossq_server *srv, int n, int is_density, int nops, cons
 int mossq_server_step(mossq_server *srv, int *active, int *queued);
 /* Tickets finished since the last call (up to max), count in *count. */
 int mossq_server_completed(mossq_server *srv, int64_t *tickets, int max, int *count);
+/* Device bytes the server's requests hold now (in_use, cached blocks
+ * included) and at most since creation (peak); both stay within the budget. */
+int mossq_server_memory(mossq_server *srv, uint64_t *in_use, uint64_t *peak);
 /* done=1 when finished: value and gradients (in the program's derivative
  * order) are copied out and the ticket is released. */
 int mossq_server_poll(mossq_server *srv, int64_t ticket, int *done, double *value, double *grads,

@@ -7,7 +7,7 @@ reference's backend, state, observable and gradient interfaces.
 from ._lib import Context, MossqError, MossqOOM, MossqUnsupported, default_context, device_count, load
 from .backend import (DeviceObservable, HIPBackend, Program, SimulationResult, UnsupportedOperation, apply, apply_,
                       backend, cross_expectation, dot, expectation, probabilities, purity, simulate, simulate_circuit,
-                      supports, trace)
+                      supports, trace, validate_positions)
 from .capability import (CapabilityIssue, CapabilityStep, ExecutionCapability, execution_capability,
                          execution_capability_backend, execution_capability_step)
 from .channels import (AmplitudeDamping, BitFlip, BitPhaseFlip, Depolarizing, GeneralizedAmplitudeDamping,

@@ -111,6 +111,9 @@ SIGNATURES = [
     ("mossq_server_step", _I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("mossq_server_completed", _I, [_P, C.POINTER(C.c_int64), _I, C.POINTER(C.c_int)]),
     ("mossq_server_poll", _I, [_P, C.c_int64, C.POINTER(C.c_int), _DP, _DP, _I]),
+    ("mossq_server_memory", _I, [_P, C.POINTER(_U64), C.POINTER(_U64)]),
+    ("mossq_batch_launch_tables", _I,
+     [_I, _IP, _I, _IP, _U64, C.POINTER(C.c_uint32), C.POINTER(C.c_int), C.POINTER(_U64)]),
     ("mossq_event_record", _I, [_P, _I]),
     ("mossq_event_elapsed_ms", _I, [_P, _I, _I, C.POINTER(C.c_float)]),
 ]

@@ -74,32 +74,67 @@ def _obs_groups(obs):
 
 
 def supports(b: HIPBackend, op, state, action: str = "apply") -> bool:
+    """Whether the backend implements `action` for this operation and state
+    (reference supports, src/api/backend_protocols.jl:79-95).  Like the
+    reference, it describes capability only: register bounds are checked
+    separately by `validate_positions`."""
     if not isinstance(b, HIPBackend):
         return False
     if action == "apply":
         if isinstance(op, Gate):
-            return op.nqubits() <= MAX_GATE_QUBITS and max(op.positions()) <= state.n
+            return op.nqubits() <= MAX_GATE_QUBITS
         if isinstance(op, QuantumMap):
-            return op.nqubits() <= MAX_CHANNEL_QUBITS and max(op.positions()) <= state.n
+            return op.nqubits() <= MAX_CHANNEL_QUBITS
         if isinstance(op, Delay):
-            return op.position <= state.n
+            return True
         if isinstance(op, QCircuit):
             return all(supports(b, o, state, "apply") for o in op.flat())
         if isinstance(op, (QubitsOperator, QubitsTerm)):
             g = _obs_groups(op)
-            return all(len(k) <= MAX_OBS_SITES and (not k or max(k) <= state.n) for k, _ in g)
+            return all(len(k) <= MAX_OBS_SITES for k, _ in g)
         return False
     if action == "expectation":
         g = _obs_groups(op)
-        return g is not None and all(len(k) <= MAX_OBS_SITES and (not k or max(k) <= state.n) for k, _ in g)
+        return g is not None and all(len(k) <= MAX_OBS_SITES for k, _ in g)
     if action == "probabilities":
         return isinstance(state, (StateVector, DensityMatrix, BatchStateVector, BatchDensityMatrix))
     return False
 
 
+def _positions(op):
+    if isinstance(op, QCircuit):
+        return [p for o in op.flat() for p in _positions(o)]
+    if isinstance(op, (Gate, QuantumMap)):
+        return list(op.positions())
+    if isinstance(op, Delay):
+        return [op.position]
+    g = _obs_groups(op)
+    if g is not None:
+        return [p for k, _ in g for p in k]
+    return []
+
+
+def validate_positions(op, n: int) -> None:
+    """Every qubit position of `op` is an integer in 1..n; raises ValueError
+    (the reference's ArgumentError, src/quantum_circuits/instructions.jl:96-103,
+    and for operators src/api/backend_protocols.jl:181-187)."""
+    if n < 0:
+        raise ValueError("the number of qubits must be nonnegative")
+    for p in _positions(op):
+        if not isinstance(p, (int, np.integer)):
+            raise ValueError("qubit positions must be integers")
+        if not 1 <= p <= n:
+            raise ValueError(f"qubit position {p} must be between 1 and {n}")
+
+
 def _require(b, op, state, action):
+    """Capability first (UnsupportedOperation), then register bounds
+    (ValueError), both before the payload is touched, in the reference's order
+    (backend_protocols.jl:218-222)."""
     if not supports(b, op, state, action):
         raise UnsupportedOperation(b, op, state, action)
+    if op is not None:
+        validate_positions(op, state.n)
 
 
 # ------------------------------------------------------------- observables ----

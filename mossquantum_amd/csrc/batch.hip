@@ -449,11 +449,13 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
     // but the result arrays, where their slots are disjoint.
     const bool overlap = ntab[0] && ntab[1];
     hipStream_t s2 = c->stream;
+    AuxJoin join(c);  // on every return, before al's blocks go back to the cache
     if (overlap) {
         MOSSQ_TRY(ctx_aux(c));
         MOSSQ_HIP(hipEventRecord(c->aux_ev[0], c->stream));
         MOSSQ_HIP(hipStreamWaitEvent(c->aux, c->aux_ev[0], 0));
         s2 = c->aux;
+        join.active = true;
     }
     if (ntab[1])
         for (int t = 0; t <= tmax; ++t)
@@ -475,6 +477,7 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         }
     }
     if (overlap) {
+        join.active = false;
         MOSSQ_HIP(hipEventRecord(c->aux_ev[1], c->aux));
         MOSSQ_HIP(hipStreamWaitEvent(c->stream, c->aux_ev[1], 0));
     }

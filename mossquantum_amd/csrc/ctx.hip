@@ -23,21 +23,61 @@ int ctx_check(mossq_ctx *ctx) {
     return MOSSQ_OK;
 }
 
-// Releases every cached block (after the context's queued work).
-static int cache_flush(Ctx *c) {
-    if (!c->cached) return MOSSQ_OK;
-    MOSSQ_HIP(hipStreamSynchronize(c->stream));
-    for (auto &kv : c->cache)
-        for (void *p : kv.second) {
-            (void)hipFree(p);
-            auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
-            if (it != c->allocs.end()) {
-                c->in_use -= it->second;
-                c->allocs.erase(it);
+// Returns `bytes` to the context's memory group (if any).
+static void group_release(Ctx *c, u64 bytes) {
+    if (!c->group || !bytes) return;
+    std::lock_guard<std::mutex> lk(c->group->mu);
+    c->group->used -= bytes;
+}
+
+// Takes `bytes` from the group; past its limit, every member's cache is
+// released first.
+static int group_reserve(Ctx *c, u64 bytes) {
+    MemGroup *g = c->group;
+    for (int pass = 0; pass < 2; ++pass) {
+        {
+            std::lock_guard<std::mutex> lk(g->mu);
+            if (g->used + bytes <= g->limit) {
+                g->used += bytes;
+                g->peak = std::max(g->peak, g->used);
+                return MOSSQ_OK;
             }
         }
-    c->cache.clear();
-    c->cached = 0;
+        if (pass == 0)
+            for (Ctx *m : g->members) MOSSQ_TRY(cache_flush(m));
+    }
+    return fail(MOSSQ_EOOM, "memory budget of the server exceeded: " + std::to_string(g->used) + " + " +
+                                std::to_string(bytes) + " > " + std::to_string(g->limit));
+}
+
+// Erases p from the allocation list (caller holds mem_mu); returns its size.
+static u64 forget_alloc(Ctx *c, void *p) {
+    auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
+    if (it == c->allocs.end()) return 0;
+    const u64 b = it->second;
+    c->in_use -= b;
+    c->allocs.erase(it);
+    return b;
+}
+
+int cache_flush(Ctx *c) {
+    u64 freed = 0;
+    {
+        std::lock_guard<std::mutex> lk(c->mem_mu);
+        if (!c->cached) return MOSSQ_OK;
+        // cached blocks may still be read by work queued on either stream
+        MOSSQ_HIP(hipStreamSynchronize(c->stream));
+        if (c->aux) MOSSQ_HIP(hipStreamSynchronize(c->aux));
+        for (auto &kv : c->cache)
+            for (void *p : kv.second) {
+                (void)hipFree(p);
+                freed += forget_alloc(c, p);
+            }
+        c->cache.clear();
+        c->in_cache.clear();
+        c->cached = 0;
+    }
+    group_release(c, freed);
     return MOSSQ_OK;
 }
 
@@ -46,16 +86,23 @@ static int raw_alloc(Ctx *c, u64 bytes, void **p) {
     if (c->budget && c->in_use + bytes > c->budget)
         return fail(MOSSQ_EOOM, "device memory budget exceeded: " + std::to_string(c->in_use) + " + " +
                                     std::to_string(bytes) + " > " + std::to_string(c->budget));
+    if (c->group) MOSSQ_TRY(group_reserve(c, bytes));
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess && c->cached) {
         (void)hipGetLastError();
-        MOSSQ_TRY(cache_flush(c));
+        const int st = cache_flush(c);
+        if (st != MOSSQ_OK) {
+            group_release(c, bytes);
+            return st;
+        }
         e = hipMalloc(p, bytes);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();
+        group_release(c, bytes);
         return fail(MOSSQ_EOOM, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
+    std::lock_guard<std::mutex> lk(c->mem_mu);
     c->allocs.emplace_back(*p, bytes);
     c->in_use += bytes;
     return MOSSQ_OK;
@@ -64,30 +111,41 @@ static int raw_alloc(Ctx *c, u64 bytes, void **p) {
 int cached_alloc(Ctx *c, u64 bytes, void **p) {
     u64 r = 4096;
     while (r < bytes) r <<= 1;
-    auto it = c->cache.find(r);
-    if (it != c->cache.end() && !it->second.empty()) {
-        *p = it->second.back();
-        it->second.pop_back();
-        c->cached -= r;
-        return MOSSQ_OK;
+    {
+        std::lock_guard<std::mutex> lk(c->mem_mu);
+        auto it = c->cache.find(r);
+        if (it != c->cache.end() && !it->second.empty()) {
+            *p = it->second.back();
+            it->second.pop_back();
+            c->in_cache.erase(*p);
+            c->cached -= r;
+            return MOSSQ_OK;
+        }
     }
     return raw_alloc(c, r, p);
 }
 
 int cached_free(Ctx *c, void *p) {
     if (!p) return MOSSQ_OK;
-    auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
-    MOSSQ_CHECK(it != c->allocs.end(), MOSSQ_EINVAL, "pointer was not allocated by this context");
-    if (c->cached + it->second <= kAllocCacheCap) {
-        // kept for reuse on this stream, which orders it after the queued work
-        c->cache[it->second].push_back(p);
-        c->cached += it->second;
-        return MOSSQ_OK;
+    u64 freed = 0;
+    {
+        std::lock_guard<std::mutex> lk(c->mem_mu);
+        auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
+        MOSSQ_CHECK(it != c->allocs.end(), MOSSQ_EINVAL, "pointer was not allocated by this context");
+        MOSSQ_CHECK(!c->in_cache.count(p), MOSSQ_EINVAL, "device block released twice");
+        if (c->cached + it->second <= kAllocCacheCap) {
+            // kept for reuse on this stream, which orders it after the queued work
+            c->cache[it->second].push_back(p);
+            c->in_cache.insert(p);
+            c->cached += it->second;
+            return MOSSQ_OK;
+        }
+        MOSSQ_HIP(hipStreamSynchronize(c->stream));
+        if (c->aux) MOSSQ_HIP(hipStreamSynchronize(c->aux));
+        MOSSQ_HIP(hipFree(p));
+        freed = forget_alloc(c, p);
     }
-    MOSSQ_HIP(hipStreamSynchronize(c->stream));
-    MOSSQ_HIP(hipFree(p));
-    c->in_use -= it->second;
-    c->allocs.erase(it);
+    group_release(c, freed);
     return MOSSQ_OK;
 }
 
@@ -100,13 +158,29 @@ int dev_alloc(Ctx *c, u64 bytes, void **p) {
 int dev_free(Ctx *c, void *p) {
     if (!p) return MOSSQ_OK;
     if (c->cache_allocs) return cached_free(c, p);
-    auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
-    MOSSQ_CHECK(it != c->allocs.end(), MOSSQ_EINVAL, "pointer was not allocated by this context");
-    // Memory may still be read by queued kernels: order the free after them.
-    MOSSQ_HIP(hipStreamSynchronize(c->stream));
-    MOSSQ_HIP(hipFree(p));
-    c->in_use -= it->second;
-    c->allocs.erase(it);
+    u64 freed = 0;
+    {
+        std::lock_guard<std::mutex> lk(c->mem_mu);
+        auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [&](auto &a) { return a.first == p; });
+        MOSSQ_CHECK(it != c->allocs.end() && !c->in_cache.count(p), MOSSQ_EINVAL,
+                    "pointer was not allocated by this context");
+        // Memory may still be read by queued kernels: order the free after them.
+        MOSSQ_HIP(hipStreamSynchronize(c->stream));
+        if (c->aux) MOSSQ_HIP(hipStreamSynchronize(c->aux));
+        MOSSQ_HIP(hipFree(p));
+        freed = forget_alloc(c, p);
+    }
+    group_release(c, freed);
+    return MOSSQ_OK;
+}
+
+int ctx_borrow(Ctx *base, Ctx **out) {
+    Ctx *c = new Ctx();
+    c->device = base->device;
+    c->num_cu = base->num_cu;
+    c->stream = base->stream;
+    c->own_stream = false;
+    *out = c;
     return MOSSQ_OK;
 }
 
@@ -238,12 +312,17 @@ int mossq_ctx_destroy(mossq_ctx *ctx) {
             if (ev) (void)hipEventDestroy(ev);
         (void)hipStreamDestroy(c->aux);
     }
-    for (auto &a : c->allocs) (void)hipFree(a.first);
+    u64 freed = 0;
+    for (auto &a : c->allocs) {
+        (void)hipFree(a.first);
+        freed += a.second;
+    }
     c->allocs.clear();
+    group_release(c, freed);
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
     for (int i = 0; i < kNumEvents; ++i)
         if (c->events[i]) (void)hipEventDestroy(c->events[i]);
-    (void)hipStreamDestroy(c->stream);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MOSSQ_OK;
 }
@@ -256,6 +335,7 @@ int mossq_sync(mossq_ctx *ctx) {
 
 int mossq_mem_in_use(mossq_ctx *ctx, uint64_t *bytes) {
     MOSSQ_CHECK(ctx && bytes, MOSSQ_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(C(ctx)->mem_mu);
     *bytes = C(ctx)->in_use;
     return MOSSQ_OK;
 }

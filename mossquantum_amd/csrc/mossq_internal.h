@@ -5,8 +5,10 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/mossq_hip.h"
@@ -94,6 +96,18 @@ __host__ __device__ __forceinline__ u64 insert0s_dyn(u64 x, const int *pos, int 
 // ------------------------------------------------------------- context ----
 constexpr int kNumEvents = 16;
 
+struct Ctx;
+
+// A device-memory limit shared by several contexts (the request server's
+// engine and worker contexts).  Every raw allocation of a member counts
+// against it, cached blocks included; an allocation that would pass the limit
+// first releases the cached blocks of every member.
+struct MemGroup {
+    std::mutex mu;
+    u64 limit = 0, used = 0, peak = 0;
+    std::vector<Ctx *> members;  // fixed before any member allocates
+};
+
 struct Ctx {
     int device = 0;
     int num_cu = 0;  // compute units of the device
@@ -121,12 +135,23 @@ struct Ctx {
     // always goes through it; with cache_allocs every dev_alloc does.
     bool cache_allocs = false;
     std::map<u64, std::vector<void *>> cache;
+    std::unordered_set<void *> in_cache;  // blocks now held by the cache
     u64 cached = 0;
+    // allocs, in_use, cache, in_cache and cached change under mem_mu: a
+    // MemGroup member may release another member's cache from its own thread
+    std::mutex mem_mu;
+    MemGroup *group = nullptr;
+    bool own_stream = true;  // false: `stream` is borrowed from another context
 };
 constexpr u64 kAllocCacheCap = u64(16) << 30;
 
 // The context's second stream and its events (created on first use).
 int ctx_aux(Ctx *c);
+// A context without a stream of its own: allocations are its own (its cache,
+// its group), launches and copies go to base's stream.
+int ctx_borrow(Ctx *base, Ctx **out);
+// Releases every cached block of c (after the work queued on its streams).
+int cache_flush(Ctx *c);
 
 int ctx_check(mossq_ctx *ctx);
 Ctx *C(mossq_ctx *ctx);
@@ -151,6 +176,21 @@ __device__ __forceinline__ const T *const_table(const T *p) {
 
 int cached_alloc(Ctx *c, u64 bytes, void **p);
 int cached_free(Ctx *c, void *p);
+
+// Orders the context's main stream after its second stream when the scope
+// ends (declared after an Allocs, it runs first, so blocks read by work on the
+// second stream return to the cache only behind that work).
+struct AuxJoin {
+    Ctx *c;
+    bool active = false;
+    explicit AuxJoin(Ctx *cc) : c(cc) {}
+    ~AuxJoin() {
+        if (active && c->aux) {
+            (void)hipEventRecord(c->aux_ev[1], c->aux);
+            (void)hipStreamWaitEvent(c->stream, c->aux_ev[1], 0);
+        }
+    }
+};
 
 // Scoped per-call scratch: everything obtained through get() goes back to the
 // context's block cache on exit.

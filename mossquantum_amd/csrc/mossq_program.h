@@ -41,6 +41,12 @@ struct Program {
     FusedPlan *fused = nullptr;
     FusedPlan *fused_rev = nullptr;  // pure-state adjoint reverse pass (with gradient probes)
     DensityAdjointPlan *dm_adj = nullptr;
+    // Graphs captured from this program hold its plan tables' device
+    // addresses: while any is alive, a plan the autotuner replaces is kept in
+    // `retired` (released when the last graph is destroyed, or with the
+    // program).
+    mutable int live_graphs = 0;
+    std::vector<FusedPlan *> retired;
 };
 
 // Host side of mossq_program_create: validate records, build steps and pool.
@@ -60,6 +66,8 @@ int fused_run(Ctx *c, const Program *p, c128 *state);
 // Time every forward-plan variant on a scratch state (reps runs each) and keep
 // the fastest as p->fused; tried = distinct variants, best_ms = its run time.
 int fused_plan_autotune(Program *p, int reps, int *tried, double *best_ms);
+// Releases the plans kept for captured graphs (call with no graph alive).
+void fused_plan_release_retired(Program *p);
 // Reverse pass of the pure-state adjoint on (y, l) with the adjoint plan;
 // gradient partials into rows of `row` entries (zeroed by the caller).
 int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, int row);
@@ -94,9 +102,12 @@ int density_adjoint_parts(const Program *p);
 // program on its own vectors.  A program qualifies when every item of its
 // forward plan (and, for adjoint sweeps, of its adjoint plan) is a sweep.
 bool fused_plan_batchable(const Program *p, bool adjoint);
+constexpr int kBatchLaunchItems = 65536;  // items per batched launch (16-bit item field)
+// Workgroup table of one batched launch (items with nt[i] tiles, round robin).
+int deal_tiles(const std::vector<u64> &nt, std::vector<unsigned> &wg);
 struct BatchSweeps {
     void *d_items = nullptr;  // device launch items (layout private to tile.hip)
-    int *d_wg = nullptr;      // workgroup -> item of its launch << 16 | tile
+    unsigned *d_wg = nullptr; // workgroup -> item of its launch << 16 | tile
     c128 *vec = nullptr, *part = nullptr;
     std::vector<int> item0, wg0, tiles;  // per launch
 };

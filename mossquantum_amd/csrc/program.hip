@@ -480,6 +480,7 @@ int mossq_program_create(mossq_ctx *ctx, int n, int is_density, int nops, const 
 int mossq_program_destroy(mossq_program *prog) {
     if (!prog) return MOSSQ_OK;
     Program *p = reinterpret_cast<Program *>(prog);
+    MOSSQ_CHECK(p->live_graphs == 0, MOSSQ_EINVAL, "a graph captured from the program is still alive");
     (void)hipSetDevice(p->ctx->device);
     fused_plan_free(p);
     int s = cached_free(p->ctx, p->d_pool);
@@ -514,6 +515,7 @@ int mossq_program_autotune(mossq_ctx *ctx, mossq_program *prog, int reps, int *v
 
 struct mossq_graph {
     Ctx *ctx = nullptr;
+    const Program *prog = nullptr;  // its plan stays alive while the graph is (Program::retired)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -543,6 +545,8 @@ int mossq_program_capture(mossq_ctx *ctx, const mossq_program *prog, void *state
     }
     auto *gr = new mossq_graph();
     gr->ctx = c;
+    gr->prog = p;
+    p->live_graphs++;
     gr->graph = g;
     gr->exec = ex;
     *out = gr;
@@ -564,6 +568,8 @@ int mossq_graph_destroy(mossq_graph *graph) {
     (void)hipStreamSynchronize(graph->ctx->stream);
     if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
     if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    Program *p = const_cast<Program *>(graph->prog);
+    if (--p->live_graphs == 0) fused_plan_release_retired(p);
     delete graph;
     return MOSSQ_OK;
 }

@@ -21,6 +21,7 @@
 // (src/differentiation/objectives.jl:183-218).
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstring>
 #include <condition_variable>
 #include <deque>
@@ -63,34 +64,6 @@ struct SRequest {
     double value = 0;
     std::vector<double> grads;
     std::string error;
-};
-
-struct BlockCache {
-    Ctx *c = nullptr;
-    std::map<u64, std::vector<void *>> free;
-    static u64 round(u64 b) {
-        u64 r = 4096;
-        while (r < b) r <<= 1;
-        return r;
-    }
-    int get(u64 bytes, void **p) {
-        u64 r = round(bytes);
-        auto &v = free[r];
-        if (!v.empty()) {
-            *p = v.back();
-            v.pop_back();
-            return MOSSQ_OK;
-        }
-        return dev_alloc(c, r, p);
-    }
-    void put(u64 bytes, void *p) {
-        if (p) free[round(bytes)].push_back(p);
-    }
-    void release() {
-        for (auto &kv : free)
-            for (void *p : kv.second) dev_free(c, p);
-        free.clear();
-    }
 };
 
 // Pinned host staging for asynchronous uploads: a ring whose regions are
@@ -150,7 +123,13 @@ struct PinnedRing {
 
 struct Server {
     Ctx *ctx = nullptr;
+    // The requests' device memory: the engine's vectors and tables come from
+    // ectx (a context on ctx's stream), the workers' from their own contexts;
+    // all of them, cached blocks included, count against `mem` (limit =
+    // budget).
+    Ctx *ectx = nullptr;
     std::vector<Ctx *> wctxs;  // worker contexts
+    MemGroup mem;
     u64 budget = 0;
     std::atomic<u64> reserved{0};
     std::mutex mu;
@@ -159,7 +138,6 @@ struct Server {
     std::vector<SRequest *> active;                   // engine items
     i64 next_ticket = 1;
     int t = 0;
-    BlockCache cache;
     PinnedRing ring;
     // engine tables
     BItemDev *d_items[2] = {nullptr, nullptr};
@@ -184,12 +162,29 @@ struct Server {
     std::condition_variable wcv;
     bool stop = false;
     int worker_busy = 0;
+    u64 finished = 0;                  // requests finished so far (under mu)
+    std::condition_variable done_cv;   // notified when one finishes
 };
 
 static int build_request_program(SRequest *r) {
     r->prog = Program();
     return program_build_host(&r->prog, r->n, r->is_density, int(r->ops.size()), r->ops.data(),
                               (const double *)r->pool.data(), r->pool.size());
+}
+
+// One worker request on worker context wc: upload, plan, adjoint, release.
+static int run_worker_request(Ctx *wc, SRequest *r) {
+    int st = program_upload(wc, &r->prog);
+    if (st == MOSSQ_OK && r->fused) st = fused_plan_build(&r->prog);
+    if (st == MOSSQ_OK && r->is_density) st = density_adjoint_plan_build(&r->prog);
+    r->grads.assign(std::max(r->prog.nderiv, 1), 0.0);
+    if (st == MOSSQ_OK)
+        st = r->is_density ? adjoint_density(wc, &r->prog, r->obs, nullptr, 0, nullptr, &r->value, r->grads.data())
+                           : adjoint_pure(wc, &r->prog, r->obs, nullptr, nullptr, &r->value, r->grads.data());
+    fused_plan_free(&r->prog);
+    if (r->prog.d_pool) dev_free(wc, r->prog.d_pool);
+    r->prog.d_pool = nullptr;
+    return st;
 }
 
 static void worker_main(Server *s, int wi) {
@@ -206,17 +201,18 @@ static void worker_main(Server *s, int wi) {
             q.pop_front();
             s->worker_busy++;
         }
-        int st = program_upload(wc, &r->prog);
-        if (st == MOSSQ_OK && r->fused) st = fused_plan_build(&r->prog);
-        if (st == MOSSQ_OK && r->is_density) st = density_adjoint_plan_build(&r->prog);
-        r->grads.assign(std::max(r->prog.nderiv, 1), 0.0);
-        if (st == MOSSQ_OK)
-            st = r->is_density
-                     ? adjoint_density(wc, &r->prog, r->obs, nullptr, 0, nullptr, &r->value, r->grads.data())
-                     : adjoint_pure(wc, &r->prog, r->obs, nullptr, nullptr, &r->value, r->grads.data());
-        fused_plan_free(&r->prog);
-        if (r->prog.d_pool) dev_free(wc, r->prog.d_pool);
-        r->prog.d_pool = nullptr;
+        int st = run_worker_request(wc, r);
+        // Admission counts each request's estimated bytes; when the blocks of
+        // the requests running beside this one leave too little room (the
+        // estimate fell short), wait for one of them to finish and run again.
+        while (st == MOSSQ_EOOM) {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->stop || s->reserved.load() <= r->bytes) break;  // nothing else holds memory
+            const u64 seen = s->finished;
+            s->done_cv.wait_for(lk, std::chrono::milliseconds(50), [&] { return s->stop || s->finished != seen; });
+            lk.unlock();
+            st = run_worker_request(wc, r);
+        }
         r->grads.resize(r->prog.nderiv);
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -224,14 +220,26 @@ static void worker_main(Server *s, int wi) {
             r->state = st == MOSSQ_OK ? 3 : -1;
             s->reserved -= r->bytes;
             s->worker_busy--;
+            s->finished++;
             s->completed.push_back(r->ticket);
         }
+        s->done_cv.notify_all();
     }
 }
 
+// Size of the block the block cache hands out for `bytes`.
+static u64 block_bytes(u64 bytes) {
+    u64 r = 4096;
+    while (r < bytes) r <<= 1;
+    return r;
+}
+
+// The engine's blocks for a request (vectors, matrix pool, one micro-op per
+// instruction), as the cache rounds them.
 static u64 engine_bytes(const SRequest *r) {
-    return 2 * (u64(1) << r->n) * sizeof(c128) + r->pool.size() * sizeof(c128) + r->ops.size() * sizeof(BOp) +
-           u64(batch_chunks(r->n)) * sizeof(c128);
+    return block_bytes(2 * (u64(1) << r->n) * sizeof(c128)) +
+           block_bytes(std::max<size_t>(r->prog.h_pool.size(), 1) * sizeof(c128)) +
+           block_bytes(std::max<size_t>(r->prog.steps.size(), 1) * sizeof(BOp));
 }
 
 static int ensure_part(Server *s, int need) {
@@ -256,9 +264,9 @@ static int admit_engine(Server *s, SRequest *r) {
     r->vec_bytes = 2 * (u64(1) << r->n) * sizeof(c128);
     r->pool_bytes = std::max<size_t>(r->prog.h_pool.size(), 1) * sizeof(c128);
     r->ops_bytes = std::max<size_t>(ops.size(), 1) * sizeof(BOp);
-    MOSSQ_TRY(s->cache.get(r->vec_bytes, (void **)&r->vec));
-    MOSSQ_TRY(s->cache.get(r->pool_bytes, (void **)&r->d_pool));
-    MOSSQ_TRY(s->cache.get(r->ops_bytes, (void **)&r->d_ops));
+    MOSSQ_TRY(cached_alloc(s->ectx, r->vec_bytes, (void **)&r->vec));
+    MOSSQ_TRY(cached_alloc(s->ectx, r->pool_bytes, (void **)&r->d_pool));
+    MOSSQ_TRY(cached_alloc(s->ectx, r->ops_bytes, (void **)&r->d_ops));
     char *stage = nullptr;
     u64 lo = 0;
     const u64 up = r->pool_bytes + r->ops_bytes;
@@ -338,9 +346,9 @@ static int retire(Server *s) {
         if (!r->done_ev) MOSSQ_HIP(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
         MOSSQ_HIP(hipEventRecord(r->done_ev, c->stream));
         // stream order keeps later users of these blocks behind this request's kernels
-        s->cache.put(r->vec_bytes, r->vec);
-        s->cache.put(r->pool_bytes, r->d_pool);
-        s->cache.put(r->ops_bytes, r->d_ops);
+        MOSSQ_TRY(cached_free(s->ectx, r->vec));
+        MOSSQ_TRY(cached_free(s->ectx, r->d_pool));
+        MOSSQ_TRY(cached_free(s->ectx, r->d_ops));
         r->vec = r->d_pool = nullptr;
         r->d_ops = nullptr;
         r->state = 2;
@@ -366,8 +374,12 @@ static void collect(Server *s) {
         r->slot = -1;
         s->reserved -= r->bytes;
         r->state = 3;
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->completed.push_back(r->ticket);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->finished++;
+            s->completed.push_back(r->ticket);
+        }
+        s->done_cv.notify_all();
     }
     s->draining.swap(keep);
 }
@@ -435,13 +447,20 @@ int mossq_server_create(mossq_ctx *ctx, uint64_t budget_bytes, mossq_server **ou
     Server *s = new Server();
     s->ctx = C(ctx);
     s->budget = budget_bytes;
-    s->cache.c = s->ctx;
-    int st = MOSSQ_OK;
+    s->mem.limit = budget_bytes;
+    int st = ctx_borrow(s->ctx, &s->ectx);
+    if (st == MOSSQ_OK) {
+        s->ectx->cache_allocs = true;
+        s->ectx->group = &s->mem;
+        s->mem.members.push_back(s->ectx);
+    }
     for (int i = 0; i < kWorkers && st == MOSSQ_OK; ++i) {
         mossq_ctx *w = nullptr;
         st = mossq_ctx_create(s->ctx->device, 0, &w);
         if (st == MOSSQ_OK) {
             C(w)->cache_allocs = true;  // no device-wide waits when a request's buffers are released
+            C(w)->group = &s->mem;
+            s->mem.members.push_back(C(w));
             s->wctxs.push_back(C(w));
         }
     }
@@ -474,6 +493,7 @@ int mossq_server_destroy(mossq_server *srv) {
         s->stop = true;
     }
     s->wcv.notify_all();
+    s->done_cv.notify_all();
     for (auto &w : s->workers)
         if (w.joinable()) w.join();
     (void)hipSetDevice(s->ctx->device);
@@ -483,7 +503,6 @@ int mossq_server_destroy(mossq_server *srv) {
         if (r->done_ev) (void)hipEventDestroy(r->done_ev);
         delete r;
     }
-    s->cache.release();
     for (int k = 0; k < 2; ++k) {
         if (s->d_items[k]) dev_free(s->ctx, s->d_items[k]);
         if (s->h_items[k]) (void)hipHostFree(s->h_items[k]);
@@ -497,6 +516,7 @@ int mossq_server_destroy(mossq_server *srv) {
     if (s->h_results) (void)hipHostFree(s->h_results);
     s->ring.destroy();
     for (Ctx *w : s->wctxs) mossq_ctx_destroy(reinterpret_cast<mossq_ctx *>(w));
+    if (s->ectx) mossq_ctx_destroy(reinterpret_cast<mossq_ctx *>(s->ectx));
     delete s;
     return MOSSQ_OK;
 }
@@ -525,17 +545,22 @@ int mossq_server_submit(mossq_server *srv, int n, int is_density, int nops, cons
     engine_ok = engine_ok && r->prog.nderiv <= kMaxGrads;
     r->fused = !r->is_density && n >= kFusedMinQubits;
     r->engine = engine_ok && !r->fused;
+    // Estimated device bytes, as the block cache rounds them: the program's
+    // pool, the plan tables (bounded by a few times the pool), the states and
+    // the partial sums.
+    const u64 pool_blocks = block_bytes(r->prog.h_pool.size() * sizeof(c128)) * 9 + (4ull << 20);
     if (r->engine) {
         r->bytes = engine_bytes(r);
     } else if (r->fused) {
         // two states, gradient partials (one per wave per tile, or per pair-kernel block), plans
         const u64 row = std::max<u64>(u64(pair_parts(n)), u64(4) << (n - 12));
-        r->bytes = 2 * (u64(1) << n) * sizeof(c128) + u64(std::max(r->prog.nderiv, 1)) * row * sizeof(c128) +
-                   u64(o->G) * 512 * sizeof(c128) + 4 * r->pool.size() * sizeof(c128) + (8ull << 20);
+        const u64 parts = std::max<u64>(u64(std::max(r->prog.nderiv, 1)) * row, u64(o->G) * 512);
+        r->bytes = 2 * block_bytes((u64(1) << n) * sizeof(c128)) + block_bytes(parts * sizeof(c128)) + pool_blocks;
     } else if (r->is_density) {
-        r->bytes = adjoint_density_bytes(&r->prog, o) + r->pool.size() * sizeof(c128);
+        // (the step-by-step layout's states; the fused chunk plans keep fewer)
+        r->bytes = adjoint_density_bytes(&r->prog, o) + pool_blocks;
     } else {
-        r->bytes = 2 * (u64(1) << n) * sizeof(c128) * 2 + r->pool.size() * sizeof(c128);
+        r->bytes = 2 * block_bytes(2 * (u64(1) << n) * sizeof(c128)) + pool_blocks;
     }
     if (r->bytes > s->budget) {
         delete r;
@@ -565,6 +590,15 @@ int mossq_server_completed(mossq_server *srv, int64_t *tickets, int max, int *co
     for (int i = 0; i < k; ++i) tickets[i] = s->completed[i];
     s->completed.erase(s->completed.begin(), s->completed.begin() + k);
     *count = k;
+    return MOSSQ_OK;
+}
+
+int mossq_server_memory(mossq_server *srv, uint64_t *in_use, uint64_t *peak) {
+    MOSSQ_CHECK(srv, MOSSQ_EINVAL, "null server");
+    Server *s = reinterpret_cast<Server *>(srv);
+    std::lock_guard<std::mutex> lk(s->mem.mu);
+    if (in_use) *in_use = s->mem.used;
+    if (peak) *peak = s->mem.peak;
     return MOSSQ_OK;
 }
 

@@ -714,7 +714,8 @@ __global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 
 // ------------------------------------------------------ batched sweeps ----
 // Sweeps of many programs in one launch: item i is one sweep of one program's
 // plan on that program's own vectors, one workgroup per tile; wg_item gives
-// each workgroup its item and tile (item << 16 | tile; tile0 is the item's
+// each workgroup its item and tile (unsigned item << 16 | tile, both below
+// 2^16; a launch holds at most 65536 items; tile0 is the item's
 // first tile in a count of the launch's tiles, used on the host).  The vectors are
 // offsets into one allocation `vec` (partials into `part`), which the host
 // checks against that allocation's size.
@@ -740,18 +741,18 @@ struct TBItem {
     TSweepArgs args;
 };
 
-__global__ __launch_bounds__(TT, 2) void k_btile(const TBItem *__restrict__ items, const int *__restrict__ wg_item,
+__global__ __launch_bounds__(TT, 2) void k_btile(const TBItem *__restrict__ items, const unsigned *__restrict__ wg_item,
                                                    c128 *__restrict__ vec) {
-    const int e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);
+    const unsigned e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);
     const TBItem &it = items[e >> 16];
     tile_sweep<false>(vec + it.x, vec + it.x, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),
                       table<TDiag>(it.diag), table<c128>(it.pool), nullptr, 0, nullptr, 0, u64(e & 0xFFFF));
 }
 
 __global__ __launch_bounds__(TT, 2) void k_btile_adj(const TBItem *__restrict__ items,
-                                                       const int *__restrict__ wg_item, c128 *__restrict__ vec,
+                                                       const unsigned *__restrict__ wg_item, c128 *__restrict__ vec,
                                                        c128 *__restrict__ part) {
-    const int e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);
+    const unsigned e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);
     const TBItem &it = items[e >> 16];
     tile_adj_sweep(vec + it.x, vec + it.l, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),
                    table<TDiag>(it.diag), table<c128>(it.pool), part + it.part, it.row, u64(e & 0xFFFF));
@@ -2019,9 +2020,15 @@ static void free_plan(Ctx *c, FusedPlan *&fp) {
     fp = nullptr;
 }
 
+void fused_plan_release_retired(Program *p) {
+    for (FusedPlan *&fp : p->retired) free_plan(p->ctx, fp);
+    p->retired.clear();
+}
+
 void fused_plan_free(Program *p) {
     free_plan(p->ctx, p->fused);
     free_plan(p->ctx, p->fused_rev);
+    fused_plan_release_retired(p);
     density_adjoint_plan_free(p);
 }
 
@@ -2099,6 +2106,23 @@ int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, in
     return MOSSQ_OK;
 }
 
+// Workgroup table of one batched launch whose items have nt[i] tiles: the
+// tiles are dealt round robin over the items (entry = item << 16 | tile), so
+// every stretch of the grid mixes the heavy tiles of small items with the
+// light ones of large items.  Items and tiles must each stay below 2^16.
+int deal_tiles(const std::vector<u64> &nt, std::vector<unsigned> &wg) {
+    MOSSQ_CHECK(nt.size() <= size_t(kBatchLaunchItems), MOSSQ_EINVAL, "too many items in one batched launch");
+    u64 maxnt = 0;
+    for (u64 t : nt) {
+        MOSSQ_CHECK(t <= 65536, MOSSQ_EINVAL, "batched sweeps take at most 28 qubits");
+        maxnt = std::max(maxnt, t);
+    }
+    for (u64 r = 0; r < maxnt; ++r)
+        for (size_t i = 0; i < nt.size(); ++i)
+            if (r < nt[i]) wg.push_back((unsigned(i) << 16) | unsigned(r));
+    return MOSSQ_OK;
+}
+
 bool fused_plan_batchable(const Program *p, bool adjoint) {
     // (the launch tables give a tile 16 bits: up to 28 qubits)
     if (!p->fused || (adjoint && !p->fused_rev) || p->nbits - TM > 16) return false;
@@ -2116,7 +2140,7 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
     std::vector<TBItem> items;
     out->vec = vec;
     out->part = part;
-    std::vector<int> wg;
+    std::vector<unsigned> wg;
     out->item0.clear();
     out->wg0.clear();
     out->tiles.clear();
@@ -2127,9 +2151,12 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
         nlaunch = std::max(nlaunch, fp->items.size());
     }
     for (size_t s = 0; s < nlaunch; ++s) {
+      for (int b0 = 0; b0 < m;) {
+        // one launch: the next kBatchLaunchItems programs that have sweep s
         const size_t i0 = items.size(), w0 = wg.size();
         u64 tiles = 0;
-        for (int b = 0; b < m; ++b) {
+        int b = b0;
+        for (; b < m && items.size() - i0 < size_t(kBatchLaunchItems); ++b) {
             const Program *p = progs[b];
             const FusedPlan *fp = adjoint ? p->fused_rev : p->fused;
             if (s >= fp->items.size()) continue;
@@ -2155,29 +2182,23 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
             tiles += nt;
             items.push_back(it);
         }
-        // Workgroups take the launch's items round robin, one tile each
-        // (entry = item << 16 | tile): every stretch of the grid mixes the
-        // heavy tiles of small items with the light ones of large items.
-        u64 maxnt = 0;
-        for (size_t i = i0; i < items.size(); ++i) {
-            const u64 nt = i + 1 < items.size() ? u64(items[i + 1].tile0 - items[i].tile0) : tiles - items[i].tile0;
-            maxnt = std::max(maxnt, nt);
-        }
-        for (u64 r = 0; r < maxnt; ++r)
-            for (size_t i = i0; i < items.size(); ++i) {
-                const u64 nt = i + 1 < items.size() ? u64(items[i + 1].tile0 - items[i].tile0) : tiles - items[i].tile0;
-                if (r < nt) wg.push_back(int(((i - i0) << 16) | r));
-            }
+        b0 = b;
+        if (items.size() == i0) continue;
+        std::vector<u64> nts;
+        for (size_t i = i0; i < items.size(); ++i)
+            nts.push_back(i + 1 < items.size() ? u64(items[i + 1].tile0 - items[i].tile0) : tiles - items[i].tile0);
+        MOSSQ_TRY(deal_tiles(nts, wg));
         out->item0.push_back(int(i0));
         out->wg0.push_back(int(w0));
         out->tiles.push_back(int(tiles));
+      }
     }
     if (items.empty()) return MOSSQ_OK;
     MOSSQ_TRY(al.get(items.size() * sizeof(TBItem), &out->d_items));
-    MOSSQ_TRY(al.get(wg.size() * sizeof(int), (void **)&out->d_wg));
+    MOSSQ_TRY(al.get(wg.size() * sizeof(unsigned), (void **)&out->d_wg));
     MOSSQ_HIP(hipMemcpyAsync(out->d_items, items.data(), items.size() * sizeof(TBItem), hipMemcpyHostToDevice,
                              c->stream));
-    MOSSQ_HIP(hipMemcpyAsync(out->d_wg, wg.data(), wg.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(out->d_wg, wg.data(), wg.size() * sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));  // the host tables go out of scope
     return MOSSQ_OK;
 }
@@ -2258,7 +2279,10 @@ int fused_plan_autotune(Program *p, int reps, int *tried, double *best_ms) {
     for (size_t i = 1; i < cand.size(); ++i)
         if (i != best) free_plan(c, cand[i]);
     if (s == MOSSQ_OK && best != 0) {
-        free_plan(c, p->fused);
+        if (p->live_graphs > 0)
+            p->retired.push_back(p->fused);  // a captured graph still reads its tables
+        else
+            free_plan(c, p->fused);
         p->fused = cand[best];
     }
     if (e0) (void)hipEventDestroy(e0);
@@ -2637,6 +2661,31 @@ static void plan_stats(const FusedPlan *fp, int *stats) {
 }
 
 extern "C" {
+
+int mossq_batch_launch_tables(int m, const int32_t *ntiles, int max_launches, int32_t *launch_items, uint64_t cap,
+                              uint32_t *entries, int *nlaunches, uint64_t *nentries) {
+    MOSSQ_CHECK(m >= 0 && (m == 0 || ntiles) && nlaunches && nentries && (max_launches == 0 || launch_items),
+                MOSSQ_EINVAL, "bad arguments");
+    std::vector<unsigned> wg;
+    int L = 0;
+    for (int b0 = 0; b0 < m; b0 += kBatchLaunchItems) {
+        const int b1 = std::min(m, b0 + kBatchLaunchItems);
+        std::vector<u64> nt;
+        for (int b = b0; b < b1; ++b) {
+            MOSSQ_CHECK(ntiles[b] >= 1, MOSSQ_EINVAL, "an item has at least one tile");
+            nt.push_back(u64(ntiles[b]));
+        }
+        MOSSQ_TRY(deal_tiles(nt, wg));
+        if (L < max_launches) launch_items[L] = b1 - b0;
+        ++L;
+    }
+    *nlaunches = L;
+    *nentries = wg.size();
+    MOSSQ_CHECK(L <= max_launches && wg.size() <= cap && (wg.empty() || entries), MOSSQ_EINVAL,
+                "output arrays too small");
+    std::copy(wg.begin(), wg.end(), entries);
+    return MOSSQ_OK;
+}
 
 int mossq_plan_emulate(int n, int is_density, int nops, const mossq_op *ops, const double *pool, uint64_t pool_len,
                        double *state, int *stats) {

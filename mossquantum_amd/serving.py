@@ -114,6 +114,12 @@ class Server:
                                                pool.size, tpl.obs.h, C.byref(t)))
         return t.value
 
+    def memory(self) -> tuple[int, int]:
+        """(bytes the requests hold now, cached blocks included; peak since creation)."""
+        u, p = C.c_uint64(0), C.c_uint64(0)
+        check(self.ctx.lib.mossq_server_memory(self.h, C.byref(u), C.byref(p)))
+        return u.value, p.value
+
     def step(self):
         a, q = C.c_int(0), C.c_int(0)
         check(self.ctx.lib.mossq_server_step(self.h, C.byref(a), C.byref(q)))

@@ -77,3 +77,13 @@ def xxz(n: int):
 
 def batch_sizes(B: int, lo: int, hi: int, seed: int) -> np.ndarray:
     return np.random.default_rng(seed).integers(lo, hi + 1, size=B)
+
+
+def c3_batch(B: int = 512, layers: int = 6, seed: int = 20260300):
+    """The batched-gradient workload: B states with n ~ U{10..20} (seeded), each
+    the ansatz with `layers` layers and its own angles, loss the XXZ chain.
+    Returns (sizes, circuits, observables)."""
+    sizes = [int(x) for x in batch_sizes(B, 10, 20, seed)]
+    rng = np.random.default_rng(seed + 1)
+    circs = [hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers)) for n in sizes]
+    return sizes, circs, [xxz(n) for n in sizes]

@@ -500,7 +500,11 @@ def test_unsupported_before_payload(ctx):
     wide = mq.QuantumGate((1, 2, 3), np.eye(8))
     assert mq.supports(mq.backend(sv), wide, sv, "apply")
     with pytest.raises(mq.UnsupportedOperation):
-        mq.apply_(mq.QuantumGate((1, 4), np.eye(4)), sv)  # position outside the register
+        mq.apply_(mq.QuantumGate((1, 2, 3, 4, 5, 6), np.eye(64)), mq.StateVector.zeros(6))  # wider than 5
+    with pytest.raises(ValueError, match="between 1 and 3"):
+        mq.apply_(mq.QuantumGate((1, 4), np.eye(4)), sv)  # position outside the register (ArgumentError)
+    with pytest.raises(ValueError, match="between 1 and 3"):
+        mq.expectation(mq.QubitsOperator([mq.QubitsTerm({4: "Z"})]), sv)
     # invalid arguments come back as status codes, not faults
     bad = np.array([2, 1], dtype=np.int32)
     eye = np.eye(4, dtype=np.complex128).reshape(-1)
@@ -551,3 +555,36 @@ def test_autotune_keeps_results(ctx):
     oracle.run_program(want, lower(c, n, False))
     assert np.max(np.abs(sv.to_host() - want)) < AMP
     assert mq.Program(c, n, False, ctx, fuse=False).autotune() == (0, 0.0)
+
+
+def test_graph_replay_after_autotune_and_churn(ctx):
+    """A graph captured before the autotuner replaces the program's plan keeps
+    replaying the plan it recorded: the replaced plan's tables stay allocated
+    while the graph lives, so the blocks that other programs, adjoints and
+    tables take from the context's cache in between never overwrite them."""
+    n, d = 15, 8
+    c = random_circuit(n, d, haar_layers(n, d, 91))
+    prog = mq.Program(c, n, False, ctx, fuse=True)
+    sv = mq.StateVector.zeros(n, ctx)
+    graph = prog.capture(sv.storage)
+    variants, _ = prog.autotune(reps=1)
+    assert variants >= 1
+    # churn the block cache: programs with plans, adjoints with scratch, all freed
+    for seed in range(6):
+        cc = random_circuit(n, 3 + seed, haar_layers(n, 3 + seed, 200 + seed))
+        p2 = mq.Program(cc, n, False, ctx, fuse=True)
+        tmp = mq.StateVector.zeros(n, ctx)
+        p2.run(tmp.storage)
+        obj = mq.ExpectationObjective(hea_circuit(12, 2), tfim(12), mq.StateVector.zeros(12, ctx))
+        mq.value_and_gradient(mq.AdjointGradient(), obj, np.linspace(0.1, 2.0, 48))
+        del p2, tmp, obj
+    want = np.zeros(1 << n, dtype=np.complex128)
+    want[0] = 1
+    oracle.run_program(want, lower(c, n, False))
+    graph.launch()
+    assert np.max(np.abs(sv.to_host() - want)) < AMP
+    prog.run(sv.storage)  # the kept plan on the same buffer
+    w2 = want.copy()
+    oracle.run_program(w2, lower(c, n, False))
+    assert np.max(np.abs(sv.to_host() - w2)) < AMP
+    del graph

@@ -1,0 +1,136 @@
+"""Parity against the oracle at the benchmark configurations' full sizes
+(GPU tests, through the C ABI).
+
+  * the 26-qubit depth-20 bandwidth circuit (770 gates, the committed Haar
+    fixture): every amplitude of the 1 GiB state, fused (autotuned plan) and
+    unfused, against cpu_ref's one-sweep-per-gate run;
+  * the 13-qubit noisy density matrix (8 layers, 1104 instructions): all
+    4^13 entries, the TFIM energy, trace and purity;
+  * the 512-state batched adjoint (n ~ U{10..20}, 6 layers, XXZ): every item of
+    17 or more qubits and a seeded sample of the others against cpu_ref's
+    adjoint (reference differentiation/circuit.jl:77-229);
+  * a served mix with 20-24-qubit state vectors and density matrices up to 12
+    qubits: values and gradients against the oracle (state vectors, density
+    matrices up to 10 qubits, whose per-channel checkpoints fit host memory)
+    and against the device's step-by-step adjoint (11 and 12 qubits).
+
+Method as the reference's kernel regression tests (test/regression/
+kernels.jl:19-43): the same seeded inputs through both paths, compared
+entrywise.  Tolerances: amplitudes 1e-12, expectations 1e-10, gradients 1e-9.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+import mossquantum_amd as mq
+from mossquantum_amd import _lib, lower
+from mossquantum_amd.serving import RequestTemplate, Server
+from mossquantum_amd.workloads import c3_batch, hea_angles, hea_circuit, random_circuit, tfim
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+AMP, EXP, GRAD = 1e-12, 1e-10, 1e-9
+
+
+@pytest.mark.timeout(300)
+def test_c1_full_circuit_against_oracle(ctx):
+    n, d = 26, 20
+    U = np.load(GOLD / "c1_haar26.npz")["unitaries"]
+    c = random_circuit(n, d, U)
+    want = np.zeros(1 << n, dtype=np.complex128)
+    want[0] = 1
+    oracle.run_program(want, lower(c, n, False))
+    for fuse in (True, False):
+        prog = mq.Program(c, n, False, ctx, fuse=fuse)
+        if fuse:
+            prog.autotune(reps=1)  # the plan the benchmark keeps
+        sv = mq.StateVector.zeros(n, ctx)
+        prog.run(sv.storage)
+        got = sv.to_host()
+        del sv, prog
+        d_amp = float(np.max(np.abs(got - want)))
+        assert d_amp < AMP, (fuse, d_amp)
+        del got
+
+
+@pytest.mark.timeout(300)
+def test_c2_full_density_matrix_against_oracle(ctx):
+    n, L = 13, 8
+    c = hea_circuit(n, L, hea_angles(n, L, 20260213), noise=(0.01, 0.02))
+    lw = lower(c, n, True)
+    want = np.zeros(4 ** n, dtype=np.complex128)
+    want[0] = 1
+    oracle.run_program(want, lw)
+    e_want = oracle.dm_expect(want, n, tfim(n).groups()).real
+    tr_want = want[:: (1 << n) + 1].sum()
+    pur_want = np.vdot(want, want).real
+    for fuse in (True, False):
+        dm = mq.DensityMatrix.zeros(n, ctx)
+        mq.Program(c, n, True, ctx, fuse=fuse).run(dm.storage)
+        e, tr, pur = mq.expectation(tfim(n), dm).real, mq.trace(dm), mq.purity(dm)
+        got = dm.to_host()
+        del dm
+        assert float(np.max(np.abs(got - want))) < AMP, fuse
+        assert abs(e - e_want) < EXP and abs(tr - tr_want) < EXP and abs(pur - pur_want) < EXP
+        assert abs(tr - 1) < 1e-12
+        del got
+
+
+@pytest.mark.timeout(600)
+def test_c3_full_batch_against_oracle(ctx):
+    B, layers = 512, 6
+    sizes, circs, obs = c3_batch(B, layers)
+    vals, grads = mq.batched_adjoint(circs, obs, sizes, ctx, fuse=True)
+    rng = np.random.default_rng(5)
+    small = [b for b in range(B) if sizes[b] < 17]
+    check = [b for b in range(B) if sizes[b] >= 17] + sorted(rng.choice(small, 32, replace=False).tolist())
+    worst_v = worst_g = 0.0
+    for b in check:
+        n = sizes[b]
+        v, g = oracle.adjoint(lower(circs[b], n, False), obs[b].groups())
+        g = lower(circs[b], n, False).slot_gradients(g)
+        worst_v = max(worst_v, abs(vals[b] - v))
+        worst_g = max(worst_g, float(np.max(np.abs(grads[b] - g))))
+    assert len(check) > 150
+    assert worst_v < EXP and worst_g < GRAD, (worst_v, worst_g)
+
+
+def _device_stepwise_adjoint(ctx, circ, n, is_dm, observable):
+    """Value and gradients of the unfused program with mossq_adjoint (the
+    step-by-step reverse pass with segment checkpoints)."""
+    prog = mq.Program(circ, n, is_dm, ctx, fuse=False)
+    dob = mq.DeviceObservable(observable, n, ctx)
+    v = C.c_double(0)
+    g = np.zeros(max(prog.lowered.nderiv, 1))
+    _lib.check(ctx.lib.mossq_adjoint(ctx.h, prog.h, dob.h, None, 0, None, C.byref(v), _lib.dptr(g)))
+    return v.value, g[:prog.lowered.nderiv]
+
+
+@pytest.mark.timeout(600)
+def test_c4_served_mix_to_full_sizes(ctx):
+    specs = [("sv", 24, 4), ("dm", 12, 4), ("sv", 8, 6), ("sv", 22, 5), ("dm", 10, 4), ("sv", 20, 8),
+             ("dm", 11, 5), ("sv", 16, 4), ("dm", 9, 6), ("sv", 12, 7), ("dm", 7, 8), ("sv", 21, 4)]
+    rng = np.random.default_rng(44)
+    reqs = []
+    for kind, n, layers in specs:
+        noise = (0.01, 0.02) if kind == "dm" else None
+        tpl = RequestTemplate(hea_circuit(n, layers, noise=noise), n, kind == "dm", tfim(n), ctx)
+        th = rng.uniform(0, 2 * np.pi, tpl.nparams)
+        reqs.append((tpl, tpl.bind(th), th, layers, noise))
+    srv = Server(ctx, 64 << 30)
+    tickets = [srv.submit(t, p) for t, p, *_ in reqs]
+    out = srv.run_until_done(tickets, [r[0].lowered.nderiv for r in reqs])
+    srv.close()
+    for t, (tpl, _, th, layers, noise) in zip(tickets, reqs):
+        vm, gm = out[t]
+        n, dm = tpl.n, tpl.is_density
+        circ = hea_circuit(n, layers, th, noise=noise)
+        if dm and n > 10:
+            v, g = _device_stepwise_adjoint(ctx, circ, n, True, tfim(n))
+        else:
+            v, g = oracle.adjoint(lower(circ, n, dm), tfim(n).groups())
+        assert abs(vm - v) < EXP, (n, dm, vm, v)
+        assert float(np.max(np.abs(gm - g))) < GRAD, (n, dm)

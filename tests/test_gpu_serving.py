@@ -85,3 +85,31 @@ def test_serve_poisson_small(ctx):
     assert all(r is not None for r in res)
     assert stats["p99_ms"] >= stats["p50_ms"] > 0
     srv.close()
+
+
+def test_server_memory_stays_within_budget(ctx):
+    """Fused state-vector requests of different sizes (worker contexts with
+    block caches) and engine requests under a budget that holds a few of the
+    largest at a time: the server's device memory, cached blocks included,
+    never passes the budget, every request completes, and each result equals
+    the same request served alone."""
+    specs = [("sv", 16, 1), ("sv", 14, 2), ("sv", 17, 1), ("sv", 15, 1), ("sv", 10, 2), ("sv", 17, 1),
+             ("sv", 14, 1), ("sv", 16, 2)]
+    reqs = make_requests(ctx, specs, 21)
+    budget = 40 << 20
+    alone = []
+    solo = Server(ctx, 8 << 30)
+    for tpl, pool, _ in reqs:
+        t = solo.submit(tpl, pool)
+        alone.append(solo.run_until_done([t], [tpl.lowered.nderiv])[t])
+    solo.close()
+    srv = Server(ctx, budget)
+    tickets = [srv.submit(t, p) for t, p, _ in reqs]
+    out = srv.run_until_done(tickets, [r[0].lowered.nderiv for r in reqs])
+    used, peak = srv.memory()
+    assert 0 < peak <= budget, (peak, budget)
+    assert used <= budget
+    for j, t in enumerate(tickets):
+        v, g = out[t]
+        assert abs(v - alone[j][0]) < EXP and np.max(np.abs(g - alone[j][1])) < GRAD
+    srv.close()

@@ -242,3 +242,58 @@ def test_density_adjoint_plan_checkpoint_segments():
             assert st["param_steps"] == lw.nderiv
             assert np.max(np.abs(rho - want_state)) < AMP
             assert np.max(np.abs(g - want)) < 1e-9
+
+
+def _launch_tables(ntiles):
+    import ctypes as C
+    from mossquantum_amd import _lib
+    lib = _lib.load()
+    nt = np.ascontiguousarray(ntiles, dtype=np.int32)
+    total = int(nt.sum())
+    items = np.zeros(8, dtype=np.int32)
+    ent = np.zeros(max(total, 1), dtype=np.uint32)
+    L, ne = C.c_int(0), C.c_uint64(0)
+    _lib.check(lib.mossq_batch_launch_tables(nt.size, _lib.iptr(nt), items.size, _lib.iptr(items), ent.size,
+                                             ent.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(L), C.byref(ne)))
+    return items[:L.value], ent[:ne.value]
+
+
+@pytest.mark.parametrize("m", [1, 32767, 32768, 65536, 65537, 131073])
+def test_batched_launch_tables_pack_items_and_tiles(m):
+    """The batched tile sweeps' workgroup tables (entry = item << 16 | tile,
+    unsigned): launches hold at most 65536 items, every (item, tile) of the
+    launch appears exactly once, round robin, and the item field decodes to
+    the item inside its launch at 32767, 32768 and 65536 items."""
+    rng = np.random.default_rng(m)
+    nt = rng.choice([1, 2, 16, 64], size=m, p=[0.85, 0.1, 0.04, 0.01]).astype(np.int32)
+    nt[m // 2] = 65536  # a 28-qubit item: its tile field uses all 16 bits
+    items, ent = _launch_tables(nt)
+    assert items.sum() == m and items.max() <= 65536 and len(items) == -(-m // 65536)
+    assert ent.size == int(nt.sum())
+    off, b0 = 0, 0
+    for L in items:
+        tiles = nt[b0:b0 + L]
+        e = ent[off:off + int(tiles.sum())]
+        item, tile = e >> 16, e & 0xFFFF
+        assert item.max() == L - 1 and item.min() == 0
+        # every (item, tile) once
+        key = item.astype(np.int64) * 65536 + tile
+        assert np.unique(key).size == e.size
+        assert np.all(tile < tiles[item])
+        # round robin: the first L entries are tile 0 of items 0..L-1
+        assert np.array_equal(item[:L], np.arange(L)) and not tile[:L].any()
+        off += e.size
+        b0 += L
+
+
+def test_batched_launch_tables_reject_oversized_items():
+    import ctypes as C
+    from mossquantum_amd import _lib
+    lib = _lib.load()
+    nt = np.array([65537], dtype=np.int32)  # a 29-qubit item does not fit the 16-bit tile field
+    items = np.zeros(1, dtype=np.int32)
+    ent = np.zeros(65537, dtype=np.uint32)
+    L, ne = C.c_int(0), C.c_uint64(0)
+    st = lib.mossq_batch_launch_tables(1, _lib.iptr(nt), 1, _lib.iptr(items), ent.size,
+                                       ent.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(L), C.byref(ne))
+    assert st == _lib.MOSSQ_EINVAL
