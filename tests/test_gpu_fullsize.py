@@ -71,7 +71,7 @@ def test_c2_full_density_matrix_against_oracle(ctx):
         dm = mq.DensityMatrix.zeros(n, ctx)
         mq.Program(c, n, True, ctx, fuse=fuse).run(dm.storage)
         e, tr, pur = mq.expectation(tfim(n), dm).real, mq.trace(dm), mq.purity(dm)
-        got = dm.to_host()
+        got = dm.storage.to_host()  # vec(rho), the oracle's layout
         del dm
         assert float(np.max(np.abs(got - want))) < AMP, fuse
         assert abs(e - e_want) < EXP and abs(tr - tr_want) < EXP and abs(pur - pur_want) < EXP
