@@ -120,7 +120,7 @@ int mossq_dm_apply_channel(mossq_ctx *ctx, void *rho, int n, int k, const int32_
                            const double *supermat, int dagger);
 
 /* ---- observables (QubitsOperator grouped by site tuple) ------------------ */
-/* ngroups groups; group g acts on group_k[g] (<=3) sites listed consecutively in
+/* ngroups groups; group g acts on group_k[g] (<=5) sites listed consecutively in
  * sites (0-based, increasing) with a 2^k x 2^k column-major matrix stored
  * consecutively in mats (reference _get_mat, src/backends/cpu/operators/
  * utils.jl:3-20; QubitsOperator data, quantum_circuits/operators/
@@ -131,7 +131,8 @@ int mossq_obs_destroy(mossq_obs *obs);
 
 /* <psi|H|psi> = sum_g sum_blocks v_b^dagger M_g v_b, complex result in out[2]
  * (reference _cpu_expectation(::QubitsOperator, ::StateVector),
- * src/observables/expectation.jl:12-33). */
+ * src/observables/expectation.jl:12-33, one sweep per site group; here one
+ * pass over psi for all groups). */
 int mossq_sv_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *psi, double *out);
 /* <phi|H|psi> (reference _cpu_cross_expectation, src/observables/expectation.jl:48-66). */
 int mossq_sv_cross_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *phi,

@@ -24,7 +24,7 @@ from .states import BatchDensityMatrix, BatchStateVector, DensityMatrix, HIPArra
 
 MAX_GATE_QUBITS = 5
 MAX_CHANNEL_QUBITS = 2
-MAX_OBS_SITES = 3
+MAX_OBS_SITES = 5
 
 
 class UnsupportedOperation(Exception):
@@ -147,7 +147,7 @@ class DeviceObservable:
             raise ValueError("observable must be a non-empty QubitsOperator or QubitsTerm")
         ks = np.array([len(k) for k, _ in groups], dtype=np.int32)
         if ks.max() > MAX_OBS_SITES:
-            raise ValueError("observable groups wider than 3 sites are not supported on the device")
+            raise ValueError(f"observable groups wider than {MAX_OBS_SITES} sites are not supported on the device")
         sites = np.array([p - 1 for k, _ in groups for p in k] or [0], dtype=np.int32)
         mats = np.concatenate([m.reshape(-1, order="F") for _, m in groups]).astype(np.complex128)
         self.ctx = ctx

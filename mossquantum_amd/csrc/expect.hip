@@ -58,98 +58,93 @@ int reduce_partials(Ctx *c, const c128 *part, int nparts, c128 *dev_out) {
 }
 
 // ------------------------------------------------------------ expectation ----
-// <phi|M_g|psi> summed over the blocks of group g (phi == psi for <M_g>).
-template <int K>
-__device__ c128 group_block_sum(const c128 *__restrict__ phi, const c128 *__restrict__ psi, const int *sites,
-                                const c128 *sm, u64 nbase, u64 t0, u64 step) {
-    constexpr int D = 1 << K;
-    u64 off[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        u64 o = 0;
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-            if (c & (1 << j)) o |= u64(1) << sites[j];
-        off[c] = o;
+// One pass over the state for every group of the observable.  Row i's action
+// (A psi)_i = sum_g sum_x A_g[r, r ^ x] psi[i ^ flip_g(x)] (r = local index of
+// i on g's sites) loads only the flip patterns in xmask, and every group's
+// diagonal term shares the one load of psi_i.  Each workgroup writes the
+// partial sum of conj(phi_i) (M psi)_i over its rows; with WRITE it also
+// stores out_i = (W psi)_i (W = M + M^dagger: the adjoint seed).  Group
+// records and matrices are staged in LDS when they fit.
+// (The reference sums one pass per site group, observables/expectation.jl:12-33.)
+constexpr int kObsLdsGroups = 64;
+constexpr int kObsLdsMats = 512;
+
+template <bool WRITE>
+__global__ __launch_bounds__(kB) void k_observe(const c128 *__restrict__ phi, const c128 *__restrict__ psi, u64 len,
+                                                int G, const ObsGroup *__restrict__ groups,
+                                                const c128 *__restrict__ mats, const c128 *__restrict__ wmats,
+                                                c128 *__restrict__ out, c128 *__restrict__ part) {
+    __shared__ ObsGroup s_groups[kObsLdsGroups];
+    __shared__ c128 s_m[kObsLdsMats], s_w[WRITE ? kObsLdsMats : 1];
+    const ObsGroup last = groups[G - 1];
+    const int nm = last.moff + (1 << (2 * last.k));
+    const bool staged = G <= kObsLdsGroups && nm <= kObsLdsMats;
+    if (staged) {
+        for (int j = threadIdx.x; j < G; j += kB) s_groups[j] = groups[j];
+        for (int j = threadIdx.x; j < nm; j += kB) {
+            s_m[j] = mats[j];
+            if (WRITE) s_w[j] = wmats[j];
+        }
+        __syncthreads();
     }
+    const ObsGroup *gs = staged ? s_groups : groups;
+    const c128 *M = staged ? s_m : mats;
+    const c128 *W = staged ? s_w : wmats;
     c128 acc = mk(0, 0);
-    for (u64 t = t0; t < nbase; t += step) {
-        u64 base = insert0s<K>(t, sites);
-        c128 v[D], w[D];
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            v[c] = psi[base + off[c]];
-            w[c] = phi[base + off[c]];
+    for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < len; i += u64(gridDim.x) * kB) {
+        c128 hp = mk(0, 0), dh = mk(0, 0), wp = mk(0, 0), dw = mk(0, 0);
+        for (int gi = 0; gi < G; ++gi) {
+            const ObsGroup g = gs[gi];
+            const int D = 1 << g.k;
+            const int r = group_row(g, i);
+            const c128 *m = M + g.moff + r;
+            const c128 *w = W + g.moff + r;
+            if (g.xmask & 1u) {
+                dh = cadd(dh, m[D * r]);
+                if (WRITE) dw = cadd(dw, w[D * r]);
+            }
+            for (int x = 1; x < D; ++x)
+                if ((g.xmask >> x) & 1u) {
+                    const c128 y = psi[i ^ group_flip(g, x)];
+                    hp = cfma(m[D * (r ^ x)], y, hp);
+                    if (WRITE) wp = cfma(w[D * (r ^ x)], y, wp);
+                }
         }
-#pragma unroll
-        for (int r = 0; r < D; ++r) {
-            c128 mv = mk(0, 0);
-#pragma unroll
-            for (int c = 0; c < D; ++c) mv = cfma(sm[r + D * c], v[c], mv);
-            acc = cfmaconj(w[r], mv, acc);
-        }
+        const c128 x0 = psi[i];
+        hp = cfma(dh, x0, hp);
+        acc = cfmaconj(phi[i], hp, acc);
+        if (WRITE) out[i] = cfma(dw, x0, wp);
     }
-    return acc;
+    const c128 s = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kB) void k_expect_groups(const c128 *__restrict__ phi, const c128 *__restrict__ psi,
-                                                      int n, const ObsGroup *__restrict__ groups,
-                                                      const c128 *__restrict__ mats, c128 *__restrict__ part,
-                                                      int nparts) {
-    const ObsGroup g = groups[blockIdx.y];
-    __shared__ c128 sm[64];
-    const int D = 1 << g.k;
-    for (int i = threadIdx.x; i < D * D; i += kB) sm[i] = mats[g.moff + i];
-    __syncthreads();
-    const u64 nbase = u64(1) << (n - g.k);
-    const u64 t0 = u64(blockIdx.x) * kB + threadIdx.x, step = u64(gridDim.x) * kB;
-    c128 acc;
-    switch (g.k) {
-        case 0:
-            acc = group_block_sum<0>(phi, psi, g.sites, sm, nbase, t0, step);
-            break;
-        case 1:
-            acc = group_block_sum<1>(phi, psi, g.sites, sm, nbase, t0, step);
-            break;
-        case 2:
-            acc = group_block_sum<2>(phi, psi, g.sites, sm, nbase, t0, step);
-            break;
-        default:
-            acc = group_block_sum<3>(phi, psi, g.sites, sm, nbase, t0, step);
-            break;
-    }
-    c128 s = block_sum(acc);
-    if (threadIdx.x == 0) part[size_t(blockIdx.y) * nparts + blockIdx.x] = s;
-}
-
-// Per-group sums to dev_out[G] (device), deterministic.
-int expect_groups_dev(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, c128 *dev_out, c128 *part) {
-    unsigned nb = blocks_for(u64(1) << o->n, 2, kMaxParts);
-    hipLaunchKernelGGL(k_expect_groups, dim3(nb, o->G), dim3(kB), 0, c->stream, phi, psi, o->n, o->d_groups,
-                       o->d_mats, part, int(nb));
+int observe_dev(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, c128 *seed, c128 *dev_out, c128 *part) {
+    const u64 len = u64(1) << o->n;
+    const unsigned nb = blocks_for(len, 4, kObsParts);
+    if (seed)
+        hipLaunchKernelGGL(k_observe<true>, dim3(nb), dim3(kB), 0, c->stream, phi, psi, len, o->G, o->d_groups,
+                           o->d_mats, o->d_mats_seed, seed, part);
+    else
+        hipLaunchKernelGGL(k_observe<false>, dim3(nb), dim3(kB), 0, c->stream, phi, psi, len, o->G, o->d_groups,
+                           o->d_mats, o->d_mats, nullptr, part);
     MOSSQ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_rows, dim3(o->G), dim3(kB), 0, c->stream, part, int(nb), int(nb), dev_out);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kB), 0, c->stream, part, int(nb), int(nb), dev_out);
     MOSSQ_HIP(hipGetLastError());
     return MOSSQ_OK;
 }
 
 int expect_total_host(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, double *out) {
     void *scr = nullptr;
-    u64 need = (u64(o->G) * kMaxParts + o->G) * sizeof(c128);
-    MOSSQ_TRY(get_scratch(c, need, &scr));
+    MOSSQ_TRY(get_scratch(c, (u64(kObsParts) + 1) * sizeof(c128), &scr));
     c128 *part = (c128 *)scr;
-    c128 *res = part + size_t(o->G) * kMaxParts;
-    MOSSQ_TRY(expect_groups_dev(c, o, phi, psi, res, part));
-    std::vector<c128> h(o->G);
-    MOSSQ_HIP(hipMemcpyAsync(h.data(), res, sizeof(c128) * o->G, hipMemcpyDeviceToHost, c->stream));
+    c128 *res = part + kObsParts;
+    MOSSQ_TRY(observe_dev(c, o, phi, psi, nullptr, res, part));
+    c128 h;
+    MOSSQ_HIP(hipMemcpyAsync(&h, res, sizeof(c128), hipMemcpyDeviceToHost, c->stream));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));
-    double re = 0, im = 0;
-    for (int g = 0; g < o->G; ++g) {
-        re += h[g].re;
-        im += h[g].im;
-    }
-    out[0] = re;
-    out[1] = im;
+    out[0] = h.re;
+    out[1] = h.im;
     return MOSSQ_OK;
 }
 
@@ -348,9 +343,9 @@ int mossq_obs_create(mossq_ctx *ctx, int n, int ngroups, const int32_t *group_k,
     for (int g = 0; g < ngroups; ++g) {
         ObsGroup gr{};
         gr.k = group_k[g];
-        if (gr.k < 0 || gr.k > 3 || gr.k > n) {
+        if (gr.k < 0 || gr.k > kMaxObsSites || gr.k > n) {
             delete o;
-            return fail(MOSSQ_EINVAL, "observable group size must be 0..3 and <= n");
+            return fail(MOSSQ_EUNSUPPORTED, "observable group size must be 0..5 and <= n");
         }
         for (int j = 0; j < gr.k; ++j) {
             gr.sites[j] = sites[so + j];
@@ -366,7 +361,7 @@ int mossq_obs_create(mossq_ctx *ctx, int n, int ngroups, const int32_t *group_k,
         for (int r = 0; r < D; ++r)
             for (int x = 0; x < D; ++x) {
                 const c128 v = M[mo + r + D * (r ^ x)];
-                if (v.re != 0.0 || v.im != 0.0) gr.xmask |= 1 << x;
+                if (v.re != 0.0 || v.im != 0.0) gr.xmask |= 1u << x;
             }
         for (int i = 0; i < D * D; ++i) o->h_mats.push_back(M[mo + i]);
         // Adjoint seed matrix M + M^dagger, column-major: the seed of a real

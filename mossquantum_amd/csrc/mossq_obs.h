@@ -5,23 +5,25 @@
 
 namespace mossq {
 
+constexpr int kMaxObsSites = 5;  // reference LARGEST_SUPPORTED_NTERMS, backends/cpu/operators/generic.jl:1
+
 struct ObsGroup {
-    int k;         // number of sites (0..3)
-    int sites[3];  // increasing 0-based sites
-    int moff;      // offset of the 2^k x 2^k column-major matrix
-    int xmask;     // bit x set: M[r, r ^ x] != 0 for some local row r
+    int k;                      // number of sites (0..5)
+    int sites[kMaxObsSites];    // increasing 0-based sites
+    int moff;                   // offset of the 2^k x 2^k column-major matrix
+    unsigned xmask;             // bit x set: M[r, r ^ x] != 0 for some local row r
 };
 
 // Row-wise action of a group sum, out_i = sum_g sum_x A_g[r, r ^ x] psi[i ^ flip_g(x)]
 // with r the local index of i on g's sites: only the flip patterns x in
 // xmask load, and every group's x = 0 (diagonal) term shares the one load of
 // psi[i].
-// (Loops over the three site slots with constant indices: a dynamic index
-// into the record would put it in scratch memory.)
+// (Loops over the site slots with constant indices: a dynamic index into the
+// record would put it in scratch memory.)
 __host__ __device__ __forceinline__ u64 group_flip(const ObsGroup &g, int x) {
     u64 o = 0;
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < kMaxObsSites; ++j)
         if (j < g.k && (x & (1 << j))) o |= u64(1) << g.sites[j];
     return o;
 }
@@ -29,7 +31,7 @@ __host__ __device__ __forceinline__ u64 group_flip(const ObsGroup &g, int x) {
 __host__ __device__ __forceinline__ int group_row(const ObsGroup &g, u64 i) {
     int r = 0;
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < kMaxObsSites; ++j)
         if (j < g.k) r |= int((i >> g.sites[j]) & 1) << j;
     return r;
 }
@@ -52,9 +54,13 @@ struct Obs {
     c128 *d_mats_adj = nullptr;
 };
 
-// Per-group <phi|M_g|psi> to dev_out[G]; part needs G * 512 entries.
-int expect_groups_dev(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, c128 *dev_out, c128 *part);
-// sum_g <phi|M_g|psi> copied to the host (out[0] re, out[1] im).
+// Partial sums the one-pass observation kernels write (at most).
+constexpr int kObsParts = 2048;
+// One pass over psi for every group: dev_out[0] = <phi|H|psi> = sum_i
+// conj(phi_i) (H psi)_i, and when seed != nullptr also seed = (H + H^dagger)
+// psi (the pure-state adjoint seed); part needs kObsParts entries.
+int observe_dev(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, c128 *seed, c128 *dev_out, c128 *part);
+// <phi|H|psi> copied to the host (out[0] re, out[1] im).
 int expect_total_host(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, double *out);
 // out = sum_g A_g psi on a 2^nbits vector with A_g = M_g, M_g + M_g^dagger or M_g^dagger.
 enum class OpMats { Plain, Seed, Adjoint };

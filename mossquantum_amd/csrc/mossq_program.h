@@ -124,8 +124,9 @@ int batch_sweeps_run(Ctx *c, const BatchSweeps &bs, bool adjoint);
 int zero_fill(Ctx *c, c128 *v, u64 count);
 
 // Pure-state adjoint enqueued on the stream, no host synchronisation.
-// in == nullptr starts from |0...0>.  Buffers a, b hold 2^n amplitudes.
-// value_dev receives G per-group partial values; grads_dev nderiv complex.
+// in == nullptr starts from |0...0>.  Buffers a, b hold 2^n amplitudes; part
+// holds max(kObsParts, nderiv * adjoint_part_row) entries.  value_dev receives
+// <H> (one complex); grads_dev nderiv complex.
 struct AdjointScratch {
     c128 *a = nullptr, *b = nullptr, *part = nullptr, *value_dev = nullptr, *grads_dev = nullptr;
 };

@@ -130,9 +130,8 @@ int adjoint_pure_enqueue(Ctx *c, const Program *p, const Obs *o, const c128 *in,
         MOSSQ_TRY(run_steps(c, p, s.a, 0, int(p->steps.size())));
     if (out_state)
         MOSSQ_HIP(hipMemcpyAsync(out_state, s.a, len * sizeof(c128), hipMemcpyDeviceToDevice, c->stream));
-    // value <H> and seed (H + H^dagger) y (differentiation/expectation.jl:32-35)
-    MOSSQ_TRY(expect_groups_dev(c, o, s.a, s.a, s.value_dev, s.part));
-    MOSSQ_TRY(apply_op_dev(c, o, s.a, s.b, p->n, OpMats::Seed));
+    // value <H> and seed (H + H^dagger) y in one pass (differentiation/expectation.jl:32-35)
+    MOSSQ_TRY(observe_dev(c, o, s.a, s.a, s.b, s.value_dev, s.part));
     // reverse: per instruction  lambda <- U^dag lambda, y <- U^dag y,
     // g = Re <y|D|lambda>  (differentiation/circuit.jl:136-144); fused into
     // LDS-tile sweeps when the program has an adjoint plan.
@@ -146,10 +145,11 @@ int adjoint_pure_enqueue(Ctx *c, const Program *p, const Obs *o, const c128 *in,
     return reduce_rows(c, s.part, p->nderiv, P, s.grads_dev);
 }
 
-static int copy_results(Ctx *c, const Program *p, const Obs *o, const c128 *value_dev, const c128 *grads_dev,
+// value = Re of the sum of nvalues device partial values; gradients copied out.
+static int copy_results(Ctx *c, const Program *p, int nvalues, const c128 *value_dev, const c128 *grads_dev,
                         double *value, double *grads) {
-    std::vector<c128> hv(o->G), hg(p->nderiv);
-    MOSSQ_HIP(hipMemcpyAsync(hv.data(), value_dev, sizeof(c128) * o->G, hipMemcpyDeviceToHost, c->stream));
+    std::vector<c128> hv(nvalues), hg(p->nderiv);
+    MOSSQ_HIP(hipMemcpyAsync(hv.data(), value_dev, sizeof(c128) * nvalues, hipMemcpyDeviceToHost, c->stream));
     if (p->nderiv)
         MOSSQ_HIP(hipMemcpyAsync(hg.data(), grads_dev, sizeof(c128) * p->nderiv, hipMemcpyDeviceToHost,
                                  c->stream));
@@ -168,14 +168,14 @@ int adjoint_pure(Ctx *c, const Program *p, const Obs *o, const c128 *in, c128 *o
     Allocs al(c);
     AdjointScratch s;
     const int P = adjoint_part_row(p);
-    u64 nparts = std::max<u64>(u64(o->G) * 512, u64(std::max(p->nderiv, 1)) * P);
+    u64 nparts = std::max<u64>(u64(kObsParts), u64(std::max(p->nderiv, 1)) * P);
     MOSSQ_TRY(al.get(len * sizeof(c128), (void **)&s.a));
     MOSSQ_TRY(al.get(len * sizeof(c128), (void **)&s.b));
     MOSSQ_TRY(al.get(nparts * sizeof(c128), (void **)&s.part));
-    MOSSQ_TRY(al.get(u64(o->G) * sizeof(c128), (void **)&s.value_dev));
+    MOSSQ_TRY(al.get(sizeof(c128), (void **)&s.value_dev));
     MOSSQ_TRY(al.get(u64(std::max(p->nderiv, 1)) * sizeof(c128), (void **)&s.grads_dev));
     MOSSQ_TRY(adjoint_pure_enqueue(c, p, o, in, out_state, s));
-    return copy_results(c, p, o, s.value_dev, s.grads_dev, value, grads);
+    return copy_results(c, p, 1, s.value_dev, s.grads_dev, value, grads);
 }
 
 // ---------------------------------------------------------- density adjoint ----
@@ -256,7 +256,7 @@ static int adjoint_density_fused(Ctx *c, const Program *p, const Obs *o, const c
     MOSSQ_TRY(density_adjoint_reverse(c, p, delta, ckpt.data(), stored.data(), d_tab, part, P));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));  // tab (host) is read by the copy above
     MOSSQ_TRY(reduce_rows(c, part, p->nderiv, P, grads_dev));
-    return copy_results(c, p, o, value_dev, grads_dev, value, grads);
+    return copy_results(c, p, o->G, value_dev, grads_dev, value, grads);
 }
 
 int adjoint_density(Ctx *c, const Program *p, const Obs *o, const c128 *in, int in_is_density,
@@ -319,7 +319,7 @@ int adjoint_density(Ctx *c, const Program *p, const Obs *o, const c128 *in, int 
         }
     }
     MOSSQ_TRY(reduce_rows(c, part, p->nderiv, P, grads_dev));
-    return copy_results(c, p, o, value_dev, grads_dev, value, grads);
+    return copy_results(c, p, o->G, value_dev, grads_dev, value, grads);
 }
 
 static int check_mat_range(u64 off, u64 count, u64 pool_len) {

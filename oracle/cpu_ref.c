@@ -149,7 +149,7 @@ void ref_cross_expect_groups(const cplx *phi, const cplx *psi, int n, int G, con
         int k = gk[g], d = 1 << k;
         const int *s = sites + so;
         const cplx *M = mats + mo;
-        i64 off[8];
+        i64 off[32];
         for (int c = 0; c < d; ++c) {
             i64 o = 0;
             for (int j = 0; j < k; ++j)
@@ -196,7 +196,7 @@ void ref_apply_op(cplx *out, const cplx *psi, int nbits, int G, const int *gk, c
         int k = gk[g], d = 1 << k;
         const int *s = sites + so;
         const cplx *M = mats + mo;
-        i64 off[8];
+        i64 off[32];
         for (int c = 0; c < d; ++c) {
             i64 o = 0;
             for (int j = 0; j < k; ++j)
@@ -230,7 +230,7 @@ void ref_dm_expect_groups(const cplx *rho, int n, int G, const int *gk, const in
         int k = gk[g], d = 1 << k;
         const int *s = sites + so;
         const cplx *M = mats + mo;
-        i64 off[8];
+        i64 off[32];
         for (int c = 0; c < d; ++c) {
             i64 o = 0;
             for (int j = 0; j < k; ++j)

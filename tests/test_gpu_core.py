@@ -147,6 +147,53 @@ def test_expectations(ctx):
     assert abs(mq.purity(dm) - np.real(np.vdot(rho, rho))) < EXP
 
 
+def wide_operator():
+    """Pauli strings on 4 and 5 sites (and a one-site term), one with ladder
+    operators (non-Hermitian group), coefficients real."""
+    return mq.QubitsOperator([mq.QubitsTerm({1: "Z", 2: "X", 4: "Y", 6: "Z"}, 0.7),
+                              mq.QubitsTerm({2: "+", 3: "-", 5: "X", 6: "Y", 7: "Z"}, -1.3),
+                              mq.QubitsTerm({2: "Y", 3: "Y", 5: "Z", 6: "X", 7: "X"}, 0.4),
+                              mq.QubitsTerm({3: "X"}, 0.25)])
+
+
+def test_four_and_five_site_groups(ctx):
+    """Observables with 4- and 5-site groups (the reference's largest group
+    width): expectation, cross expectation, H psi, tr(H rho), H rho, the
+    adjoint value and gradient, and the batched engine's observe step, against
+    the oracle."""
+    rng = np.random.default_rng(46)
+    H = wide_operator()
+    g = H.groups()
+    assert max(len(k) for k, _ in g) == 5
+    for n in (7, 13, 18):
+        psi, phi = rand_state(n, rng), rand_state(n, rng)
+        sv = mq.StateVector.from_host(psi)
+        assert abs(mq.expectation(H, sv) - oracle.expect(psi, n, g)) < EXP
+        assert abs(mq.cross_expectation(mq.StateVector.from_host(phi), H, sv) - oracle.cross_expect(phi, psi, n, g)) < EXP
+        assert np.max(np.abs(mq.apply(H, sv).to_host() - oracle.apply_op(psi, n, g))) < AMP
+    n = 7
+    rho = rand_rho(n, rng)
+    dm = mq.DensityMatrix.from_host(rho)
+    assert abs(mq.expectation(H, dm) - oracle.dm_expect(rho.reshape(-1, order="F"), n, g)) < EXP
+    assert np.max(np.abs(mq.apply(H, dm).to_host().reshape(-1, order="F")
+                         - oracle.apply_op(rho.reshape(-1, order="F"), 2 * n, g))) < AMP
+    # adjoint: the seed (H + H^dagger) psi of a non-Hermitian observable, fused and unfused
+    for n, fuse in ((8, False), (13, True)):
+        tpl = hea_circuit(n, 2)
+        th = rng.uniform(0, 2 * np.pi, tpl.nparameters())
+        obj = mq.ExpectationObjective(tpl, H, mq.StateVector.zeros(n, ctx), backend_=mq.HIPBackend(ctx, fuse=fuse))
+        v, gr = mq.value_and_gradient(mq.AdjointGradient(), obj, th)
+        wv, wg = oracle.adjoint(lower(tpl.bind(th), n, False), g)
+        assert abs(v - wv) < EXP and np.max(np.abs(gr - wg)) < GRAD
+    # batched engine (gate-by-gate and tiled items) with the wide observable
+    sizes = [9, 12, 14]
+    circs = [hea_circuit(n, 2, rng.uniform(0, 2 * np.pi, 4 * n)) for n in sizes]
+    vals, grads = mq.batched_adjoint(circs, [H] * 3, sizes, ctx, fuse=True)
+    for i, n in enumerate(sizes):
+        wv, wg = oracle.adjoint(lower(circs[i], n, False), g)
+        assert abs(vals[i] - wv) < EXP and np.max(np.abs(grads[i] - wg)) < GRAD
+
+
 def test_probabilities(ctx):
     rng = np.random.default_rng(12)
     n = 10

@@ -151,6 +151,35 @@ def test_expectations_vs_dense():
                                (Hd @ rho).reshape(-1, order="F"), atol=1e-13)
 
 
+def test_four_and_five_site_groups_vs_dense():
+    """Groups of 4 and 5 sites (the reference's largest, backends/cpu/operators/
+    generic.jl:1): expectation, cross expectation, H psi, tr(H rho) and H rho
+    against the full Kronecker embedding of each group matrix; random
+    (non-Hermitian) matrices and a Pauli-string operator."""
+    n = 7
+    rng = np.random.default_rng(45)
+
+    def rm(d):
+        return rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+
+    groups = [((1, 2, 4, 6), rm(16)), ((2, 3, 5, 6, 7), rm(32)), ((3,), rm(2)), ((1, 7), rm(4))]
+    Hd = sum(dense.embed(n, k, m) for k, m in groups)
+    psi, phi = rand_state(n), rand_state(n)
+    assert abs(oracle.expect(psi, n, groups) - np.vdot(psi, Hd @ psi)) < 1e-12
+    assert abs(oracle.cross_expect(phi, psi, n, groups) - np.vdot(phi, Hd @ psi)) < 1e-12
+    assert np.allclose(oracle.apply_op(psi, n, groups), Hd @ psi, atol=1e-12)
+    rho = rand_rho(n)
+    assert abs(oracle.dm_expect(rho.reshape(-1, order="F"), n, groups) - np.trace(Hd @ rho)) < 1e-12
+    assert np.allclose(oracle.apply_op(rho.reshape(-1, order="F"), 2 * n, groups),
+                       (Hd @ rho).reshape(-1, order="F"), atol=1e-12)
+    # a Pauli-string operator: QubitsOperator group matrices against matrix(n, x)
+    from mossquantum_amd import QubitsOperator, QubitsTerm
+    H = QubitsOperator([QubitsTerm({1: "Z", 2: "X", 4: "Y", 6: "Z"}, 0.7),
+                        QubitsTerm({2: "+", 3: "-", 5: "X", 6: "Y", 7: "Z"}, -1.3),
+                        QubitsTerm({2: "Y", 3: "Y", 5: "Z", 6: "X", 7: "X"}, 0.4)])
+    assert abs(oracle.expect(psi, n, H.groups()) - np.vdot(psi, H.matrix(n) @ psi)) < 1e-12
+
+
 def test_non_hermitian_group():
     n = 4
     psi = rand_state(n)
