@@ -261,6 +261,21 @@ int mossq_adjoint(mossq_ctx *ctx, const mossq_program *prog, const mossq_obs *ob
                   const void *in, int in_is_density, void *out_state, double *value,
                   double *grads);
 
+/* mossq_adjoint for a state-vector program that allocates nothing: the same
+ * kernels, with the forward state, lambda, gradient partials and sums in a
+ * 64 MiB device array the library holds for the life of the process (one
+ * per device, set up by mossq_ctx_create).  The caller passes the program,
+ * the observable, the input (or NULL for |0...0>) and the outputs.  Programs
+ * whose scratch does not fit (beyond 20 qubits, or 20 qubits with very many
+ * parameters) return MOSSQ_EINVAL with the size needed; density programs
+ * return MOSSQ_EUNSUPPORTED.  All calls on one device share the array:
+ * calls that overlap on one device race (serialise them, e.g. one context
+ * per device).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving loops, code run under stream
+ * capture, harnesses that forbid hipMalloc). */
+int mossq_adjoint_noalloc(mossq_ctx *ctx, const mossq_program *prog, const mossq_obs *obs, const void *in,
+                          void *out_state, double *value, double *grads);
+
 /* Batched adjoint gradients of B independent programs (uneven qubit counts,
  * own parameters, own observable), all started from |0...0>.  grads of item b
  * start at grads[grad_offset[b]].  Programs created with fuse=1 of 12 or more

@@ -103,6 +103,7 @@ SIGNATURES = [
     ("mossq_graph_launch", _I, [_P, _P]),
     ("mossq_graph_destroy", _I, [_P]),
     ("mossq_adjoint", _I, [_P, _P, _P, _P, _I, _P, _DP, _DP]),
+    ("mossq_adjoint_noalloc", _I, [_P, _P, _P, _P, _P, _DP, _DP]),
     ("mossq_batch_expectation", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _DP]),
     ("mossq_batch_adjoint", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _DP, _DP]),
     ("mossq_server_create", _I, [_P, _U64, C.POINTER(_P)]),

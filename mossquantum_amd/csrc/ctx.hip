@@ -293,9 +293,16 @@ int mossq_ctx_create(int device, uint64_t mem_budget_bytes, mossq_ctx **out) {
     for (int i = 0; i < kNumEvents; ++i) {
         e = hipEventCreate(&c->events[i]);
         if (e != hipSuccess) {
-            delete c;
+            mossq_ctx_destroy(reinterpret_cast<mossq_ctx *>(c));
             return fail(MOSSQ_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
         }
+    }
+    // the allocation-free adjoint's scratch address on this device
+    const int st = noalloc_scratch_init(device);
+    if (st != MOSSQ_OK) {
+        const std::string msg = mossq_last_error();
+        mossq_ctx_destroy(reinterpret_cast<mossq_ctx *>(c));
+        return fail(st, msg);
     }
     *out = reinterpret_cast<mossq_ctx *>(c);
     return MOSSQ_OK;

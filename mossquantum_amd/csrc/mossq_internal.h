@@ -152,6 +152,9 @@ int ctx_aux(Ctx *c);
 int ctx_borrow(Ctx *base, Ctx **out);
 // Releases every cached block of c (after the work queued on its streams).
 int cache_flush(Ctx *c);
+// Address of the allocation-free adjoint's device array on `device` (which
+// must be current), taken once per device at context creation (program.hip).
+int noalloc_scratch_init(int device);
 
 int ctx_check(mossq_ctx *ctx);
 Ctx *C(mossq_ctx *ctx);

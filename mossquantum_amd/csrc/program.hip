@@ -10,6 +10,7 @@
 // so every launch is one sweep of a dense or monomial block kernel.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "mossq_program.h"
 
@@ -159,6 +160,65 @@ static int copy_results(Ctx *c, const Program *p, int nvalues, const c128 *value
     if (value) *value = v;
     if (grads)
         for (int j = 0; j < p->nderiv; ++j) grads[j] = hg[j].re;
+    return MOSSQ_OK;
+}
+
+// ------------------------------------------------- allocation-free adjoint ----
+// mossq_adjoint_noalloc keeps its scratch (the forward state, lambda, the
+// gradient partials, the value and the gradient sums) in this file-scope
+// device array instead of allocating: 64 MiB per device, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (its
+// name is a host-side stand-in): set-up takes each device's address with
+// hipGetSymbolAddress, once per device with that device current, and the
+// entry point uses that address.  Every call on a device shares the array:
+// calls that overlap on one device race.
+constexpr u64 kNoAllocBytes = u64(64) << 20;
+static __device__ __align__(256) c128 scratch[kNoAllocBytes / sizeof(c128)];
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static c128 *g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch_init(int device) {
+    MOSSQ_CHECK(device >= 0 && device < kMaxDevices, MOSSQ_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return MOSSQ_OK;
+    int cur = -1;
+    MOSSQ_HIP(hipGetDevice(&cur));
+    MOSSQ_CHECK(cur == device, MOSSQ_EINVAL, "scratch set-up needs its device current");
+    void *ptr = nullptr;
+    MOSSQ_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    MOSSQ_CHECK(ptr != nullptr, MOSSQ_EDEVICE, "no device address for the allocation-free scratch");
+    g_noalloc_ptr[device] = static_cast<c128 *>(ptr);
+    return MOSSQ_OK;
+}
+
+// Carves the pure adjoint's buffers for program p out of the device's array.
+static int noalloc_layout(const Ctx *c, const Program *p, AdjointScratch *s) {
+    c128 *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        base = c->device >= 0 && c->device < kMaxDevices ? g_noalloc_ptr[c->device] : nullptr;
+    }
+    MOSSQ_CHECK(base, MOSSQ_EDEVICE, "allocation-free scratch was not set up for this device");
+    const u64 len = u64(1) << p->n;
+    const u64 nparts = std::max<u64>(u64(kObsParts), u64(std::max(p->nderiv, 1)) * adjoint_part_row(p));
+    const u64 align = 256 / sizeof(c128);
+    auto up = [&](u64 x) { return (x + align - 1) / align * align; };
+    const u64 need = up(len) * 2 + up(nparts) + up(1) + up(u64(std::max(p->nderiv, 1)));
+    MOSSQ_CHECK(need * sizeof(c128) <= kNoAllocBytes, MOSSQ_EINVAL,
+                "the allocation-free adjoint needs " + std::to_string(need * sizeof(c128)) +
+                    " bytes of scratch; it holds " + std::to_string(kNoAllocBytes) +
+                    " (state vectors up to 20 qubits); use mossq_adjoint");
+    u64 off = 0;
+    s->a = base + off;
+    off += up(len);
+    s->b = base + off;
+    off += up(len);
+    s->part = base + off;
+    off += up(nparts);
+    s->value_dev = base + off;
+    off += up(1);
+    s->grads_dev = base + off;
     return MOSSQ_OK;
 }
 
@@ -588,6 +648,22 @@ int mossq_adjoint(mossq_ctx *ctx, const mossq_program *prog, const mossq_obs *ob
         return adjoint_pure(c, p, o, (const c128 *)in, (c128 *)out_state, value, grads);
     }
     return adjoint_density(c, p, o, (const c128 *)in, in_is_density, (c128 *)out_state, value, grads);
+}
+
+int mossq_adjoint_noalloc(mossq_ctx *ctx, const mossq_program *prog, const mossq_obs *obs, const void *in,
+                          void *out_state, double *value, double *grads) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(prog && obs, MOSSQ_EINVAL, "null argument");
+    const Program *p = reinterpret_cast<const Program *>(prog);
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    Ctx *c = C(ctx);
+    MOSSQ_CHECK(p->ctx == c && o->ctx == c, MOSSQ_EINVAL, "program/observable belong to another context");
+    MOSSQ_CHECK(o->n == p->n, MOSSQ_EINVAL, "observable and program qubit counts differ");
+    MOSSQ_CHECK(!p->is_density, MOSSQ_EUNSUPPORTED, "the allocation-free adjoint takes state-vector programs");
+    AdjointScratch s;
+    MOSSQ_TRY(noalloc_layout(c, p, &s));
+    MOSSQ_TRY(adjoint_pure_enqueue(c, p, o, (const c128 *)in, (c128 *)out_state, s));
+    return copy_results(c, p, 1, s.value_dev, s.grads_dev, value, grads);
 }
 
 }  // extern "C"
