@@ -151,23 +151,21 @@ def fuse_gates(circuit: QCircuit, max_qubits: int = 5) -> QCircuit:
 
 
 def _embed(g: Gate, support):
-    pos = g.ordered_positions()
-    data = g.ordered_mat()
-    if tuple(pos) == tuple(support):
+    """g's ordered matrix acting on the sorted qubit list `support` (which
+    holds g's positions), identity on the rest, in the ordered basis of
+    `support` (its first qubit is the least significant local bit).  Built as
+    kron(I, U) over (g's qubits, then the others) and a permutation of the
+    tensor legs into support order."""
+    pos = list(g.ordered_positions())
+    data = np.asarray(g.ordered_mat(), dtype=np.complex128)
+    if pos == list(support):
         return data
-    bits = [support.index(p) for p in pos]
-    dim = 1 << len(support)
-    out = np.zeros((dim, dim), dtype=np.complex128)
-    mask = sum(1 << b for b in bits)
-    for col in range(dim):
-        lc = sum(((col >> b) & 1) << j for j, b in enumerate(bits))
-        rem = col & ~mask
-        for lr in range(1 << len(bits)):
-            row = rem
-            for j, b in enumerate(bits):
-                row |= ((lr >> j) & 1) << b
-            out[row, col] = data[lr, lc]
-    return out
+    m = len(support)
+    order = pos + [q for q in support if q not in pos]  # qubit at each bit of kron(I, U), low to high
+    T = np.kron(np.eye(1 << (m - len(pos)), dtype=np.complex128), data).reshape([2] * (2 * m))
+    # axis a of the row (and column) legs is bit m-1-a
+    axes = [m - 1 - order.index(support[m - 1 - a]) for a in range(m)]
+    return T.transpose(axes + [m + x for x in axes]).reshape(1 << m, 1 << m)
 
 
 class Lowered:

@@ -200,14 +200,62 @@ def value_and_gradient(method, objective, values, rng=None):
     return v, pm.accumulate(g)
 
 
+def _check_gradient_input(values) -> np.ndarray:
+    """Real, finite parameter values (reference _check_gradient_input,
+    api/gradient_methods.jl:64-68)."""
+    v = np.asarray(values)
+    if v.ndim != 1 or not (np.issubdtype(v.dtype, np.floating) or np.issubdtype(v.dtype, np.integer)):
+        raise TypeError("parameter values must be a real vector")
+    v = v.astype(np.float64)
+    if not np.all(np.isfinite(v)):
+        raise ValueError("parameter values must be finite.")
+    return v
+
+
+def _check_objective_value(value) -> float:
+    """A real scalar (reference _check_objective_value, gradient_methods.jl:70-73)."""
+    if isinstance(value, (complex, np.complexfloating)) or not np.isscalar(value):
+        raise ValueError("a differentiable objective must return a real scalar.")
+    return float(value)
+
+
+def _check_parameter_gradient(gradient, values) -> np.ndarray:
+    """A real vector with one entry per parameter (reference
+    _check_parameter_gradient, gradient_methods.jl:75-82)."""
+    g = np.asarray(gradient)
+    if g.ndim != 1 or np.iscomplexobj(g):
+        raise ValueError("a parameter gradient must be a real vector.")
+    if g.size != np.asarray(values).size:
+        raise ValueError(f"expected {np.asarray(values).size} parameter gradients, received {g.size}.")
+    return g.astype(np.float64)
+
+
 def value_and_pullback(method, objective, values, rng=None):
-    v, g = value_and_gradient(method, objective, values, rng)
-    saved = np.array(g, copy=True)
-    return v, (lambda seed: seed * saved)
+    """(value, back) with back(seed) = seed * gradient (reference
+    value_and_pullback, api/gradient_methods.jl:100-115): support is checked
+    first, the gradient is computed once, and back returns a new array on
+    every call, so repeated calls are independent and consume nothing."""
+    if not supports_gradient(method, objective, values):
+        raise UnsupportedGradient(method, objective)
+    x = _check_gradient_input(values)
+    v, g = value_and_gradient(method, objective, x, rng)
+    v = _check_objective_value(v)
+    saved = _check_parameter_gradient(g, x).copy()
+
+    def back(seed):
+        if isinstance(seed, (complex, np.complexfloating)) or not np.isscalar(seed):
+            raise TypeError("the pullback seed must be a real scalar")
+        return float(seed) * saved
+
+    return v, back
 
 
 def differentiable_value(method, objective, values):
-    return value_and_gradient(method, objective, values)[0]
+    """The objective's value, forward execution only (reference
+    differentiable_value, api/gradient_methods.jl:117-138); AD adapters
+    attach the reverse rule (torch_differentiable_value here)."""
+    x = _check_gradient_input(values)
+    return _check_objective_value(evaluate(objective, x))
 
 
 def batched_adjoint(circuits, observables, nqubits, ctx: _lib.Context | None = None, fuse: bool = True):
@@ -239,9 +287,17 @@ _torch_rule = None
 
 
 def torch_differentiable_value(method, objective, values):
-    """AD adapter (the analogue of the reference's ChainRulesCore rrule): a
-    torch.autograd rule whose forward is the objective value and whose backward
-    is seed * gradient from value_and_pullback.  `values` is a real torch tensor."""
+    """AD adapter, the torch.autograd counterpart of the reference's
+    ChainRulesCore rrule on differentiable_value
+    (ext/MOSSQuantumChainRulesCoreExt.jl:7-28): the forward checks the input,
+    prepares value_and_pullback once and returns the value (a 0-d tensor of
+    `values`' dtype on its device); the backward feeds the output cotangent
+    to the prepared pullback and returns seed * gradient shaped like
+    `values` (the projection onto the input).  A zero cotangent gives a zero
+    gradient without calling the pullback.  Only `values` is differentiable;
+    the method and the objective are fixed configuration.  The prepared
+    pullback is reusable, so backward passes with retain_graph are
+    independent."""
     global _torch_rule
     if _torch_rule is None:
         import torch
@@ -249,13 +305,21 @@ def torch_differentiable_value(method, objective, values):
         class _Rule(torch.autograd.Function):
             @staticmethod
             def forward(ctx, vals, m, obj):
-                v, back = value_and_pullback(m, obj, vals.detach().cpu().numpy())
+                if vals.is_complex() or not vals.is_floating_point() or vals.dim() != 1:
+                    raise TypeError("parameter values must be a real vector tensor")
+                host = vals.detach().to("cpu", torch.float64).numpy()
+                v, back = value_and_pullback(m, obj, host)
                 ctx.back = back
-                return vals.new_tensor(v)
+                return torch.tensor(v, dtype=vals.dtype, device=vals.device)
 
             @staticmethod
             def backward(ctx, seed):
-                return seed.new_tensor(ctx.back(float(seed))), None, None
+                s = float(seed.detach().to("cpu"))
+                if s == 0.0:
+                    g = np.zeros_like(ctx.back(1.0))
+                else:
+                    g = ctx.back(s)
+                return torch.as_tensor(g, dtype=seed.dtype, device=seed.device), None, None
 
         _torch_rule = _Rule
     return _torch_rule.apply(values, method, objective)

@@ -59,6 +59,8 @@ def lib():
             "ref_run_program": (None, [P, I, I, I, C.POINTER(RefOp), P]),
             "ref_adjoint": (I, [I, I, I, C.POINTER(RefOp), P, I, IP, IP, P, P, P, DP, DP]),
             "ref_probabilities": (None, [P, I, I, I, IP, DP]),
+            "ref_expect_groups_compensated": (None, [P, I, I, IP, IP, P, DP]),
+            "ref_energy_ld": (I, [I, I, C.POINTER(RefOp), P, I, IP, IP, P, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -191,3 +193,92 @@ def probabilities(state, n, is_dm, qubits1):
     out = np.zeros(1 << len(q))
     lib().ref_probabilities(_p(state), n, int(is_dm), len(q), _ip(q), _dp(out))
     return out
+
+
+def expect_compensated(psi, n, groups):
+    """<psi|H|psi> with compensated (Neumaier) sums (cpu_ref ref_expect_groups_compensated)."""
+    ks, sites, mats = groups_arrays(groups)
+    out = np.zeros(2)
+    lib().ref_expect_groups_compensated(_p(psi), n, len(ks), _ip(ks), _ip(sites), _p(mats), _dp(out))
+    return complex(out[0], out[1])
+
+
+_ROTATIONS = ("RxGate", "RyGate", "RzGate", "PHASEGate")
+
+
+def _pool_ld(lowered, slot_values):
+    """The program's matrix pool in extended precision, every rotation matrix
+    (RX, RY, RZ, PHASE) rebuilt from its angle in extended precision."""
+    pool = np.concatenate(lowered.pool).astype(np.clongdouble)
+    th = np.asarray(slot_values, dtype=np.longdouble)
+    for _, op, s0, off in lowered.param_gates:
+        kind = type(op).__name__
+        if kind not in _ROTATIONS or op.nqubits() != 1:
+            raise ValueError(f"extended-precision energies take one-qubit rotations, not {kind}")
+        t = th[s0]
+        c, s = np.cos(t / 2), np.sin(t / 2)
+        j = np.clongdouble(1j)
+        if kind == "RxGate":  # column-major [[c, -is], [-is, c]]
+            vals = (c, -j * s, -j * s, c)
+        elif kind == "RyGate":
+            vals = (c, s, -s, c)
+        elif kind == "RzGate":
+            vals = (c - j * s, 0, 0, c + j * s)
+        else:
+            vals = (1, 0, 0, np.cos(t) + j * np.sin(t))
+        for q, x in enumerate(vals):
+            pool[off + q] = x
+    return np.ascontiguousarray(pool)
+
+
+def energy_ld(lowered, slot_values, groups):
+    """Re<H> of a state-vector program from |0...0> in extended precision
+    (cpu_ref ref_energy_ld), as np.longdouble."""
+    recs, _ = _records(lowered)
+    pool = _pool_ld(lowered, slot_values)
+    ks, sites, mats = groups_arrays(groups)
+    out = np.zeros(2, dtype=np.longdouble)
+    st = lib().ref_energy_ld(lowered.n, len(lowered.records), recs, _p(pool), len(ks), _ip(ks), _ip(sites),
+                             _p(mats), _p(out))
+    if st != 0:
+        raise ValueError("extended-precision energies take gate-only state-vector programs")
+    return out[0]
+
+
+def central_fd(lowered, slot_values, groups, slots, h=1e-6, extended=True):
+    """Central finite differences (E(theta + h e_j) - E(theta - h e_j)) / 2h of
+    Re<H> for the active slots `slots` of a state-vector program from
+    |0...0> (reference test/support/finite_differences.jl:8-46 with h = 1e-6).
+    extended=True evaluates energies and angles in extended precision;
+    extended=False in double, with compensated expectation sums and the
+    quotient taken over the steps actually represented, (theta_j + h) - theta_j
+    and theta_j - (theta_j - h)."""
+    th = np.asarray(slot_values, dtype=np.float64)
+    out = np.zeros(len(slots))
+    for i, j in enumerate(slots):
+        if extended:
+            t = th.astype(np.longdouble)
+            hh = np.longdouble(h)
+            tp, tm = t.copy(), t.copy()
+            tp[j] += hh
+            tm[j] -= hh
+            out[i] = float((energy_ld(lowered, tp, groups) - energy_ld(lowered, tm, groups)) / (2 * hh))
+        else:
+            tp, tm = th.copy(), th.copy()
+            tp[j] += h
+            tm[j] -= h
+            e = []
+            for t in (tp, tm):
+                psi = np.zeros(1 << lowered.n, dtype=np.complex128)
+                psi[0] = 1
+                pool = lowered.pool_with_slots(t)
+                _run_with_pool(psi, lowered, pool)
+                e.append(expect_compensated(psi, lowered.n, groups).real)
+            out[i] = (e[0] - e[1]) / ((tp[j] - th[j]) + (th[j] - tm[j]))
+    return out
+
+
+def _run_with_pool(state, lowered, pool):
+    recs, _ = _records(lowered)
+    pool = np.ascontiguousarray(pool, dtype=np.complex128)
+    lib().ref_run_program(_p(state), lowered.n, int(lowered.is_density), len(lowered.records), recs, _p(pool))

@@ -428,3 +428,164 @@ void ref_probabilities(const cplx *s, int n, int is_dm, int k, const int *qubits
         out[m] += p;
     }
 }
+
+/* ------------------------------------------------------ finite differences ----
+ * Energies for central finite differences, the method of the reference's
+ * gradient regression tests (test/regression/gradients.jl:4-83,
+ * test/support/finite_differences.jl:8-46).  Two forms:
+ *   ref_expect_groups_compensated: <psi|H|psi> with every sum (the blocks of a
+ *     group, the threads' partials, the groups) compensated (Neumaier), so the
+ *     summation adds no error beyond the terms' own rounding;
+ *   ref_energy_ld: the whole forward run and <H> of a state-vector program in
+ *     x87 extended precision (64-bit significand), with the gate matrices
+ *     given in extended precision, so an FD quotient at h = 1e-6 keeps about
+ *     13 significant digits instead of about 7. */
+typedef struct {
+    double s, c;
+} neum;
+
+static void neum_add(neum *a, double x) {
+    double t = a->s + x;
+    if (fabs(a->s) >= fabs(x))
+        a->c += (a->s - t) + x;
+    else
+        a->c += (x - t) + a->s;
+    a->s = t;
+}
+
+void ref_expect_groups_compensated(const cplx *psi, int n, int G, const int *gk, const int *sites, const cplx *mats,
+                                   double *out) {
+    neum tre = {0, 0}, tim = {0, 0};
+    int so = 0, mo = 0;
+    for (int g = 0; g < G; ++g) {
+        int k = gk[g], d = 1 << k;
+        const int *s = sites + so;
+        const cplx *M = mats + mo;
+        i64 off[32];
+        for (int c = 0; c < d; ++c) {
+            i64 o = 0;
+            for (int j = 0; j < k; ++j)
+                if (c & (1 << j)) o |= ((i64)1) << s[j];
+            off[c] = o;
+        }
+        const i64 nb = ((i64)1) << (n - k);
+        int nt = 1;
+#ifdef _OPENMP
+        nt = omp_get_max_threads();
+#endif
+        neum *pr = (neum *)calloc((size_t)nt, sizeof(neum)), *pi = (neum *)calloc((size_t)nt, sizeof(neum));
+#pragma omp parallel num_threads(nt)
+        {
+            int tid = 0;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+#endif
+#pragma omp for schedule(static)
+            for (i64 t = 0; t < nb; ++t) {
+                i64 base = t;
+                for (int j = 0; j < k; ++j) base = insert0(base, s[j]);
+                for (int r = 0; r < d; ++r) {
+                    cplx mv = 0;
+                    for (int c = 0; c < d; ++c) mv += M[r + d * c] * psi[base + off[c]];
+                    cplx term = conj(psi[base + off[r]]) * mv;
+                    neum_add(&pr[tid], creal(term));
+                    neum_add(&pi[tid], cimag(term));
+                }
+            }
+        }
+        for (int t = 0; t < nt; ++t) {  /* fixed thread order */
+            neum_add(&tre, pr[t].s);
+            neum_add(&tre, pr[t].c);
+            neum_add(&tim, pi[t].s);
+            neum_add(&tim, pi[t].c);
+        }
+        free(pr);
+        free(pi);
+        so += k;
+        mo += d * d;
+    }
+    out[0] = tre.s + tre.c;
+    out[1] = tim.s + tim.c;
+}
+
+typedef long double complex lcplx;
+
+/* ref_apply_gate in extended precision. */
+static void ld_apply_gate(lcplx *v, int nbits, int k, const int *pos, const lcplx *U) {
+    const int d = 1 << k;
+    const i64 nblocks = ((i64)1) << (nbits - k);
+    i64 off[32];
+    for (int c = 0; c < d; ++c) {
+        i64 o = 0;
+        for (int j = 0; j < k; ++j)
+            if (c & (1 << j)) o |= ((i64)1) << pos[j];
+        off[c] = o;
+    }
+#pragma omp parallel for schedule(static) if (nblocks >= 4096)
+    for (i64 t = 0; t < nblocks; ++t) {
+        i64 base = t;
+        for (int j = 0; j < k; ++j) base = insert0(base, pos[j]);
+        lcplx in[32];
+        for (int c = 0; c < d; ++c) in[c] = v[base + off[c]];
+        for (int r = 0; r < d; ++r) {
+            lcplx acc = 0;
+            for (int c = 0; c < d; ++c) acc += U[r + d * c] * in[c];
+            v[base + off[r]] = acc;
+        }
+    }
+}
+
+/* <H> after a state-vector program started from |0...0>, in extended
+ * precision: pool holds the program's matrices as long double complex (same
+ * offsets as the double pool); mats are the group matrices (double, exact for
+ * Pauli strings with binary coefficients).  out[0], out[1] = Re, Im. */
+int ref_energy_ld(int n, int nops, const ref_op *ops, const lcplx *pool, int G, const int *gk, const int *sites,
+                  const cplx *mats, long double *out) {
+    const i64 len = ((i64)1) << n;
+    lcplx *v = (lcplx *)calloc((size_t)len, sizeof(lcplx));
+    if (!v) return 2;
+    v[0] = 1;
+    for (int i = 0; i < nops; ++i) {
+        if (ops[i].kind != 0) {
+            free(v);
+            return 1;
+        }
+        ld_apply_gate(v, n, ops[i].k, ops[i].pos, pool + ops[i].mat);
+    }
+    long double re = 0, im = 0;
+    int so = 0, mo = 0;
+    for (int g = 0; g < G; ++g) {
+        int k = gk[g], d = 1 << k;
+        const int *s = sites + so;
+        const cplx *M = mats + mo;
+        i64 off[32];
+        for (int c = 0; c < d; ++c) {
+            i64 o = 0;
+            for (int j = 0; j < k; ++j)
+                if (c & (1 << j)) o |= ((i64)1) << s[j];
+            off[c] = o;
+        }
+        const i64 nb = ((i64)1) << (n - k);
+        long double gre = 0, gim = 0;
+#pragma omp parallel for schedule(static) reduction(+ : gre, gim) if (nb >= 4096)
+        for (i64 t = 0; t < nb; ++t) {
+            i64 base = t;
+            for (int j = 0; j < k; ++j) base = insert0(base, s[j]);
+            for (int r = 0; r < d; ++r) {
+                lcplx mv = 0;
+                for (int c = 0; c < d; ++c) mv += (lcplx)M[r + d * c] * v[base + off[c]];
+                lcplx term = conjl(v[base + off[r]]) * mv;
+                gre += creall(term);
+                gim += cimagl(term);
+            }
+        }
+        re += gre;
+        im += gim;
+        so += k;
+        mo += d * d;
+    }
+    out[0] = re;
+    out[1] = im;
+    free(v);
+    return 0;
+}

@@ -242,6 +242,38 @@ def test_adjoint_pure_vs_fd():
     assert np.max(np.abs(g - _fd(f, ang))) < 1e-8
 
 
+def test_finite_difference_energies():
+    """The FD helpers the device gradients are checked against: the
+    extended-precision energy equals the dense energy; a one-qubit RY/RZ/RX
+    chain's extended-precision central FD matches the closed-form derivative
+    to 1e-12 (RY(t) on |0> under Z: E = cos t, dE = -sin t); on the ansatz it
+    matches the oracle adjoint and, in double precision with compensated sums,
+    to within the double FD's rounding floor; compensated and plain
+    expectations agree."""
+    from mossquantum_amd import RxGate
+    from mossquantum_amd.workloads import hea_circuit
+    t = 0.83
+    Z = [((1,), np.diag([1.0, -1.0]).astype(np.complex128))]
+    lw = lower(QCircuit([RyGate(1, t, isparas=True)]), 1, False)
+    assert abs(float(oracle.energy_ld(lw, [t], Z)) - np.cos(t)) < 1e-15
+    assert abs(oracle.central_fd(lw, [t], Z, [0])[0] + np.sin(t)) < 1e-12
+    # X(rx) then Z(rz): <Z> = cos a, independent of the RZ angle
+    lw = lower(QCircuit([RxGate(1, t, isparas=True), RzGate(1, 0.4, isparas=True)]), 1, False)
+    fd = oracle.central_fd(lw, [t, 0.4], Z, [0, 1])
+    assert abs(fd[0] + np.sin(t)) < 1e-12 and abs(fd[1]) < 1e-12
+    n, L = 5, 2
+    ang = RNG.uniform(0, 2 * np.pi, 2 * n * L)
+    H = xxz_chain(n)
+    Hd = dense.operator_matrix(n, dense.xxz_terms(n))
+    lw = lower(hea_circuit(n, L, ang), n, False)
+    assert abs(float(oracle.energy_ld(lw, ang, H.groups())) - dense.hea_energy(n, L, ang, Hd)[0]) < 1e-13
+    v, g = oracle.adjoint(lw, H.groups())
+    assert np.max(np.abs(oracle.central_fd(lw, ang, H.groups(), range(len(ang))) - g)) < 1e-11
+    assert np.max(np.abs(oracle.central_fd(lw, ang, H.groups(), range(len(ang)), extended=False) - g)) < 1e-8
+    psi = rand_state(n)
+    assert abs(oracle.expect_compensated(psi, n, H.groups()) - oracle.expect(psi, n, H.groups())) < 1e-14
+
+
 def test_adjoint_density_vs_fd():
     n, L = 3, 1
     ang = RNG.uniform(0, 2 * np.pi, 2 * n * L)
