@@ -8,8 +8,7 @@ from ._lib import Context, MossqError, MossqOOM, MossqUnsupported, default_conte
 from .backend import (DeviceObservable, HIPBackend, Program, SimulationResult, UnsupportedOperation, apply, apply_,
                       backend, cross_expectation, dot, expectation, probabilities, purity, simulate, simulate_circuit,
                       supports, trace, validate_positions)
-from .capability import (CapabilityIssue, CapabilityStep, ExecutionCapability, execution_capability,
-                         execution_capability_backend, execution_capability_step)
+from .capability import CapabilityStep, execution_capability_backend, execution_capability_step
 from .channels import (AmplitudeDamping, BitFlip, BitPhaseFlip, Depolarizing, GeneralizedAmplitudeDamping,
                        MixedUnitary, PauliChannel, PhaseDamping, PhaseFlip, QuantumMap, ResetChannel,
                        ThermalRelaxation)
@@ -20,7 +19,7 @@ from .gates import (CNOTGate, CPHASEGate, CRxGate, CRyGate, CRzGate, CZGate, FRE
 from .gradients import (AdjointGradient, BatchExpectationObjective, ExpectationObjective, ParameterMap,
                         UnsupportedGradient, batched_adjoint, differentiable_value, evaluate, supports_gradient,
                         value_and_gradient, value_and_pullback)
-from .noise import NoiseModel, NoiseRule, NoisyCircuit, ReadoutError, readout_probabilities, with_noise
+from .noise import NoiseModel, NoiseRule, NoisyCircuit, with_noise
 from .operators import QubitsOperator, QubitsTerm, tfim_chain, xxz_chain
 from .readout import counts, discrete_sample, draw_samples, sample
 from .sampled_gradients import (FiniteDifference, GradientEstimate, ParameterShift, SPSAGradient, estimate_gradient,

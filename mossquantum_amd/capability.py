@@ -1,23 +1,21 @@
-"""Static execution-capability analysis: the HIP backend's declarations and
-the walk that assembles them (reference src/api/execution_capability.jl).
-
-An analysis never touches a state's payload, never expands noise factories
-and never draws random numbers.  The HIP backend supports the `apply` and
-`simulate` modes for device-resident pure and mixed states (single or batch);
-shot execution (`execute`, `estimate`) stays with the reference's CPU
-backend.
+"""The HIP backend's two execution-capability hooks (reference
+src/api/execution_capability.jl:31-75): which execution modes it runs, and
+what each instruction does to the state representation.  The reference's own
+capability analysis walks circuits with these hooks; the declarations never
+touch a state's payload.  The backend runs the `apply` and `simulate` modes
+for device-resident pure and mixed states (single or batch); shot execution
+(`execute`, `estimate`) stays with the reference's CPU backend.
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 from .backend import MAX_CHANNEL_QUBITS, MAX_GATE_QUBITS, HIPBackend
 from .channels import QuantumMap
-from .circuit import Delay, QCircuit
+from .circuit import Delay
 from .gates import Gate
 from .states import BatchDensityMatrix, BatchStateVector, DensityMatrix, HIPArray, StateVector
 
-_MODES = ("apply", "simulate", "execute", "estimate")
 _STATUSES = ("supported", "unsupported", "unknown")
 
 
@@ -34,20 +32,6 @@ class CapabilityStep:
             raise ValueError("invalid capability status")
         if (self.status == "supported") != bool(self.state_types):
             raise ValueError("only supported steps have resulting state types")
-
-
-@dataclass
-class CapabilityIssue:
-    status: str
-    path: tuple
-    reason: str
-
-
-@dataclass
-class ExecutionCapability:
-    status: str
-    issues: list = field(default_factory=list)
-    final_state_types: list = field(default_factory=list)
 
 
 def _promoted(state_type):
@@ -95,54 +79,3 @@ def execution_capability_step(b, op, state_type, mode="simulate",
             return CapabilityStep("unsupported", reason=f"channels act on at most {MAX_CHANNEL_QUBITS} qubits")
         return CapabilityStep("supported", (_promoted(state_type),))
     return CapabilityStep("unsupported", reason=f"{type(op).__name__} is not supported in {mode}")
-
-
-def _walk(issues, b, circuit, types, mode, method, n, path):
-    for index, op in enumerate(circuit, start=1):
-        itempath = path + (index,)
-        if isinstance(op, QCircuit):
-            types = _walk(issues, b, op, types, mode, method, n, itempath)
-        else:
-            if max(op.positions()) > n:
-                issues.append(CapabilityIssue("unsupported", itempath, f"position outside a {n}-qubit register"))
-                return []
-            nxt = []
-            for T in types:
-                step = execution_capability_step(b, op, T, mode, method)
-                if step.status == "supported":
-                    nxt.extend(t for t in step.state_types if t not in nxt)
-                else:
-                    issues.append(CapabilityIssue(step.status, itempath, step.reason or "instruction capability not declared"))
-            types = nxt
-        if not types:
-            return types
-    return types
-
-
-def execution_capability(b, circuit, state, mode="simulate", method="density_matrix_evolution",
-                         noise_model=None) -> ExecutionCapability:
-    """Static assessment of running `circuit` on `state` (reference
-    execution_capability, execution_capability.jl:123-282, for the modes and
-    options this backend has).  Noise models are not expanded here and make
-    the report unknown, as in the reference."""
-    from .noise import NoisyCircuit
-    if mode not in _MODES:
-        raise ValueError("invalid execution capability mode")
-    if isinstance(circuit, NoisyCircuit):
-        if mode == "apply":
-            return ExecutionCapability("unsupported", [CapabilityIssue("unsupported", (),
-                                                                       "NoisyCircuit has no apply entry point")])
-        circuit = circuit.circuit
-    issues: list = []
-    modecap = execution_capability_backend(b, state, mode, method)
-    if modecap.status != "supported":
-        issues.append(CapabilityIssue(modecap.status, (), modecap.reason))
-    if noise_model is not None:
-        issues.append(CapabilityIssue("unknown", (), "noise models are not expanded during capability queries"))
-        status = "unsupported" if any(i.status == "unsupported" for i in issues) else "unknown"
-        return ExecutionCapability(status, issues, [])
-    initial = list(modecap.state_types) if modecap.status == "supported" else [type(state)]
-    types = _walk(issues, b, circuit, initial, mode, method, state.n, ())
-    status = ("unsupported" if any(i.status == "unsupported" for i in issues)
-              else "unknown" if any(i.status == "unknown" for i in issues) else "supported")
-    return ExecutionCapability(status, issues, types)

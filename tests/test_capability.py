@@ -1,11 +1,10 @@
-"""Static capability declarations of the HIP backend (reference
-src/api/execution_capability.jl).  No device needed: the analysis only
-inspects types and instruction shapes."""
+"""The HIP backend's capability hooks (reference
+src/api/execution_capability.jl:31-75).  No device needed: the declarations
+only inspect types and instruction shapes."""
 import numpy as np
 import pytest
 
 import mossquantum_amd as mq
-from mossquantum_amd.capability import _walk
 
 
 def test_step_declarations():
@@ -28,18 +27,8 @@ def test_step_declarations():
         mq.CapabilityStep("maybe", (mq.StateVector,))
 
 
-def test_walk_paths_and_promotion():
-    b = mq.HIPBackend.__new__(mq.HIPBackend)
-    big = mq.QuantumGate(tuple(range(1, 7)), np.eye(64))
-    c = mq.QCircuit([mq.HGate(1), mq.QCircuit([mq.CNOTGate(1, 2), mq.BitFlip(2, 0.1)]), mq.RyGate(3, 0.2)])
-    issues = []
-    assert _walk(issues, b, c, [mq.StateVector], "simulate", "density_matrix_evolution", 3, ()) == [mq.DensityMatrix]
-    assert issues == []
-    c2 = mq.QCircuit([mq.HGate(1), mq.QCircuit([mq.XGate(1), big])])
-    issues = []
-    assert _walk(issues, b, c2, [mq.StateVector], "simulate", "density_matrix_evolution", 6, ()) == []
-    assert issues[0].status == "unsupported" and issues[0].path == (2, 2)
-    issues = []
-    assert _walk(issues, b, mq.QCircuit([mq.HGate(4)]), [mq.StateVector], "apply", "density_matrix_evolution", 3,
-                 ()) == []
-    assert issues[0].path == (1,)
+def test_backend_modes():
+    ctx_free = mq.HIPBackend.__new__(mq.HIPBackend)
+    host_state = type("HostState", (), {"storage": np.zeros(4)})()
+    assert mq.execution_capability_backend(ctx_free, host_state).status == "unsupported"
+    assert mq.execution_capability_backend(object(), host_state).status == "unknown"

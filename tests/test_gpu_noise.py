@@ -1,6 +1,6 @@
 """Noise models executed on the device: with_noise programs through the
-fused LDS-tile planner, readout errors on device marginals, idle relaxation
-(reference test/unit/noise_execution.jl:1-62)."""
+fused LDS-tile planner, idle relaxation (reference
+test/unit/noise_execution.jl:1-62)."""
 import numpy as np
 import pytest
 
@@ -50,27 +50,6 @@ def test_noisy_template_gradient(ctx):
     assert np.max(np.abs(g - lw.slot_gradients(want_g))) < GRAD
 
 
-def test_readout_probabilities_on_device(ctx):
-    zero = mq.StateVector.zeros(2)
-    err = mq.ReadoutError(1, p0_to_1=0.2, p1_to_0=0.4)
-    assert np.allclose(mq.readout_probabilities(zero, err, qubits=[1]), [0.8, 0.2], atol=1e-15)
-    assert mq.probabilities(zero).tolist() == [1, 0, 0, 0]
-    m = np.zeros((4, 4))
-    for actual in range(4):
-        m[actual, ((actual >> 1) & 1) * 3] = 1
-    corr = mq.ReadoutError((1, 2), m)
-    x2 = mq.apply_(mq.XGate(2), mq.StateVector.zeros(2))
-    assert mq.readout_probabilities(x2, corr, qubits=[1]).tolist() == [0, 1]
-    assert mq.readout_probabilities(x2, corr, qubits=[2, 1]).tolist() == [0, 0, 0, 1]
-    assert mq.readout_probabilities(x2, corr, qubits=[]).tolist() == [1]
-    h = mq.apply_(mq.HGate(1), mq.StateVector.zeros(1))
-    flip = mq.ReadoutError(1, p0_to_1=1, p1_to_0=1)
-    s = mq.apply_(mq.RyGate(1, 2 * np.arccos(np.sqrt(0.25))), mq.StateVector.zeros(1))
-    assert np.allclose(mq.readout_probabilities(s, flip), [0.75, 0.25], atol=1e-14)
-    batch = mq.BatchStateVector([h, mq.StateVector.zeros(1)])
-    assert mq.readout_probabilities(batch, flip).shape == (2, 2)
-
-
 def test_idle_relaxation(ctx):
     timed = mq.NoiseModel(rules=[mq.NoiseRule(mq.Delay, lambda g: mq.ThermalRelaxation(g.position, 1, 2,
                                                                                         g.duration))])
@@ -86,13 +65,10 @@ def test_execution_capability_and_simulate(ctx):
     b = mq.HIPBackend(ctx)
     sv = mq.StateVector.zeros(3, ctx)
     c = mq.QCircuit([mq.HGate(1), mq.CNOTGate(1, 2)])
-    rep = mq.execution_capability(b, c, sv)
-    assert rep.status == "supported" and rep.final_state_types == [mq.StateVector]
-    noisy = mq.QCircuit([mq.HGate(1), mq.AmplitudeDamping(1, 0.3)])
-    assert mq.execution_capability(b, noisy, sv).final_state_types == [mq.DensityMatrix]
-    assert mq.execution_capability(b, c, sv, mode="execute").status == "unsupported"
+    step = mq.execution_capability_backend(b, sv, "simulate")
+    assert step.status == "supported" and step.state_types == (mq.StateVector,)
+    assert mq.execution_capability_backend(b, sv, "execute").status == "unsupported"
     model = mq.NoiseModel(rules=[mq.NoiseRule(mq.HGate, mq.BitFlip(1, 0.1))])
-    assert mq.execution_capability(b, c, sv, noise_model=model).status == "unknown"
     res = mq.simulate(c, sv, b, noise_model=model)
     assert isinstance(res, mq.SimulationResult) and isinstance(res.state, mq.DensityMatrix)
     assert res.measurements == []

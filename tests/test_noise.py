@@ -1,13 +1,12 @@
-"""Noise models (with_noise expansion, readout errors) on the host, following
-the reference's noise tests (test/unit/noise_execution.jl:1-62,
-test/unit/noise_channels.jl).  No device needed."""
+"""Noise models (with_noise expansion) on the host, following the reference's
+noise tests (test/unit/noise_execution.jl:1-62, test/unit/noise_channels.jl).
+No device needed."""
 import numpy as np
 import pytest
 
 import oracle
 import mossquantum_amd as mq
 from mossquantum_amd import lower
-from mossquantum_amd.noise import apply_readout_errors, readout_register
 from mossquantum_amd.workloads import hea_circuit
 
 
@@ -29,13 +28,11 @@ def test_with_noise_matches_the_noisy_ansatz_builder():
 
 
 def test_expansion_structure():
-    model = mq.NoiseModel(rules=[mq.NoiseRule(mq.XGate, lambda g: mq.AmplitudeDamping(g.positions()[0], 1.0))],
-                          readout=mq.ReadoutError(1, p0_to_1=1, p1_to_0=1))
+    model = mq.NoiseModel(rules=[mq.NoiseRule(mq.XGate, lambda g: mq.AmplitudeDamping(g.positions()[0], 1.0))])
     c = mq.QCircuit([mq.QCircuit([mq.XGate(1)])])
     prog = mq.with_noise(c, model)
     assert len(prog.circuit) == 2 and len(c) == 1
     assert isinstance(prog.circuit.ops[0], mq.XGate) and isinstance(prog.circuit.ops[1], mq.QuantumMap)
-    assert len(prog.readout) == 1
     with pytest.raises(ValueError):
         mq.with_noise(prog, model)
     before = mq.NoiseModel(rules=[mq.NoiseRule(mq.XGate, lambda g: mq.ResetChannel(1), when="before")])
@@ -57,18 +54,15 @@ def test_expansion_structure():
 
 def test_validation():
     with pytest.raises(ValueError):
-        mq.ReadoutError(1, [[0.8, 0.1], [0.2, 0.8]])
+        mq.NoiseModel(readout=[object()])  # classical readout errors are outside this backend
     with pytest.raises(ValueError):
-        mq.ReadoutError(1, [[-0.1, 1.1], [0.2, 0.8]])
+        mq.NoiseModel(preparation=mq.QuantumMap(1, [2 * np.eye(2)]))  # not trace preserving
     with pytest.raises(ValueError):
-        mq.ReadoutError((1, 2), np.eye(4), scope="all")
-    err = mq.ReadoutError(1, p0_to_1=0.2, p1_to_0=0.4)
-    with pytest.raises(ValueError):
-        mq.NoiseModel(readout=[err, err])
-    with pytest.raises(ValueError):
-        mq.NoiseModel(preparation=mq.QuantumMap(1, [2 * np.eye(2)]))
+        mq.NoiseModel(terminal=[mq.BitFlip(1, 0.1), mq.QuantumMap(1, [np.diag([1.0, 0.5])])])
     with pytest.raises(ValueError):
         mq.NoiseRule(mq.XGate, lambda g: None, when="during")
+    with pytest.raises(ValueError):
+        mq.NoiseRule(mq.XGate, mq.BitFlip(1, 0.1), qubits=(1, 1))
     bad = mq.NoiseModel(rules=[mq.NoiseRule(lambda g: 1, lambda g: mq.BitFlip(1, 0.1))])
     with pytest.raises(ValueError):
         mq.with_noise(mq.QCircuit([mq.XGate(1)]), bad)
@@ -79,26 +73,25 @@ def test_validation():
         mq.Delay(1, -1.0)
 
 
-def test_readout_error_distributions():
-    err = mq.ReadoutError(1, p0_to_1=0.2, p1_to_0=0.4)
-    zero = np.array([1.0, 0, 0, 0])
-    exp = readout_register([err], [1], 2)
-    assert exp == [1]
-    p1 = np.array([1.0, 0.0])  # qubit-1 marginal of |00>
-    assert np.allclose(apply_readout_errors(p1, exp, [err], 1), [0.8, 0.2])
-    # correlated error copying bit 2 into bit 1; reading qubit 1 alone must
-    # marginalize the whole block
-    m = np.zeros((4, 4))
-    for actual in range(4):
-        m[actual, ((actual >> 1) & 1) * 3] = 1
-    corr = mq.ReadoutError((1, 2), m)
-    x2 = np.array([0.0, 0, 1, 0])  # |q2=1, q1=0>
-    assert readout_register([corr], [1], 2) == [1, 2]
-    assert apply_readout_errors(x2, [1, 2], [corr], 1).tolist() == [0, 1]
-    x2_21 = np.array([0.0, 1, 0, 0])  # the same state on the register ordered (2, 1)
-    assert apply_readout_errors(x2_21, [2, 1], [corr], 2).tolist() == [0, 0, 0, 1]
-    assert apply_readout_errors(x2, [1, 2], [corr], 0).tolist() == [1]
-    assert np.allclose(apply_readout_errors(zero, [1, 2], [err], 2), [0.8, 0.2, 0, 0])
+def test_rules_see_independent_copies_and_keep_order():
+    """Factories and predicates receive copies (mutating them changes neither
+    the circuit nor the expansion); rules apply in registration order; the
+    input circuit is unchanged."""
+    seen = []
+
+    def predicate(g):
+        seen.append(g)
+        g.q = 99  # a mutation of the copy
+        return True
+
+    model = mq.NoiseModel(rules=[mq.NoiseRule(predicate, lambda g: [mq.PhaseFlip(1, 0.1)]),
+                                 mq.NoiseRule(mq.Gate, lambda g: mq.BitFlip(1, 0.2))])
+    c = mq.QCircuit([mq.XGate(1)])
+    out = mq.with_noise(c, model).circuit.ops
+    assert [type(o).__name__ for o in out] == ["XGate", "QuantumMap", "QuantumMap"]
+    assert np.allclose(out[1].ordered_supermat(), mq.PhaseFlip(1, 0.1).ordered_supermat())
+    assert np.allclose(out[2].ordered_supermat(), mq.BitFlip(1, 0.2).ordered_supermat())
+    assert seen[0] is not c.ops[0] and c.ops[0].positions() == (1,)
 
 
 def test_thermal_relaxation_closed_form():
