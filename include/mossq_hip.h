@@ -140,6 +140,15 @@ int mossq_sv_cross_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void 
 /* out = H psi (reference apply(::QubitsOperator, ::StateVector),
  * src/execution/operators.jl:3-64, src/backends/cpu/operators/generic.jl:12-28). */
 int mossq_sv_apply_operator(mossq_ctx *ctx, const mossq_obs *obs, const void *psi, void *out);
+/* <psi_b|H|psi_b> for the B columns of a batch of state vectors stored
+ * contiguously ((2^n, B) column-major: column b at psi + b 2^n), one launch
+ * over all columns; out[2b], out[2b+1] = Re, Im of column b (reference
+ * batch expectation, src/observables/batch_expectation.jl:1-179,
+ * backends/cpu/expectations/batch.jl:17-273).  B <= 65535. */
+int mossq_sv_expectation_batch(mossq_ctx *ctx, const mossq_obs *obs, const void *psi, int B, double *out);
+/* tr(H rho_b) for B density matrices stored contiguously (column b at
+ * rho + b 4^n), one launch (reference observables/batch_density_expectation.jl:4-63). */
+int mossq_dm_expectation_batch(mossq_ctx *ctx, const mossq_obs *obs, const void *rho, int B, double *out);
 /* tr(H rho) (reference _cpu_expectation(::QubitsOperator, ::DensityMatrix),
  * src/observables/density_matrix_expectation.jl:12-25). */
 int mossq_dm_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *rho, double *out);
@@ -287,6 +296,16 @@ int mossq_adjoint_noalloc(mossq_ctx *ctx, const mossq_program *prog, const mossq
 int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs,
                         mossq_obs *const *obs, const int64_t *grad_offset, double *values,
                         double *grads);
+
+/* mossq_batch_adjoint with item b starting from init[b] (a device vector of
+ * 2^n_b amplitudes, not modified) instead of |0...0> (NULL init or init[b]:
+ * |0...0>).  The gradients of a batch objective over many initial states of
+ * one circuit (reference _batch_circuit_adjoint / _batch_expectation_adjoint,
+ * src/differentiation/batch.jl:35-66,128-204, one column at a time) in one
+ * call: the same program may appear in every item. */
+int mossq_batch_adjoint_init(mossq_ctx *ctx, int B, mossq_program *const *progs, mossq_obs *const *obs,
+                             const void *const *init, const int64_t *grad_offset, double *values,
+                             double *grads);
 
 /* Values Re <H_b> of B independent state-vector programs in one engine run
  * (forward and observe steps only), item b starting from init[b] (a device

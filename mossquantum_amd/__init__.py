@@ -21,7 +21,7 @@ from .gradients import (AdjointGradient, BatchExpectationObjective, ExpectationO
                         value_and_gradient, value_and_pullback)
 from .noise import NoiseModel, NoiseRule, NoisyCircuit, with_noise
 from .operators import QubitsOperator, QubitsTerm, tfim_chain, xxz_chain
-from .readout import counts, discrete_sample, draw_samples, sample
+from .readout import counts, discrete_sample, draw_samples, measure_, sample
 from .sampled_gradients import (FiniteDifference, GradientEstimate, ParameterShift, SPSAGradient, estimate_gradient,
                                 parameter_shift_rule)
 from .states import BatchDensityMatrix, BatchStateVector, DensityMatrix, HIPArray, StateVector

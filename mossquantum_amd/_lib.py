@@ -106,6 +106,10 @@ SIGNATURES = [
     ("mossq_adjoint_noalloc", _I, [_P, _P, _P, _P, _P, _DP, _DP]),
     ("mossq_batch_expectation", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _DP]),
     ("mossq_batch_adjoint", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _DP, _DP]),
+    ("mossq_batch_adjoint_init", _I,
+     [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _DP, _DP]),
+    ("mossq_sv_expectation_batch", _I, [_P, _P, _P, _I, _DP]),
+    ("mossq_dm_expectation_batch", _I, [_P, _P, _P, _I, _DP]),
     ("mossq_server_create", _I, [_P, _U64, C.POINTER(_P)]),
     ("mossq_server_destroy", _I, [_P]),
     ("mossq_server_submit", _I, [_P, _I, _I, _I, C.POINTER(OpRecord), _DP, _U64, _P, C.POINTER(C.c_int64)]),

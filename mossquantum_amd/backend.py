@@ -314,11 +314,7 @@ def apply_(op, state, b: HIPBackend | None = None):
     """reference apply!(op, state): mutate and return the resulting state."""
     b = b or backend(state)
     if isinstance(state, (BatchStateVector, BatchDensityMatrix)):
-        cols = [apply_(op, c, b) for c in state.columns]
-        if isinstance(state, BatchStateVector) and isinstance(cols[0], DensityMatrix):
-            return BatchDensityMatrix(cols)
-        state.columns = cols
-        return state
+        return _apply_batch(op, state, b)
     _require(b, op, state, "apply")
     ctx = state.storage.ctx
     if isinstance(op, Gate):
@@ -348,6 +344,45 @@ def apply_(op, state, b: HIPBackend | None = None):
             Program(QCircuit(flat), state.n, isinstance(state, DensityMatrix), ctx, b.fuse).run(state.storage)
         return state
     raise UnsupportedOperation(b, op, state, "apply")
+
+
+def _pow2_chunks(B):
+    """(first column, log2 count) of the power-of-two column runs of B columns."""
+    out, c0 = [], 0
+    for j in reversed(range(B.bit_length())):
+        if (B >> j) & 1:
+            out.append((c0, j))
+            c0 += 1 << j
+    return out
+
+
+def _apply_batch(op, batch, b):
+    """apply! on a batch (reference backends/cpu/gates/batch.jl:22-82): on a
+    (2^n, B) state-vector batch a gate or a channel-free circuit runs over each
+    power-of-two run of 2^j columns as one vector of n + j qubits (columns are
+    the high index bits, which no instruction touches), so every instruction is
+    one launch per run; density-matrix batches and channels go column by column
+    (a channel first promotes the whole batch)."""
+    _require(b, op, batch.columns[0], "apply")
+    flat = list(op.flat()) if isinstance(op, QCircuit) else [op]
+    if isinstance(batch, BatchStateVector) and any(isinstance(o, QuantumMap) for o in flat):
+        first = next(i for i, o in enumerate(flat) if isinstance(o, QuantumMap))
+        if first:
+            _apply_batch(QCircuit(flat[:first]), batch, b)
+        dm = BatchDensityMatrix.from_states(batch)
+        return _apply_batch(QCircuit(flat[first:]), dm, b)
+    if isinstance(batch, BatchDensityMatrix) or not flat or all(isinstance(o, Delay) for o in flat):
+        for c in batch.columns:
+            apply_(op, c, b)
+        return batch
+    ctx, n, L = batch.storage.ctx, batch.n, 1 << batch.n
+    for c0, j in _pow2_chunks(batch.B):
+        run = StateVector(batch.storage.view(c0 * L, L << j), n + j)
+        if isinstance(op, Gate):
+            apply_(op, run, b)
+        else:
+            Program(QCircuit(flat), n + j, False, ctx, b.fuse).run(run.storage)
+    return batch
 
 
 class SimulationResult:
@@ -392,7 +427,15 @@ def expectation(obs, state, b: HIPBackend | None = None):
     """<psi|O|psi> or tr(O rho) as a complex number; a vector for batches."""
     b = b or backend(state)
     if isinstance(state, (BatchStateVector, BatchDensityMatrix)):
-        return np.array([expectation(obs, c, b) for c in state.columns])
+        # one launch over every column (reference observables/batch_expectation.jl)
+        _require(b, obs, state.columns[0], "expectation")
+        ctx = state.storage.ctx
+        dob = DeviceObservable(obs, state.n, ctx)
+        out = np.zeros(2 * state.B)
+        fn = (ctx.lib.mossq_sv_expectation_batch if isinstance(state, BatchStateVector)
+              else ctx.lib.mossq_dm_expectation_batch)
+        check(fn(ctx.h, dob.h, state.storage.ptr, state.B, _lib.dptr(out)))
+        return out[0::2] + 1j * out[1::2]
     _require(b, obs, state, "expectation")
     ctx = state.storage.ctx
     dob = DeviceObservable(obs, state.n, ctx)
@@ -460,6 +503,17 @@ def _normalize(values, tol):
     if not abs(total - 1) <= max(tol, tol * max(abs(total), 1.0)):
         raise ValueError(f"state probabilities must sum to one; normalize the state first (sum = {total})")
     return values / total
+
+
+def _device_p0(state, position):
+    """Unnormalised probability that qubit `position` reads 0 (the device
+    marginal; reference compute_probability_zero, execution/measurement.jl:156-192)."""
+    ctx = state.storage.ctx
+    q = np.array([int(position) - 1], dtype=np.int32)
+    out = np.zeros(2)
+    check(ctx.lib.mossq_probabilities(ctx.h, state.storage.ptr, state.n, int(isinstance(state, DensityMatrix)), 1,
+                                      _lib.iptr(q), _lib.dptr(out)))
+    return float(out[0])
 
 
 def probabilities(state, qubits=None, b: HIPBackend | None = None):

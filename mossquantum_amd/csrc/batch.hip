@@ -528,6 +528,20 @@ int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs, moss
     return batch_adjoint_engine(c, B, P.data(), O.data(), grad_offset, values, grads, false, nullptr);
 }
 
+int mossq_batch_adjoint_init(mossq_ctx *ctx, int B, mossq_program *const *progs, mossq_obs *const *obs,
+                             const void *const *init, const int64_t *grad_offset, double *values, double *grads) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(B >= 0 && (B == 0 || (progs && obs && grad_offset && values && grads)), MOSSQ_EINVAL,
+                "bad batch arguments");
+    Ctx *c = C(ctx);
+    std::vector<const Program *> P;
+    std::vector<const Obs *> O;
+    MOSSQ_TRY(batch_items_check(c, B, progs, obs, P, O));
+    for (int b = 0; b < B; ++b) MOSSQ_CHECK(grad_offset[b] >= 0, MOSSQ_EINVAL, "negative gradient offset");
+    if (B == 0) return MOSSQ_OK;
+    return batch_adjoint_engine(c, B, P.data(), O.data(), grad_offset, values, grads, false, init);
+}
+
 int mossq_batch_expectation(mossq_ctx *ctx, int B, mossq_program *const *progs, mossq_obs *const *obs,
                             const void *const *init, double *values) {
     MOSSQ_TRY(ctx_check(ctx));

@@ -90,6 +90,10 @@ __global__ __launch_bounds__(kB) void k_observe(const c128 *__restrict__ phi, co
     const ObsGroup *gs = staged ? s_groups : groups;
     const c128 *M = staged ? s_m : mats;
     const c128 *W = staged ? s_w : wmats;
+    // blockIdx.y: the column of a batch (consecutive vectors of len entries)
+    phi += u64(blockIdx.y) * len;
+    psi += u64(blockIdx.y) * len;
+    if (WRITE) out += u64(blockIdx.y) * len;
     c128 acc = mk(0, 0);
     for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < len; i += u64(gridDim.x) * kB) {
         c128 hp = mk(0, 0), dh = mk(0, 0), wp = mk(0, 0), dw = mk(0, 0);
@@ -116,7 +120,7 @@ __global__ __launch_bounds__(kB) void k_observe(const c128 *__restrict__ phi, co
         if (WRITE) out[i] = cfma(dw, x0, wp);
     }
     const c128 s = block_sum(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (threadIdx.x == 0) part[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
 }
 
 int observe_dev(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, c128 *seed, c128 *dev_out, c128 *part) {
@@ -189,6 +193,7 @@ __global__ __launch_bounds__(kB) void k_dm_expect_groups(const c128 *__restrict_
     const int D = 1 << g.k;
     const u64 N = u64(1) << n;
     const u64 nbase = u64(1) << (n - g.k);
+    rho += u64(blockIdx.z) * N * N;  // the column of a batch
     c128 acc = mk(0, 0);
     for (u64 t = u64(blockIdx.x) * kB + threadIdx.x; t < nbase; t += u64(gridDim.x) * kB) {
         u64 b = insert0s_dyn(t, g.sites, g.k);
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(kB) void k_dm_expect_groups(const c128 *__restrict_
         }
     }
     c128 s = block_sum(acc);
-    if (threadIdx.x == 0) part[size_t(blockIdx.y) * nparts + blockIdx.x] = s;
+    if (threadIdx.x == 0) part[(size_t(blockIdx.z) * gridDim.y + blockIdx.y) * nparts + blockIdx.x] = s;
 }
 
 // ------------------------------------------------------------ dot / trace ----
@@ -461,6 +466,68 @@ int mossq_dm_expectation(mossq_ctx *ctx, const mossq_obs *obs, const void *rho, 
     }
     out[0] = re;
     out[1] = im;
+    return MOSSQ_OK;
+}
+
+int mossq_sv_expectation_batch(mossq_ctx *ctx, const mossq_obs *obs, const void *psi, int B, double *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(obs && (psi || B == 0) && (out || B == 0) && B >= 0 && B <= 65535, MOSSQ_EINVAL,
+                "bad batch expectation arguments");
+    if (B == 0) return MOSSQ_OK;
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    Ctx *c = C(ctx);
+    const u64 len = u64(1) << o->n;
+    // partial sums: nb per column, the batch's total bounded by 4 kObsParts
+    const unsigned nb = std::max(1u, std::min(blocks_for(len, 4, kObsParts), unsigned(4 * kObsParts / B)));
+    void *scr = nullptr;
+    MOSSQ_TRY(get_scratch(c, (u64(B) * nb + B) * sizeof(c128), &scr));
+    c128 *part = (c128 *)scr, *res = part + size_t(B) * nb;
+    const c128 *v = (const c128 *)psi;
+    hipLaunchKernelGGL(k_observe<false>, dim3(nb, B), dim3(kB), 0, c->stream, v, v, len, o->G, o->d_groups,
+                       o->d_mats, o->d_mats, nullptr, part);
+    MOSSQ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_rows, dim3(B), dim3(kB), 0, c->stream, part, int(nb), int(nb), res);
+    MOSSQ_HIP(hipGetLastError());
+    std::vector<c128> h(B);
+    MOSSQ_HIP(hipMemcpyAsync(h.data(), res, sizeof(c128) * B, hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < B; ++b) {
+        out[2 * b] = h[b].re;
+        out[2 * b + 1] = h[b].im;
+    }
+    return MOSSQ_OK;
+}
+
+int mossq_dm_expectation_batch(mossq_ctx *ctx, const mossq_obs *obs, const void *rho, int B, double *out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(obs && (rho || B == 0) && (out || B == 0) && B >= 0 && B <= 65535, MOSSQ_EINVAL,
+                "bad batch expectation arguments");
+    if (B == 0) return MOSSQ_OK;
+    const Obs *o = reinterpret_cast<const Obs *>(obs);
+    MOSSQ_CHECK(o->n <= 20, MOSSQ_EINVAL, "density matrix too large");
+    Ctx *c = C(ctx);
+    const unsigned nb = blocks_for(u64(1) << o->n, 1, 64);
+    const u64 rows = u64(B) * o->G;
+    void *scr = nullptr;
+    MOSSQ_TRY(get_scratch(c, (rows * nb + rows) * sizeof(c128), &scr));
+    c128 *part = (c128 *)scr, *res = part + rows * nb;
+    hipLaunchKernelGGL(k_dm_expect_groups, dim3(nb, o->G, B), dim3(kB), 0, c->stream, (const c128 *)rho, o->n,
+                       o->d_groups, o->d_mats, part, int(nb));
+    MOSSQ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_rows, dim3(unsigned(rows)), dim3(kB), 0, c->stream, part, int(nb), int(nb), res);
+    MOSSQ_HIP(hipGetLastError());
+    std::vector<c128> h(rows);
+    MOSSQ_HIP(hipMemcpyAsync(h.data(), res, sizeof(c128) * rows, hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < B; ++b) {
+        double re = 0, im = 0;
+        for (int g = 0; g < o->G; ++g) {
+            re += h[size_t(b) * o->G + g].re;
+            im += h[size_t(b) * o->G + g].im;
+        }
+        out[2 * b] = re;
+        out[2 * b + 1] = im;
+    }
     return MOSSQ_OK;
 }
 

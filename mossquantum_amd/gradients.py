@@ -143,6 +143,23 @@ def _adjoint_one(b: HIPBackend, prog: Program, observable, state):
     return value.value, prog.lowered.slot_gradients(grads[:nd])
 
 
+def _adjoint_columns(prog: Program, observable, batch: BatchStateVector):
+    """(values[B], slot gradients per column) of real(<obs>) for one program
+    on every column of a state-vector batch (mossq_batch_adjoint_init)."""
+    ctx = batch.storage.ctx
+    dob = DeviceObservable(observable, batch.n, ctx)
+    B, nd = batch.B, prog.lowered.nderiv
+    offs = np.arange(B, dtype=np.int64) * nd
+    vals = np.zeros(B)
+    grads = np.zeros(max(B * nd, 1))
+    ph = (C.c_void_p * B)(*([prog.h.value] * B))
+    oh = (C.c_void_p * B)(*([dob.h.value] * B))
+    ih = (C.c_void_p * B)(*[c.storage.ptr.value for c in batch.columns])
+    check(ctx.lib.mossq_batch_adjoint_init(ctx.h, B, ph, oh, ih, offs.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           _lib.dptr(vals), _lib.dptr(grads)))
+    return vals, [prog.lowered.slot_gradients(grads[o:o + nd]) for o in offs]
+
+
 def _sampled(method):
     from .sampled_gradients import FiniteDifference, ParameterShift, SPSAGradient
     return isinstance(method, (ParameterShift, FiniteDifference, SPSAGradient))
@@ -189,8 +206,21 @@ def value_and_gradient(method, objective, values, rng=None):
     b = objective.backend
     if isinstance(objective, BatchExpectationObjective):
         w = objective.sample_weights()
+        init = objective.initial_state
+        if isinstance(init, BatchStateVector) and not objective.template.has_channel():
+            # every column in one batched engine call, each from its own state
+            # (reference differentiation/batch.jl:35-66 reverses column by column);
+            # programs the engine does not take (gates on three or more qubits
+            # below the tiled size) go column by column below
+            try:
+                vals, slot_grads = _adjoint_columns(_objective_program(objective, flat, init.columns[0], b.fuse),
+                                                    objective.observable, init)
+                grad = sum(wi * pm.accumulate(g) for wi, g in zip(w, slot_grads))
+                return float(np.dot(w, vals)), grad
+            except _lib.MossqUnsupported:
+                pass
         total, grad = 0.0, np.zeros(pm.count)
-        for wi, col in zip(w, objective.initial_state.columns):
+        for wi, col in zip(w, init.columns):
             v, g = _adjoint_one(b, _objective_program(objective, flat, col, b.fuse), objective.observable, col)
             total += wi * v
             grad += wi * pm.accumulate(g)

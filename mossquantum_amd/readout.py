@@ -113,3 +113,52 @@ def counts(state, shots: int, rng=None, qubits=None):
     if draws.ndim == 2:
         return _histogram(draws)
     return [_histogram(draws[:, :, b]) for b in range(draws.shape[2])]
+
+
+def measure_(state, position: int, auto_reset: bool = True, rng=None, b=None):
+    """Mid-circuit measurement of one qubit, in place (reference
+    measure!(backend, state, position; auto_reset, rng),
+    src/execution/simulation.jl:42-51, _cpu_apply!(::QMeasure),
+    src/execution/measurement.jl:4-17,171-201).
+
+    The device sums the qubit's zero-outcome probability p0; the host draws
+    the outcome from [p0, 1 - p0] (normalised as the reference does, one
+    uniform from `rng`) and the device applies the collapse as a one-qubit
+    gate: |0><o| / sqrt(p) with auto_reset (the qubit returns to 0), else
+    |o><o| / sqrt(p).  A density matrix collapses as m rho m^dagger.
+    Returns (outcome, probability); a batch returns (outcomes, probabilities)
+    in column order, the draws taken column by column after every column's
+    distribution is known (measurement.jl:69-91)."""
+    from .backend import _normalize, _probability_tolerance, apply_, backend
+    from .gates import QuantumGate
+    from .states import DensityMatrix, StateVector
+    if isinstance(state, (BatchStateVector, BatchDensityMatrix)):
+        if not 1 <= int(position) <= state.n:
+            raise ValueError(f"qubit position must be between 1 and {state.n}")
+        from .backend import _device_p0
+        p0s = [_device_p0(c, position) for c in state.columns]
+        outs, probs = [], []
+        for c, p0 in zip(state.columns, p0s):
+            o, p = _collapse(c, position, p0, auto_reset, rng, b, apply_, QuantumGate, _normalize,
+                             _probability_tolerance())
+            outs.append(o)
+            probs.append(p)
+        return np.array(outs, dtype=np.int64), np.array(probs)
+    if not isinstance(state, (StateVector, DensityMatrix)):
+        raise TypeError("measure_ takes state vectors and density matrices")
+    if not 1 <= int(position) <= state.n:
+        raise ValueError(f"qubit position must be between 1 and {state.n}")
+    b = b or backend(state)
+    from .backend import _device_p0
+    return _collapse(state, position, _device_p0(state, position), auto_reset, rng, b, apply_, QuantumGate,
+                     _normalize, _probability_tolerance())
+
+
+def _collapse(state, position, p0, auto_reset, rng, b, apply_, QuantumGate, normalize, tol):
+    dist = normalize(np.array([p0, 1.0 - p0]), tol)
+    i = discrete_sample(rng, dist)  # 1-based
+    prob = float(dist[i - 1])
+    m = np.zeros((2, 2), dtype=np.complex128)
+    m[0 if auto_reset else i - 1, i - 1] = 1.0 / np.sqrt(prob)
+    apply_(QuantumGate((int(position),), m), state, b)
+    return i - 1, prob

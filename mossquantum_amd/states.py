@@ -26,11 +26,6 @@ class HIPArray:
         check(self.ctx.lib.mossq_alloc(self.ctx.h, self.nbytes, C.byref(p)))
         self.ptr = p
 
-    def free(self):
-        if getattr(self, "ptr", None) and self.ctx.h:
-            check(self.ctx.lib.mossq_free(self.ctx.h, self.ptr))
-        self.ptr = None
-
     def __del__(self):
         try:
             self.free()
@@ -57,6 +52,25 @@ class HIPArray:
 
     def similar(self) -> "HIPArray":
         return HIPArray(self.size, self.ctx)
+
+    def view(self, offset: int, size: int) -> "HIPArray":
+        """Entries [offset, offset + size) of this buffer, not owning them
+        (it keeps this buffer alive): the columns of a batch."""
+        if offset < 0 or size < 0 or offset + size > self.size:
+            raise ValueError("view outside the buffer")
+        v = HIPArray.__new__(HIPArray)
+        v.ctx, v.size, v.nbytes = self.ctx, int(size), int(size) * 16
+        v.ptr = C.c_void_p(self.ptr.value + 16 * int(offset))
+        v._owner = self
+        return v
+
+    def free(self):
+        if getattr(self, "_owner", None) is not None:  # a view: the owner frees
+            self.ptr = None
+            return
+        if getattr(self, "ptr", None) and self.ctx.h:
+            check(self.ctx.lib.mossq_free(self.ctx.h, self.ptr))
+        self.ptr = None
 
     def __array__(self, *a, **k):
         raise TypeError("HIPArray has no implicit host conversion; call to_host()")
@@ -135,53 +149,97 @@ class DensityMatrix:
         return self.storage.to_host().reshape((1 << self.n, 1 << self.n), order="F")
 
 
-class BatchStateVector:
-    """B pure states of one qubit count, columns of a (2^n, B) array (reference batch_state_vector.jl:18-80)."""
+class _Batch:
+    """B states of one qubit count in one device buffer, column b at entries
+    [b L, (b + 1) L) (L = 2^n amplitudes, or 4^n for density matrices): the
+    reference's (2^n, B) / (2^n, 2^n, B) storage (batch_state_vector.jl:18-30,
+    batch_density_matrix.jl:17-28).  `columns` are views of that buffer."""
 
-    def __init__(self, columns: list[StateVector]):
-        if not columns:
-            raise ValueError("a batch needs at least one state")
-        n = columns[0].n
-        if any(c.n != n for c in columns):
-            raise ValueError("batch states must share a qubit count")
-        self.columns = columns
-        self.n = n
+    _single = None  # StateVector or DensityMatrix
+    _len_bits = 1   # L = 2^(_len_bits n)
+
+    def __init__(self, columns_or_storage, n: int | None = None, B: int | None = None):
+        if isinstance(columns_or_storage, HIPArray):
+            storage = columns_or_storage
+            if n is None or B is None or B < 1 or storage.size != B << (self._len_bits * n):
+                raise ValueError("batch storage must hold B columns of the state size")
+        else:
+            cols = list(columns_or_storage)
+            if not cols:
+                raise ValueError("a batch needs at least one state")
+            n = cols[0].n
+            if any(not isinstance(c, self._single) or c.n != n for c in cols):
+                raise ValueError("batch states must share a qubit count and kind")
+            B = len(cols)
+            L = 1 << (self._len_bits * n)
+            storage = HIPArray(B * L, cols[0].storage.ctx)
+            for b, c in enumerate(cols):
+                check(storage.ctx.lib.mossq_copy_d2d(storage.ctx.h, C.c_void_p(storage.ptr.value + 16 * b * L),
+                                                     c.storage.ptr, L * 16))
+        self.storage, self.n, self.B = storage, int(n), int(B)
+        L = 1 << (self._len_bits * self.n)
+        self.columns = [self._single(storage.view(b * L, L), self.n) for b in range(self.B)]
+
+    def batch_size(self):
+        return self.B
+
+    def nqubits(self):
+        return self.n
+
+    def copy(self):
+        return type(self)(self.storage.copy(), self.n, self.B)
+
+
+class BatchStateVector(_Batch):
+    """B pure states of one qubit count, columns of one (2^n, B) device array
+    (reference batch_state_vector.jl:18-80)."""
+
+    _single = StateVector
+    _len_bits = 1
 
     @classmethod
     def zeros(cls, n, B, ctx=None):
-        return cls([StateVector.zeros(n, ctx) for _ in range(B)])
+        s = HIPArray(B << n, ctx)
+        out = cls(s, n, B)
+        for c in out.columns:
+            check(s.ctx.lib.mossq_sv_init_zero(s.ctx.h, c.storage.ptr, n))
+        return out
 
-    def batch_size(self):
-        return len(self.columns)
-
-    def copy(self):
-        return BatchStateVector([c.copy() for c in self.columns])
+    @classmethod
+    def from_host(cls, amps, ctx=None):
+        """From a (2^n, B) host array."""
+        a = np.asarray(amps, dtype=np.complex128)
+        n = int(round(np.log2(a.shape[0])))
+        return cls(HIPArray(a.size, ctx).copy_from_host(a), n, a.shape[1])
 
     def to_host(self):
-        return np.stack([c.to_host() for c in self.columns], axis=1)
+        return self.storage.to_host().reshape((1 << self.n, self.B), order="F")
 
 
-class BatchDensityMatrix:
-    """B density matrices of one qubit count (reference batch_density_matrix.jl:17-134)."""
+class BatchDensityMatrix(_Batch):
+    """B density matrices of one qubit count in one (2^n, 2^n, B) device
+    array (reference batch_density_matrix.jl:17-134)."""
 
-    def __init__(self, columns: list[DensityMatrix]):
-        if not columns:
-            raise ValueError("a batch needs at least one state")
-        n = columns[0].n
-        if any(c.n != n for c in columns):
-            raise ValueError("batch states must share a qubit count")
-        self.columns = columns
-        self.n = n
+    _single = DensityMatrix
+    _len_bits = 2
 
     @classmethod
     def zeros(cls, n, B, ctx=None):
-        return cls([DensityMatrix.zeros(n, ctx) for _ in range(B)])
+        s = HIPArray(B << (2 * n), ctx)
+        out = cls(s, n, B)
+        for c in out.columns:
+            check(s.ctx.lib.mossq_dm_init_zero(s.ctx.h, c.storage.ptr, n))
+        return out
 
-    def batch_size(self):
-        return len(self.columns)
-
-    def copy(self):
-        return BatchDensityMatrix([c.copy() for c in self.columns])
+    @classmethod
+    def from_states(cls, batch: BatchStateVector):
+        """psi_b psi_b^dagger for every column (the promotion at a batch's first channel)."""
+        s = HIPArray(batch.B << (2 * batch.n), batch.storage.ctx)
+        out = cls(s, batch.n, batch.B)
+        for c, sv in zip(out.columns, batch.columns):
+            check(s.ctx.lib.mossq_dm_from_sv(s.ctx.h, c.storage.ptr, sv.storage.ptr, batch.n))
+        return out
 
     def to_host(self):
-        return np.stack([c.to_host() for c in self.columns], axis=2)
+        N = 1 << self.n
+        return self.storage.to_host().reshape((N, N, self.B), order="F")

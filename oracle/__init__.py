@@ -71,6 +71,8 @@ def lib():
 
 
 def _p(a):
+    if not a.flags.c_contiguous:
+        raise ValueError("the oracle reads contiguous arrays (pass np.ascontiguousarray of a view)")
     return a.ctypes.data_as(C.c_void_p)
 
 

@@ -110,3 +110,62 @@ def test_validation(ctx):
         mq.probabilities(mq.DensityMatrix.from_host(np.diag([0.5j, 1.0])))
     with pytest.raises(ValueError, match="nonnegative"):
         mq.probabilities(mq.DensityMatrix.from_host(np.diag([1.1, -0.1])))
+
+
+def test_measure_collapses_like_the_reference(ctx):
+    """measure_ (reference measure!(backend, state, position),
+    execution/simulation.jl:42-51, measurement.jl:4-17): outcome drawn from
+    [p0, 1 - p0] with the given uniform, probability returned, and the state
+    collapsed by |0><o|/sqrt(p) (auto_reset) or |o><o|/sqrt(p); the same
+    uniforms through the oracle's sampler and a host collapse give the same
+    outcome and state.  Density matrices, batches and a bad position too."""
+    rng = np.random.default_rng(81)
+    n, pos = 7, 3
+    for auto_reset in (True, False):
+        for seed in (1, 2, 3):
+            psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+            psi /= np.linalg.norm(psi)
+            sv = mq.StateVector.from_host(psi, ctx)
+            o, p = mq.measure_(sv, pos, auto_reset=auto_reset, rng=np.random.default_rng(seed))
+            u = np.random.default_rng(seed).random()
+            bit = (np.arange(1 << n) >> (pos - 1)) & 1
+            p0 = float(np.sum(np.abs(psi[bit == 0]) ** 2))
+            i = ref.discrete_sample(u, [p0, 1 - p0])
+            assert o == i - 1 and abs(p - [p0, 1 - p0][i - 1]) < 1e-13
+            m = np.zeros((2, 2), dtype=np.complex128)
+            m[0 if auto_reset else o, o] = 1 / np.sqrt(p)
+            want = psi.copy()
+            oracle.apply_gate(want, n, [pos - 1], m)
+            assert np.max(np.abs(sv.to_host() - want)) < 1e-12
+            got = mq.probabilities(sv, [pos])
+            assert abs(got[0 if auto_reset else o] - 1) < 1e-12
+    # density matrix
+    m_ = 5
+    A = rng.standard_normal((1 << m_, 1 << m_)) + 1j * rng.standard_normal((1 << m_, 1 << m_))
+    rho = A @ A.conj().T
+    rho /= np.trace(rho)
+    dm = mq.DensityMatrix.from_host(rho, ctx)
+    o, p = mq.measure_(dm, 2, auto_reset=False, rng=np.random.default_rng(5))
+    d = np.real(np.diag(rho))
+    p0 = float(np.sum(d[((np.arange(1 << m_) >> 1) & 1) == 0]))
+    i = ref.discrete_sample(np.random.default_rng(5).random(), [p0, 1 - p0])
+    assert o == i - 1 and abs(p - [p0, 1 - p0][i - 1]) < 1e-13
+    mm = np.zeros((2, 2), dtype=np.complex128)
+    mm[o, o] = 1 / np.sqrt(p)
+    want = rho.reshape(-1, order="F").copy()
+    oracle.dm_apply_gate(want, m_, [1], mm)
+    assert np.max(np.abs(dm.to_host().reshape(-1, order="F") - want)) < 1e-12
+    assert abs(mq.trace(dm) - 1) < 1e-12
+    # batch: one draw per column, in column order
+    cols = [mq.apply_(mq.RyGate(1, t), mq.StateVector.zeros(2, ctx)) for t in (0.3, 2.9, np.pi)]
+    batch = mq.BatchStateVector(cols)  # one buffer, the columns copied in
+    cols = batch.columns
+    outs, probs = mq.measure_(batch, 1, rng=np.random.default_rng(9))
+    us = np.random.default_rng(9).random(3)
+    for j, (t, u) in enumerate(zip((0.3, 2.9, np.pi), us)):
+        p0 = np.cos(t / 2) ** 2
+        i = ref.discrete_sample(u, [p0, 1 - p0])
+        assert outs[j] == i - 1 and abs(probs[j] - [p0, 1 - p0][i - 1]) < 1e-12
+        assert abs(mq.probabilities(cols[j], [1])[0] - 1) < 1e-12  # reset to 0
+    with pytest.raises(ValueError):
+        mq.measure_(mq.StateVector.zeros(2, ctx), 3)
