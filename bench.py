@@ -74,13 +74,14 @@ def time_program(ctx, prog, sv, steps, warmup, s0, s1):
     return ctx.elapsed_ms(s0, s1)
 
 
-def cpu_baseline(circ, n, budget_s=12.0):
-    """cpu_ref on the host cores: the circuit's gates in order on a 2^n host
-    vector until the time budget is spent; gates/s over the gates run."""
+def cpu_baseline(circ, n, budget_s=12.0, is_dm=False):
+    """cpu_ref on the host cores: the circuit's instructions in order on a
+    host vector (2^n amplitudes, or 4^n entries for a density matrix) until
+    the time budget is spent; returns (instructions run, seconds, threads)."""
     import oracle
     from mossquantum_amd import lower
-    lw = lower(circ, n, False)
-    psi = np.zeros(1 << n, dtype=np.complex128)
+    lw = lower(circ, n, is_dm)
+    psi = np.zeros(1 << (2 * n if is_dm else n), dtype=np.complex128)
     psi[0] = 1
     recs, pool = oracle._records(lw)
     lib = oracle.lib()
@@ -89,10 +90,48 @@ def cpu_baseline(circ, n, budget_s=12.0):
     one = (oracle.RefOp * 1)()
     while done < len(lw.records) and time.perf_counter() - t0 < budget_s:
         one[0] = recs[done]
-        lib.ref_run_program(oracle._p(psi), n, 0, 1, one, oracle._p(pool))
+        lib.ref_run_program(oracle._p(psi), n, int(is_dm), 1, one, oracle._p(pool))
         done += 1
     dt = time.perf_counter() - t0
     return done, dt, oracle.num_threads()
+
+
+def cpu_baselines(circ_c1):
+    """The CPU oracle (cpu_ref) on every configuration, on this host's cores
+    and on one thread where stated; bounded samples, each named."""
+    import oracle
+    from mossquantum_amd import lower
+    from mossquantum_amd.workloads import hea_angles, hea_circuit, tfim
+    cores = oracle.num_threads()
+    out = {"cores": cores, "kind": "port"}
+    # C1 on one thread (the all-core figure is the headline's cpu_baseline)
+    oracle.set_threads(1)
+    try:
+        done, dt, _ = cpu_baseline(circ_c1, N, budget_s=6.0)
+        out["c1_26q_one_thread"] = {"gates_per_s": done / dt, "sample": f"first {done} gates, {dt:.1f} s"}
+        # C0: value and full adjoint gradient, 12 qubits, 10 layers
+        n, L = 12, 10
+        th = hea_angles(n, L, 20260012)
+        lw = lower(hea_circuit(n, L, th), n, False)
+        t0 = time.perf_counter()
+        oracle.adjoint(lw, tfim(n).groups())
+        one = time.perf_counter() - t0
+    finally:
+        oracle.set_threads(cores)
+    t0 = time.perf_counter()
+    oracle.adjoint(lw, tfim(n).groups())
+    out["c0_12q_adjoint_ms"] = {"one_thread": one * 1e3, "all_cores": (time.perf_counter() - t0) * 1e3}
+    # C2: the 13-qubit noisy density matrix, instructions in order for a bounded time
+    n, L = 13, 8
+    c2 = hea_circuit(n, L, hea_angles(n, L, 20260213), noise=(0.01, 0.02))
+    done, dt, _ = cpu_baseline(c2, n, budget_s=8.0, is_dm=True)
+    nops = len(list(c2.flat()))
+    out["c2_13q_density"] = {"instructions_per_s": done / dt, "full_run_s_at_that_rate": nops / (done / dt),
+                             "sample": f"first {done} of {nops} instructions, {dt:.1f} s"}
+    log(f"cpu baselines: C1 one thread {out['c1_26q_one_thread']['gates_per_s']:.1f} gates/s; C0 adjoint "
+        f"{out['c0_12q_adjoint_ms']['one_thread']:.1f} / {out['c0_12q_adjoint_ms']['all_cores']:.1f} ms; "
+        f"C2 {out['c2_13q_density']['instructions_per_s']:.1f} instructions/s")
+    return out
 
 
 def batched_bytes(n, layers=6):
@@ -177,19 +216,18 @@ def bench_batched(ctx, B=512, layers=6, seed=20260300, cpu=True):
         from mossquantum_amd import lower
         # one state of every size, timed with the oracle; the batch estimate sums per-size times
         per_size = {}
-        for n in range(10, 19):
+        for n in range(10, 21):
             c = hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers))
             lw = lower(c, n, False)
             t1 = time.perf_counter()
             oracle.adjoint(lw, xxz(n).groups())
             per_size[n] = time.perf_counter() - t1
-        for n in (19, 20):  # scaled from n=18 by the work ratio 2^n (3n-1)
-            per_size[n] = per_size[18] * 2 ** (n - 18) * (3 * n - 1) / (3 * 18 - 1)
         est = sum(per_size[n] for n in sizes)
         out["cpu_estimate_s"] = est
         out["cpu_grads_per_s"] = B / est
-        out["cpu_sample"] = ("one adjoint per size n=10..18 timed with cpu_ref (OpenMP), n=19,20 scaled from "
-                             "n=18 by 2^n(3n-1); summed over the batch's sizes")
+        out["cpu_per_size_s"] = per_size
+        out["cpu_sample"] = ("one adjoint per size n=10..20 timed with cpu_ref (OpenMP, all cores); the batch "
+                             "time sums the per-size times over the batch's sizes")
         log(f"cpu estimate: {est:.1f} s for the batch")
     return out
 
@@ -320,15 +358,12 @@ def bench_density_adjoint(ctx, n=12, layers=8, seed=20260412):
     return out
 
 
-def bench_serving(ctx, rates=(50.0, 200.0, 2000.0), nreq=300, seed=20260400, budget=200 << 30):
-    """Poisson request stream: 70% pure states n ~ U{8..24}, 30% density
-    matrices n ~ U{6..12} with the noisy channels; 4-8 layers; TFIM energy +
-    adjoint gradient; 200 GB admission budget.  A bounded sample of `nreq`
-    requests per rate (the full 20k-request sweep is `--serving-requests 20000`)."""
-    from mossquantum_amd.serving import RequestTemplate, Server, serve_poisson
+def serving_mix(ctx, nreq, rng, tpls):
+    """`nreq` requests of the serving mix: 70% pure states n ~ U{8..24}, 30%
+    density matrices n ~ U{6..12} with the noisy channels, layers ~ U{4..8},
+    seeded angles; [(template, pool, angles, layers)]."""
+    from mossquantum_amd.serving import RequestTemplate
     from mossquantum_amd.workloads import hea_circuit, tfim
-    rng = np.random.default_rng(seed)
-    tpls = {}
 
     def template(kind, n, layers):
         key = (kind, n, layers)
@@ -337,21 +372,104 @@ def bench_serving(ctx, rates=(50.0, 200.0, 2000.0), nreq=300, seed=20260400, bud
             tpls[key] = RequestTemplate(hea_circuit(n, layers, noise=noise), n, kind == "dm", tfim(n), ctx)
         return tpls[key]
 
+    reqs = []
+    for _ in range(nreq):
+        dm = rng.random() < 0.3
+        n = int(rng.integers(6, 13)) if dm else int(rng.integers(8, 25))
+        layers = int(rng.integers(4, 9))
+        tpl = template("dm" if dm else "sv", n, layers)
+        th = rng.uniform(0, 2 * np.pi, tpl.nparams)
+        reqs.append((tpl, lambda tpl=tpl, th=th: tpl.bind(th), th, layers))  # pools bound when submitted
+    return reqs
+
+
+def per_circuit_path(ctx, reqs):
+    """The plain per-circuit GPU path on the same requests, one after the
+    other: lower, plan (fused) and mossq_adjoint each request; req/s."""
+    import ctypes as C
+    from mossquantum_amd import _lib
+    from mossquantum_amd.backend import Program
+    t0 = time.perf_counter()
+    for tpl, pool, _, _ in reqs:
+        prog = Program(None, tpl.n, tpl.is_density, ctx, fuse=True, lowered=tpl.lowered, pool=pool())
+        v = C.c_double()
+        g = np.zeros(max(tpl.lowered.nderiv, 1))
+        _lib.check(ctx.lib.mossq_adjoint(ctx.h, prog.h, tpl.obs.h, None, 0, None, C.byref(v), _lib.dptr(g)))
+        del prog
+    dt = time.perf_counter() - t0
+    return {"requests": len(reqs), "seconds": dt, "throughput_req_s": len(reqs) / dt}
+
+
+def cpu_serving(reqs, budget_s=40.0):
+    """cpu_ref's serial per-request loop (value and adjoint gradient) on the
+    mix: for every (kind, qubits) class in the sample one request is timed
+    (smallest first, until the budget is spent), and each request counts that
+    time scaled by its layer count (the work is linear in the layers).
+    Density requests above 10 qubits are not run: the reference's checkpoint
+    before every channel needs 4^n x 16 B x the channel count of host memory
+    (about 64 GiB at 11 qubits, 256 GiB at 12), so their times are scaled
+    from the 10-qubit class by 4^(n-10) and the instruction count."""
+    import oracle
+    from mossquantum_amd import lower
+    from mossquantum_amd.workloads import hea_circuit, tfim
+    classes = {}
+    for tpl, _, th, layers in reqs:
+        classes.setdefault((tpl.is_density, tpl.n), (th, layers))
+    times, t_all = {}, time.perf_counter()
+    for key in sorted(classes, key=lambda k: (k[1], k[0])):
+        dm, n = key
+        if (dm and n > 10) or time.perf_counter() - t_all > budget_s:
+            continue
+        th, layers = classes[key]
+        lw = lower(hea_circuit(n, layers, th, noise=(0.01, 0.02) if dm else None), n, dm)
+        t0 = time.perf_counter()
+        oracle.adjoint(lw, tfim(n).groups())
+        times[key] = (time.perf_counter() - t0) / layers  # per layer
+    total, scaled, missing = 0.0, 0, 0
+    for tpl, _, _, layers in reqs:
+        key = (tpl.is_density, tpl.n)
+        if key in times:
+            total += times[key] * layers
+        elif tpl.is_density and (True, 10) in times:
+            inst = lambda m: 11 * m - 5  # noisy ansatz instructions per layer
+            total += times[(True, 10)] * layers * 4 ** (tpl.n - 10) * inst(tpl.n) / inst(10)
+            scaled += 1
+        else:
+            missing += 1
+    counted = len(reqs) - missing
+    return {"throughput_req_s": counted / total if total else None, "requests_counted": counted,
+            "classes_timed": len(times), "density_scaled_from_10q": scaled, "cores": oracle.num_threads(),
+            "sample": f"{len(reqs)} requests of the mix; one oracle adjoint per (kind, qubits) class timed "
+                      f"and scaled by layers ({time.perf_counter() - t_all:.1f} s)"}
+
+
+def bench_serving(ctx, rates=(50.0, 200.0, 2000.0), nreq=300, seed=20260400, budget=200 << 30, cpu=True):
+    """Poisson request stream: 70% pure states n ~ U{8..24}, 30% density
+    matrices n ~ U{6..12} with the noisy channels; 4-8 layers; TFIM energy +
+    adjoint gradient; 200 GB admission budget.  `nreq` requests per rate; the
+    plain per-circuit GPU path and the CPU oracle's serial loop run on a
+    sample of the same mix."""
+    from mossquantum_amd.serving import Server, serve_poisson
+    rng = np.random.default_rng(seed)
+    tpls = {}
     srv = Server(ctx, budget)
     out = {"requests_per_rate": nreq, "mix": "70% SV n~U{8..24}, 30% DM n~U{6..12}, layers~U{4..8}",
            "budget_bytes": budget, "rates": []}
     for rate in rates:
-        reqs = []
-        for _ in range(nreq):
-            dm = rng.random() < 0.3
-            n = int(rng.integers(6, 13)) if dm else int(rng.integers(8, 25))
-            tpl = template("dm" if dm else "sv", n, int(rng.integers(4, 9)))
-            reqs.append((tpl, tpl.bind(rng.uniform(0, 2 * np.pi, tpl.nparams))))
-        _, stats = serve_poisson(srv, reqs, rate, seed=int(rate))
+        reqs = serving_mix(ctx, nreq, rng, tpls)
+        log(f"serving at {rate:.0f} req/s: {nreq} requests")
+        _, stats = serve_poisson(srv, [(t, p) for t, p, _, _ in reqs], rate, seed=int(rate), progress=log)
         log(f"serving at {rate:.0f} req/s: {stats['throughput_req_s']:.1f} req/s, "
             f"p50 {stats['p50_ms']:.1f} ms, p99 {stats['p99_ms']:.1f} ms")
         out["rates"].append(stats)
+    out["peak_device_bytes"] = srv.memory()[1]
     srv.close()
+    sample = serving_mix(ctx, min(nreq, 300), np.random.default_rng(seed + 1), tpls)
+    out["per_circuit_gpu"] = per_circuit_path(ctx, sample)
+    log(f"serving mix through the per-circuit GPU path: {out['per_circuit_gpu']['throughput_req_s']:.1f} req/s")
+    if cpu:
+        out["cpu_serial"] = cpu_serving(sample)
+        log(f"serving mix through the CPU oracle, serially: {out['cpu_serial']['throughput_req_s']:.2f} req/s")
     return out
 
 
@@ -366,6 +484,8 @@ def main():
     ap.add_argument("--no-density", action="store_true", help="skip the 13-qubit density-matrix leg")
     ap.add_argument("--serving-requests", type=int, default=300,
                     help="requests per arrival rate in the serving leg (0 skips it)")
+    ap.add_argument("--serving-rates", default="50,200,2000",
+                    help="arrival rates (req/s) of the serving leg, comma separated")
     ap.add_argument("--no-autotune", action="store_true", help="keep the planner's cost-model plan for the headline")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the fused 26-qubit headline (every other leg skipped), e.g. under a profiler")
@@ -385,11 +505,13 @@ def main():
     import mossquantum_amd as mq
     from mossquantum_amd.workloads import haar_layers, random_circuit
 
+    rates = tuple(float(r) for r in args.serving_rates.split(","))
     if args.leg:
         ctx = mq.default_context()
         legs = {"c0": lambda: bench_c0(ctx), "batched": lambda: bench_batched(ctx, cpu=not args.no_cpu),
                 "density": lambda: bench_density(ctx), "density-adjoint": lambda: bench_density_adjoint(ctx),
-                "serving": lambda: bench_serving(ctx, nreq=args.serving_requests)}
+                "serving": lambda: bench_serving(ctx, rates=rates, nreq=args.serving_requests,
+                                                 cpu=not args.no_cpu)}
         print(json.dumps({"leg": args.leg, "result": legs[args.leg]()}))
         return
 
@@ -483,7 +605,7 @@ def main():
         extra["density_13q"] = bench_density(ctx)
         extra["density_adjoint_12q"] = bench_density_adjoint(ctx)
     if args.serving_requests > 0:
-        extra["serving"] = bench_serving(ctx, nreq=args.serving_requests)
+        extra["serving"] = bench_serving(ctx, rates=rates, nreq=args.serving_requests, cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -492,6 +614,7 @@ def main():
                "sample": f"first {done} of the {ngates} gates of this circuit at n=26 "
                          f"({dt:.1f} s, cpu_ref OpenMP, one sweep per gate)"}
         log(f"cpu baseline: {done} gates in {dt:.1f} s on {cores} threads")
+        extra["cpu_baselines"] = cpu_baselines(circ)
 
     out = {
         "metric": METRIC,
@@ -508,11 +631,12 @@ def main():
         "data": "synthetic (seeded Haar unitaries, state |0...0>)",
         "config": {"workload": "26q c128 SV random circuit, depth 20 (Haar 1q + alternating CZ), fused plan",
                    "n_qubits": N, "depth": DEPTH, "gates": ngates, "state_bytes": 16 * 2 ** N},
-        "roofline": {"bound": "mfma", "achieved": tile_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64-valu", "achieved": tile_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": tile_tflops / FP64_PEAK_TFLOPS, "traffic": read_traffic(), "kernel": "k_tile",
-                     "note": "fp64 FMA on the vector ALUs (MI355X dense fp64 peak 78.6 TFLOP/s is the same for "
-                             "vector and matrix units); a fused sweep carries ~35 dense gates per amplitude "
-                             "pass (~17 flop/B), past the fp64 ridge (~10 flop/B)",
+                     "note": "FP64 FMA on the vector ALUs, no MFMA (MI355X's fp64 peak, 78.6 TFLOP/s, is the "
+                             "same for the vector and the matrix units); a fused sweep carries ~40 dense gates "
+                             "per amplitude pass (~17 flop/B), past the fp64 ridge (~10 flop/B); the HBM view "
+                             "is hbm_achieved_gbps / hbm_frac",
                      "avg_launch_ms": tile_avg_ms, "flops_per_launch": tile_flops,
                      "bytes_per_launch": tile_bytes, "hbm_achieved_gbps": achieved,
                      "hbm_frac": achieved / HBM_PEAK_GBPS},

@@ -154,10 +154,14 @@ class Server:
         return out
 
 
-def serve_poisson(server: Server, requests, rate: float, seed: int = 0, idle_sleep: float = 2e-4):
+def serve_poisson(server: Server, requests, rate: float, seed: int = 0, idle_sleep: float = 2e-4, progress=None,
+                  progress_every_s: float = 30.0):
     """Replay `requests` = [(template, pool)] arriving as a Poisson stream of
-    `rate` per second.  Latency = completion seen by the host - scheduled
-    arrival.  Returns (results in request order, stats dict)."""
+    `rate` per second (`pool` may be a callable returning it, bound at
+    submission).  Latency = completion seen by the host - scheduled
+    arrival.  `progress(message)`, when given, is called every
+    `progress_every_s` seconds with the count finished.  Returns (results in
+    request order, stats dict)."""
     rng = np.random.default_rng(seed)
     N = len(requests)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=N))
@@ -168,11 +172,15 @@ def serve_poisson(server: Server, requests, rate: float, seed: int = 0, idle_sle
     start = time.perf_counter()
     finished = 0
     max_active = 0
+    last_note = 0.0
     while finished < N:
+        if progress is not None and time.perf_counter() - start - last_note >= progress_every_s:
+            last_note = time.perf_counter() - start
+            progress(f"{finished} of {N} requests finished after {last_note:.0f} s")
         now = time.perf_counter() - start
         while i < N and arrivals[i] <= now:
             tpl, pool = requests[i]
-            by_ticket[server.submit(tpl, pool)] = i
+            by_ticket[server.submit(tpl, pool() if callable(pool) else pool)] = i
             i += 1
         active, queued = server.step()
         max_active = max(max_active, active + queued)
