@@ -99,9 +99,8 @@ __device__ __forceinline__ void bgate_chunk(const BItemDev &it, const BOp &op, b
 // Observe step on rows [i0, i1) of one item: E partial = sum_i conj(psi_i)
 // (H psi)_i and lambda = (H + H^dagger) psi.  Only the nonzero flip patterns
 // of each group load; the diagonal terms of all groups share psi[i].  The
-// group records and matrices come from LDS when they fit, else from the
-// observable's device tables.
-constexpr int kObsGroupsLds = 64;
+// group records are read as uniform (scalar) loads; the matrices come from
+// LDS when they fit, else from the observable's device tables.
 constexpr int kObsMatsLds = 512;
 
 __device__ __forceinline__ c128 observe_rows(const BItemDev &it, u64 i0, u64 i1, const ObsGroup *groups,
@@ -110,22 +109,7 @@ __device__ __forceinline__ c128 observe_rows(const BItemDev &it, u64 i0, u64 i1,
     c128 acc = mk(0, 0);
     for (u64 i = i0 + threadIdx.x; i < i1; i += kBT) {
         c128 hp = mk(0, 0), sd = mk(0, 0), dh = mk(0, 0), ds = mk(0, 0);
-        for (int gi = 0; gi < it.G; ++gi) {
-            const ObsGroup g = groups[gi];
-            const int D = 1 << g.k;
-            const int r = group_row(g, i);
-            const c128 *m = mats + g.moff + r, *sm = seed + g.moff + r;
-            if (g.xmask & 1) {
-                dh = cadd(dh, m[D * r]);
-                ds = cadd(ds, sm[D * r]);
-            }
-            for (int x = 1; x < D; ++x)
-                if ((g.xmask >> x) & 1) {
-                    const c128 y = psi[i ^ group_flip(g, x)];
-                    hp = cfma(m[D * (r ^ x)], y, hp);
-                    sd = cfma(sm[D * (r ^ x)], y, sd);
-                }
-        }
+        obs_row<true>(i, psi, it.G, groups, mats, seed, hp, dh, sd, ds);
         const c128 x0 = psi[i];
         hp = cfma(dh, x0, hp);
         if (!it.value_only) it.lam[i] = cfma(ds, x0, sd);
@@ -136,20 +120,18 @@ __device__ __forceinline__ c128 observe_rows(const BItemDev &it, u64 i0, u64 i1,
 
 // Observe step on chunk c of item it, its partial energy into part.
 __device__ __forceinline__ void observe_chunk(const BItemDev &it, int c, c128 *red, c128 *__restrict__ part) {
-    __shared__ ObsGroup s_groups[kObsGroupsLds];
     __shared__ c128 s_mats[kObsMatsLds], s_seed[kObsMatsLds];
     const u64 len = u64(1) << it.n, per = len / u64(it.nchunk);
     const ObsGroup last = it.groups[it.G - 1];
     const int nm = last.moff + (1 << (2 * last.k));
     c128 acc;
-    if (it.G <= kObsGroupsLds && nm <= kObsMatsLds) {
-        for (int j = threadIdx.x; j < it.G; j += kBT) s_groups[j] = it.groups[j];
+    if (nm <= kObsMatsLds) {
         for (int j = threadIdx.x; j < nm; j += kBT) {
             s_mats[j] = it.mats[j];
             s_seed[j] = it.seed[j];
         }
         __syncthreads();
-        acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, s_groups, s_mats, s_seed);
+        acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, const_table(it.groups), s_mats, s_seed);
     } else {
         acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, const_table(it.groups), const_table(it.mats),
                            const_table(it.seed));

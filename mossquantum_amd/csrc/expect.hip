@@ -64,9 +64,9 @@ int reduce_partials(Ctx *c, const c128 *part, int nparts, c128 *dev_out) {
 // diagonal term shares the one load of psi_i.  Each workgroup writes the
 // partial sum of conj(phi_i) (M psi)_i over its rows; with WRITE it also
 // stores out_i = (W psi)_i (W = M + M^dagger: the adjoint seed).  Group
-// records and matrices are staged in LDS when they fit.
+// records are read as uniform (scalar) loads; the matrices, gathered at each
+// row's local index, are staged in LDS when they fit.
 // (The reference sums one pass per site group, observables/expectation.jl:12-33.)
-constexpr int kObsLdsGroups = 64;
 constexpr int kObsLdsMats = 512;
 
 template <bool WRITE>
@@ -74,20 +74,17 @@ __global__ __launch_bounds__(kB) void k_observe(const c128 *__restrict__ phi, co
                                                 int G, const ObsGroup *__restrict__ groups,
                                                 const c128 *__restrict__ mats, const c128 *__restrict__ wmats,
                                                 c128 *__restrict__ out, c128 *__restrict__ part) {
-    __shared__ ObsGroup s_groups[kObsLdsGroups];
     __shared__ c128 s_m[kObsLdsMats], s_w[WRITE ? kObsLdsMats : 1];
     const ObsGroup last = groups[G - 1];
     const int nm = last.moff + (1 << (2 * last.k));
-    const bool staged = G <= kObsLdsGroups && nm <= kObsLdsMats;
+    const bool staged = nm <= kObsLdsMats;
     if (staged) {
-        for (int j = threadIdx.x; j < G; j += kB) s_groups[j] = groups[j];
         for (int j = threadIdx.x; j < nm; j += kB) {
             s_m[j] = mats[j];
             if (WRITE) s_w[j] = wmats[j];
         }
         __syncthreads();
     }
-    const ObsGroup *gs = staged ? s_groups : groups;
     const c128 *M = staged ? s_m : mats;
     const c128 *W = staged ? s_w : wmats;
     // blockIdx.y: the column of a batch (consecutive vectors of len entries)
@@ -97,23 +94,7 @@ __global__ __launch_bounds__(kB) void k_observe(const c128 *__restrict__ phi, co
     c128 acc = mk(0, 0);
     for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < len; i += u64(gridDim.x) * kB) {
         c128 hp = mk(0, 0), dh = mk(0, 0), wp = mk(0, 0), dw = mk(0, 0);
-        for (int gi = 0; gi < G; ++gi) {
-            const ObsGroup g = gs[gi];
-            const int D = 1 << g.k;
-            const int r = group_row(g, i);
-            const c128 *m = M + g.moff + r;
-            const c128 *w = W + g.moff + r;
-            if (g.xmask & 1u) {
-                dh = cadd(dh, m[D * r]);
-                if (WRITE) dw = cadd(dw, w[D * r]);
-            }
-            for (int x = 1; x < D; ++x)
-                if ((g.xmask >> x) & 1u) {
-                    const c128 y = psi[i ^ group_flip(g, x)];
-                    hp = cfma(m[D * (r ^ x)], y, hp);
-                    if (WRITE) wp = cfma(w[D * (r ^ x)], y, wp);
-                }
-        }
+        obs_row<WRITE>(i, psi, G, groups, M, W, hp, dh, wp, dw);
         const c128 x0 = psi[i];
         hp = cfma(dh, x0, hp);
         acc = cfmaconj(phi[i], hp, acc);
@@ -160,16 +141,8 @@ __global__ __launch_bounds__(kB) void k_apply_op(c128 *__restrict__ out, const c
                                                  int G, const ObsGroup *__restrict__ groups,
                                                  const c128 *__restrict__ mats) {
     for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < len; i += u64(gridDim.x) * kB) {
-        c128 acc = mk(0, 0), dg = mk(0, 0);
-        for (int gi = 0; gi < G; ++gi) {
-            const ObsGroup g = groups[gi];
-            const int D = 1 << g.k;
-            const int r = group_row(g, i);
-            const c128 *m = mats + g.moff + r;
-            if (g.xmask & 1) dg = cadd(dg, m[size_t(D) * r]);
-            for (int x = 1; x < D; ++x)
-                if ((g.xmask >> x) & 1) acc = cfma(m[size_t(D) * (r ^ x)], psi[i ^ group_flip(g, x)], acc);
-        }
+        c128 acc = mk(0, 0), dg = mk(0, 0), unused0 = mk(0, 0), unused1 = mk(0, 0);
+        obs_row<false>(i, psi, G, groups, mats, mats, acc, dg, unused0, unused1);
         out[i] = cfma(dg, psi[i], acc);
     }
 }
@@ -367,6 +340,16 @@ int mossq_obs_create(mossq_ctx *ctx, int n, int ngroups, const int32_t *group_k,
             for (int x = 0; x < D; ++x) {
                 const c128 v = M[mo + r + D * (r ^ x)];
                 if (v.re != 0.0 || v.im != 0.0) gr.xmask |= 1u << x;
+            }
+        for (int x = 1; x < D; ++x)
+            if ((gr.xmask >> x) & 1u) {
+                if (gr.nf >= kObsFlips) {
+                    gr.nf = -1;
+                    break;
+                }
+                gr.fx[gr.nf] = x;
+                gr.fm[gr.nf] = group_flip(gr, x);
+                ++gr.nf;
             }
         for (int i = 0; i < D * D; ++i) o->h_mats.push_back(M[mo + i]);
         // Adjoint seed matrix M + M^dagger, column-major: the seed of a real

@@ -75,6 +75,45 @@ __device__ __forceinline__ void store_nt(c128 *p, c128 x) {
 }
 
 
+// ------------------------------------------------------ wave reductions ----
+// Lane exchanges of a double without LDS: DPP moves inside 16-lane rows and
+// gfx950's permlane swaps across rows, two 32-bit halves each.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(v >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Sum over the 64 lanes of a wave, the same in every lane for a given lane
+// position and run (the order is fixed): rotations by 8, 4, 2, 1 inside each
+// row (DPP row_ror), then the rows pairwise (v_permlane16_swap) and the
+// halves (v_permlane32_swap).  Every step is a VALU op, where a shuffle
+// through ds_bpermute waits on LDS each time.
+__device__ __forceinline__ double wave_sum_f64(double x) {
+    x += dpp_f64<0x128>(x);  // row_ror:8
+    x += dpp_f64<0x124>(x);  // row_ror:4
+    x += dpp_f64<0x122>(x);  // row_ror:2
+    x += dpp_f64<0x121>(x);  // row_ror:1
+    {
+        const long long v = __double_as_longlong(x);
+        const auto a = __builtin_amdgcn_permlane16_swap(int(v), int(v), false, false);
+        const auto b = __builtin_amdgcn_permlane16_swap(int(v >> 32), int(v >> 32), false, false);
+        // a[0]: even rows keep x, odd rows take the even row below; a[1] the other way
+        x = __longlong_as_double((long long)(unsigned)a[0] | ((long long)b[0] << 32)) +
+            __longlong_as_double((long long)(unsigned)a[1] | ((long long)b[1] << 32));
+    }
+    {
+        const long long v = __double_as_longlong(x);
+        const auto a = __builtin_amdgcn_permlane32_swap(int(v), int(v), false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap(int(v >> 32), int(v >> 32), false, false);
+        x = __longlong_as_double((long long)(unsigned)a[0] | ((long long)b[0] << 32)) +
+            __longlong_as_double((long long)(unsigned)a[1] | ((long long)b[1] << 32));
+    }
+    return x;
+}
+
 // ----------------------------------------------------------------- bits ----
 // Insert a zero bit at position q of x (bits >= q move up by one).
 __host__ __device__ __forceinline__ u64 insert0(u64 x, int q) {

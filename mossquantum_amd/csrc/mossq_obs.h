@@ -6,12 +6,19 @@
 namespace mossq {
 
 constexpr int kMaxObsSites = 5;  // reference LARGEST_SUPPORTED_NTERMS, backends/cpu/operators/generic.jl:1
+constexpr int kObsFlips = 4;     // nonzero flip patterns listed per group
 
 struct ObsGroup {
     int k;                      // number of sites (0..5)
     int sites[kMaxObsSites];    // increasing 0-based sites
     int moff;                   // offset of the 2^k x 2^k column-major matrix
     unsigned xmask;             // bit x set: M[r, r ^ x] != 0 for some local row r
+    // The nonzero flip patterns x != 0 of xmask with their global flip masks
+    // (a Pauli term flips one pattern: an XX+YY pair or an X site has one),
+    // or nf = -1 when there are more than kObsFlips.
+    int nf;
+    int fx[kObsFlips];
+    u64 fm[kObsFlips];
 };
 
 // Row-wise action of a group sum, out_i = sum_g sum_x A_g[r, r ^ x] psi[i ^ flip_g(x)]
@@ -34,6 +41,43 @@ __host__ __device__ __forceinline__ int group_row(const ObsGroup &g, u64 i) {
     for (int j = 0; j < kMaxObsSites; ++j)
         if (j < g.k) r |= int((i >> g.sites[j]) & 1) << j;
     return r;
+}
+
+// Row i of the group sum: dh += the diagonal entries M_g[r, r] (their term
+// shares psi_i), hp += sum over the nonzero flip patterns x of
+// M_g[r, r ^ x] psi[i ^ flip_g(x)]; with TWO the same with the matrices W
+// into dw, wp.  Group records are read uniformly (one per group for the
+// whole wave), matrices at the row's local index from M / W.
+template <bool TWO>
+__device__ __forceinline__ void obs_row(u64 i, const c128 *__restrict__ psi, int G, const ObsGroup *groups,
+                                        const c128 *M, const c128 *W, c128 &hp, c128 &dh, c128 &wp, c128 &dw) {
+    for (int gi = 0; gi < G; ++gi) {
+        const ObsGroup g = groups[gi];
+        const int D = 1 << g.k;
+        const int r = group_row(g, i);
+        const c128 *m = M + g.moff + r;
+        const c128 *w = W + g.moff + r;
+        if (g.xmask & 1u) {
+            dh = cadd(dh, m[D * r]);
+            if (TWO) dw = cadd(dw, w[D * r]);
+        }
+        if (g.nf >= 0) {
+#pragma unroll
+            for (int q = 0; q < kObsFlips; ++q)
+                if (q < g.nf) {
+                    const c128 y = psi[i ^ g.fm[q]];
+                    hp = cfma(m[D * (r ^ g.fx[q])], y, hp);
+                    if (TWO) wp = cfma(w[D * (r ^ g.fx[q])], y, wp);
+                }
+        } else {
+            for (int x = 1; x < D; ++x)
+                if ((g.xmask >> x) & 1u) {
+                    const c128 y = psi[i ^ group_flip(g, x)];
+                    hp = cfma(m[D * (r ^ x)], y, hp);
+                    if (TWO) wp = cfma(w[D * (r ^ x)], y, wp);
+                }
+        }
+    }
 }
 
 struct ObsSites {

@@ -624,11 +624,7 @@ __device__ __forceinline__ double probe_op_re(int code, unsigned mask, const c12
     }
 }
 
-__device__ __forceinline__ double wave_sum_re(double a) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
-    return a;
-}
+__device__ __forceinline__ double wave_sum_re(double a) { return wave_sum_f64(a); }
 
 
 __device__ __forceinline__ void transpose_lds(c128 *v, char *ldsb, int t, const TSeg &sg) {
