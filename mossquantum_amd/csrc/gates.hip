@@ -13,6 +13,8 @@
 
 namespace mossq {
 
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
 constexpr int kBlock = 256;
 
 // Grid size for a sweep of `work` items with `items` per thread.
@@ -94,6 +96,59 @@ __global__ __launch_bounds__(kBlock) void k_dense(c128 *__restrict__ v, u64 nblo
             for (int c = 0; c < D; ++c) acc = cfma(sm[r + D * c], in[c], acc);
             store_nt(v + base + block_off<K>(r, bits), acc);
         }
+    }
+}
+
+// K = 4, 5 dense blocks on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
+// The complex update out = U in is the real product
+//     [out_re; out_im] = [[U_re, -U_im], [U_im, U_re]] [in_re; in_im]
+// of a (2D x 2D) matrix with 2D-long columns, D = 2^K.  Each wave takes 16
+// blocks at once (the 16 columns of the MFMA's B and D operands), so one
+// wave's matrix fragments (2D/16 row tiles x D/2 k-steps, held in VGPRs for
+// the whole sweep) serve every block it visits.  Lane l = 16 q + j loads
+// amplitudes c = 4 t + q (t < D/4) of block j; k-step s consumes the real
+// parts of c = 4 s + q for s < D/4 and the imaginary parts after that, which
+// is exactly operand B[k = 4 s + q][j].  Result element e of row tile rb is
+// real-matrix row 16 rb + q + 4 e: the real part of amplitude row
+// 16 rb + q + 4 e for rb < D/16, its imaginary part D/16 tiles later.
+// The matrix cores give the same FP64 peak as the vector units but take the
+// 2^K-wide complex dot products off the VALU, which leaves the sweep HBM
+// bound (measured at 26 qubits: 0.38-0.45 ms against 0.48-3.3 ms for the
+// vector kernel over target sets from {0..4} to {21..25}).
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_dense_mfma(c128 *__restrict__ v, u64 ngroups, Bits bits,
+                                                        const c128 *__restrict__ mat) {
+    constexpr int D = 1 << K, RB = D / 8, S = D / 2, T = D / 4;
+    const int l = threadIdx.x & 63, q = l >> 4, j = l & 15;
+    double afr[RB][S];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int i = 16 * rb + j, k = 4 * s + q;
+            const c128 m = mat[(i & (D - 1)) + D * (k & (D - 1))];
+            afr[rb][s] = i < D ? (k < D ? m.re : -m.im) : (k < D ? m.im : m.re);
+        }
+    const u64 stride = u64(gridDim.x) * (kBlock / 64);
+    for (u64 g = u64(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6); g < ngroups; g += stride) {
+        const u64 base = insert0s<K>(g * 16 + j, bits.b);
+        c128 x[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) x[t] = load_nt(v + base + block_off<K>(4 * t + q, bits));
+        double4_t acc[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) acc[rb] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+                acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(afr[rb][s], s < T ? x[s].re : x[s - T].im, acc[rb],
+                                                               0, 0, 0);
+#pragma unroll
+        for (int rb = 0; rb < RB / 2; ++rb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                store_nt(v + base + block_off<K>(16 * rb + q + 4 * e, bits), mk(acc[rb][e], acc[rb + RB / 2][e]));
     }
 }
 
@@ -233,6 +288,14 @@ static int launch_mono(Ctx *c, c128 *v, int nbits, int k, const int *bits, const
     return MOSSQ_OK;
 }
 
+// Workgroups for k_dense_mfma over `nblocks` blocks: one 16-block group per
+// wave, at most 4096 workgroups (16384 waves, so each wave loads its matrix
+// fragments once and reuses them over several groups on large states).
+static inline unsigned mfma_grid(u64 nblocks) {
+    u64 waves = nblocks / 16, g = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    return unsigned(g < 1 ? 1 : (g > 4096 ? 4096 : g));
+}
+
 int launch_dense(Ctx *c, c128 *v, int nbits, int k, const int *bits, const c128 *dmat,
                  const MatInfo &info) {
     MOSSQ_CHECK(k >= 1 && k <= 5 && k <= nbits, MOSSQ_EINVAL, "dense block size out of range");
@@ -253,10 +316,18 @@ int launch_dense(Ctx *c, c128 *v, int nbits, int k, const int *bits, const c128 
             hipLaunchKernelGGL(k_dense<3>, dim3(grid), dim3(kBlock), 0, c->stream, v, nblocks, b, dmat);
             break;
         case 4:
-            hipLaunchKernelGGL(k_dense<4>, dim3(grid), dim3(kBlock), 0, c->stream, v, nblocks, b, dmat);
+            if (nblocks >= 16)
+                hipLaunchKernelGGL(k_dense_mfma<4>, dim3(mfma_grid(nblocks)), dim3(kBlock), 0, c->stream, v,
+                                   nblocks / 16, b, dmat);
+            else
+                hipLaunchKernelGGL(k_dense<4>, dim3(grid), dim3(kBlock), 0, c->stream, v, nblocks, b, dmat);
             break;
         default:
-            hipLaunchKernelGGL(k_dense<5>, dim3(grid), dim3(kBlock), 0, c->stream, v, nblocks, b, dmat);
+            if (nblocks >= 16)
+                hipLaunchKernelGGL(k_dense_mfma<5>, dim3(mfma_grid(nblocks)), dim3(kBlock), 0, c->stream, v,
+                                   nblocks / 16, b, dmat);
+            else
+                hipLaunchKernelGGL(k_dense<5>, dim3(grid), dim3(kBlock), 0, c->stream, v, nblocks, b, dmat);
             break;
     }
     MOSSQ_HIP(hipGetLastError());

@@ -47,6 +47,25 @@ def test_gate_positions(ctx, k):
         assert np.max(np.abs(sv.to_host() - want)) < AMP, pos
 
 
+@pytest.mark.parametrize("k", [4, 5])
+def test_wide_gates_matrix_core_path(ctx, k):
+    """Four- and five-qubit gates on states with at least 16 blocks take the
+    FP64 matrix-core kernel: random ordered target sets (low, high, adjacent,
+    scattered, unsorted), on 12 qubits and, at the 16-block threshold, on k + 4."""
+    rng = np.random.default_rng(40 + k)
+    for n in (12, k + 4):
+        psi = rand_state(n, rng)
+        sv = mq.StateVector.from_host(psi)
+        want = psi.copy()
+        sets = [tuple(range(1, k + 1)), tuple(range(n - k + 1, n + 1))]
+        sets += [tuple(int(p) + 1 for p in rng.permutation(n)[:k]) for _ in range(10)]
+        for pos in sets:
+            g = mq.QuantumGate(pos, haar_unitary(rng, 1 << k))
+            mq.apply_(g, sv)
+            oracle.apply_gate(want, n, [p - 1 for p in g.ordered_positions()], g.ordered_mat())
+            assert np.max(np.abs(sv.to_host() - want)) < AMP, (n, pos)
+
+
 def test_large_qubit_gates(ctx):
     rng = np.random.default_rng(3)
     n = 20
