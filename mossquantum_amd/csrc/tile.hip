@@ -1238,11 +1238,108 @@ static bool safe_layout(const int *reg) {
 
 static int encode_ops(FusedPlan *fp, std::string *err);
 
+// The ops one sweep over tile bits T executes, in program order, from the
+// not-yet-done ops: an op runs when it fits (dense ops and probes need their
+// bits in T; diagonals are applied from the index and fit any T) and no
+// earlier op left undone fails to commute with it.  Undone ops are tracked per
+// bit: bd[q] = an undone non-diagonal op on bit q (blocks every later op on
+// q), bg[q] = an undone diagonal on q (blocks later non-diagonal ops on q);
+// unfusable steps and fixed probes commute with nothing, so an undone one
+// ends the scan.  Returns the work executed (2 per dense op or probe, 1 per
+// diagonal); `out` (if given) receives the ops.
+static int sweep_ops(const std::vector<POp> &ops, const std::vector<char> &done, size_t first, int nbits, u64 T,
+                     std::vector<int> *out) {
+    u64 bd = 0, bg = 0;
+    const u64 all = nbits >= 64 ? ~u64(0) : (u64(1) << nbits) - 1;
+    int work = 0;
+    for (size_t i = first; i < ops.size(); ++i) {
+        if (done[i]) continue;
+        const POp &o = ops[i];
+        const u64 m = o.mask();
+        if (o.kind == 2 || o.fixed) {
+            // commutes with nothing: runs only if nothing before it is undone
+            if (o.kind != 2 && !bd && !bg && (m & ~T) == 0) {
+                work += 2;
+                if (out) out->push_back(int(i));
+                continue;
+            }
+            break;
+        }
+        if (o.kind == 1) {
+            if (m & bd) {
+                bg |= m;
+                continue;
+            }
+            work += 1;
+            if (out) out->push_back(int(i));
+            continue;
+        }
+        if ((m & (bd | bg)) || (m & ~T)) {
+            bd |= m;
+            if ((bd & all) == all) break;  // every later op is blocked
+            continue;
+        }
+        work += 2;
+        if (out) out->push_back(int(i));
+    }
+    return work;
+}
+
+// Tile bits of the next sweep: the lowest kCoalesceBits bits plus up to
+// TM - kCoalesceBits more.  A first choice takes the bits of the undone dense
+// ops in program order while they fit; then single-bit exchanges (one chosen
+// bit out, one other in) are applied while one increases the work the sweep
+// executes (a few rounds).  The exchanges let a sweep follow the diagonal
+// front of layered circuits (CNOT ladders) instead of the bits that happen to
+// come first.
+static u64 choose_tile_bits(const std::vector<POp> &ops, const std::vector<char> &done, size_t first, int nbits) {
+    const u64 coal = (u64(1) << kCoalesceBits) - 1;
+    u64 T = coal, bd = 0, bg = 0;
+    for (size_t i = first; i < ops.size() && popc(T) < TM; ++i) {
+        if (done[i]) continue;
+        const POp &o = ops[i];
+        const u64 m = o.mask();
+        if (o.kind == 2 || o.fixed) {
+            if (o.kind != 2 && !bd && !bg && popc(T | m) <= TM) T |= m;
+            break;
+        }
+        if (o.kind == 1) {
+            if (m & bd) bg |= m;
+            continue;
+        }
+        if (!(m & (bd | bg)) && popc(T | m) <= TM) {
+            T |= m;
+            continue;
+        }
+        bd |= m;
+    }
+    for (int b = 0; b < nbits && popc(T) < TM; ++b) T |= u64(1) << b;
+    if (nbits <= TM) return T;
+    int best = sweep_ops(ops, done, first, nbits, T, nullptr);
+    for (int round = 0; round < 6; ++round) {
+        u64 bestT = T;
+        for (int b = kCoalesceBits; b < nbits; ++b) {
+            if (!((T >> b) & 1)) continue;
+            for (int c = kCoalesceBits; c < nbits; ++c) {
+                if ((T >> c) & 1) continue;
+                const u64 T2 = (T & ~(u64(1) << b)) | (u64(1) << c);
+                const int w = sweep_ops(ops, done, first, nbits, T2, nullptr);
+                if (w > best) {
+                    best = w;
+                    bestT = T2;
+                }
+            }
+        }
+        if (bestT == T) break;
+        T = bestT;
+    }
+    return T;
+}
+
 static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, int min_tail, bool adjoint, bool kron,
                                  std::string *err) {
     FusedPlan *fp = new FusedPlan();
     fp->adjoint = adjoint;
-    const u64 coal = (u64(1) << kCoalesceBits) - 1;
     std::vector<char> done(ops.size(), 0);
     size_t first = 0;
     auto failp = [&](const char *m) {
@@ -1261,29 +1358,9 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
             done[first] = 1;
             continue;
         }
-        // Greedy sweep with commutation lookahead.
-        u64 T = coal;
-        std::vector<int> chosen, deferred;
-        for (size_t i = first; i < ops.size(); ++i) {
-            if (done[i]) continue;
-            const POp &o = ops[i];
-            bool blocked = false;
-            for (int d : deferred)
-                if (!commutes(ops[d], o)) {
-                    blocked = true;
-                    break;
-                }
-            if (!blocked && o.kind != 2) {
-                u64 need = o.kind == 0 || o.kind == 3 ? o.mask() : 0;
-                if (popc(T | need) <= TM) {
-                    T |= need;
-                    chosen.push_back(int(i));
-                    continue;
-                }
-            }
-            deferred.push_back(int(i));
-            if (deferred.size() > 64) break;  // bounded lookahead
-        }
+        u64 T = choose_tile_bits(ops, done, first, p->nbits);
+        std::vector<int> chosen;
+        sweep_ops(ops, done, first, p->nbits, T, &chosen);
         for (int i : chosen) done[i] = 1;
         for (int b = 0; b < p->nbits && popc(T) < TM; ++b) T |= u64(1) << b;
         PlanItem it;
