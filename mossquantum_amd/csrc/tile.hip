@@ -878,6 +878,23 @@ static void lower_steps(const Program *p, std::vector<POp> &out) { lower_range(p
 
 static bool sign_only(const std::vector<c128> &d, int *negmask);
 
+// A B == B A for column-major d x d matrices, to rounding.
+static bool commute_mats(const std::vector<c128> &A, const std::vector<c128> &B, int d) {
+    double scale = 0.0, diff = 0.0;
+    for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c) {
+            c128 ab = mk(0, 0), ba = mk(0, 0);
+            for (int k = 0; k < d; ++k) {
+                ab = cfma(A[r + d * k], B[k + d * c], ab);
+                ba = cfma(B[r + d * k], A[k + d * c], ba);
+            }
+            diff = std::max(diff, std::max(std::fabs(ab.re - ba.re), std::fabs(ab.im - ba.im)));
+            scale = std::max(scale, std::max(std::hypot(A[r + d * c].re, A[r + d * c].im),
+                                             std::hypot(B[r + d * c].re, B[r + d * c].im)));
+        }
+    return diff <= 1e-14 * std::max(scale * scale, 1e-300);
+}
+
 // Adjoint plan of a pure-state program: the steps in reverse order with
 // their inverse matrices, each parameterized step followed by the probe of its
 // derivative matrix D (differentiation/circuit.jl:136-144: y <- U^dag y,
@@ -925,6 +942,20 @@ static void lower_steps_rev(const Program *p, std::vector<POp> &out) {
             pr.probe = st.grad_index;
             out.push_back(pr);
         }
+    }
+    // A probe whose D commutes with its own step's reverse matrix U^dag takes
+    // the same value before U^dag as after it: Re<U^dag y|D|U^dag l> =
+    // Re<y|U D U^dag|l> = Re<y|D|l>.  Where the next op is a dense op on the
+    // same bits, the probe moves ahead of its own op, so the two dense ops
+    // meet and the segment peephole multiplies them into one (a rotation
+    // layer's R_z^dag and R_y^dag on one qubit: one 2x2 instead of two).
+    for (size_t i = 0; i + 2 < out.size(); ++i) {
+        POp &A = out[i], &P = out[i + 1];
+        const POp &B = out[i + 2];
+        if (A.kind != 0 || P.kind != 3 || B.kind != 0 || P.step != A.step || A.mask() != P.mask() ||
+            B.mask() != A.mask() || A.nb != P.nb || B.nb != A.nb)
+            continue;
+        if (commute_mats(A.m, P.m, 1 << A.nb)) std::swap(A, P);
     }
 }
 
