@@ -109,7 +109,7 @@ struct BatchSweeps {
     void *d_items = nullptr;  // device launch items (layout private to tile.hip)
     unsigned *d_wg = nullptr; // workgroup -> item of its launch << 16 | tile
     c128 *vec = nullptr, *part = nullptr;
-    std::vector<int> item0, wg0, tiles;  // per launch
+    std::vector<int> item0, wg0, tiles, rb;  // per launch (rb: register bits of its plans)
 };
 // Builds and uploads the launch tables into al's memory (synchronises c's
 // stream).  Every vector lies in the allocation vec (vec_len entries), given

@@ -31,12 +31,23 @@
 
 namespace mossq {
 
-constexpr int TM = 12;   // tile bits
-constexpr int TRB = 4;   // register bits
-constexpr int TT = 256;  // threads per tile
-constexpr int TN = 16;   // amplitudes per thread
-constexpr int TTB = 8;   // thread-index bits (TM - TRB)
+constexpr int TM = 12;         // tile bits
+constexpr int kRegMax = 4;     // register bits of a layout: 4 (16 slots, 256 threads) or 3 (8 slots, 512 threads)
+constexpr int kThreadMax = 9;  // thread-index bits: TM - register bits
 constexpr int kCoalesceBits = 3;
+
+// Tile geometry for RB register bits: TN amplitudes per thread, TT threads
+// per tile (TTB thread-index bits), WPT waves per tile.  With 3 register bits a
+// thread holds half the amplitudes, so a tile of the same 2^12 amplitudes has
+// twice the threads at about half the registers per thread: four waves per
+// SIMD instead of two.
+template <int RB>
+struct TGeo {
+    static constexpr int TN = 1 << RB;
+    static constexpr int TTB = TM - RB;
+    static constexpr int TT = 1 << TTB;
+    static constexpr int WPT = TT / 64;
+};
 
 enum TOpType { T_DENSE1 = 0, T_DENSE2 = 1, T_SIGN = 2, T_SELECT = 3, T_GATHER = 4, T_PROBE1 = 5, T_PROBE2 = 6 };
 
@@ -61,7 +72,8 @@ struct TOp {
 // Device form of a segment's ops: groups, each one out-of-place step
 // (in -> out) of the two-register-set op loop, so the loop never copies.
 //   C_DENSE1 + a, C_DENSE2 + pair: dense 2x2 on register bit a / 4x4 on a
-//     register-bit pair (kPairA/kPairB);
+//     register-bit pair (kPairA/kPairB; with 3 register bits only the pairs
+//     of bits 0..2);
 //   C_FLIP + any of those: the dense op, then sign flips of its outputs;
 //   C_COPY: copy with sign flips (flips at the start of a segment);
 //   C_DIAG: a general diagonal (TDiag record x);
@@ -71,7 +83,7 @@ struct TOp {
 //   C_DIAG1 + a (adjoint plans): a diagonal 2x2 on register bit a, one
 //     complex product per slot (no flips ride on it).
 // Sign flips are controlled-Z-type diagonals (-1 exactly where all their bits
-// are 1), up to kGroupCz per group: flip i negates the slots in the 16-bit slot
+// are 1), up to kGroupCz per group: flip i negates the slots in the slot
 // pattern pat[i] wherever its condition bits are all 1.  cond[i] packs up to
 // three condition bits, 7 bits each from bit 0: 0x7F none; 0x40 | b: global bit
 // b outside the tile; else tile-local bit b.  A segment with an odd number of
@@ -114,6 +126,8 @@ struct TDiag {
 // A register layout and its instructions.  The layout's index maps are
 // precomputed on the host: thread-index bit k sits at tile-local bit tpos[k]
 // (global bit gpos[k]); register bit i at tile-local bit reg[i] (global gbit[i]).
+// A plan's layouts all hold the same number of register bits (3 or 4); the
+// arrays are sized for the larger thread and register counts.
 //
 // Entering a segment whose layout differs from the previous one transposes
 // the tile through LDS.  The LDS slot of tile-local index l is the XOR of one
@@ -123,15 +137,15 @@ struct TDiag {
 // layout) and the read (new layout) side.  wt/wr and rt/rr are those columns
 // at the old and new layouts' thread and register bits.
 struct TSeg {
-    int reg[TRB];  // tile-local bits held in registers (increasing)
-    int op0, op1;  // planner ops (TOp) of the segment
-    int g0, g1;    // device groups of the segment
-    int tpos[TTB];
-    int gpos[TTB];
-    int gbit[TRB];
+    int reg[kRegMax];  // tile-local bits held in registers (increasing; -1 past the layout's count)
+    int op0, op1;      // planner ops (TOp) of the segment
+    int g0, g1;        // device groups of the segment
+    int tpos[kThreadMax];
+    int gpos[kThreadMax];
+    int gbit[kRegMax];
     int transpose;
-    int wt[TTB], wr[TRB];
-    int rt[TTB], rr[TRB];
+    int wt[kThreadMax], wr[kRegMax];
+    int rt[kThreadMax], rr[kRegMax];
 };
 
 struct TSweepArgs {
@@ -151,11 +165,11 @@ __device__ __forceinline__ c128 flip_slot(c128 x, unsigned F, int j) {
 
 // Dense ops are out of place (in -> out); with FLIP the outputs are then
 // sign-flipped per the lane's pattern F.
-template <int A, bool FLIP>
+template <int RB, int A, bool FLIP>
 __device__ __forceinline__ void reg_apply1(const c128 *in, c128 *out, const c128 *__restrict__ M, unsigned F) {
     const c128 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
+    for (int j = 0; j < TGeo<RB>::TN; ++j) {
         if (j & (1 << A)) continue;
         const c128 x = in[j], y = in[j | (1 << A)];
         const c128 y0 = cfma(m01, y, cmul(m00, x)), y1 = cfma(m11, y, cmul(m10, x));
@@ -164,14 +178,14 @@ __device__ __forceinline__ void reg_apply1(const c128 *in, c128 *out, const c128
     }
 }
 
-template <int A, int B, bool FLIP>
+template <int RB, int A, int B, bool FLIP>
 __device__ __forceinline__ void reg_apply2(const c128 *in, c128 *out, const c128 *__restrict__ M, unsigned F) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const c128 m0 = M[r], m1 = M[r + 4], m2 = M[r + 8], m3 = M[r + 12];
         const int orow = ((r & 1) << A) | ((r >> 1) << B);
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
+        for (int j = 0; j < TGeo<RB>::TN; ++j) {
             if (j & ((1 << A) | (1 << B))) continue;
             c128 acc = cmul(m0, in[j]);
             acc = cfma(m1, in[j | (1 << A)], acc);
@@ -199,8 +213,9 @@ __host__ __device__ constexpr int perm_src(int j, int A, int B, int kind) {
 // In place: each instantiation reads through its own local copy (with one
 // source array for all 36 cases, the compiler merges their moves into moves
 // from a selected address, which takes the register arrays to scratch).
-template <int A, int B, int K, bool FLIP>
+template <int RB, int A, int B, int K, bool FLIP>
 __device__ __forceinline__ void reg_perm(const c128 *in, c128 *out, unsigned F) {
+    constexpr int TN = TGeo<RB>::TN;
     c128 t[TN];
 #pragma unroll
     for (int j = 0; j < TN; ++j) t[j] = in[j];
@@ -209,6 +224,7 @@ __device__ __forceinline__ void reg_perm(const c128 *in, c128 *out, unsigned F) 
 }
 
 // Permutation groups (code >= C_PERM): pk = code - C_PERM below kPermFlip.
+// Pairs that name register bit 3 exist only in 4-bit layouts.
 #define MOSSQ_PERM6(P, STMT) \
     STMT(P, 0)               \
     STMT(P, 1)               \
@@ -224,42 +240,55 @@ __device__ __forceinline__ void reg_perm(const c128 *in, c128 *out, unsigned F) 
     MOSSQ_PERM6(4, STMT)   \
     MOSSQ_PERM6(5, STMT)
 
-template <bool FLIP>
+template <int RB, bool FLIP>
 __device__ __forceinline__ void perm_ip(int pk, c128 *v, unsigned F) {
-#define MOSSQ_PERM_IP(P, K)                               \
-    case 6 * P + K:                                       \
-        reg_perm<kPairA[P], kPairB[P], K, FLIP>(v, v, F); \
+#define MOSSQ_PERM_IP(P, K)                                                                      \
+    case 6 * P + K:                                                                              \
+        if constexpr (kPairB[P] < RB) reg_perm<RB, kPairA[P], kPairB[P], K, FLIP>(v, v, F); \
         break;
     switch (pk) { MOSSQ_PERM36(MOSSQ_PERM_IP) }
 #undef MOSSQ_PERM_IP
 }
 
 // Tile-local index of thread t's register slot 0 in a layout.
+template <int RB>
 __host__ __device__ __forceinline__ int seg_tdep(int t, const TSeg &sg) {
     int d = 0;
 #pragma unroll
-    for (int k = 0; k < TTB; ++k) d |= ((t >> k) & 1) << sg.tpos[k];
+    for (int k = 0; k < TGeo<RB>::TTB; ++k) d |= ((t >> k) & 1) << sg.tpos[k];
     return d;
 }
 
 // Global offset of thread t's register slot 0 within its tile.
+template <int RB>
 __host__ __device__ __forceinline__ u64 seg_gthread(int t, const TSeg &sg) {
     u64 g = 0;
 #pragma unroll
-    for (int k = 0; k < TTB; ++k) g |= u64((t >> k) & 1) << sg.gpos[k];
+    for (int k = 0; k < TGeo<RB>::TTB; ++k) g |= u64((t >> k) & 1) << sg.gpos[k];
     return g;
 }
 
-// LDS byte offsets of thread t's 16 register slots: the XOR of the columns of
-// the set thread bits, then one column per set bit of j (15 XORs for all 16).
+// Global offset of register slot j (its register bits) in a layout.
+template <int RB>
+__host__ __device__ __forceinline__ u64 slot_goff(int j, const TSeg &sg) {
+    u64 o = 0;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+        if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
+    return o;
+}
+
+// LDS byte offsets of thread t's register slots: the XOR of the columns of
+// the set thread bits, then one column per set bit of j (TN - 1 XORs in all).
+template <int RB>
 __host__ __device__ __forceinline__ void lds_slots(int t, const int *tcol, const int *rcol, int *a) {
     int x = 0;
 #pragma unroll
-    for (int k = 0; k < TTB; ++k)
+    for (int k = 0; k < TGeo<RB>::TTB; ++k)
         if ((t >> k) & 1) x ^= tcol[k];
     a[0] = x * int(sizeof(c128));
 #pragma unroll
-    for (int j = 1; j < TN; ++j) a[j] = a[j & (j - 1)] ^ (rcol[__builtin_ctz(j)] * int(sizeof(c128)));
+    for (int j = 1; j < TGeo<RB>::TN; ++j) a[j] = a[j & (j - 1)] ^ (rcol[__builtin_ctz(j)] * int(sizeof(c128)));
 }
 
 // Non-register part of a diagonal index.
@@ -284,8 +313,10 @@ __device__ __forceinline__ double flip_sign(double x, unsigned flip) {
 }
 
 // General diagonal, out of place.
+template <int RB>
 __device__ __forceinline__ void diag_op(const TDiag &d, const c128 *__restrict__ pool, const c128 *in, c128 *out,
                                         int tdep, u64 base) {
+    constexpr int TN = TGeo<RB>::TN;
     const int f = diag_f(d, tdep, base);
     if (d.code == D_SIGN) {
         const unsigned neg = unsigned(d.negmask);
@@ -331,73 +362,80 @@ __host__ __device__ __forceinline__ unsigned group_flips(const TGroup &g, int td
     return F;
 }
 
+// Dense cases of a group code (code < C_COPY) for RB register bits; pair
+// cases that name bit 3 exist only with 4 register bits.
+#define MOSSQ_DENSE_CASES(RB, CALL1, CALL2)                                             \
+    case C_DENSE1 + 0: CALL1(RB, 0, false); break;                                      \
+    case C_DENSE1 + 1: CALL1(RB, 1, false); break;                                      \
+    case C_DENSE1 + 2: CALL1(RB, 2, false); break;                                      \
+    case C_DENSE1 + 3: if constexpr (RB > 3) CALL1(RB, 3, false); break;                \
+    case C_DENSE2 + 0: CALL2(RB, 0, 1, false); break;                                   \
+    case C_DENSE2 + 1: CALL2(RB, 0, 2, false); break;                                   \
+    case C_DENSE2 + 2: if constexpr (RB > 3) CALL2(RB, 0, 3, false); break;             \
+    case C_DENSE2 + 3: CALL2(RB, 1, 2, false); break;                                   \
+    case C_DENSE2 + 4: if constexpr (RB > 3) CALL2(RB, 1, 3, false); break;             \
+    case C_DENSE2 + 5: if constexpr (RB > 3) CALL2(RB, 2, 3, false); break;             \
+    case C_FLIP + C_DENSE1 + 0: CALL1(RB, 0, true); break;                              \
+    case C_FLIP + C_DENSE1 + 1: CALL1(RB, 1, true); break;                              \
+    case C_FLIP + C_DENSE1 + 2: CALL1(RB, 2, true); break;                              \
+    case C_FLIP + C_DENSE1 + 3: if constexpr (RB > 3) CALL1(RB, 3, true); break;        \
+    case C_FLIP + C_DENSE2 + 0: CALL2(RB, 0, 1, true); break;                           \
+    case C_FLIP + C_DENSE2 + 1: CALL2(RB, 0, 2, true); break;                           \
+    case C_FLIP + C_DENSE2 + 2: if constexpr (RB > 3) CALL2(RB, 0, 3, true); break;     \
+    case C_FLIP + C_DENSE2 + 3: CALL2(RB, 1, 2, true); break;                           \
+    case C_FLIP + C_DENSE2 + 4: if constexpr (RB > 3) CALL2(RB, 1, 3, true); break;     \
+    case C_FLIP + C_DENSE2 + 5: if constexpr (RB > 3) CALL2(RB, 2, 3, true); break;
+
 // One group, out of place.
 // `in` is scratch afterwards (permutations run in place on it).
+template <int RB>
 __device__ __forceinline__ void group_op(const TGroup &g, const TDiag *__restrict__ diag,
                                          const c128 *__restrict__ pool, c128 *in, c128 *out, int tdep,
                                          u64 base) {
+    constexpr int TN = TGeo<RB>::TN;
     const c128 *M = pool + g.moff;
     const unsigned F = group_flips(g, tdep, base);
     if (g.code >= C_PERM) {  // C_DIAG1 groups occur in adjoint plans only (dense_ip)
         // permuted in place, then moved: out-of-place permutation switches
         // spill (see reg_perm)
         if (g.code >= C_PERM + kPermFlip)
-            perm_ip<true>(g.code - C_PERM - kPermFlip, in, F);
+            perm_ip<RB, true>(g.code - C_PERM - kPermFlip, in, F);
         else
-            perm_ip<false>(g.code - C_PERM, in, F);
+            perm_ip<RB, false>(g.code - C_PERM, in, F);
 #pragma unroll
         for (int j = 0; j < TN; ++j) out[j] = in[j];
         return;
     }
+#define MOSSQ_OOP1(R, A, FL) reg_apply1<R, A, FL>(in, out, M, F)
+#define MOSSQ_OOP2(R, A, B, FL) reg_apply2<R, A, B, FL>(in, out, M, F)
     switch (g.code) {
-        case C_DENSE1 + 0: reg_apply1<0, false>(in, out, M, F); break;
-        case C_DENSE1 + 1: reg_apply1<1, false>(in, out, M, F); break;
-        case C_DENSE1 + 2: reg_apply1<2, false>(in, out, M, F); break;
-        case C_DENSE1 + 3: reg_apply1<3, false>(in, out, M, F); break;
-        case C_DENSE2 + 0: reg_apply2<0, 1, false>(in, out, M, F); break;
-        case C_DENSE2 + 1: reg_apply2<0, 2, false>(in, out, M, F); break;
-        case C_DENSE2 + 2: reg_apply2<0, 3, false>(in, out, M, F); break;
-        case C_DENSE2 + 3: reg_apply2<1, 2, false>(in, out, M, F); break;
-        case C_DENSE2 + 4: reg_apply2<1, 3, false>(in, out, M, F); break;
-        case C_DENSE2 + 5: reg_apply2<2, 3, false>(in, out, M, F); break;
-        case C_FLIP + C_DENSE1 + 0: reg_apply1<0, true>(in, out, M, F); break;
-        case C_FLIP + C_DENSE1 + 1: reg_apply1<1, true>(in, out, M, F); break;
-        case C_FLIP + C_DENSE1 + 2: reg_apply1<2, true>(in, out, M, F); break;
-        case C_FLIP + C_DENSE1 + 3: reg_apply1<3, true>(in, out, M, F); break;
-        case C_FLIP + C_DENSE2 + 0: reg_apply2<0, 1, true>(in, out, M, F); break;
-        case C_FLIP + C_DENSE2 + 1: reg_apply2<0, 2, true>(in, out, M, F); break;
-        case C_FLIP + C_DENSE2 + 2: reg_apply2<0, 3, true>(in, out, M, F); break;
-        case C_FLIP + C_DENSE2 + 3: reg_apply2<1, 2, true>(in, out, M, F); break;
-        case C_FLIP + C_DENSE2 + 4: reg_apply2<1, 3, true>(in, out, M, F); break;
-        case C_FLIP + C_DENSE2 + 5: reg_apply2<2, 3, true>(in, out, M, F); break;
+        MOSSQ_DENSE_CASES(RB, MOSSQ_OOP1, MOSSQ_OOP2)
         case C_COPY:
 #pragma unroll
             for (int j = 0; j < TN; ++j) out[j] = flip_slot(in[j], F, j);
             break;
-        default: diag_op(diag[g.x], pool, in, out, tdep, base); break;
+        default: diag_op<RB>(diag[g.x], pool, in, out, tdep, base); break;
     }
+#undef MOSSQ_OOP1
+#undef MOSSQ_OOP2
 }
 
+template <int RB>
 __device__ __forceinline__ double probe_op_re(int code, unsigned mask, const c128 *y, const c128 *l,
                                               const c128 *__restrict__ D);
-__device__ __forceinline__ double wave_sum_re(double a);
+__device__ __forceinline__ double wave_sum_re(double a) { return wave_sum_f64(a); }
 
 // Probe of a group against a fixed state: this wave's share of
 // sum conj(cur) (D fixed) over the tile, fixed read in the current layout
 // into f (the register set the group's output left free).
+template <int RB>
 __device__ __forceinline__ void fixed_probe(const TGroup &g, const c128 *cur, c128 *f, const c128 *fixed,
                                             const TSeg &sg, int t, u64 base, const c128 *__restrict__ pool,
                                             c128 *part, int row, int slot) {
-    const u64 gt = base | seg_gthread(t, sg);
+    const u64 gt = base | seg_gthread<RB>(t, sg);
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-        u64 o = 0;
-#pragma unroll
-        for (int i = 0; i < TRB; ++i)
-            if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-        f[j] = load_nt(fixed + (gt | o));
-    }
-    const double acc = wave_sum_re(probe_op_re(g.pcode, g.pmask, cur, f, pool + g.pmoff));
+    for (int j = 0; j < TGeo<RB>::TN; ++j) f[j] = load_nt(fixed + (gt | slot_goff<RB>(j, sg)));
+    const double acc = wave_sum_re(probe_op_re<RB>(g.pcode, g.pmask, cur, f, pool + g.pmoff));
     if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = c128{acc, 0.0};
 }
 
@@ -409,106 +447,85 @@ __device__ __forceinline__ u64 tile_base(const TSweepArgs &sw, u64 tile) {
     return base;
 }
 
+// LDS transposition of one register set into the layout of segment sg.
+template <int RB>
+__device__ __forceinline__ void transpose_lds(c128 *v, char *ldsb, int t, const TSeg &sg) {
+    constexpr int TN = TGeo<RB>::TN;
+    int a[TN];
+    lds_slots<RB>(t, sg.wt, sg.wr, a);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) *reinterpret_cast<c128 *>(ldsb + a[j]) = v[j];
+    __syncthreads();
+    lds_slots<RB>(t, sg.rt, sg.rr, a);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) v[j] = *reinterpret_cast<const c128 *>(ldsb + a[j]);
+    __syncthreads();
+}
+
 // Reads the tile from src and writes it to dst (src == dst for an in-place
 // sweep; a workgroup stores only after it has loaded its whole tile).  With
 // FIXP, a group's probe adds Re<cur|D|S> partials (one per wave per tile)
 // for a state S = ftab[probe - fbase] that the sweep does not change.
-template <bool FIXP>
+template <int RB, bool FIXP>
 __device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSweepArgs &sw,
                                            const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
                                            const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
                                            const c128 *const *ftab, int fbase, c128 *part, int row, u64 tile) {
+    constexpr int TN = TGeo<RB>::TN;
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
     const int t = threadIdx.x;
     const u64 base = tile_base(sw, tile);
     const int seg0 = sw.seg0, seg1 = sw.seg1;
+    const int slot = int(tile) * TGeo<RB>::WPT + (t >> 6);
 
     TSeg sg = segs[seg0];
-    int tdep = seg_tdep(t, sg);
+    int tdep = seg_tdep<RB>(t, sg);
     c128 v[TN], w[TN];
     {
-        const u64 g = base | seg_gthread(t, sg);
+        const u64 g = base | seg_gthread<RB>(t, sg);
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            u64 o = 0;
-#pragma unroll
-            for (int i = 0; i < TRB; ++i)
-                if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            v[j] = load_nt(src + (g | o));
-        }
+        for (int j = 0; j < TN; ++j) v[j] = load_nt(src + (g | slot_goff<RB>(j, sg)));
     }
     for (int s = seg0; s < seg1; ++s) {
         sg = segs[s];
         if (sg.transpose) {
-            int a[TN];
-            lds_slots(t, sg.wt, sg.wr, a);
-#pragma unroll
-            for (int j = 0; j < TN; ++j) *reinterpret_cast<c128 *>(ldsb + a[j]) = v[j];
-            __syncthreads();
-            tdep = seg_tdep(t, sg);
-            lds_slots(t, sg.rt, sg.rr, a);
-#pragma unroll
-            for (int j = 0; j < TN; ++j) v[j] = *reinterpret_cast<const c128 *>(ldsb + a[j]);
-            __syncthreads();
+            transpose_lds<RB>(v, ldsb, t, sg);
+            tdep = seg_tdep<RB>(t, sg);
         }
         // Groups in pairs, v -> w -> v (a segment holds an even number): the
         // loop carries v only.
         for (int g = sg.g0; g < sg.g1; g += 2) {
             const TGroup ga = groups[g], gb = groups[g + 1];
-            group_op(ga, diag, pool, v, w, tdep, base);
-            if (FIXP && ga.probe >= 0)
-                fixed_probe(ga, w, v, ftab[ga.probe - fbase], sg, t, base, pool, part, row, int(tile) * 4 + (t >> 6));
-            group_op(gb, diag, pool, w, v, tdep, base);
-            if (FIXP && gb.probe >= 0)
-                fixed_probe(gb, v, w, ftab[gb.probe - fbase], sg, t, base, pool, part, row, int(tile) * 4 + (t >> 6));
+            group_op<RB>(ga, diag, pool, v, w, tdep, base);
+            if (FIXP && ga.probe >= 0) fixed_probe<RB>(ga, w, v, ftab[ga.probe - fbase], sg, t, base, pool, part, row, slot);
+            group_op<RB>(gb, diag, pool, w, v, tdep, base);
+            if (FIXP && gb.probe >= 0) fixed_probe<RB>(gb, v, w, ftab[gb.probe - fbase], sg, t, base, pool, part, row, slot);
         }
     }
     {
         // sg is the last segment: its layout is the current one.
-        const u64 g = base | seg_gthread(t, sg);
+        const u64 g = base | seg_gthread<RB>(t, sg);
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            u64 o = 0;
-#pragma unroll
-            for (int i = 0; i < TRB; ++i)
-                if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            store_nt(dst + (g | o), v[j]);
-        }
+        for (int j = 0; j < TN; ++j) store_nt(dst + (g | slot_goff<RB>(j, sg)), v[j]);
     }
-}
-
-__global__ __launch_bounds__(TT, 2) void k_tile(const c128 *src, c128 *dst, TSweepArgs sw,
-                                                  const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
-                                                  const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {
-    tile_sweep<false>(src, dst, sw, segs, groups, diag, pool, nullptr, 0, nullptr, 0, blockIdx.x);
-}
-
-// The density adjoint's reverse chunks: Delta in place, each probe against
-// the state before its parameterized gate (ftab[probe - fbase]).
-__global__ __launch_bounds__(TT, 2) void k_tile_fixp(c128 *delta, TSweepArgs sw, const TSeg *__restrict__ segs,
-                                                       const TGroup *__restrict__ groups,
-                                                       const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
-                                                       const c128 *const *__restrict__ ftab, int fbase,
-                                                       c128 *__restrict__ part, int row) {
-    tile_sweep<true>(delta, delta, sw, segs, groups, diag, pool, ftab, fbase, part, row, blockIdx.x);
 }
 
 // --------------------------------------------------------- adjoint sweep ----
 // The reverse pass of the pure-state adjoint over LDS tiles: each thread holds
-// 16 amplitudes of y and the same 16 of l; every op applies to both (in
+// TN amplitudes of y and the same TN of l; every op applies to both (in
 // place), and a probe adds this wave's share of Re<y|D|l> to the partials of
 // its parameter (one entry per wave per tile, reduced afterwards in a fixed
 // order, so gradients are reproducible).  Adjoint plans hold dense ops,
 // controlled-Z-type sign flips and probes only (no general diagonals).
 
-template <int A, int B, bool FLIP>
+template <int RB, int A, int B, bool FLIP>
 __device__ __forceinline__ void ip_apply2(c128 *v, const c128 *__restrict__ M, unsigned F) {
     c128 m[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) m[i] = M[i];
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
+    for (int j = 0; j < TGeo<RB>::TN; ++j) {
         if (j & ((1 << A) | (1 << B))) continue;
         const int idx[4] = {j, j | (1 << A), j | (1 << B), j | (1 << A) | (1 << B)};
         const c128 x0 = v[idx[0]], x1 = v[idx[1]], x2 = v[idx[2]], x3 = v[idx[3]];
@@ -523,56 +540,44 @@ __device__ __forceinline__ void ip_apply2(c128 *v, const c128 *__restrict__ M, u
     }
 }
 
-template <int A>
+template <int RB, int A>
 __device__ __forceinline__ void diag1_ip(c128 *v, const c128 *__restrict__ M) {
     const c128 d0 = M[0], d1 = M[3];
 #pragma unroll
-    for (int j = 0; j < TN; ++j) v[j] = cmul((j >> A) & 1 ? d1 : d0, v[j]);
+    for (int j = 0; j < TGeo<RB>::TN; ++j) v[j] = cmul((j >> A) & 1 ? d1 : d0, v[j]);
 }
 
+template <int RB>
 __device__ __forceinline__ void dense_ip(int code, c128 *v, const c128 *__restrict__ M, unsigned F) {
     if (code >= C_DIAG1) {
         switch (code - C_DIAG1) {
-            case 0: diag1_ip<0>(v, M); break;
-            case 1: diag1_ip<1>(v, M); break;
-            case 2: diag1_ip<2>(v, M); break;
-            default: diag1_ip<3>(v, M); break;
+            case 0: diag1_ip<RB, 0>(v, M); break;
+            case 1: diag1_ip<RB, 1>(v, M); break;
+            case 2: diag1_ip<RB, 2>(v, M); break;
+            default:
+                if constexpr (RB > 3) diag1_ip<RB, 3>(v, M);
+                break;
         }
         return;
     }
     if (code >= C_PERM) {
         if (code >= C_PERM + kPermFlip)
-            perm_ip<true>(code - C_PERM - kPermFlip, v, F);
+            perm_ip<RB, true>(code - C_PERM - kPermFlip, v, F);
         else
-            perm_ip<false>(code - C_PERM, v, F);
+            perm_ip<RB, false>(code - C_PERM, v, F);
         return;
     }
+#define MOSSQ_IP1(R, A, FL) reg_apply1<R, A, FL>(v, v, M, F)
+#define MOSSQ_IP2(R, A, B, FL) ip_apply2<R, A, B, FL>(v, M, F)
     switch (code) {
-        case C_DENSE1 + 0: reg_apply1<0, false>(v, v, M, F); break;
-        case C_DENSE1 + 1: reg_apply1<1, false>(v, v, M, F); break;
-        case C_DENSE1 + 2: reg_apply1<2, false>(v, v, M, F); break;
-        case C_DENSE1 + 3: reg_apply1<3, false>(v, v, M, F); break;
-        case C_DENSE2 + 0: ip_apply2<0, 1, false>(v, M, F); break;
-        case C_DENSE2 + 1: ip_apply2<0, 2, false>(v, M, F); break;
-        case C_DENSE2 + 2: ip_apply2<0, 3, false>(v, M, F); break;
-        case C_DENSE2 + 3: ip_apply2<1, 2, false>(v, M, F); break;
-        case C_DENSE2 + 4: ip_apply2<1, 3, false>(v, M, F); break;
-        case C_DENSE2 + 5: ip_apply2<2, 3, false>(v, M, F); break;
-        case C_FLIP + C_DENSE1 + 0: reg_apply1<0, true>(v, v, M, F); break;
-        case C_FLIP + C_DENSE1 + 1: reg_apply1<1, true>(v, v, M, F); break;
-        case C_FLIP + C_DENSE1 + 2: reg_apply1<2, true>(v, v, M, F); break;
-        case C_FLIP + C_DENSE1 + 3: reg_apply1<3, true>(v, v, M, F); break;
-        case C_FLIP + C_DENSE2 + 0: ip_apply2<0, 1, true>(v, M, F); break;
-        case C_FLIP + C_DENSE2 + 1: ip_apply2<0, 2, true>(v, M, F); break;
-        case C_FLIP + C_DENSE2 + 2: ip_apply2<0, 3, true>(v, M, F); break;
-        case C_FLIP + C_DENSE2 + 3: ip_apply2<1, 2, true>(v, M, F); break;
-        case C_FLIP + C_DENSE2 + 4: ip_apply2<1, 3, true>(v, M, F); break;
-        case C_FLIP + C_DENSE2 + 5: ip_apply2<2, 3, true>(v, M, F); break;
+        MOSSQ_DENSE_CASES(RB, MOSSQ_IP1, MOSSQ_IP2)
         default:  // C_COPY: flips only
 #pragma unroll
-            for (int j = 0; j < TN; ++j) v[j] = flip_slot(v[j], F, j);
+            for (int j = 0; j < TGeo<RB>::TN; ++j) v[j] = flip_slot(v[j], F, j);
             break;
     }
+#undef MOSSQ_IP1
+#undef MOSSQ_IP2
 }
 
 // Re sum over the thread's slots of conj(y) (D l), accumulating only the
@@ -580,11 +585,11 @@ __device__ __forceinline__ void dense_ip(int code, c128 *v, const c128 *__restri
 // entry e = r + dim * c, which pairs conj(y) at row r with l at column c):
 // Re(D conj(a) b) = Re D Re(conj(a) b) - Im D Im(conj(a) b).  Gradient probes
 // of rotations touch two of the eight parts of a 2x2 D.
-template <int NB>
+template <int RB>
 __device__ __forceinline__ double probe_part(const c128 *y, const c128 *l, int msk, int orow, int ocol, bool imag) {
     double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
+    for (int j = 0; j < TGeo<RB>::TN; ++j) {
         if (j & msk) continue;
         const c128 a = y[j | orow], b = l[j | ocol];
         acc = imag ? fma(a.re, b.im, fma(-a.im, b.re, acc)) : fma(a.re, b.re, fma(a.im, b.im, acc));
@@ -592,7 +597,7 @@ __device__ __forceinline__ double probe_part(const c128 *y, const c128 *l, int m
     return acc;
 }
 
-template <int A, int B>
+template <int RB, int A, int B>
 __device__ __forceinline__ double probe_re(const c128 *y, const c128 *l, const c128 *__restrict__ D, unsigned mask) {
     constexpr int dim = B < 0 ? 2 : 4;
     constexpr int msk = B < 0 ? (1 << A) : ((1 << A) | (1 << B));
@@ -602,47 +607,41 @@ __device__ __forceinline__ double probe_re(const c128 *y, const c128 *l, const c
         const int r = e % dim, cc = e / dim;
         const int orow = B < 0 ? (r << A) : (((r & 1) << A) | ((r >> 1) << (B < 0 ? 0 : B)));
         const int ocol = B < 0 ? (cc << A) : (((cc & 1) << A) | ((cc >> 1) << (B < 0 ? 0 : B)));
-        if ((mask >> (2 * e)) & 1u) acc = fma(D[e].re, probe_part<0>(y, l, msk, orow, ocol, false), acc);
-        if ((mask >> (2 * e + 1)) & 1u) acc = fma(-D[e].im, probe_part<0>(y, l, msk, orow, ocol, true), acc);
+        if ((mask >> (2 * e)) & 1u) acc = fma(D[e].re, probe_part<RB>(y, l, msk, orow, ocol, false), acc);
+        if ((mask >> (2 * e + 1)) & 1u) acc = fma(-D[e].im, probe_part<RB>(y, l, msk, orow, ocol, true), acc);
     }
     return acc;
 }
 
+template <int RB>
 __device__ __forceinline__ double probe_op_re(int code, unsigned mask, const c128 *y, const c128 *l,
                                               const c128 *__restrict__ D) {
     switch (code) {
-        case C_DENSE1 + 0: return probe_re<0, -1>(y, l, D, mask);
-        case C_DENSE1 + 1: return probe_re<1, -1>(y, l, D, mask);
-        case C_DENSE1 + 2: return probe_re<2, -1>(y, l, D, mask);
-        case C_DENSE1 + 3: return probe_re<3, -1>(y, l, D, mask);
-        case C_DENSE2 + 0: return probe_re<0, 1>(y, l, D, mask);
-        case C_DENSE2 + 1: return probe_re<0, 2>(y, l, D, mask);
-        case C_DENSE2 + 2: return probe_re<0, 3>(y, l, D, mask);
-        case C_DENSE2 + 3: return probe_re<1, 2>(y, l, D, mask);
-        case C_DENSE2 + 4: return probe_re<1, 3>(y, l, D, mask);
-        default: return probe_re<2, 3>(y, l, D, mask);
+        case C_DENSE1 + 0: return probe_re<RB, 0, -1>(y, l, D, mask);
+        case C_DENSE1 + 1: return probe_re<RB, 1, -1>(y, l, D, mask);
+        case C_DENSE1 + 2: return probe_re<RB, 2, -1>(y, l, D, mask);
+        case C_DENSE2 + 0: return probe_re<RB, 0, 1>(y, l, D, mask);
+        case C_DENSE2 + 1: return probe_re<RB, 0, 2>(y, l, D, mask);
+        case C_DENSE2 + 3: return probe_re<RB, 1, 2>(y, l, D, mask);
+        default:
+            if constexpr (RB > 3) {
+                switch (code) {
+                    case C_DENSE1 + 3: return probe_re<RB, 3, -1>(y, l, D, mask);
+                    case C_DENSE2 + 2: return probe_re<RB, 0, 3>(y, l, D, mask);
+                    case C_DENSE2 + 4: return probe_re<RB, 1, 3>(y, l, D, mask);
+                    default: return probe_re<RB, 2, 3>(y, l, D, mask);
+                }
+            }
+            return 0.0;
     }
 }
 
-__device__ __forceinline__ double wave_sum_re(double a) { return wave_sum_f64(a); }
-
-
-__device__ __forceinline__ void transpose_lds(c128 *v, char *ldsb, int t, const TSeg &sg) {
-    int a[TN];
-    lds_slots(t, sg.wt, sg.wr, a);
-#pragma unroll
-    for (int j = 0; j < TN; ++j) *reinterpret_cast<c128 *>(ldsb + a[j]) = v[j];
-    __syncthreads();
-    lds_slots(t, sg.rt, sg.rr, a);
-#pragma unroll
-    for (int j = 0; j < TN; ++j) v[j] = *reinterpret_cast<const c128 *>(ldsb + a[j]);
-    __syncthreads();
-}
-
+template <int RB>
 __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__restrict__ ls, const TSweepArgs &sw,
                                                const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
-                                               const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
-                                               c128 *__restrict__ part, int row, u64 tile) {
+                                               const c128 *__restrict__ pool, c128 *__restrict__ part, int row,
+                                               u64 tile) {
+    constexpr int TN = TGeo<RB>::TN;
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
     const int t = threadIdx.x;
@@ -650,71 +649,81 @@ __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__re
     const int seg0 = sw.seg0, seg1 = sw.seg1;
 
     TSeg sg = segs[seg0];
-    int tdep = seg_tdep(t, sg);
+    int tdep = seg_tdep<RB>(t, sg);
     c128 y[TN], l[TN];
     {
-        const u64 g = base | seg_gthread(t, sg);
+        const u64 g = base | seg_gthread<RB>(t, sg);
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-            u64 o = 0;
-#pragma unroll
-            for (int i = 0; i < TRB; ++i)
-                if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            y[j] = load_nt(ys + (g | o));
-            l[j] = load_nt(ls + (g | o));
+            const u64 o = g | slot_goff<RB>(j, sg);
+            y[j] = load_nt(ys + o);
+            l[j] = load_nt(ls + o);
         }
     }
-    const int slot = int(tile) * 4 + (t >> 6);
+    const int slot = int(tile) * TGeo<RB>::WPT + (t >> 6);
     for (int s = seg0; s < seg1; ++s) {
         sg = segs[s];
         if (sg.transpose) {
-            transpose_lds(y, ldsb, t, sg);
-            transpose_lds(l, ldsb, t, sg);
-            tdep = seg_tdep(t, sg);
+            transpose_lds<RB>(y, ldsb, t, sg);
+            transpose_lds<RB>(l, ldsb, t, sg);
+            tdep = seg_tdep<RB>(t, sg);
         }
         for (int gi = sg.g0; gi < sg.g1; ++gi) {
             const TGroup g = groups[gi];
             if (g.code != C_COPY || g.ncz > 0) {
                 const unsigned F = group_flips(g, tdep, base);
-                dense_ip(g.code, y, pool + g.moff, F);
-                dense_ip(g.code, l, pool + g.moff, F);
+                dense_ip<RB>(g.code, y, pool + g.moff, F);
+                dense_ip<RB>(g.code, l, pool + g.moff, F);
             }
             if (g.probe >= 0) {
-                const double acc = wave_sum_re(probe_op_re(g.pcode, g.pmask, y, l, pool + g.pmoff));
+                const double acc = wave_sum_re(probe_op_re<RB>(g.pcode, g.pmask, y, l, pool + g.pmoff));
                 if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = c128{acc, 0.0};
             }
         }
     }
     {
-        const u64 g = base | seg_gthread(t, sg);
+        const u64 g = base | seg_gthread<RB>(t, sg);
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-            u64 o = 0;
-#pragma unroll
-            for (int i = 0; i < TRB; ++i)
-                if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-            store_nt(ys + (g | o), y[j]);
-            store_nt(ls + (g | o), l[j]);
+            const u64 o = g | slot_goff<RB>(j, sg);
+            store_nt(ys + o, y[j]);
+            store_nt(ls + o, l[j]);
         }
     }
 }
 
-__global__ __launch_bounds__(TT, 2) void k_tile_adj(c128 *__restrict__ ys, c128 *__restrict__ ls, TSweepArgs sw,
-                                                      const TSeg *__restrict__ segs,
-                                                      const TGroup *__restrict__ groups,
-                                                      const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
-                                                      c128 *__restrict__ part, int row) {
-    tile_adj_sweep(ys, ls, sw, segs, groups, diag, pool, part, row, blockIdx.x);
-}
+// Kernels per register-bit count: 4 bits, 256 threads (two waves per SIMD);
+// 3 bits, 512 threads (four waves per SIMD).
+#define MOSSQ_TILE_KERNELS(RB, TTHREADS, WAVES)                                                                      \
+    __global__ __launch_bounds__(TTHREADS, WAVES) void k_tile##RB(                                                   \
+        const c128 *src, c128 *dst, TSweepArgs sw, const TSeg *__restrict__ segs, const TGroup *__restrict__ groups, \
+        const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {                                             \
+        tile_sweep<RB, false>(src, dst, sw, segs, groups, diag, pool, nullptr, 0, nullptr, 0, blockIdx.x);          \
+    }                                                                                                                \
+    /* the density adjoint's reverse chunks: Delta in place, each probe against */                                   \
+    /* the state before its parameterized gate (ftab[probe - fbase]) */                                              \
+    __global__ __launch_bounds__(TTHREADS, WAVES) void k_tile_fixp##RB(                                              \
+        c128 *delta, TSweepArgs sw, const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,                \
+        const TDiag *__restrict__ diag, const c128 *__restrict__ pool, const c128 *const *__restrict__ ftab,         \
+        int fbase, c128 *__restrict__ part, int row) {                                                               \
+        tile_sweep<RB, true>(delta, delta, sw, segs, groups, diag, pool, ftab, fbase, part, row, blockIdx.x);       \
+    }                                                                                                                \
+    __global__ __launch_bounds__(TTHREADS, WAVES) void k_tile_adj##RB(                                               \
+        c128 *__restrict__ ys, c128 *__restrict__ ls, TSweepArgs sw, const TSeg *__restrict__ segs,                  \
+        const TGroup *__restrict__ groups, const c128 *__restrict__ pool, c128 *__restrict__ part, int row) {        \
+        tile_adj_sweep<RB>(ys, ls, sw, segs, groups, pool, part, row, blockIdx.x);                                  \
+    }
+MOSSQ_TILE_KERNELS(4, 256, 2)
+MOSSQ_TILE_KERNELS(3, 512, 4)
 
 // ------------------------------------------------------ batched sweeps ----
 // Sweeps of many programs in one launch: item i is one sweep of one program's
 // plan on that program's own vectors, one workgroup per tile; wg_item gives
 // each workgroup its item and tile (unsigned item << 16 | tile, both below
-// 2^16; a launch holds at most 65536 items; tile0 is the item's
-// first tile in a count of the launch's tiles, used on the host).  The vectors are
-// offsets into one allocation `vec` (partials into `part`), which the host
-// checks against that allocation's size.
+// 2^16; a launch holds at most 65536 items, all with plans of one register-bit
+// count; tile0 is the item's first tile in a count of the launch's tiles, used
+// on the host).  The vectors are offsets into one allocation `vec` (partials
+// into `part`), which the host checks against that allocation's size.
 // The plan tables' addresses become constant-memory pointers in the kernel
 // (the tables are read-only while a sweep runs).  Generic pointers read from
 // memory would load through the flat path, into vector registers; constant
@@ -737,22 +746,25 @@ struct TBItem {
     TSweepArgs args;
 };
 
-__global__ __launch_bounds__(TT, 2) void k_btile(const TBItem *__restrict__ items, const unsigned *__restrict__ wg_item,
-                                                   c128 *__restrict__ vec) {
-    const unsigned e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);
-    const TBItem &it = items[e >> 16];
-    tile_sweep<false>(vec + it.x, vec + it.x, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),
-                      table<TDiag>(it.diag), table<c128>(it.pool), nullptr, 0, nullptr, 0, u64(e & 0xFFFF));
-}
-
-__global__ __launch_bounds__(TT, 2) void k_btile_adj(const TBItem *__restrict__ items,
-                                                       const unsigned *__restrict__ wg_item, c128 *__restrict__ vec,
-                                                       c128 *__restrict__ part) {
-    const unsigned e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);
-    const TBItem &it = items[e >> 16];
-    tile_adj_sweep(vec + it.x, vec + it.l, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),
-                   table<TDiag>(it.diag), table<c128>(it.pool), part + it.part, it.row, u64(e & 0xFFFF));
-}
+#define MOSSQ_BTILE_KERNELS(RB, TTHREADS, WAVES)                                                                    \
+    __global__ __launch_bounds__(TTHREADS, WAVES) void k_btile##RB(                                                 \
+        const TBItem *__restrict__ items, const unsigned *__restrict__ wg_item, c128 *__restrict__ vec) {            \
+        const unsigned e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);                                      \
+        const TBItem &it = items[e >> 16];                                                                          \
+        tile_sweep<RB, false>(vec + it.x, vec + it.x, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),      \
+                              table<TDiag>(it.diag), table<c128>(it.pool), nullptr, 0, nullptr, 0, u64(e & 0xFFFF)); \
+    }                                                                                                               \
+    __global__ __launch_bounds__(TTHREADS, WAVES) void k_btile_adj##RB(const TBItem *__restrict__ items,             \
+                                                                       const unsigned *__restrict__ wg_item,         \
+                                                                       c128 *__restrict__ vec,                      \
+                                                                       c128 *__restrict__ part) {                   \
+        const unsigned e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);                                      \
+        const TBItem &it = items[e >> 16];                                                                          \
+        tile_adj_sweep<RB>(vec + it.x, vec + it.l, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),         \
+                           table<c128>(it.pool), part + it.part, it.row, u64(e & 0xFFFF));                          \
+    }
+MOSSQ_BTILE_KERNELS(4, 256, 2)
+MOSSQ_BTILE_KERNELS(3, 512, 4)
 
 // ------------------------------------------------------------------ planner ----
 // kind 3 (adjoint plans): probe Re<y|D|l> of parameter `probe` on the two
@@ -791,6 +803,7 @@ struct FusedPlan {
     TGroup *d_groups = nullptr;
     TDiag *d_diag = nullptr;
     c128 *d_pool = nullptr;
+    int rb = 4;            // register bits of every layout (4: 256 threads per tile, 3: 512)
     bool adjoint = false;  // reverse plan of the pure-state adjoint (probes, no general diagonals)
     bool reverse_steps = false;  // unfused items apply the steps' reverse matrices
     bool shared_device = false;  // device arrays live in a block owned elsewhere
@@ -1169,15 +1182,16 @@ static bool sign_only(const std::vector<c128> &d, int *negmask) {
     return true;
 }
 
-static TSeg make_seg(const int *reg, int op0, int op1, const int *tbits) {
+static TSeg make_seg(const int *reg, int rb, int op0, int op1, const int *tbits) {
     TSeg sg{};
-    for (int i = 0; i < TRB; ++i) {
+    for (int i = 0; i < kRegMax; ++i) sg.reg[i] = sg.gbit[i] = -1;
+    for (int i = 0; i < rb; ++i) {
         sg.reg[i] = reg[i];
         sg.gbit[i] = tbits[reg[i]];
     }
     int k = 0;
     for (int b = 0; b < TM; ++b) {
-        if (b == reg[0] || b == reg[1] || b == reg[2] || b == reg[3]) continue;
+        if (std::find(reg, reg + rb, b) != reg + rb) continue;
         sg.tpos[k] = b;
         sg.gpos[k] = tbits[b];
         ++k;
@@ -1187,7 +1201,12 @@ static TSeg make_seg(const int *reg, int op0, int op1, const int *tbits) {
     return sg;
 }
 
-static const int kStoreLayout[TRB] = {8, 9, 10, 11};
+// Layout of a sweep's loads and stores when no segment layout keeps the
+// lowest tile bits on lanes: the top rb tile bits in registers.
+static const int *store_layout(int rb) {
+    static const int l4[kRegMax] = {8, 9, 10, 11}, l3[kRegMax] = {9, 10, 11, -1};
+    return rb == 3 ? l3 : l4;
+}
 
 // Rank of up to 4 vectors over GF(2).
 static int gf2_rank(int *v, int n) {
@@ -1235,7 +1254,7 @@ static bool swizzle_search(const TSeg &A, const TSeg &B, const std::vector<int> 
 }
 
 // Returns false when no conflict-free columns exist (the identity map is used).
-static bool set_transpose(const TSeg &prev, TSeg &next) {
+static bool set_transpose(const TSeg &prev, TSeg &next, int rb) {
     int col[TM];
     std::vector<int> vars;
     for (int b = 0; b < TM; ++b) col[b] = 1 << b;
@@ -1250,19 +1269,19 @@ static bool set_transpose(const TSeg &prev, TSeg &next) {
     if (!found)
         for (int b = 0; b < TM; ++b) col[b] = 1 << b;  // plain layout (bank conflicts, still correct)
     next.transpose = 1;
-    for (int k = 0; k < TTB; ++k) {
+    for (int k = 0; k < TM - rb; ++k) {
         next.wt[k] = col[prev.tpos[k]];
         next.rt[k] = col[next.tpos[k]];
     }
-    for (int i = 0; i < TRB; ++i) {
+    for (int i = 0; i < rb; ++i) {
         next.wr[i] = col[prev.reg[i]];
         next.rr[i] = col[next.reg[i]];
     }
     return found;
 }
 
-static bool safe_layout(const int *reg) {
-    for (int i = 0; i < TRB; ++i)
+static bool safe_layout(const int *reg, int rb) {
+    for (int i = 0; i < rb; ++i)
         if (reg[i] < kCoalesceBits) return false;
     return true;
 }
@@ -1368,9 +1387,11 @@ static u64 choose_tile_bits(const std::vector<POp> &ops, const std::vector<char>
 }
 
 static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, int min_tail, bool adjoint, bool kron,
-                                 std::string *err) {
+                                 int rb, std::string *err) {
     FusedPlan *fp = new FusedPlan();
     fp->adjoint = adjoint;
+    fp->rb = rb;
+    const int tn = 1 << rb;
     std::vector<char> done(ops.size(), 0);
     size_t first = 0;
     auto failp = [&](const char *m) {
@@ -1429,10 +1450,8 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
         auto fits = [&](size_t ci, const int *reg) {
             const POp &o = ops[chosen[ci]];
             if (o.kind == 1) return true;
-            for (int q = 0; q < o.nb; ++q) {
-                int l = local_of[o.bits[q]];
-                if (l != reg[0] && l != reg[1] && l != reg[2] && l != reg[3]) return false;
-            }
+            for (int q = 0; q < o.nb; ++q)
+                if (std::find(reg, reg + rb, local_of[o.bits[q]]) == reg + rb) return false;
             return true;
         };
         // Ops a segment with register set `reg` would schedule from the current state.
@@ -1469,17 +1488,17 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                         int l = local_of[o.bits[q]];
                         if (std::find(want.begin(), want.end(), l) == want.end()) add.push_back(l);
                     }
-                    if (want.size() + add.size() <= TRB) want.insert(want.end(), add.begin(), add.end());
+                    if (want.size() + add.size() <= size_t(rb)) want.insert(want.end(), add.begin(), add.end());
                 };
                 try_add(ops[chosen[c0]]);
-                for (size_t ci = 0; ci < chosen.size() && want.size() < TRB; ++ci)
+                for (size_t ci = 0; ci < chosen.size() && want.size() < size_t(rb); ++ci)
                     if (!sched[ci] && ci != c0 && ops[chosen[ci]].kind != 1) try_add(ops[chosen[ci]]);
-                for (int i = 0; i < TRB && want.size() < TRB; ++i)  // keep current bits
+                for (int i = 0; i < rb && want.size() < size_t(rb); ++i)  // keep current bits
                     if (std::find(want.begin(), want.end(), reg[i]) == want.end()) want.push_back(reg[i]);
                 std::sort(want.begin(), want.end());
                 int score = simulate(want.data());
                 int keep = 0;
-                for (int i = 0; i < TRB; ++i) keep += std::find(want.begin(), want.end(), reg[i]) != want.end();
+                for (int i = 0; i < rb; ++i) keep += std::find(want.begin(), want.end(), reg[i]) != want.end();
                 if (score > best_score || (score == best_score && keep > best_keep)) {
                     best = want;
                     best_score = score;
@@ -1487,20 +1506,21 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                 }
             }
             if (best.empty()) {  // no dense op is ready: any layout works
-                best.assign(reg, reg + TRB);
+                best.assign(reg, reg + rb);
                 std::sort(best.begin(), best.end());
             }
-            if (best.size() != TRB || best[0] < 0 || best[TRB - 1] >= TM ||
+            if (best.size() != size_t(rb) || best[0] < 0 || best[rb - 1] >= TM ||
                 std::adjacent_find(best.begin(), best.end()) != best.end())
                 return false;
-            for (int i = 0; i < TRB; ++i) reg[i] = best[i];
+            for (int i = 0; i < rb; ++i) reg[i] = best[i];
             return true;
         };
-        int reg0[TRB] = {8, 9, 10, 11};
+        int reg0[kRegMax];
+        std::copy(store_layout(rb), store_layout(rb) + kRegMax, reg0);
         if (!next_reg(reg0)) return failp("fusion planner produced an invalid register set");
         // Schedule: segments as (register set, chosen indices in execution order).
         struct SegPlan {
-            int reg[TRB];
+            int reg[kRegMax];
             std::vector<int> idx;
             int work;
         };
@@ -1533,18 +1553,18 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
             plan.pop_back();
         }
         it.args.seg0 = int(fp->h_segs.size());
-        if (!safe_layout(plan.front().reg)) {
+        if (!safe_layout(plan.front().reg, rb)) {
             const int n0 = int(fp->h_ops.size());
-            fp->h_segs.push_back(make_seg(kStoreLayout, n0, n0, it.args.tbits));
+            fp->h_segs.push_back(make_seg(store_layout(rb), rb, n0, n0, it.args.tbits));
         }
         for (const SegPlan &sp : plan) {
             const int *reg = sp.reg;
             const int op0 = int(fp->h_ops.size());
             std::vector<POp> seg_ops;
             for (int ci : sp.idx) seg_ops.push_back(ops[chosen[ci]]);
-            int reg_global[TRB];
-            for (int i = 0; i < TRB; ++i) reg_global[i] = it.args.tbits[reg[i]];
-            peephole(seg_ops, reg_global, TRB, kron);
+            int reg_global[kRegMax];
+            for (int i = 0; i < rb; ++i) reg_global[i] = it.args.tbits[reg[i]];
+            peephole(seg_ops, reg_global, rb, kron);
             for (const POp &o : seg_ops) {
                 TOp t{};
                 t.moff = int(fp->h_pool.size());
@@ -1554,7 +1574,7 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                     for (int q = 0; q < 4; ++q) t.qbit[q] = t.lbit[q] = -1;
                     for (int q = 0; q < o.nb; ++q) {
                         int l = local_of[o.bits[q]];
-                        for (int i = 0; i < TRB; ++i)
+                        for (int i = 0; i < rb; ++i)
                             if (reg[i] == l && l >= 0) ridx[q] = i;
                         if (ridx[q] < 0) {
                             if (l >= 0)
@@ -1567,7 +1587,7 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                     for (int q = 0; q < o.nb; ++q)
                         if (ridx[q] >= 0) t.rmask |= 1 << ridx[q];
                     t.rp_lo = t.rp_hi = 0;
-                    for (int j = 0; j < TN; ++j) {
+                    for (int j = 0; j < tn; ++j) {
                         unsigned rp = 0;
                         for (int q = 0; q < o.nb; ++q)
                             if (ridx[q] >= 0 && ((j >> ridx[q]) & 1)) rp |= 1u << q;
@@ -1592,7 +1612,7 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                     fp->h_pool.insert(fp->h_pool.end(), o.m.begin(), o.m.end());
                     auto ridx = [&](int gb) {
                         int l = local_of[gb];
-                        for (int i = 0; i < TRB; ++i)
+                        for (int i = 0; i < rb; ++i)
                             if (reg[i] == l) return i;
                         return -1;
                     };
@@ -1610,11 +1630,11 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                 }
                 fp->h_ops.push_back(t);
             }
-            fp->h_segs.push_back(make_seg(reg, op0, int(fp->h_ops.size()), it.args.tbits));
+            fp->h_segs.push_back(make_seg(reg, rb, op0, int(fp->h_ops.size()), it.args.tbits));
         }
-        if (!safe_layout(fp->h_segs.back().reg)) {
+        if (!safe_layout(fp->h_segs.back().reg, rb)) {
             const int n1 = int(fp->h_ops.size());
-            fp->h_segs.push_back(make_seg(kStoreLayout, n1, n1, it.args.tbits));
+            fp->h_segs.push_back(make_seg(store_layout(rb), rb, n1, n1, it.args.tbits));
         }
         it.args.seg1 = int(fp->h_segs.size());
         if (getenv("MOSSQ_PLAN_DEBUG")) {
@@ -1625,9 +1645,9 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                 const TSeg &g = fp->h_segs[q];
                 int changed = 0;
                 if (q > it.args.seg0)
-                    for (int i = 0; i < TRB; ++i) {
+                    for (int i = 0; i < rb; ++i) {
                         bool found = false;
-                        for (int k = 0; k < TRB; ++k) found |= fp->h_segs[q - 1].reg[k] == g.reg[i];
+                        for (int k = 0; k < rb; ++k) found |= fp->h_segs[q - 1].reg[k] == g.reg[i];
                         changed += !found;
                     }
                 int nd1 = 0, nd2 = 0, ndg = 0;
@@ -1655,8 +1675,8 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
             }
         }
         for (int q = it.args.seg0 + 1; q < it.args.seg1; ++q)
-            if (memcmp(fp->h_segs[q].reg, fp->h_segs[q - 1].reg, sizeof(int) * TRB) != 0) {
-                if (!set_transpose(fp->h_segs[q - 1], fp->h_segs[q])) fp->nconflicted++;
+            if (memcmp(fp->h_segs[q].reg, fp->h_segs[q - 1].reg, sizeof(int) * rb) != 0) {
+                if (!set_transpose(fp->h_segs[q - 1], fp->h_segs[q], rb)) fp->nconflicted++;
                 fp->ntransitions++;
             }
         fp->items.push_back(it);
@@ -1673,7 +1693,7 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
 // Packed form of a controlled-Z-type sign flip (a sign-only diagonal whose
 // only -1 is at the all-ones index): its slot pattern and condition bits.
 // False for other diagonals or more than three bits outside the registers.
-static bool pack_flip(const TOp &o, unsigned *pat, int *cond) {
+static bool pack_flip(const TOp &o, int tn, unsigned *pat, int *cond) {
     if (o.type != T_SIGN || o.negmask != (1 << ((1 << o.k) - 1))) return false;
     int w = 0, n = 0;
     for (int q = 0; q < o.k; ++q) {
@@ -1689,7 +1709,7 @@ static bool pack_flip(const TOp &o, unsigned *pat, int *cond) {
     }
     for (; n < 3; ++n) w |= kCzNone << (7 * n);
     unsigned pm = 0;
-    for (int j = 0; j < TN; ++j)
+    for (int j = 0; j < tn; ++j)
         if ((j & o.rmask) == o.rmask) pm |= 1u << j;
     *pat = pm;
     *cond = w;
@@ -1762,7 +1782,7 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
             if (o.type == T_SIGN && o.negmask == 0) continue;  // identity (e.g. two equal CZs merged)
             unsigned pat;
             int cond;
-            if (pack_flip(o, &pat, &cond)) {
+            if (pack_flip(o, 1 << fp->rb, &pat, &cond)) {
                 TGroup *g = here ? &fp->h_groups.back() : nullptr;
                 if (!g || g->code == C_DIAG || g->code >= C_DIAG1 || g->ncz == kGroupCz || g->probe >= 0) {
                     add(C_COPY, 0);
@@ -1822,15 +1842,15 @@ static double plan_cost(const FusedPlan *fp) {
 // The tail threshold trades a transposition against the ops it carries into
 // the next sweep; which side wins depends on the circuit, so each candidate is
 // planned and the cheapest plan under the cost model is kept.
-static FusedPlan *best_plan(const Program *p, const std::vector<POp> &ops, bool adjoint, std::string *err) {
-    if (const char *mt = getenv("MOSSQ_PLAN_MIN_TAIL")) return plan_with_tail(p, ops, atoi(mt), adjoint, true, err);
+static FusedPlan *best_plan(const Program *p, const std::vector<POp> &ops, bool adjoint, int rb, std::string *err) {
+    if (const char *mt = getenv("MOSSQ_PLAN_MIN_TAIL")) return plan_with_tail(p, ops, atoi(mt), adjoint, true, rb, err);
     // One tile holds the whole state: every op fits the first sweep, and a
     // tail threshold could only push ops into an extra pass.
-    if (p->nbits <= TM) return plan_with_tail(p, ops, 0, adjoint, true, err);
+    if (p->nbits <= TM) return plan_with_tail(p, ops, 0, adjoint, true, rb, err);
     FusedPlan *best = nullptr;
     double best_cost = 0;
     for (int min_tail : {0, 3, 5, 8}) {
-        FusedPlan *fp = plan_with_tail(p, ops, min_tail, adjoint, true, err);
+        FusedPlan *fp = plan_with_tail(p, ops, min_tail, adjoint, true, rb, err);
         if (!fp) {
             delete best;
             return nullptr;
@@ -1850,13 +1870,19 @@ static FusedPlan *best_plan(const Program *p, const std::vector<POp> &ops, bool 
     return best;
 }
 
-FusedPlan *fused_plan_build_host(const Program *p, std::string *err, bool adjoint) {
+// Register bits of the plans a program gets.
+static int plan_rb(bool adjoint) {
+    const char *e = getenv(adjoint ? "MOSSQ_TILE_RB_ADJ" : "MOSSQ_TILE_RB_FWD");
+    return e && atoi(e) == 3 ? 3 : 4;
+}
+
+FusedPlan *fused_plan_build_host(const Program *p, std::string *err, bool adjoint, int rb) {
     std::vector<POp> ops;
     if (adjoint)
         lower_steps_rev(p, ops);
     else
         lower_steps(p, ops);
-    return best_plan(p, ops, adjoint, err);
+    return best_plan(p, ops, adjoint, rb > 0 ? rb : plan_rb(adjoint), err);
 }
 
 // Host execution of a plan with the kernel's index arithmetic (tiles, layouts,
@@ -1887,8 +1913,10 @@ static void host_block(c128 *v, int nbits, int k, const int *bits, const c128 *m
 
 // One tile of a sweep for one state (forward plans) or the two states y, l of
 // an adjoint plan (probes add Re<y|D|l> contributions to grads).
-static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw, const FusedPlan *fp, u64 tile,
-                         c128 *grads, const c128 *const *ftab = nullptr, int fbase = 0) {
+template <int RB>
+static void emulate_tile_rb(const std::vector<c128 *> &states, const TSweepArgs &sw, const FusedPlan *fp, u64 tile,
+                            c128 *grads, const c128 *const *ftab, int fbase) {
+    constexpr int TN = TGeo<RB>::TN, TT = TGeo<RB>::TT;
     const size_t ns = states.size();
     std::vector<c128> lds(1 << TM);
     std::vector<std::vector<c128>> V(ns, std::vector<c128>(size_t(TT) * TN));
@@ -1896,17 +1924,12 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
     for (int i = 0; i < sw.ncbits; ++i)
         if ((tile >> i) & 1) base |= u64(1) << sw.cbits[i];
     // Slot j's register-bit offsets, as the kernel forms them.
-    auto slot_g = [](int j, const TSeg &sg) {
-        u64 o = 0;
-        for (int i = 0; i < TRB; ++i)
-            if (j & (1 << i)) o |= u64(1) << sg.gbit[i];
-        return o;
-    };
+    auto slot_g = [](int j, const TSeg &sg) { return slot_goff<RB>(j, sg); };
     const TSeg *lay = &fp->h_segs[sw.seg0];
     std::vector<int> tdep(TT);
     for (int t = 0; t < TT; ++t) {
-        tdep[t] = seg_tdep(t, *lay);
-        const u64 gth = base | seg_gthread(t, *lay);
+        tdep[t] = seg_tdep<RB>(t, *lay);
+        const u64 gth = base | seg_gthread<RB>(t, *lay);
         for (size_t q = 0; q < ns; ++q)
             for (int j = 0; j < TN; ++j) V[q][t * TN + j] = states[q][gth | slot_g(j, *lay)];
     }
@@ -1940,7 +1963,7 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
             for (size_t q = 0; q < ns; ++q) {
                 std::vector<char> written(lds.size(), 0);
                 for (int t = 0; t < TT; ++t) {
-                    lds_slots(t, sg.wt, sg.wr, a);
+                    lds_slots<RB>(t, sg.wt, sg.wr, a);
                     for (int j = 0; j < TN; ++j) {
                         lds[a[j] / sizeof(c128)] = V[q][t * TN + j];
                         written[a[j] / sizeof(c128)] = 1;
@@ -1949,11 +1972,11 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
                 if (std::count(written.begin(), written.end(), 1) != int(lds.size()))
                     fprintf(stderr, "mossq: LDS slot map is not a bijection\n");
                 for (int t = 0; t < TT; ++t) {
-                    lds_slots(t, sg.rt, sg.rr, a);
+                    lds_slots<RB>(t, sg.rt, sg.rr, a);
                     for (int j = 0; j < TN; ++j) V[q][t * TN + j] = lds[a[j] / sizeof(c128)];
                 }
             }
-            for (int t = 0; t < TT; ++t) tdep[t] = seg_tdep(t, sg);
+            for (int t = 0; t < TT; ++t) tdep[t] = seg_tdep<RB>(t, sg);
         }
         // Executes the device encoding (groups, packed flips, diagonal records,
         // probes), as the kernels do.
@@ -2004,7 +2027,7 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
                     if (ns == 2) {
                         dl.assign(V[1].begin() + t * TN, V[1].begin() + (t + 1) * TN);
                     } else {
-                        const u64 gth = base | seg_gthread(t, sg);
+                        const u64 gth = base | seg_gthread<RB>(t, sg);
                         const c128 *fixed = ftab[g.probe - fbase];
                         for (int j = 0; j < TN; ++j) dl[j] = fixed[gth | slot_g(j, sg)];
                     }
@@ -2018,10 +2041,18 @@ static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw
     }
     const TSeg &last = fp->h_segs[sw.seg1 - 1];
     for (int t = 0; t < TT; ++t) {
-        const u64 gth = base | seg_gthread(t, last);
+        const u64 gth = base | seg_gthread<RB>(t, last);
         for (size_t q = 0; q < ns; ++q)
             for (int j = 0; j < TN; ++j) states[q][gth | slot_g(j, last)] = V[q][t * TN + j];
     }
+}
+
+static void emulate_tile(const std::vector<c128 *> &states, const TSweepArgs &sw, const FusedPlan *fp, u64 tile,
+                         c128 *grads, const c128 *const *ftab = nullptr, int fbase = 0) {
+    if (fp->rb == 3)
+        emulate_tile_rb<3>(states, sw, fp, tile, grads, ftab, fbase);
+    else
+        emulate_tile_rb<4>(states, sw, fp, tile, grads, ftab, fbase);
 }
 
 void emulate_plan(const Program *p, const FusedPlan *fp, c128 *state, const c128 *const *ftab = nullptr,
@@ -2096,7 +2127,7 @@ int fused_plan_build(Program *p) {
     }
     for (int adj = 0; adj < (p->is_density ? 1 : 2); ++adj) {
         std::string err;
-        FusedPlan *fp = fused_plan_build_host(p, &err, adj != 0);
+        FusedPlan *fp = fused_plan_build_host(p, &err, adj != 0, 0);
         if (!fp && adj) break;  // no adjoint plan: the reverse pass runs step by step
         if (!fp) {
             fused_plan_free(p);
@@ -2158,8 +2189,12 @@ static int fused_run_plan(Ctx *c, const Program *p, const FusedPlan *fp, const c
                 MOSSQ_TRY(apply_step(c, p, it.step, dst, fp->reverse_steps));
                 continue;
             }
-            hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(TT), 0, c->stream, from, dst, it.args, fp->d_segs,
-                               fp->d_groups, fp->d_diag, fp->d_pool);
+            if (fp->rb == 3)
+                hipLaunchKernelGGL(k_tile3, dim3(ntiles), dim3(TGeo<3>::TT), 0, c->stream, from, dst, it.args,
+                                   fp->d_segs, fp->d_groups, fp->d_diag, fp->d_pool);
+            else
+                hipLaunchKernelGGL(k_tile4, dim3(ntiles), dim3(TGeo<4>::TT), 0, c->stream, from, dst, it.args,
+                                   fp->d_segs, fp->d_groups, fp->d_diag, fp->d_pool);
             MOSSQ_HIP(hipGetLastError());
             from = dst;
         }
@@ -2178,8 +2213,12 @@ static int fused_run_fixp(Ctx *c, const Program *p, const FusedPlan *fp, c128 *d
             MOSSQ_TRY(apply_step(c, p, it.step, delta, fp->reverse_steps));
             continue;
         }
-        hipLaunchKernelGGL(k_tile_fixp, dim3(ntiles), dim3(TT), 0, c->stream, delta, it.args, fp->d_segs,
-                           fp->d_groups, fp->d_diag, fp->d_pool, ftab, fbase, part, row);
+        if (fp->rb == 3)
+            hipLaunchKernelGGL(k_tile_fixp3, dim3(ntiles), dim3(TGeo<3>::TT), 0, c->stream, delta, it.args,
+                               fp->d_segs, fp->d_groups, fp->d_diag, fp->d_pool, ftab, fbase, part, row);
+        else
+            hipLaunchKernelGGL(k_tile_fixp4, dim3(ntiles), dim3(TGeo<4>::TT), 0, c->stream, delta, it.args,
+                               fp->d_segs, fp->d_groups, fp->d_diag, fp->d_pool, ftab, fbase, part, row);
         MOSSQ_HIP(hipGetLastError());
     }
     return MOSSQ_OK;
@@ -2190,9 +2229,12 @@ int fused_run(Ctx *c, const Program *p, c128 *state) {
     return fused_run_plan(c, p, p->fused, state, state);
 }
 
+// Waves per tile of a plan's sweeps (one gradient partial each).
+static int plan_wpt(const FusedPlan *fp) { return fp->rb == 3 ? TGeo<3>::WPT : TGeo<4>::WPT; }
+
 int fused_adjoint_parts(const Program *p) {
     if (!p->fused_rev) return 0;
-    return int(u64(4) << (p->nbits - TM));  // one partial per wave per tile
+    return int(u64(plan_wpt(p->fused_rev)) << (p->nbits - TM));  // one partial per wave per tile
 }
 
 int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, int row) {
@@ -2203,8 +2245,12 @@ int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, in
             MOSSQ_TRY(adjoint_rev_step(c, p, it.step, y, l, part, row));
             continue;
         }
-        hipLaunchKernelGGL(k_tile_adj, dim3(ntiles), dim3(TT), 0, c->stream, y, l, it.args, fp->d_segs,
-                           fp->d_groups, fp->d_diag, fp->d_pool, part, row);
+        if (fp->rb == 3)
+            hipLaunchKernelGGL(k_tile_adj3, dim3(ntiles), dim3(TGeo<3>::TT), 0, c->stream, y, l, it.args,
+                               fp->d_segs, fp->d_groups, fp->d_pool, part, row);
+        else
+            hipLaunchKernelGGL(k_tile_adj4, dim3(ntiles), dim3(TGeo<4>::TT), 0, c->stream, y, l, it.args,
+                               fp->d_segs, fp->d_groups, fp->d_pool, part, row);
         MOSSQ_HIP(hipGetLastError());
     }
     return MOSSQ_OK;
@@ -2248,22 +2294,25 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
     out->item0.clear();
     out->wg0.clear();
     out->tiles.clear();
+    out->rb.clear();
     size_t nlaunch = 0;
     for (int b = 0; b < m; ++b) {
         MOSSQ_CHECK(fused_plan_batchable(progs[b], adjoint), MOSSQ_EINVAL, "program has no all-sweep fused plan");
         const FusedPlan *fp = adjoint ? progs[b]->fused_rev : progs[b]->fused;
         nlaunch = std::max(nlaunch, fp->items.size());
     }
-    for (size_t s = 0; s < nlaunch; ++s) {
+    for (size_t s = 0; s < nlaunch; ++s)
+     for (int rb : {4, 3}) {
       for (int b0 = 0; b0 < m;) {
-        // one launch: the next kBatchLaunchItems programs that have sweep s
+        // one launch: the next kBatchLaunchItems programs whose plans have
+        // sweep s and rb register bits
         const size_t i0 = items.size(), w0 = wg.size();
         u64 tiles = 0;
         int b = b0;
         for (; b < m && items.size() - i0 < size_t(kBatchLaunchItems); ++b) {
             const Program *p = progs[b];
             const FusedPlan *fp = adjoint ? p->fused_rev : p->fused;
-            if (s >= fp->items.size()) continue;
+            if (s >= fp->items.size() || fp->rb != rb) continue;
             TBItem it{};
             const u64 len = u64(1) << p->nbits, nt = len >> TM;
             it.x = x[b];
@@ -2279,7 +2328,7 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
             MOSSQ_CHECK(tiles + nt <= u64(INT32_MAX), MOSSQ_EINVAL, "batched sweep launch too large");
             MOSSQ_CHECK(x[b] + len <= vec_len && (!adjoint || l[b] + len <= vec_len), MOSSQ_EINVAL,
                         "batched sweep vector outside its allocation");
-            MOSSQ_CHECK(!adjoint || (row[b] >= int(4 * nt) &&
+            MOSSQ_CHECK(!adjoint || (row[b] >= int(u64(plan_wpt(fp)) * nt) &&
                                      part_off[b] + u64(std::max(p->nderiv, 1)) * u64(row[b]) <= part_len),
                         MOSSQ_EINVAL, "gradient partial rows outside their allocation");
             MOSSQ_CHECK(nt <= 65536, MOSSQ_EINVAL, "batched sweeps take at most 28 qubits");
@@ -2295,8 +2344,9 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
         out->item0.push_back(int(i0));
         out->wg0.push_back(int(w0));
         out->tiles.push_back(int(tiles));
+        out->rb.push_back(rb);
       }
-    }
+     }
     if (items.empty()) return MOSSQ_OK;
     MOSSQ_TRY(al.get(items.size() * sizeof(TBItem), &out->d_items));
     MOSSQ_TRY(al.get(wg.size() * sizeof(unsigned), (void **)&out->d_wg));
@@ -2311,12 +2361,17 @@ int batch_sweeps_run(Ctx *c, const BatchSweeps &bs, bool adjoint) {
     const TBItem *items = static_cast<const TBItem *>(bs.d_items);
     for (size_t s = 0; s < bs.tiles.size(); ++s) {
         if (bs.tiles[s] == 0) continue;
-        if (adjoint)
-            hipLaunchKernelGGL(k_btile_adj, dim3(unsigned(bs.tiles[s])), dim3(TT), 0, c->stream, items + bs.item0[s],
-                               bs.d_wg + bs.wg0[s], bs.vec, bs.part);
+        const dim3 grid(unsigned(bs.tiles[s]));
+        const TBItem *it = items + bs.item0[s];
+        const unsigned *wg = bs.d_wg + bs.wg0[s];
+        if (adjoint && bs.rb[s] == 3)
+            hipLaunchKernelGGL(k_btile_adj3, grid, dim3(TGeo<3>::TT), 0, c->stream, it, wg, bs.vec, bs.part);
+        else if (adjoint)
+            hipLaunchKernelGGL(k_btile_adj4, grid, dim3(TGeo<4>::TT), 0, c->stream, it, wg, bs.vec, bs.part);
+        else if (bs.rb[s] == 3)
+            hipLaunchKernelGGL(k_btile3, grid, dim3(TGeo<3>::TT), 0, c->stream, it, wg, bs.vec);
         else
-            hipLaunchKernelGGL(k_btile, dim3(unsigned(bs.tiles[s])), dim3(TT), 0, c->stream, items + bs.item0[s],
-                               bs.d_wg + bs.wg0[s], bs.vec);
+            hipLaunchKernelGGL(k_btile4, grid, dim3(TGeo<4>::TT), 0, c->stream, it, wg, bs.vec);
         MOSSQ_HIP(hipGetLastError());
     }
     return MOSSQ_OK;
@@ -2327,7 +2382,7 @@ int batch_sweeps_run(Ctx *c, const BatchSweeps &bs, bool adjoint) {
 // variant (tail thresholds 0/3/5/8, with and without pairing one-qubit gates
 // into 4x4 ops), times each distinct one on the device, and keeps the fastest.
 static bool same_plan(const FusedPlan *a, const FusedPlan *b) {
-    if (a->nsweeps != b->nsweeps || a->ntransitions != b->ntransitions || a->items.size() != b->items.size() ||
+    if (a->rb != b->rb || a->nsweeps != b->nsweeps || a->ntransitions != b->ntransitions || a->items.size() != b->items.size() ||
         a->h_groups.size() != b->h_groups.size() || a->h_segs.size() != b->h_segs.size())
         return false;
     for (size_t i = 0; i < a->h_groups.size(); ++i)
@@ -2344,9 +2399,10 @@ int fused_plan_autotune(Program *p, int reps, int *tried, double *best_ms) {
     lower_steps(p, ops);
     std::vector<FusedPlan *> cand;
     std::string err;
-    for (bool kron : {true, false})
+    for (int rb : {4, 3})
+      for (bool kron : {true, false})
         for (int tail : {0, 3, 5, 8}) {
-            FusedPlan *fp = plan_with_tail(p, ops, tail, false, kron, &err);
+            FusedPlan *fp = plan_with_tail(p, ops, tail, false, kron, rb, &err);
             if (!fp) continue;
             bool dup = same_plan(fp, p->fused);
             for (const FusedPlan *q : cand) dup = dup || same_plan(q, fp);
@@ -2447,7 +2503,7 @@ static FusedPlan *plan_range(const Program *p, int lo, int hi, bool reverse, std
     std::vector<POp> ops;
     lower_range(p, lo, hi, reverse, ops);
     if (ops.empty()) return nullptr;
-    FusedPlan *fp = best_plan(p, ops, false, err);
+    FusedPlan *fp = best_plan(p, ops, false, 4, err);
     if (!fp) {
         *ok = false;
         return nullptr;
@@ -2491,7 +2547,7 @@ static FusedPlan *plan_reverse_probe(const Program *p, int si, int hi, std::stri
     if (!probe_fits(p->steps[si])) return nullptr;
     std::vector<POp> ops;
     lower_reverse_probe(p, si, hi, ops);
-    FusedPlan *fp = best_plan(p, ops, false, err);
+    FusedPlan *fp = best_plan(p, ops, false, 4, err);
     if (!fp) {
         *ok = false;
         return nullptr;
@@ -2545,7 +2601,7 @@ static DensityAdjointPlan *dm_plan_build_host(const Program *p, std::string *err
         if (ok && fits) {
             std::vector<POp> ops;
             for (int j = e; j >= a; --j) lower_reverse_probe(p, dp->psteps[j - 1], b(j + 1), ops);
-            dp->segrev[k] = best_plan(p, ops, false, err);
+            dp->segrev[k] = best_plan(p, ops, false, 4, err);
             if (!dp->segrev[k]) {
                 ok = false;
                 break;
@@ -2698,7 +2754,9 @@ int density_adjoint_reverse(Ctx *c, const Program *p, c128 *delta, c128 *const *
 
 int density_adjoint_parts(const Program *p) {
     if (!p->dm_adj) return 0;
-    return std::max(pair_parts(p->nbits), int(u64(4) << (p->nbits - TM)));
+    int wpt = 1;
+    for (const FusedPlan *fp : dm_plans(p->dm_adj)) wpt = std::max(wpt, plan_wpt(fp));
+    return std::max(pair_parts(p->nbits), int(u64(wpt) << (p->nbits - TM)));
 }
 
 // Host execution of the fused density adjoint with the same schedule:
@@ -2798,7 +2856,7 @@ int mossq_plan_emulate(int n, int is_density, int nops, const mossq_op *ops, con
     MOSSQ_TRY(program_build_host(&p, n, is_density, nops, ops, pool, pool_len));
     MOSSQ_CHECK(p.nbits >= TM, MOSSQ_EINVAL, "plans tile registers of at least 12 bits");
     std::string err;
-    FusedPlan *fp = fused_plan_build_host(&p, &err, false);
+    FusedPlan *fp = fused_plan_build_host(&p, &err, false, 0);
     MOSSQ_CHECK(fp, MOSSQ_EINVAL, err);
     if (state) emulate_plan(&p, fp, (c128 *)state);
     plan_stats(fp, stats);
@@ -2813,7 +2871,7 @@ int mossq_plan_emulate_adjoint(int n, int nops, const mossq_op *ops, const doubl
     MOSSQ_TRY(program_build_host(&p, n, 0, nops, ops, pool, pool_len));
     MOSSQ_CHECK(p.nbits >= TM, MOSSQ_EINVAL, "plans tile registers of at least 12 bits");
     std::string err;
-    FusedPlan *fp = fused_plan_build_host(&p, &err, true);
+    FusedPlan *fp = fused_plan_build_host(&p, &err, true, 0);
     MOSSQ_CHECK(fp, MOSSQ_EINVAL, err);
     std::vector<c128> g(std::max(p.nderiv, 1), mk(0, 0));
     emulate_adjoint_plan(&p, fp, (c128 *)y, (c128 *)l, g.data());
