@@ -96,9 +96,13 @@ constexpr int kPairB[6] = {1, 2, 3, 2, 3, 3};
 constexpr int kGroupCz = 3;
 constexpr int kCzNone = 0x7F;
 
-// Adjoint plans add, after a group's op and flips, at most one probe: the
-// contribution of this tile to Re<y|D|l> for parameter `probe`, D a 2x2 / 4x4
-// on register bits (pcode as a dense code) at pool offset pmoff.
+// After a group's op and flips come its probes, np of them at probes[p0 ..]
+// (up to kGroupProbes in adjoint plans, one fixed probe in density plans):
+// the contribution of this tile to Re<y|D|l> (or Re<v|D|fixed>) for
+// parameter `probe`, D a 2x2 / 4x4 on register bits (pcode as a dense code) at
+// pool offset pmoff.  The probes of one group are summed over the wave
+// together.
+constexpr int kGroupProbes = 4;
 struct TGroup {
     int code;
     int moff;  // dense: matrix offset in the pool
@@ -106,10 +110,23 @@ struct TGroup {
     int x;     // C_DIAG: TDiag index
     unsigned pat[kGroupCz];
     int cond[kGroupCz];
-    int probe;  // -1: none
-    int pcode;
+    int np;  // probes after the op
+    int p0;  // first probe (TProbe index)
+};
+
+// Probe kinds (pcode bits 8 and up; the low bits are the dense code): the
+// general form walks the nonzero parts of D; a rotation's generator is one
+// of two two-part forms with its own code path.
+enum : int { PK_GENERAL = 0, PK_IMDIAG = 1, PK_REANTI = 2 };
+struct TProbe {
+    int probe;  // parameter (gradient row)
+    int pcode;  // dense code | kind << 8
     int pmoff;
     unsigned pmask;  // nonzero parts of D: bit 2e real, bit 2e + 1 imaginary part of entry e
+    // the factors of the first two nonzero parts in mask order (Re D for a
+    // real part, -Im D for an imaginary one), so a rotation's probe (two
+    // parts) reads no matrix
+    double coef[2];
 };
 
 struct TDiag {
@@ -422,21 +439,48 @@ __device__ __forceinline__ void group_op(const TGroup &g, const TDiag *__restric
 
 template <int RB>
 __device__ __forceinline__ double probe_op_re(int code, unsigned mask, const c128 *y, const c128 *l,
-                                              const c128 *__restrict__ D);
+                                              const c128 *__restrict__ D, double c0, double c1);
 __device__ __forceinline__ double wave_sum_re(double a) { return wave_sum_f64(a); }
 
 // Probe of a group against a fixed state: this wave's share of
 // sum conj(cur) (D fixed) over the tile, fixed read in the current layout
 // into f (the register set the group's output left free).
 template <int RB>
-__device__ __forceinline__ void fixed_probe(const TGroup &g, const c128 *cur, c128 *f, const c128 *fixed,
+__device__ __forceinline__ void fixed_probe(const TProbe &pr, const c128 *cur, c128 *f, const c128 *fixed,
                                             const TSeg &sg, int t, u64 base, const c128 *__restrict__ pool,
                                             c128 *part, int row, int slot) {
     const u64 gt = base | seg_gthread<RB>(t, sg);
 #pragma unroll
     for (int j = 0; j < TGeo<RB>::TN; ++j) f[j] = load_nt(fixed + (gt | slot_goff<RB>(j, sg)));
-    const double acc = wave_sum_re(probe_op_re<RB>(g.pcode, g.pmask, cur, f, pool + g.pmoff));
-    if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = c128{acc, 0.0};
+    const double acc = wave_sum_re(probe_op_re<RB>(pr.pcode, pr.pmask, cur, f, pool + pr.pmoff, pr.coef[0],
+                                                   pr.coef[1]));
+    if ((t & 63) == 0) part[size_t(pr.probe) * row + slot] = c128{acc, 0.0};
+}
+
+// Four per-lane values summed over the wave in one pass: the halves exchange
+// (a, b) against (c, d), the row pairs one of the two remaining values, then
+// each row of 16 lanes sums its own value.  Afterwards lane 16 k holds the
+// sum of the k-th argument (k < 4); one reduction chain for four probes.
+__device__ __forceinline__ double swap_add_f64(double p, double q, bool halves) {
+    const long long pv = __double_as_longlong(p), qv = __double_as_longlong(q);
+    const auto lo = halves ? __builtin_amdgcn_permlane32_swap(int(pv), int(qv), false, false)
+                           : __builtin_amdgcn_permlane16_swap(int(pv), int(qv), false, false);
+    const auto hi = halves ? __builtin_amdgcn_permlane32_swap(int(pv >> 32), int(qv >> 32), false, false)
+                           : __builtin_amdgcn_permlane16_swap(int(pv >> 32), int(qv >> 32), false, false);
+    return __longlong_as_double((long long)(unsigned)lo[0] | ((long long)hi[0] << 32)) +
+           __longlong_as_double((long long)(unsigned)lo[1] | ((long long)hi[1] << 32));
+}
+
+__device__ __forceinline__ double wave_sum4(double a, double b, double c, double d) {
+    // lanes 0..31: a (+ lane + 32), lanes 32..63: c (+ lane - 32); likewise b / d
+    const double u = swap_add_f64(a, c, true), w = swap_add_f64(b, d, true);
+    // rows 0 and 2 keep u summed over rows (0, 1) / (2, 3); rows 1 and 3 keep w
+    double x = swap_add_f64(u, w, false);
+    x += dpp_f64<0x128>(x);  // row_ror:8
+    x += dpp_f64<0x124>(x);  // row_ror:4
+    x += dpp_f64<0x122>(x);  // row_ror:2
+    x += dpp_f64<0x121>(x);  // row_ror:1
+    return x;
 }
 
 // Global index bits of a tile: tile-id bit i sits at global bit cbits[i].
@@ -470,7 +514,8 @@ template <int RB, bool FIXP>
 __device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSweepArgs &sw,
                                            const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
                                            const TDiag *__restrict__ diag, const c128 *__restrict__ pool,
-                                           const c128 *const *ftab, int fbase, c128 *part, int row, u64 tile) {
+                                           const TProbe *__restrict__ probes, const c128 *const *ftab, int fbase,
+                                           c128 *part, int row, u64 tile) {
     constexpr int TN = TGeo<RB>::TN;
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
@@ -498,9 +543,16 @@ __device__ __forceinline__ void tile_sweep(const c128 *src, c128 *dst, const TSw
         for (int g = sg.g0; g < sg.g1; g += 2) {
             const TGroup ga = groups[g], gb = groups[g + 1];
             group_op<RB>(ga, diag, pool, v, w, tdep, base);
-            if (FIXP && ga.probe >= 0) fixed_probe<RB>(ga, w, v, ftab[ga.probe - fbase], sg, t, base, pool, part, row, slot);
+            // (a density plan's group holds at most one probe)
+            if (FIXP && ga.np > 0) {
+                const TProbe pr = probes[ga.p0];
+                fixed_probe<RB>(pr, w, v, ftab[pr.probe - fbase], sg, t, base, pool, part, row, slot);
+            }
             group_op<RB>(gb, diag, pool, w, v, tdep, base);
-            if (FIXP && gb.probe >= 0) fixed_probe<RB>(gb, v, w, ftab[gb.probe - fbase], sg, t, base, pool, part, row, slot);
+            if (FIXP && gb.np > 0) {
+                const TProbe pr = probes[gb.p0];
+                fixed_probe<RB>(pr, v, w, ftab[pr.probe - fbase], sg, t, base, pool, part, row, slot);
+            }
         }
     }
     {
@@ -597,39 +649,87 @@ __device__ __forceinline__ double probe_part(const c128 *y, const c128 *l, int m
     return acc;
 }
 
+// One-bit probe forms on register bit A, factors c0, c1 of the two parts:
+// imaginary diagonal (R_z generators), c0 Im sum_{a=0} conj(y) l + c1 Im
+// sum_{a=1} conj(y) l; real antidiagonal (R_y generators), c0 Re sum
+// conj(y_{j|a}) l_j + c1 Re sum conj(y_j) l_{j|a}.
+template <int RB, int A>
+__device__ __forceinline__ double probe_imdiag(const c128 *y, const c128 *l, double c0, double c1) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < TGeo<RB>::TN; ++j) {
+        const c128 a = y[j], b = l[j];
+        if (j & (1 << A))
+            s1 = fma(a.re, b.im, fma(-a.im, b.re, s1));
+        else
+            s0 = fma(a.re, b.im, fma(-a.im, b.re, s0));
+    }
+    return fma(c0, s0, c1 * s1);
+}
+
+template <int RB, int A>
+__device__ __forceinline__ double probe_reanti(const c128 *y, const c128 *l, double c0, double c1) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < TGeo<RB>::TN; ++j) {
+        if (j & (1 << A)) continue;
+        const c128 y0 = y[j], y1 = y[j | (1 << A)], l0 = l[j], l1 = l[j | (1 << A)];
+        s0 = fma(y1.re, l0.re, fma(y1.im, l0.im, s0));
+        s1 = fma(y0.re, l1.re, fma(y0.im, l1.im, s1));
+    }
+    return fma(c0, s0, c1 * s1);
+}
+
+// The first two nonzero parts take their factors from c0, c1; further ones
+// read D.
 template <int RB, int A, int B>
-__device__ __forceinline__ double probe_re(const c128 *y, const c128 *l, const c128 *__restrict__ D, unsigned mask) {
+__device__ __forceinline__ double probe_re(const c128 *y, const c128 *l, const c128 *__restrict__ D, unsigned mask,
+                                           double c0, double c1) {
     constexpr int dim = B < 0 ? 2 : 4;
     constexpr int msk = B < 0 ? (1 << A) : ((1 << A) | (1 << B));
     double acc = 0.0;
+    int used = 0;
 #pragma unroll
     for (int e = 0; e < dim * dim; ++e) {
         const int r = e % dim, cc = e / dim;
         const int orow = B < 0 ? (r << A) : (((r & 1) << A) | ((r >> 1) << (B < 0 ? 0 : B)));
         const int ocol = B < 0 ? (cc << A) : (((cc & 1) << A) | ((cc >> 1) << (B < 0 ? 0 : B)));
-        if ((mask >> (2 * e)) & 1u) acc = fma(D[e].re, probe_part<RB>(y, l, msk, orow, ocol, false), acc);
-        if ((mask >> (2 * e + 1)) & 1u) acc = fma(-D[e].im, probe_part<RB>(y, l, msk, orow, ocol, true), acc);
+#pragma unroll
+        for (int im = 0; im < 2; ++im) {
+            if (!((mask >> (2 * e + im)) & 1u)) continue;
+            const double c = used == 0 ? c0 : used == 1 ? c1 : (im ? -D[e].im : D[e].re);
+            ++used;
+            acc = fma(c, probe_part<RB>(y, l, msk, orow, ocol, im != 0), acc);
+        }
     }
     return acc;
 }
 
 template <int RB>
 __device__ __forceinline__ double probe_op_re(int code, unsigned mask, const c128 *y, const c128 *l,
-                                              const c128 *__restrict__ D) {
+                                              const c128 *__restrict__ D, double c0, double c1) {
     switch (code) {
-        case C_DENSE1 + 0: return probe_re<RB, 0, -1>(y, l, D, mask);
-        case C_DENSE1 + 1: return probe_re<RB, 1, -1>(y, l, D, mask);
-        case C_DENSE1 + 2: return probe_re<RB, 2, -1>(y, l, D, mask);
-        case C_DENSE2 + 0: return probe_re<RB, 0, 1>(y, l, D, mask);
-        case C_DENSE2 + 1: return probe_re<RB, 0, 2>(y, l, D, mask);
-        case C_DENSE2 + 3: return probe_re<RB, 1, 2>(y, l, D, mask);
+        case C_DENSE1 + 0 + (PK_IMDIAG << 8): return probe_imdiag<RB, 0>(y, l, c0, c1);
+        case C_DENSE1 + 1 + (PK_IMDIAG << 8): return probe_imdiag<RB, 1>(y, l, c0, c1);
+        case C_DENSE1 + 2 + (PK_IMDIAG << 8): return probe_imdiag<RB, 2>(y, l, c0, c1);
+        case C_DENSE1 + 0 + (PK_REANTI << 8): return probe_reanti<RB, 0>(y, l, c0, c1);
+        case C_DENSE1 + 1 + (PK_REANTI << 8): return probe_reanti<RB, 1>(y, l, c0, c1);
+        case C_DENSE1 + 2 + (PK_REANTI << 8): return probe_reanti<RB, 2>(y, l, c0, c1);
+        case C_DENSE1 + 0: return probe_re<RB, 0, -1>(y, l, D, mask, c0, c1);
+        case C_DENSE1 + 1: return probe_re<RB, 1, -1>(y, l, D, mask, c0, c1);
+        case C_DENSE1 + 2: return probe_re<RB, 2, -1>(y, l, D, mask, c0, c1);
+        case C_DENSE2 + 0: return probe_re<RB, 0, 1>(y, l, D, mask, c0, c1);
+        case C_DENSE2 + 1: return probe_re<RB, 0, 2>(y, l, D, mask, c0, c1);
+        case C_DENSE2 + 3: return probe_re<RB, 1, 2>(y, l, D, mask, c0, c1);
         default:
             if constexpr (RB > 3) {
                 switch (code) {
-                    case C_DENSE1 + 3: return probe_re<RB, 3, -1>(y, l, D, mask);
-                    case C_DENSE2 + 2: return probe_re<RB, 0, 3>(y, l, D, mask);
-                    case C_DENSE2 + 4: return probe_re<RB, 1, 3>(y, l, D, mask);
-                    default: return probe_re<RB, 2, 3>(y, l, D, mask);
+                    case C_DENSE1 + 3 + (PK_IMDIAG << 8): return probe_imdiag<RB, 3>(y, l, c0, c1);
+                    case C_DENSE1 + 3 + (PK_REANTI << 8): return probe_reanti<RB, 3>(y, l, c0, c1);
+                    case C_DENSE1 + 3: return probe_re<RB, 3, -1>(y, l, D, mask, c0, c1);
+                    case C_DENSE2 + 2: return probe_re<RB, 0, 3>(y, l, D, mask, c0, c1);
+                    case C_DENSE2 + 4: return probe_re<RB, 1, 3>(y, l, D, mask, c0, c1);
+                    default: return probe_re<RB, 2, 3>(y, l, D, mask, c0, c1);
                 }
             }
             return 0.0;
@@ -639,8 +739,8 @@ __device__ __forceinline__ double probe_op_re(int code, unsigned mask, const c12
 template <int RB>
 __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__restrict__ ls, const TSweepArgs &sw,
                                                const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
-                                               const c128 *__restrict__ pool, c128 *__restrict__ part, int row,
-                                               u64 tile) {
+                                               const c128 *__restrict__ pool, const TProbe *__restrict__ probes,
+                                               c128 *__restrict__ part, int row, u64 tile) {
     constexpr int TN = TGeo<RB>::TN;
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
@@ -661,6 +761,11 @@ __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__re
         }
     }
     const int slot = int(tile) * TGeo<RB>::WPT + (t >> 6);
+    // The sweep's groups are consecutive; each iteration loads the next
+    // group's record before its own op, so the record's round trip overlaps
+    // the op's matrix load.
+    const int glast = segs[seg1 - 1].g1 - 1;
+    TGroup gnext = groups[sg.g0];
     for (int s = seg0; s < seg1; ++s) {
         sg = segs[s];
         if (sg.transpose) {
@@ -675,9 +780,26 @@ __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__re
                 dense_ip<RB>(g.code, y, pool + g.moff, F);
                 dense_ip<RB>(g.code, l, pool + g.moff, F);
             }
-            if (g.probe >= 0) {
-                const double acc = wave_sum_re(probe_op_re<RB>(g.pcode, g.pmask, y, l, pool + g.pmoff));
-                if ((t & 63) == 0) part[size_t(g.probe) * row + slot] = c128{acc, 0.0};
+            if (g.np > 0) {
+                // all of the group's probe records in one round trip (the
+                // table holds kGroupProbes entries of padding), this thread's
+                // share of each probe, then one reduction for all
+                TProbe pr[kGroupProbes];
+#pragma unroll
+                for (int k = 0; k < kGroupProbes; ++k) pr[k] = probes[g.p0 + k];
+                double v[kGroupProbes];
+#pragma unroll
+                for (int k = 0; k < kGroupProbes; ++k)
+                    v[k] = k < g.np ? probe_op_re<RB>(pr[k].pcode, pr[k].pmask, y, l, pool + pr[k].pmoff, pr[k].coef[0],
+                                                      pr[k].coef[1])
+                                    : 0.0;
+                const int id0 = pr[0].probe, id1 = pr[1].probe, id2 = pr[2].probe, id3 = pr[3].probe;
+                const double acc = wave_sum4(v[0], v[1], v[2], v[3]);
+                const int k = (t >> 4) & 3;
+                if ((t & 15) == 0 && k < g.np) {
+                    const int id = k == 0 ? id0 : k == 1 ? id1 : k == 2 ? id2 : id3;
+                    part[size_t(id) * row + slot] = c128{acc, 0.0};
+                }
             }
         }
     }
@@ -698,20 +820,22 @@ __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__re
     __global__ __launch_bounds__(TTHREADS, WAVES) void k_tile##RB(                                                   \
         const c128 *src, c128 *dst, TSweepArgs sw, const TSeg *__restrict__ segs, const TGroup *__restrict__ groups, \
         const TDiag *__restrict__ diag, const c128 *__restrict__ pool) {                                             \
-        tile_sweep<RB, false>(src, dst, sw, segs, groups, diag, pool, nullptr, 0, nullptr, 0, blockIdx.x);          \
+        tile_sweep<RB, false>(src, dst, sw, segs, groups, diag, pool, nullptr, nullptr, 0, nullptr, 0, blockIdx.x); \
     }                                                                                                                \
     /* the density adjoint's reverse chunks: Delta in place, each probe against */                                   \
     /* the state before its parameterized gate (ftab[probe - fbase]) */                                              \
     __global__ __launch_bounds__(TTHREADS, WAVES) void k_tile_fixp##RB(                                              \
         c128 *delta, TSweepArgs sw, const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,                \
-        const TDiag *__restrict__ diag, const c128 *__restrict__ pool, const c128 *const *__restrict__ ftab,         \
-        int fbase, c128 *__restrict__ part, int row) {                                                               \
-        tile_sweep<RB, true>(delta, delta, sw, segs, groups, diag, pool, ftab, fbase, part, row, blockIdx.x);       \
+        const TDiag *__restrict__ diag, const c128 *__restrict__ pool, const TProbe *__restrict__ probes,            \
+        const c128 *const *__restrict__ ftab, int fbase, c128 *__restrict__ part, int row) {                         \
+        tile_sweep<RB, true>(delta, delta, sw, segs, groups, diag, pool, probes, ftab, fbase, part, row,            \
+                             blockIdx.x);                                                                            \
     }                                                                                                                \
     __global__ __launch_bounds__(TTHREADS, WAVES) void k_tile_adj##RB(                                               \
         c128 *__restrict__ ys, c128 *__restrict__ ls, TSweepArgs sw, const TSeg *__restrict__ segs,                  \
-        const TGroup *__restrict__ groups, const c128 *__restrict__ pool, c128 *__restrict__ part, int row) {        \
-        tile_adj_sweep<RB>(ys, ls, sw, segs, groups, pool, part, row, blockIdx.x);                                  \
+        const TGroup *__restrict__ groups, const c128 *__restrict__ pool, const TProbe *__restrict__ probes,         \
+        c128 *__restrict__ part, int row) {                                                                          \
+        tile_adj_sweep<RB>(ys, ls, sw, segs, groups, pool, probes, part, row, blockIdx.x);                          \
     }
 MOSSQ_TILE_KERNELS(4, 256, 2)
 MOSSQ_TILE_KERNELS(3, 512, 4)
@@ -740,7 +864,7 @@ struct TBItem {
     u64 x;  // the state (forward sweeps, in place) or y (adjoint sweeps)
     u64 l;  // lambda (adjoint sweeps)
     u64 part;  // gradient partial rows (adjoint sweeps)
-    u64 segs, groups, diag, pool;  // plan tables (device addresses)
+    u64 segs, groups, diag, pool, probes;  // plan tables (device addresses)
     int row;
     int tile0;
     TSweepArgs args;
@@ -752,7 +876,8 @@ struct TBItem {
         const unsigned e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);                                      \
         const TBItem &it = items[e >> 16];                                                                          \
         tile_sweep<RB, false>(vec + it.x, vec + it.x, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),      \
-                              table<TDiag>(it.diag), table<c128>(it.pool), nullptr, 0, nullptr, 0, u64(e & 0xFFFF)); \
+                              table<TDiag>(it.diag), table<c128>(it.pool), nullptr, nullptr, 0, nullptr, 0,        \
+                              u64(e & 0xFFFF));                                                                     \
     }                                                                                                               \
     __global__ __launch_bounds__(TTHREADS, WAVES) void k_btile_adj##RB(const TBItem *__restrict__ items,             \
                                                                        const unsigned *__restrict__ wg_item,         \
@@ -761,7 +886,8 @@ struct TBItem {
         const unsigned e = __builtin_amdgcn_readfirstlane(wg_item[blockIdx.x]);                                      \
         const TBItem &it = items[e >> 16];                                                                          \
         tile_adj_sweep<RB>(vec + it.x, vec + it.l, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),         \
-                           table<c128>(it.pool), part + it.part, it.row, u64(e & 0xFFFF));                          \
+                           table<c128>(it.pool), table<TProbe>(it.probes), part + it.part, it.row,                  \
+                           u64(e & 0xFFFF));                                                                        \
     }
 MOSSQ_BTILE_KERNELS(4, 256, 2)
 MOSSQ_BTILE_KERNELS(3, 512, 4)
@@ -799,10 +925,12 @@ struct FusedPlan {
     std::vector<c128> h_pool;
     std::vector<TGroup> h_groups;  // device encoding of h_ops
     std::vector<TDiag> h_diag;
+    std::vector<TProbe> h_probes;
     TSeg *d_segs = nullptr;
     TGroup *d_groups = nullptr;
     TDiag *d_diag = nullptr;
     c128 *d_pool = nullptr;
+    TProbe *d_probes = nullptr;
     int rb = 4;            // register bits of every layout (4: 256 threads per tile, 3: 512)
     bool adjoint = false;  // reverse plan of the pure-state adjoint (probes, no general diagonals)
     bool reverse_steps = false;  // unfused items apply the steps' reverse matrices
@@ -1040,7 +1168,7 @@ static void absorb_diag(POp &M, const POp &D) {
 // Peephole on one segment's op list (in execution order): merge dense ops on
 // identical bits, absorb register-bit diagonals into the dense ops before them
 // (pairing two one-bit gates into a two-bit gate when the diagonal spans both).
-static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, bool kron = true) {
+static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, bool kron = true, bool adjoint = false) {
     auto in_reg = [&](int gb) {
         for (int i = 0; i < nreg; ++i)
             if (reg_global[i] == gb) return true;
@@ -1108,6 +1236,21 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, boo
             }
         }
     }
+    // Adjoint plans: a probe changes neither state, and an op on other bits
+    // (applied to both states) leaves its value alone, so probes move ahead
+    // of such ops and of other probes.  Gathered probes share a group (and
+    // one wave reduction), and the dense ops they no longer separate can be
+    // paired below.
+    if (adjoint)
+        for (size_t i = 1; i < ops.size(); ++i) {
+            if (ops[i].kind != 3 || ops[i].fixed) continue;
+            for (size_t j = i; j > 0; --j) {
+                const POp &prev = ops[j - 1];
+                const bool moves = prev.kind == 3 ? !prev.fixed : (prev.mask() & ops[j].mask()) == 0;
+                if (!moves) break;
+                std::swap(ops[j - 1], ops[j]);
+            }
+        }
     // Fold one-bit dense ops into the two-bit dense op they meet first on
     // their bit (same flops, one op fewer; not into permutations, which cost
     // no arithmetic), then pair the remaining one-bit dense ops on different
@@ -1152,8 +1295,14 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, boo
         u64 touched = 0;
         for (size_t k = i + 1; k < ops.size(); ++k) {
             const POp &B = ops[k];
-            if (B.kind == 0 && B.nb == 1 && !diag1(B) && B.bits[0] != a && in_reg(B.bits[0]) &&
-                ((touched | (u64(1) << a)) & B.mask()) == 0) {
+            const bool pairable = B.kind == 0 && B.nb == 1 && !diag1(B) && B.bits[0] != a && in_reg(B.bits[0]) &&
+                                  ((u64(1) << a) & B.mask()) == 0;
+            // the pair runs where B runs (A moves later past ops off its bit)
+            // or, in adjoint plans, where A runs (B moves earlier past ops
+            // off its bit, such as the probes that follow A)
+            const bool at_b = pairable && (touched & ((u64(1) << a) | B.mask())) == 0;
+            const bool at_a = pairable && adjoint && (touched & B.mask()) == 0;
+            if (at_b || at_a) {
                 POp F;
                 F.kind = 0;
                 F.nb = 2;
@@ -1161,13 +1310,19 @@ static void peephole(std::vector<POp> &ops, const int *reg_global, int nreg, boo
                 F.bits[0] = lo.bits[0];
                 F.bits[1] = hi.bits[0];
                 F.m = kron2(lo.m, hi.m);
-                ops[k] = F;
-                ops.erase(ops.begin() + i);
-                --i;
+                F.step = ops[i].step;
+                if (at_b) {
+                    ops[k] = F;
+                    ops.erase(ops.begin() + i);
+                    --i;
+                } else {
+                    ops[i] = F;
+                    ops.erase(ops.begin() + k);
+                }
                 break;
             }
             touched |= B.mask();
-            if (touched & (u64(1) << a)) break;
+            if (!adjoint && (touched & (u64(1) << a))) break;
         }
     }
 }
@@ -1564,7 +1719,7 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
             for (int ci : sp.idx) seg_ops.push_back(ops[chosen[ci]]);
             int reg_global[kRegMax];
             for (int i = 0; i < rb; ++i) reg_global[i] = it.args.tbits[reg[i]];
-            peephole(seg_ops, reg_global, rb, kron);
+            peephole(seg_ops, reg_global, rb, kron, adjoint);
             for (const POp &o : seg_ops) {
                 TOp t{};
                 t.moff = int(fp->h_pool.size());
@@ -1722,13 +1877,13 @@ static bool pack_flip(const TOp &o, int tn, unsigned *pat, int *cond) {
 static int encode_ops(FusedPlan *fp, std::string *err) {
     fp->h_groups.clear();
     fp->h_diag.clear();
+    fp->h_probes.clear();
     for (TSeg &sg : fp->h_segs) {
         sg.g0 = int(fp->h_groups.size());
         auto add = [&](int code, int moff) {
             TGroup g{};
             g.code = code;
             g.moff = moff;
-            g.probe = -1;
             fp->h_groups.push_back(g);
         };
         auto pair_code = [&](const TOp &o, int *code) {
@@ -1758,25 +1913,51 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
             }
             const bool here = int(fp->h_groups.size()) > sg.g0;
             if (o.type == T_PROBE1 || o.type == T_PROBE2) {
+                // probes join the group before them (adjoint plans: up to
+                // kGroupProbes, summed together; fixed probes: one)
                 TGroup *g = here ? &fp->h_groups.back() : nullptr;
-                if (!g || g->code == C_DIAG || g->probe >= 0) {
+                if (!g || g->code == C_DIAG || g->np >= (fp->adjoint ? kGroupProbes : 1)) {
                     add(C_COPY, 0);
                     g = &fp->h_groups.back();
                 }
+                TProbe pr{};
                 if (o.type == T_PROBE1) {
-                    g->pcode = C_DENSE1 + o.a;
-                } else if (!pair_code(o, &g->pcode)) {
+                    pr.pcode = C_DENSE1 + o.a;
+                } else if (!pair_code(o, &pr.pcode)) {
                     return MOSSQ_EINVAL;
                 }
-                g->probe = o.probe;
-                g->pmoff = o.moff;
-                g->pmask = 0;
+                pr.probe = o.probe;
+                pr.pmoff = o.moff;
+                // D = (dU)^dagger U formed as a product leaves rounding
+                // residues (~1e-17) where the exact generator is zero (R_y:
+                // i Y / 2, R_z: i Z / 2); parts below 64 eps of D's largest
+                // entry count as zero, so a rotation's probe takes its two-part
+                // form.  The dropped terms change a gradient by ~1e-17.
                 const int ne = o.type == T_PROBE1 ? 4 : 16;
+                double dmax = 0.0;
+                for (int e = 0; e < ne; ++e)
+                    dmax = std::max(dmax, std::max(std::fabs(fp->h_pool[o.moff + e].re),
+                                                   std::fabs(fp->h_pool[o.moff + e].im)));
+                const double tiny = 64.0 * 2.220446049250313e-16 * dmax;
+                int nc = 0;
                 for (int e = 0; e < ne; ++e) {
                     const c128 d = fp->h_pool[o.moff + e];
-                    if (d.re != 0.0) g->pmask |= 1u << (2 * e);
-                    if (d.im != 0.0) g->pmask |= 1u << (2 * e + 1);
+                    if (std::fabs(d.re) > tiny) {
+                        pr.pmask |= 1u << (2 * e);
+                        if (nc < 2) pr.coef[nc++] = d.re;
+                    }
+                    if (std::fabs(d.im) > tiny) {
+                        pr.pmask |= 1u << (2 * e + 1);
+                        if (nc < 2) pr.coef[nc++] = -d.im;
+                    }
                 }
+                if (o.type == T_PROBE1 && pr.pmask == 0x82u)
+                    pr.pcode |= PK_IMDIAG << 8;  // parts (0, 0) and (1, 1), imaginary
+                else if (o.type == T_PROBE1 && pr.pmask == 0x14u)
+                    pr.pcode |= PK_REANTI << 8;  // parts (1, 0) and (0, 1), real
+                if (g->np == 0) g->p0 = int(fp->h_probes.size());
+                fp->h_probes.push_back(pr);
+                g->np++;
                 continue;
             }
             if (o.type == T_SIGN && o.negmask == 0) continue;  // identity (e.g. two equal CZs merged)
@@ -1784,7 +1965,7 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
             int cond;
             if (pack_flip(o, 1 << fp->rb, &pat, &cond)) {
                 TGroup *g = here ? &fp->h_groups.back() : nullptr;
-                if (!g || g->code == C_DIAG || g->code >= C_DIAG1 || g->ncz == kGroupCz || g->probe >= 0) {
+                if (!g || g->code == C_DIAG || g->code >= C_DIAG1 || g->ncz == kGroupCz || g->np > 0) {
                     add(C_COPY, 0);
                     g = &fp->h_groups.back();
                 }
@@ -1822,6 +2003,8 @@ static int encode_ops(FusedPlan *fp, std::string *err) {
     }
     if (fp->h_groups.empty()) fp->h_groups.push_back(TGroup{});
     if (fp->h_diag.empty()) fp->h_diag.push_back(TDiag{});
+    // padding: a group reads kGroupProbes records from its first probe on
+    fp->h_probes.resize(fp->h_probes.size() + kGroupProbes, TProbe{});
     return MOSSQ_OK;
 }
 
@@ -2019,22 +2202,23 @@ static void emulate_tile_rb(const std::vector<c128 *> &states, const TSweepArgs 
                         }
                     }
                 }
-                if (g.probe >= 0 && grads && (ns == 2 || ftab)) {
+                for (int k = 0; k < g.np && grads && (ns == 2 || ftab); ++k) {
                     // adjoint plans: Re<y|D|l>; fixed probes: Re<v|D|fixed>, fixed
                     // read in the current layout
-                    const c128 *y = &V[0][t * TN], *D = fp->h_pool.data() + g.pmoff;
+                    const TProbe &pr = fp->h_probes[g.p0 + k];
+                    const c128 *y = &V[0][t * TN], *D = fp->h_pool.data() + pr.pmoff;
                     std::vector<c128> dl(TN);
                     if (ns == 2) {
                         dl.assign(V[1].begin() + t * TN, V[1].begin() + (t + 1) * TN);
                     } else {
                         const u64 gth = base | seg_gthread<RB>(t, sg);
-                        const c128 *fixed = ftab[g.probe - fbase];
+                        const c128 *fixed = ftab[pr.probe - fbase];
                         for (int j = 0; j < TN; ++j) dl[j] = fixed[gth | slot_g(j, sg)];
                     }
-                    dense_host(g.pcode, D, dl.data());
+                    dense_host(pr.pcode & 0xFF, D, dl.data());
                     c128 acc = mk(0, 0);
                     for (int j = 0; j < TN; ++j) acc = cfmaconj(y[j], dl[j], acc);
-                    grads[g.probe] = cadd(grads[g.probe], acc);
+                    grads[pr.probe] = cadd(grads[pr.probe], acc);
                 }
             }
         }
@@ -2104,6 +2288,7 @@ int fused_plan_upload(Ctx *c, FusedPlan *fp) {
     if (s == MOSSQ_OK) s = cached_alloc(c, fp->h_groups.size() * sizeof(TGroup), (void **)&fp->d_groups);
     if (s == MOSSQ_OK) s = cached_alloc(c, std::max<size_t>(fp->h_diag.size(), 1) * sizeof(TDiag), (void **)&fp->d_diag);
     if (s == MOSSQ_OK) s = cached_alloc(c, fp->h_pool.size() * sizeof(c128), (void **)&fp->d_pool);
+    if (s == MOSSQ_OK) s = cached_alloc(c, fp->h_probes.size() * sizeof(TProbe), (void **)&fp->d_probes);
     if (s != MOSSQ_OK) return s;
     if (!fp->h_segs.empty())
         MOSSQ_HIP(hipMemcpyAsync(fp->d_segs, fp->h_segs.data(), fp->h_segs.size() * sizeof(TSeg),
@@ -2114,6 +2299,8 @@ int fused_plan_upload(Ctx *c, FusedPlan *fp) {
                              hipMemcpyHostToDevice, c->stream));
     MOSSQ_HIP(hipMemcpyAsync(fp->d_pool, fp->h_pool.data(), fp->h_pool.size() * sizeof(c128), hipMemcpyHostToDevice,
                              c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(fp->d_probes, fp->h_probes.data(), fp->h_probes.size() * sizeof(TProbe),
+                             hipMemcpyHostToDevice, c->stream));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));
     return MOSSQ_OK;
 }
@@ -2150,6 +2337,7 @@ static void free_plan(Ctx *c, FusedPlan *&fp) {
         if (fp->d_groups) cached_free(c, fp->d_groups);
         if (fp->d_diag) cached_free(c, fp->d_diag);
         if (fp->d_pool) cached_free(c, fp->d_pool);
+        if (fp->d_probes) cached_free(c, fp->d_probes);
     }
     delete fp;
     fp = nullptr;
@@ -2215,10 +2403,12 @@ static int fused_run_fixp(Ctx *c, const Program *p, const FusedPlan *fp, c128 *d
         }
         if (fp->rb == 3)
             hipLaunchKernelGGL(k_tile_fixp3, dim3(ntiles), dim3(TGeo<3>::TT), 0, c->stream, delta, it.args,
-                               fp->d_segs, fp->d_groups, fp->d_diag, fp->d_pool, ftab, fbase, part, row);
+                               fp->d_segs, fp->d_groups, fp->d_diag, fp->d_pool, fp->d_probes, ftab, fbase, part,
+                               row);
         else
             hipLaunchKernelGGL(k_tile_fixp4, dim3(ntiles), dim3(TGeo<4>::TT), 0, c->stream, delta, it.args,
-                               fp->d_segs, fp->d_groups, fp->d_diag, fp->d_pool, ftab, fbase, part, row);
+                               fp->d_segs, fp->d_groups, fp->d_diag, fp->d_pool, fp->d_probes, ftab, fbase, part,
+                               row);
         MOSSQ_HIP(hipGetLastError());
     }
     return MOSSQ_OK;
@@ -2247,10 +2437,10 @@ int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, in
         }
         if (fp->rb == 3)
             hipLaunchKernelGGL(k_tile_adj3, dim3(ntiles), dim3(TGeo<3>::TT), 0, c->stream, y, l, it.args,
-                               fp->d_segs, fp->d_groups, fp->d_pool, part, row);
+                               fp->d_segs, fp->d_groups, fp->d_pool, fp->d_probes, part, row);
         else
             hipLaunchKernelGGL(k_tile_adj4, dim3(ntiles), dim3(TGeo<4>::TT), 0, c->stream, y, l, it.args,
-                               fp->d_segs, fp->d_groups, fp->d_pool, part, row);
+                               fp->d_segs, fp->d_groups, fp->d_pool, fp->d_probes, part, row);
         MOSSQ_HIP(hipGetLastError());
     }
     return MOSSQ_OK;
@@ -2322,6 +2512,7 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
             it.groups = reinterpret_cast<u64>(fp->d_groups);
             it.diag = reinterpret_cast<u64>(fp->d_diag);
             it.pool = reinterpret_cast<u64>(fp->d_pool);
+            it.probes = reinterpret_cast<u64>(fp->d_probes);
             it.row = adjoint ? row[b] : 0;
             it.tile0 = int(tiles);
             it.args = fp->items[s].args;
@@ -2668,7 +2859,8 @@ static int upload_plan_block(Ctx *c, const std::vector<FusedPlan *> &plans, void
     size_t total = 0;
     for (const FusedPlan *fp : plans)
         total += al(fp->h_segs.size() * sizeof(TSeg)) + al(fp->h_groups.size() * sizeof(TGroup)) +
-                 al(fp->h_diag.size() * sizeof(TDiag)) + al(fp->h_pool.size() * sizeof(c128));
+                 al(fp->h_diag.size() * sizeof(TDiag)) + al(fp->h_pool.size() * sizeof(c128)) +
+                 al(fp->h_probes.size() * sizeof(TProbe));
     if (total == 0) return MOSSQ_OK;
     std::vector<char> h(total);
     MOSSQ_TRY(cached_alloc(c, total, block));
@@ -2685,6 +2877,7 @@ static int upload_plan_block(Ctx *c, const std::vector<FusedPlan *> &plans, void
         fp->d_groups = reinterpret_cast<TGroup *>(put(fp->h_groups.data(), fp->h_groups.size() * sizeof(TGroup)));
         fp->d_diag = reinterpret_cast<TDiag *>(put(fp->h_diag.data(), fp->h_diag.size() * sizeof(TDiag)));
         fp->d_pool = reinterpret_cast<c128 *>(put(fp->h_pool.data(), fp->h_pool.size() * sizeof(c128)));
+        fp->d_probes = reinterpret_cast<TProbe *>(put(fp->h_probes.data(), fp->h_probes.size() * sizeof(TProbe)));
         fp->shared_device = true;
     }
     MOSSQ_HIP(hipMemcpyAsync(d, h.data(), total, hipMemcpyHostToDevice, c->stream));
