@@ -439,9 +439,9 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         s2 = c->aux;
         join.active = true;
     }
-    if (ntab[1])
-        for (int t = 0; t <= tmax; ++t)
-            MOSSQ_TRY(batch_iteration_on(s2, tab[1], ntab[1], nchunks[1], t, part2, d_vals, d_grads));
+    // The tiled sweeps (a few long launches) are enqueued before the
+    // gate-by-gate items' many short ones, so the device starts on them while
+    // the host is still enqueueing the second stream's launches.
     if (ntab[0]) {
         MOSSQ_TRY(batch_sweeps_run(c, fwd, false));
         // observe (the tiled items have no engine gate steps: t = 0 is their observe step)
@@ -458,6 +458,9 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
             MOSSQ_HIP(hipGetLastError());
         }
     }
+    if (ntab[1])
+        for (int t = 0; t <= tmax; ++t)
+            MOSSQ_TRY(batch_iteration_on(s2, tab[1], ntab[1], nchunks[1], t, part2, d_vals, d_grads));
     if (overlap) {
         join.active = false;
         MOSSQ_HIP(hipEventRecord(c->aux_ev[1], c->aux));
