@@ -183,8 +183,25 @@ int mossq_program_sweeps(const mossq_program *prog, int *sweeps);
 /* Apply the program in place (reference apply!(circuit, state),
  * src/api/backend_protocols.jl:246-254). */
 int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state);
+/* New matrix values for a program: ops/pool as for mossq_program_create, with
+ * the same instructions (kinds, positions, offsets) and new matrices, such as
+ * the template's rotations at new angles: the device form of the reference's
+ * in-place reset_parameters! (src/quantum_circuits/instructions.jl:62-80) and
+ * of bind_parameters (src/api/parameter_binding.jl:29) on a lowered circuit.
+ * Every fused plan is rebuilt with the variant it came from and must keep its
+ * tables (a value that turns a gate into the identity or a permutation
+ * changes them): the new values are then written over the program's device
+ * tables in place, so graphs captured from it (mossq_program_capture,
+ * mossq_batch_capture) replay the new values.  MOSSQ_EINVAL and the program
+ * unchanged when the records describe another program or a plan would change;
+ * also when a live program graph launches a monomial step on its own (an
+ * unfused step: its entries are kernel arguments) whose entries change, and
+ * after an autotune replaced a plan that a live graph still uses. */
+int mossq_program_update(mossq_ctx *ctx, mossq_program *prog, int nops, const mossq_op *ops, const double *pool,
+                         uint64_t pool_len);
 /* Autotune a fused program: plan every forward-plan variant (tail thresholds
- * 0/3/5/8, with and without pairing one-qubit gates into 4x4 ops), time each
+ * 0/3/5/8, with and without pairing one-qubit gates into 4x4 ops, with 4 or 3
+ * register bits per layout: 256- or 512-thread tiles), time each
  * distinct one `reps` times on a scratch state of the program's size, and keep
  * the fastest (the planner's cost-model choice is one candidate).  variants =
  * distinct plans timed (0 for an unfused program), best_ms = the kept plan's
@@ -306,6 +323,20 @@ int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs,
 int mossq_batch_adjoint_init(mossq_ctx *ctx, int B, mossq_program *const *progs, mossq_obs *const *obs,
                              const void *const *init, const int64_t *grad_offset, double *values,
                              double *grads);
+
+/* Record one mossq_batch_adjoint call (items from |0...0>) as a HIP graph: the
+ * launch tables, item tables and scratch vectors are built once and held by
+ * the graph; mossq_batch_graph_run replays the whole batch (tiled sweeps,
+ * observe, gate-by-gate items on the second stream) with one launch call and
+ * copies the values and gradients out as mossq_batch_adjoint does.  With
+ * mossq_program_update between runs, the same graph evaluates new parameters
+ * (an optimizer's repeated batch gradients).  The programs and observables
+ * must outlive the graph; mossq_graph_destroy releases it.  No counterpart in
+ * the reference (its batch reverse runs per column, src/differentiation/
+ * batch.jl:35-66). */
+int mossq_batch_capture(mossq_ctx *ctx, int B, mossq_program *const *progs, mossq_obs *const *obs,
+                        const int64_t *grad_offset, mossq_graph **out);
+int mossq_batch_graph_run(mossq_ctx *ctx, mossq_graph *graph, double *values, double *grads);
 
 /* Values Re <H_b> of B independent state-vector programs in one engine run
  * (forward and observe steps only), item b starting from init[b] (a device

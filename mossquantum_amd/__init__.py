@@ -16,9 +16,9 @@ from .circuit import Delay, QCircuit, fuse_gates, lower
 from .gates import (CNOTGate, CPHASEGate, CRxGate, CRyGate, CRzGate, CZGate, FREDKINGate, Gate, HGate, PHASEGate,
                     QuantumGate, RxGate, RyGate, RzGate, SGate, SWAPGate, TGate, TOFFOLIGate, XGate, YGate, ZGate,
                     iSWAPGate)
-from .gradients import (AdjointGradient, BatchExpectationObjective, ExpectationObjective, ParameterMap,
-                        UnsupportedGradient, batched_adjoint, differentiable_value, evaluate, supports_gradient,
-                        value_and_gradient, value_and_pullback)
+from .gradients import (AdjointGradient, BatchAdjointGraph, BatchExpectationObjective, ExpectationObjective,
+                        ParameterMap, UnsupportedGradient, batched_adjoint, batched_adjoint_graph,
+                        differentiable_value, evaluate, supports_gradient, value_and_gradient, value_and_pullback)
 from .noise import NoiseModel, NoiseRule, NoisyCircuit, with_noise
 from .operators import QubitsOperator, QubitsTerm, tfim_chain, xxz_chain
 from .readout import counts, discrete_sample, draw_samples, measure_, sample

@@ -98,6 +98,7 @@ SIGNATURES = [
     ("mossq_plan_emulate_adjoint", _I, [_I, _I, C.POINTER(OpRecord), _DP, _U64, _DP, _DP, _DP, C.POINTER(C.c_int)]),
     ("mossq_plan_emulate_density_adjoint", _I,
      [_I, _I, C.POINTER(OpRecord), _DP, _U64, _I, _DP, _DP, _DP, C.POINTER(C.c_int)]),
+    ("mossq_program_update", _I, [_P, _P, _I, C.POINTER(OpRecord), _DP, _U64]),
     ("mossq_program_autotune", _I, [_P, _P, _I, C.POINTER(C.c_int), _DP]),
     ("mossq_program_capture", _I, [_P, _P, _P, C.POINTER(_P)]),
     ("mossq_graph_launch", _I, [_P, _P]),
@@ -108,6 +109,8 @@ SIGNATURES = [
     ("mossq_batch_adjoint", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _DP, _DP]),
     ("mossq_batch_adjoint_init", _I,
      [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), _DP, _DP]),
+    ("mossq_batch_capture", _I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64), C.POINTER(_P)]),
+    ("mossq_batch_graph_run", _I, [_P, _P, _DP, _DP]),
     ("mossq_sv_expectation_batch", _I, [_P, _P, _P, _I, _DP]),
     ("mossq_dm_expectation_batch", _I, [_P, _P, _P, _I, _DP]),
     ("mossq_server_create", _I, [_P, _U64, C.POINTER(_P)]),

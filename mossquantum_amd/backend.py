@@ -202,6 +202,22 @@ class Program:
         check(self.ctx.lib.mossq_program_autotune(self.ctx.h, self.h, int(reps), C.byref(nv), C.byref(best)))
         return nv.value, best.value
 
+    def update(self, circuit: QCircuit | None = None, pool=None):
+        """Rebind in place (mossq_program_update): new gate matrices for the
+        same records, from a circuit of the same structure with other
+        parameter values, or a pool for the current lowering.  Plans and
+        device tables stay where they are, so graphs captured from this
+        program (capture, BatchAdjointGraph) replay the new values.  Raises
+        when the new values lower to another program or change a fused plan;
+        the program is unchanged then."""
+        lw = self.lowered if circuit is None else lower(circuit, self.n, self.is_density)
+        recs, base_pool = lw.arrays()
+        pool = base_pool if pool is None else np.ascontiguousarray(pool, dtype=np.complex128)
+        check(self.ctx.lib.mossq_program_update(self.ctx.h, self.h, len(lw.records), recs,
+                                                pool.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)),
+                                                pool.size))
+        self.lowered = lw
+
     def capture(self, storage: HIPArray) -> "ProgramGraph":
         """A HIP graph replaying this program on `storage` (one launch call per run)."""
         h = C.c_void_p()

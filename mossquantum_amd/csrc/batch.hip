@@ -300,29 +300,56 @@ __global__ __launch_bounds__(kBT) void k_brows(const BRows *__restrict__ rows, i
     if (threadIdx.x == 0) grads[r.gslot0 + j] = s.re;
 }
 
-// Batch of B items; one host synchronisation at the end.  Items whose programs
-// carry all-sweep fused plans (12 or more qubits) run their forward and
-// reverse passes as batched LDS-tile sweeps (launch s: sweep s of every such
-// item) around one engine iteration for the observe step; the others run the
-// gate-by-gate engine micro-program.
+// Everything one batch needs on the device, built on the host once: the
+// item tables, the launch tables of the tiled sweeps, and the scratch
+// vectors.  An engine call is prepare, enqueue, finish; a captured batch graph
+// keeps its BatchRun (and the memory) until the graph is destroyed.
+struct BatchRun {
+    Ctx *c;
+    int B = 0;
+    bool value_only = false;
+    std::vector<const Program *> progs;
+    std::vector<int64_t> grad_offset;
+    Allocs al;
+    c128 *vec = nullptr, *part = nullptr, *part2 = nullptr, *rpart = nullptr;
+    BOp *d_ops = nullptr;
+    BItemDev *d_items = nullptr;
+    BRows *d_rows = nullptr;
+    double *d_vals = nullptr, *d_grads = nullptr;
+    int nchunks[2] = {0, 0}, ntab[2] = {0, 0}, tmax = 0, np = 0, nrows = 0;
+    i64 dtot = 0, rev_parts = 0;
+    BatchSweeps fwd, rev;
+    std::vector<std::pair<c128 *, const void *>> presets;  // (psi, init) copies of each call
+    std::vector<u64> preset_len;
+    explicit BatchRun(Ctx *cc) : c(cc), al(cc) {}
+};
+
+// Batch of B items.  Items whose programs carry all-sweep fused plans (12 or
+// more qubits) run their forward and reverse passes as batched LDS-tile sweeps
+// (launch s: sweep s of every such item) around one observe launch; the others
+// run the gate-by-gate engine micro-program.
 // value_only: forward and observe steps only (no lambda vectors, no gradients);
 // init[b], when given, is item b's initial state (else |0...0>).
-static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, const Obs *const *obs,
-                                const int64_t *grad_offset, double *values, double *grads, bool value_only,
-                                const void *const *init) {
+static int batch_prepare(BatchRun &R, int B, const Program *const *progs, const Obs *const *obs,
+                         const int64_t *grad_offset, bool value_only, const void *const *init) {
+    Ctx *c = R.c;
+    R.B = B;
+    R.value_only = value_only;
+    R.progs.assign(progs, progs + B);
+    R.grad_offset.assign(B, 0);
+    if (!value_only) R.grad_offset.assign(grad_offset, grad_offset + B);
     std::vector<BItemHost> items(B);
     std::vector<BOp> all_ops;
     std::vector<int> op_off(B), nder(B);
     std::vector<char> tiled(B);
     std::vector<int> order;  // tiled items first, then stepped ones
-    i64 total_amps = 0, dtot = 0, rev_parts = 0;
-    int nchunks[2] = {0, 0}, ntab[2] = {0, 0}, tmax = 0;
+    i64 total_amps = 0;
     for (int b = 0; b < B; ++b) {
         tiled[b] = fused_plan_batchable(progs[b], !value_only);
         if (tiled[b]) {
             items[b].ops.clear();
             nder[b] = progs[b]->nderiv;
-            if (!value_only) rev_parts += i64(std::max(progs[b]->nderiv, 1)) * fused_adjoint_parts(progs[b]);
+            if (!value_only) R.rev_parts += i64(std::max(progs[b]->nderiv, 1)) * fused_adjoint_parts(progs[b]);
         } else {
             MOSSQ_TRY(batch_lower(progs[b], items[b].ops, &nder[b]));
         }
@@ -332,36 +359,32 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         items[b].nchunk = batch_chunks(progs[b]->n);
         items[b].gslot0 = value_only ? 0 : int(grad_offset[b]);
         total_amps += i64(1) << progs[b]->n;
-        if (!value_only) dtot = std::max<i64>(dtot, grad_offset[b] + nder[b]);
+        if (!value_only) R.dtot = std::max<i64>(R.dtot, grad_offset[b] + nder[b]);
         const int k = tiled[b] ? 0 : 1;
-        nchunks[k] += items[b].nchunk;
-        ++ntab[k];
-        if (!tiled[b]) tmax = std::max(tmax, (value_only ? 1 : 2) * items[b].nfwd);
+        R.nchunks[k] += items[b].nchunk;
+        ++R.ntab[k];
+        if (!tiled[b]) R.tmax = std::max(R.tmax, (value_only ? 1 : 2) * items[b].nfwd);
     }
     for (int k = 0; k < 2; ++k)
         for (int b = 0; b < B; ++b)
             if ((tiled[b] != 0) == (k == 0)) order.push_back(b);
-    Allocs al(c);
-    c128 *vec = nullptr, *part = nullptr, *part2 = nullptr, *rpart = nullptr;
-    BOp *d_ops = nullptr;
-    BItemDev *d_items = nullptr;
-    BRows *d_rows = nullptr;
-    double *d_vals = nullptr, *d_grads = nullptr;
-    MOSSQ_TRY(al.get(u64(total_amps) * (value_only ? 1 : 2) * sizeof(c128), (void **)&vec));
-    MOSSQ_TRY(al.get(u64(std::max(nchunks[0], 1)) * sizeof(c128), (void **)&part));
-    MOSSQ_TRY(al.get(u64(std::max(nchunks[1], 1)) * sizeof(c128), (void **)&part2));
-    MOSSQ_TRY(al.get(std::max<size_t>(all_ops.size(), 1) * sizeof(BOp), (void **)&d_ops));
-    MOSSQ_TRY(al.get(u64(B) * sizeof(BItemDev), (void **)&d_items));
-    MOSSQ_TRY(al.get(u64(B) * sizeof(double), (void **)&d_vals));
-    MOSSQ_TRY(al.get(u64(std::max<i64>(dtot, 1)) * sizeof(double), (void **)&d_grads));
-    if (rev_parts) MOSSQ_TRY(al.get(u64(rev_parts) * sizeof(c128), (void **)&rpart));
+    Allocs &al = R.al;
+    MOSSQ_TRY(al.get(u64(total_amps) * (value_only ? 1 : 2) * sizeof(c128), (void **)&R.vec));
+    MOSSQ_TRY(al.get(u64(std::max(R.nchunks[0], 1)) * sizeof(c128), (void **)&R.part));
+    MOSSQ_TRY(al.get(u64(std::max(R.nchunks[1], 1)) * sizeof(c128), (void **)&R.part2));
+    MOSSQ_TRY(al.get(std::max<size_t>(all_ops.size(), 1) * sizeof(BOp), (void **)&R.d_ops));
+    MOSSQ_TRY(al.get(u64(B) * sizeof(BItemDev), (void **)&R.d_items));
+    MOSSQ_TRY(al.get(u64(B) * sizeof(double), (void **)&R.d_vals));
+    MOSSQ_TRY(al.get(u64(std::max<i64>(R.dtot, 1)) * sizeof(double), (void **)&R.d_grads));
+    if (R.rev_parts) MOSSQ_TRY(al.get(u64(R.rev_parts) * sizeof(c128), (void **)&R.rpart));
+    c128 *vec = R.vec;
     std::vector<BItemDev> hd(B);
     std::vector<const Program *> tprogs;
     std::vector<u64> tx, tl, tpart;
     std::vector<int> trow;
     std::vector<BRows> rows;
     i64 vo = 0, ro = 0;
-    int ch[2] = {0, 0}, np = 0;
+    int ch[2] = {0, 0};
     for (int pos = 0; pos < B; ++pos) {
         const int b = order[pos];
         const int k = tiled[b] ? 0 : 1;
@@ -370,7 +393,7 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         d.psi = vec + vo;
         d.lam = value_only ? nullptr : vec + vo + len;
         vo += (value_only ? 1 : 2) * len;
-        d.ops = d_ops + op_off[b];
+        d.ops = R.d_ops + op_off[b];
         d.pool = progs[b]->d_pool;
         d.groups = obs[b]->d_groups;
         d.mats = obs[b]->d_mats;
@@ -386,8 +409,10 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
         d.value_only = value_only ? 1 : 0;
         d.preset = (init && init[b]) ? 1 : 0;
         ch[k] += d.nchunk;
-        if (d.preset)
-            MOSSQ_HIP(hipMemcpyAsync(d.psi, init[b], u64(len) * sizeof(c128), hipMemcpyDeviceToDevice, c->stream));
+        if (d.preset) {
+            R.presets.push_back({d.psi, init[b]});
+            R.preset_len.push_back(u64(len));
+        }
         if (tiled[b]) {
             tprogs.push_back(progs[b]);
             tx.push_back(u64(d.psi - vec));
@@ -396,42 +421,56 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
             tpart.push_back(u64(ro));
             trow.push_back(row);
             if (!value_only && nder[b] > 0) {
-                rows.push_back(BRows{rpart + ro, row, nder[b], d.gslot0, np});
-                np += nder[b];
+                rows.push_back(BRows{R.rpart + ro, row, nder[b], d.gslot0, R.np});
+                R.np += nder[b];
             }
             ro += i64(std::max(nder[b], 1)) * row;
         }
     }
-    BatchSweeps fwd, rev;
     const int nt = int(tprogs.size());
     if (nt) {
         const u64 vec_len = u64(total_amps) * (value_only ? 1 : 2);
         MOSSQ_TRY(batch_sweeps_build(c, al, nt, tprogs.data(), vec, vec_len, tx.data(), nullptr, nullptr, 0, nullptr,
-                                     nullptr, false, &fwd));
+                                     nullptr, false, &R.fwd));
         if (!value_only)
-            MOSSQ_TRY(batch_sweeps_build(c, al, nt, tprogs.data(), vec, vec_len, tx.data(), tl.data(), rpart,
-                                         u64(rev_parts), tpart.data(), trow.data(), true, &rev));
+            MOSSQ_TRY(batch_sweeps_build(c, al, nt, tprogs.data(), vec, vec_len, tx.data(), tl.data(), R.rpart,
+                                         u64(R.rev_parts), tpart.data(), trow.data(), true, &R.rev));
     }
+    R.nrows = int(rows.size());
     if (!rows.empty()) {
-        MOSSQ_TRY(al.get(rows.size() * sizeof(BRows), (void **)&d_rows));
-        MOSSQ_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * sizeof(BRows), hipMemcpyHostToDevice, c->stream));
+        MOSSQ_TRY(al.get(rows.size() * sizeof(BRows), (void **)&R.d_rows));
+        MOSSQ_HIP(hipMemcpyAsync(R.d_rows, rows.data(), rows.size() * sizeof(BRows), hipMemcpyHostToDevice,
+                                 c->stream));
     }
     if (!all_ops.empty())
-        MOSSQ_HIP(hipMemcpyAsync(d_ops, all_ops.data(), all_ops.size() * sizeof(BOp), hipMemcpyHostToDevice,
+        MOSSQ_HIP(hipMemcpyAsync(R.d_ops, all_ops.data(), all_ops.size() * sizeof(BOp), hipMemcpyHostToDevice,
                                  c->stream));
-    MOSSQ_HIP(hipMemcpyAsync(d_items, hd.data(), hd.size() * sizeof(BItemDev), hipMemcpyHostToDevice, c->stream));
-    const BItemDev *tab[2] = {d_items, d_items + ntab[0]};
+    MOSSQ_HIP(hipMemcpyAsync(R.d_items, hd.data(), hd.size() * sizeof(BItemDev), hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));  // the host tables go out of scope
+    return MOSSQ_OK;
+}
+
+// The device work of one engine call on the context stream: initial states,
+// tiled sweeps and observe, gate-by-gate items on the second stream (joined
+// before return).  Kernel launches, event waits and device-to-device copies of
+// the initial states only, so a stream capture of it replays as a whole.
+static int batch_enqueue(BatchRun &R) {
+    Ctx *c = R.c;
+    for (size_t i = 0; i < R.presets.size(); ++i)
+        MOSSQ_HIP(hipMemcpyAsync(R.presets[i].first, R.presets[i].second, R.preset_len[i] * sizeof(c128),
+                                 hipMemcpyDeviceToDevice, c->stream));
+    const BItemDev *tab[2] = {R.d_items, R.d_items + R.ntab[0]};
     for (int k = 0; k < 2; ++k)
-        if (ntab[k]) {
-            hipLaunchKernelGGL(k_binit, dim3(nchunks[k]), dim3(kBT), 0, c->stream, tab[k], ntab[k]);
+        if (R.ntab[k]) {
+            hipLaunchKernelGGL(k_binit, dim3(R.nchunks[k]), dim3(kBT), 0, c->stream, tab[k], R.ntab[k]);
             MOSSQ_HIP(hipGetLastError());
         }
     // With both kinds of items, the gate-by-gate items (many small launches)
     // run on the second stream beside the tiled sweeps; they share no memory
     // but the result arrays, where their slots are disjoint.
-    const bool overlap = ntab[0] && ntab[1];
+    const bool overlap = R.ntab[0] && R.ntab[1];
     hipStream_t s2 = c->stream;
-    AuxJoin join(c);  // on every return, before al's blocks go back to the cache
+    AuxJoin join(c);  // on every return, before the scratch can go back to the cache
     if (overlap) {
         MOSSQ_TRY(ctx_aux(c));
         MOSSQ_HIP(hipEventRecord(c->aux_ev[0], c->stream));
@@ -442,41 +481,59 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
     // The tiled sweeps (a few long launches) are enqueued before the
     // gate-by-gate items' many short ones, so the device starts on them while
     // the host is still enqueueing the second stream's launches.
-    if (ntab[0]) {
-        MOSSQ_TRY(batch_sweeps_run(c, fwd, false));
+    if (R.ntab[0]) {
+        MOSSQ_TRY(batch_sweeps_run(c, R.fwd, false));
         // observe (the tiled items have no engine gate steps: t = 0 is their observe step)
-        hipLaunchKernelGGL(k_bobserve, dim3(nchunks[0]), dim3(kBT), 0, c->stream, tab[0], ntab[0], part);
+        hipLaunchKernelGGL(k_bobserve, dim3(R.nchunks[0]), dim3(kBT), 0, c->stream, tab[0], R.ntab[0], R.part);
         MOSSQ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_breduce, dim3(ntab[0]), dim3(kBT), 0, c->stream, tab[0], 0, part, d_vals, d_grads);
+        hipLaunchKernelGGL(k_breduce, dim3(R.ntab[0]), dim3(kBT), 0, c->stream, tab[0], 0, R.part, R.d_vals,
+                           R.d_grads);
         MOSSQ_HIP(hipGetLastError());
     }
-    if (ntab[0] && !value_only) {
-        MOSSQ_TRY(zero_fill(c, rpart, u64(rev_parts)));
-        MOSSQ_TRY(batch_sweeps_run(c, rev, true));
-        if (np) {
-            hipLaunchKernelGGL(k_brows, dim3(np), dim3(kBT), 0, c->stream, d_rows, int(rows.size()), d_grads);
+    if (R.ntab[0] && !R.value_only) {
+        MOSSQ_TRY(zero_fill(c, R.rpart, u64(R.rev_parts)));
+        MOSSQ_TRY(batch_sweeps_run(c, R.rev, true));
+        if (R.np) {
+            hipLaunchKernelGGL(k_brows, dim3(R.np), dim3(kBT), 0, c->stream, R.d_rows, R.nrows, R.d_grads);
             MOSSQ_HIP(hipGetLastError());
         }
     }
-    if (ntab[1])
-        for (int t = 0; t <= tmax; ++t)
-            MOSSQ_TRY(batch_iteration_on(s2, tab[1], ntab[1], nchunks[1], t, part2, d_vals, d_grads));
+    if (R.ntab[1])
+        for (int t = 0; t <= R.tmax; ++t)
+            MOSSQ_TRY(batch_iteration_on(s2, tab[1], R.ntab[1], R.nchunks[1], t, R.part2, R.d_vals, R.d_grads));
     if (overlap) {
         join.active = false;
         MOSSQ_HIP(hipEventRecord(c->aux_ev[1], c->aux));
         MOSSQ_HIP(hipStreamWaitEvent(c->stream, c->aux_ev[1], 0));
     }
-    std::vector<double> hv(B), hg(std::max<i64>(dtot, 1));
-    MOSSQ_HIP(hipMemcpyAsync(hv.data(), d_vals, B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    MOSSQ_HIP(hipMemcpyAsync(hg.data(), d_grads, hg.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return MOSSQ_OK;
+}
+
+// Values and gradients to the host (one synchronisation).
+static int batch_finish(BatchRun &R, double *values, double *grads) {
+    Ctx *c = R.c;
+    std::vector<double> hv(R.B), hg(std::max<i64>(R.dtot, 1));
+    MOSSQ_HIP(hipMemcpyAsync(hv.data(), R.d_vals, R.B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(hg.data(), R.d_grads, hg.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));
-    for (int b = 0; b < B; ++b) {
+    for (int b = 0; b < R.B; ++b) {
         values[b] = hv[b];
-        if (!value_only)
-            for (int j = 0; j < progs[b]->nderiv; ++j) grads[grad_offset[b] + j] = hg[grad_offset[b] + j];
+        if (!R.value_only)
+            for (int j = 0; j < R.progs[b]->nderiv; ++j) grads[R.grad_offset[b] + j] = hg[R.grad_offset[b] + j];
     }
     return MOSSQ_OK;
 }
+
+static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, const Obs *const *obs,
+                                const int64_t *grad_offset, double *values, double *grads, bool value_only,
+                                const void *const *init) {
+    BatchRun R(c);
+    MOSSQ_TRY(batch_prepare(R, B, progs, obs, grad_offset, value_only, init));
+    MOSSQ_TRY(batch_enqueue(R));
+    return batch_finish(R, values, grads);
+}
+
+void batch_run_free(BatchRun *r) { delete r; }
 
 static int batch_items_check(Ctx *c, int B, mossq_program *const *progs, mossq_obs *const *obs,
                              std::vector<const Program *> &P, std::vector<const Obs *> &O) {
@@ -525,6 +582,57 @@ int mossq_batch_adjoint_init(mossq_ctx *ctx, int B, mossq_program *const *progs,
     for (int b = 0; b < B; ++b) MOSSQ_CHECK(grad_offset[b] >= 0, MOSSQ_EINVAL, "negative gradient offset");
     if (B == 0) return MOSSQ_OK;
     return batch_adjoint_engine(c, B, P.data(), O.data(), grad_offset, values, grads, false, init);
+}
+
+int mossq_batch_capture(mossq_ctx *ctx, int B, mossq_program *const *progs, mossq_obs *const *obs,
+                        const int64_t *grad_offset, mossq_graph **out) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(B >= 1 && progs && obs && grad_offset && out, MOSSQ_EINVAL, "bad batch arguments");
+    Ctx *c = C(ctx);
+    std::vector<const Program *> P;
+    std::vector<const Obs *> O;
+    MOSSQ_TRY(batch_items_check(c, B, progs, obs, P, O));
+    for (int b = 0; b < B; ++b) MOSSQ_CHECK(grad_offset[b] >= 0, MOSSQ_EINVAL, "negative gradient offset");
+    auto *R = new BatchRun(c);
+    int s = batch_prepare(*R, B, P.data(), O.data(), grad_offset, false, nullptr);
+    if (s == MOSSQ_OK) s = ctx_aux(c);  // created outside the capture
+    if (s != MOSSQ_OK) {
+        delete R;
+        return s;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        s = batch_enqueue(*R);
+        e = hipStreamEndCapture(c->stream, &g);
+    }
+    hipGraphExec_t ex = nullptr;
+    if (s == MOSSQ_OK && e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (s != MOSSQ_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        delete R;
+        if (s != MOSSQ_OK) return s;
+        return fail(MOSSQ_EDEVICE, std::string("batch capture: ") + hipGetErrorString(e));
+    }
+    auto *gr = new mossq_graph();
+    gr->ctx = c;
+    gr->batch = R;
+    for (const Program *p : P)
+        if (std::find(gr->progs.begin(), gr->progs.end(), p) == gr->progs.end()) gr->progs.push_back(p);
+    for (const Program *p : gr->progs) p->live_graphs++;
+    gr->graph = g;
+    gr->exec = ex;
+    *out = gr;
+    return MOSSQ_OK;
+}
+
+int mossq_batch_graph_run(mossq_ctx *ctx, mossq_graph *graph, double *values, double *grads) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(graph && graph->batch && values && grads, MOSSQ_EINVAL, "bad batch graph arguments");
+    Ctx *c = C(ctx);
+    MOSSQ_CHECK(graph->ctx == c, MOSSQ_EINVAL, "graph belongs to another context");
+    MOSSQ_HIP(hipGraphLaunch(graph->exec, c->stream));
+    return batch_finish(*graph->batch, values, grads);
 }
 
 int mossq_batch_expectation(mossq_ctx *ctx, int B, mossq_program *const *progs, mossq_obs *const *obs,

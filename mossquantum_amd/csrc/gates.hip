@@ -7,6 +7,7 @@
 // inserting zero bits at the k target positions, so consecutive lanes touch
 // consecutive amplitudes whenever the lowest target bit is >= 2 and runs of
 // 2^q amplitudes otherwise; every byte of the state is read and written once.
+#include <cmath>
 #include <cstring>
 
 #include "mossq_internal.h"
@@ -240,7 +241,12 @@ MatInfo classify(const c128 *m, int dim) {
     info.nchanged = 0;
     for (int r = 0; r < dim; ++r) {
         const c128 &e = m[r + dim * src_of_row[r]];
-        bool ident = (src_of_row[r] == r) && e.re == 1.0 && e.im == 0.0;
+        // A diagonal entry within a few ulps of 1 counts as 1: |u|^2 = c^2 + s^2
+        // in the density superoperator of a phase gate rounds to 1 - 2^-53,
+        // 1 or 1 + 2^-52 depending on the angle, and the kernels chosen (and
+        // the tables an in-place rebinding must keep) would follow that rounding.
+        const double tol = 4 * 2.220446049250313e-16;
+        bool ident = (src_of_row[r] == r) && std::fabs(e.re - 1.0) <= tol && std::fabs(e.im) <= tol;
         if (!ident) {
             int j = info.nchanged++;
             info.rows[j] = r;

@@ -46,8 +46,29 @@ struct Program {
     // `retired` (released when the last graph is destroyed, or with the
     // program).
     mutable int live_graphs = 0;
+    // of which program graphs, which may launch unfused steps with their
+    // monomial entries as kernel arguments (not rewritten by an update)
+    mutable int live_step_graphs = 0;
     std::vector<FusedPlan *> retired;
 };
+
+struct BatchRun;  // a prepared batch of the batched engine (batch.hip)
+void batch_run_free(BatchRun *r);
+
+}  // namespace mossq
+
+// A captured graph: one program run (prog) or one batched-engine call (batch,
+// over the programs in progs), with what it keeps alive.
+struct mossq_graph {
+    mossq::Ctx *ctx = nullptr;
+    const mossq::Program *prog = nullptr;  // its plan stays alive while the graph is (Program::retired)
+    std::vector<const mossq::Program *> progs;
+    mossq::BatchRun *batch = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace mossq {
 
 // Host side of mossq_program_create: validate records, build steps and pool.
 int program_build_host(Program *p, int n, int is_density, int nops, const mossq_op *ops, const double *pool,
@@ -66,6 +87,15 @@ int fused_run(Ctx *c, const Program *p, c128 *state);
 // Time every forward-plan variant on a scratch state (reps runs each) and keep
 // the fastest as p->fused; tried = distinct variants, best_ms = its run time.
 int fused_plan_autotune(Program *p, int reps, int *tried, double *best_ms);
+// New matrix values (q: the program rebuilt from the new records): every plan
+// of p is rebuilt with the variant it came from and must keep its tables; the
+// values are then written over p's device arrays in place (MOSSQ_EINVAL, p
+// unchanged, when a plan's structure would change).
+int fused_plan_update(Program *p, const Program *q);
+// Whether the forward run of p launches step i on its own (an unfused item of
+// the fused plan, or every step of an unfused program), passing a monomial
+// step's entries as kernel arguments.
+bool fused_run_launches_step(const Program *p, int i);
 // Releases the plans kept for captured graphs (call with no graph alive).
 void fused_plan_release_retired(Program *p);
 // Reverse pass of the pure-state adjoint on (y, l) with the adjoint plan;

@@ -10,6 +10,7 @@
 // so every launch is one sweep of a dense or monomial block kernel.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 
 #include "mossq_program.h"
@@ -537,6 +538,62 @@ int mossq_program_create(mossq_ctx *ctx, int n, int is_density, int nops, const 
     return MOSSQ_OK;
 }
 
+// Same steps up to matrix values: kinds, bits, pool offsets and, for
+// monomial matrices, the pattern of changed rows.  moved[i] reports whether
+// monomial step i's entries differ.
+static bool same_steps(const Program &a, const Program &b, std::vector<char> &moved) {
+    moved.assign(a.steps.size(), 0);
+    if (a.steps.size() != b.steps.size() || a.h_pool.size() != b.h_pool.size() || a.nderiv != b.nderiv) return false;
+    bool m = false;
+    auto same_info = [&](const MatInfo &x, const MatInfo &y) {
+        if (x.monomial != y.monomial || x.nchanged != y.nchanged) return false;
+        for (int j = 0; x.monomial && j < x.nchanged; ++j) {
+            if (x.rows[j] != y.rows[j] || x.src[j] != y.src[j]) return false;
+            if (x.val[j].re != y.val[j].re || x.val[j].im != y.val[j].im) m = true;
+        }
+        return true;
+    };
+    for (size_t i = 0; i < a.steps.size(); ++i) {
+        const Step &x = a.steps[i], &y = b.steps[i];
+        m = false;
+        if (x.channel != y.channel || x.k != y.k || x.kb != y.kb || memcmp(x.bits, y.bits, sizeof(x.bits)) != 0 ||
+            x.fwd != y.fwd || x.rev != y.rev || x.grad != y.grad || x.grad_index != y.grad_index ||
+            x.u_src != y.u_src || x.split != y.split || x.fwd_cols != y.fwd_cols || x.rev_cols != y.rev_cols)
+            return false;
+        if (!same_info(x.ifwd, y.ifwd) || !same_info(x.irev, y.irev) || !same_info(x.ifwd_cols, y.ifwd_cols) ||
+            !same_info(x.irev_cols, y.irev_cols))
+            return false;
+        moved[i] = m ? 1 : 0;
+    }
+    return true;
+}
+
+int mossq_program_update(mossq_ctx *ctx, mossq_program *prog, int nops, const mossq_op *ops, const double *pool,
+                         uint64_t pool_len) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(prog, MOSSQ_EINVAL, "null argument");
+    Program *p = reinterpret_cast<Program *>(prog);
+    Ctx *c = C(ctx);
+    MOSSQ_CHECK(p->ctx == c, MOSSQ_EINVAL, "program belongs to another context");
+    Program q;
+    MOSSQ_TRY(program_build_host(&q, p->n, p->is_density, nops, ops, pool, pool_len));
+    std::vector<char> moved;
+    MOSSQ_CHECK(same_steps(*p, q, moved), MOSSQ_EINVAL, "the records describe another program");
+    // A program graph recorded the entries of the monomial steps its run
+    // launches on their own (kernel arguments); those must not move under it.
+    for (size_t i = 0; i < moved.size() && p->live_step_graphs > 0; ++i)
+        MOSSQ_CHECK(!(moved[i] && fused_run_launches_step(p, int(i))), MOSSQ_EINVAL,
+                    "a program graph launches a step whose entries would change: destroy it first");
+    MOSSQ_TRY(fused_plan_update(p, &q));
+    p->steps.swap(q.steps);
+    p->h_pool.swap(q.h_pool);
+    if (!p->h_pool.empty())
+        MOSSQ_HIP(hipMemcpyAsync(p->d_pool, p->h_pool.data(), p->h_pool.size() * sizeof(c128),
+                                 hipMemcpyHostToDevice, c->stream));
+    MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    return MOSSQ_OK;
+}
+
 int mossq_program_destroy(mossq_program *prog) {
     if (!prog) return MOSSQ_OK;
     Program *p = reinterpret_cast<Program *>(prog);
@@ -573,12 +630,6 @@ int mossq_program_autotune(mossq_ctx *ctx, mossq_program *prog, int reps, int *v
     return fused_plan_autotune(p, reps, variants, best_ms);
 }
 
-struct mossq_graph {
-    Ctx *ctx = nullptr;
-    const Program *prog = nullptr;  // its plan stays alive while the graph is (Program::retired)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
 
 int mossq_program_capture(mossq_ctx *ctx, const mossq_program *prog, void *state, mossq_graph **out) {
     MOSSQ_TRY(ctx_check(ctx));
@@ -607,6 +658,7 @@ int mossq_program_capture(mossq_ctx *ctx, const mossq_program *prog, void *state
     gr->ctx = c;
     gr->prog = p;
     p->live_graphs++;
+    p->live_step_graphs++;
     gr->graph = g;
     gr->exec = ex;
     *out = gr;
@@ -628,8 +680,16 @@ int mossq_graph_destroy(mossq_graph *graph) {
     (void)hipStreamSynchronize(graph->ctx->stream);
     if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
     if (graph->graph) (void)hipGraphDestroy(graph->graph);
-    Program *p = const_cast<Program *>(graph->prog);
-    if (--p->live_graphs == 0) fused_plan_release_retired(p);
+    if (graph->prog) {
+        Program *p = const_cast<Program *>(graph->prog);
+        p->live_step_graphs--;
+        if (--p->live_graphs == 0) fused_plan_release_retired(p);
+    }
+    for (const Program *cp : graph->progs) {
+        Program *p = const_cast<Program *>(cp);
+        if (--p->live_graphs == 0) fused_plan_release_retired(p);
+    }
+    if (graph->batch) batch_run_free(graph->batch);
     delete graph;
     return MOSSQ_OK;
 }

@@ -932,6 +932,8 @@ struct FusedPlan {
     c128 *d_pool = nullptr;
     TProbe *d_probes = nullptr;
     int rb = 4;            // register bits of every layout (4: 256 threads per tile, 3: 512)
+    int min_tail = 0;      // planner variant the plan came from (rebuilt alike by
+    bool kron = true;      // fused_plan_update)
     bool adjoint = false;  // reverse plan of the pure-state adjoint (probes, no general diagonals)
     bool reverse_steps = false;  // unfused items apply the steps' reverse matrices
     bool shared_device = false;  // device arrays live in a block owned elsewhere
@@ -1546,6 +1548,8 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
     FusedPlan *fp = new FusedPlan();
     fp->adjoint = adjoint;
     fp->rb = rb;
+    fp->min_tail = min_tail;
+    fp->kron = kron;
     const int tn = 1 << rb;
     std::vector<char> done(ops.size(), 0);
     size_t first = 0;
@@ -2988,6 +2992,102 @@ static void emulate_density_adjoint(const Program *p, const DensityAdjointPlan *
         return MOSSQ_OK;
     });
 }
+// ------------------------------------------------------ parameter update ----
+// New matrix values for a program whose structure stays: every plan is
+// rebuilt from q (the program rebuilt from the new records) with the variant
+// it came from, and must come out with the same tables but for matrix values,
+// diagonal values and probe factors, which are then written over the old
+// plan's device arrays in place (captured graphs keep reading them).
+static bool same_tables(const FusedPlan *a, const FusedPlan *b) {
+    if (a->rb != b->rb || a->adjoint != b->adjoint || a->items.size() != b->items.size() ||
+        a->h_segs.size() != b->h_segs.size() || a->h_groups.size() != b->h_groups.size() ||
+        a->h_diag.size() != b->h_diag.size() || a->h_probes.size() != b->h_probes.size() ||
+        a->h_pool.size() != b->h_pool.size() || a->reverse_steps != b->reverse_steps)
+        return false;
+    for (size_t i = 0; i < a->items.size(); ++i)
+        if (a->items[i].sweep != b->items[i].sweep || a->items[i].step != b->items[i].step ||
+            memcmp(&a->items[i].args, &b->items[i].args, sizeof(TSweepArgs)) != 0)
+            return false;
+    if (memcmp(a->h_segs.data(), b->h_segs.data(), a->h_segs.size() * sizeof(TSeg)) != 0 ||
+        memcmp(a->h_groups.data(), b->h_groups.data(), a->h_groups.size() * sizeof(TGroup)) != 0)
+        return false;
+    for (size_t i = 0; i < a->h_diag.size(); ++i) {
+        const TDiag &x = a->h_diag[i], &y = b->h_diag[i];
+        if (x.code != y.code || memcmp(x.lbit, y.lbit, sizeof(x.lbit)) != 0 || memcmp(x.qbit, y.qbit, sizeof(x.qbit)) ||
+            x.rp_lo != y.rp_lo || x.rp_hi != y.rp_hi || x.negmask != y.negmask || x.moff != y.moff)
+            return false;
+    }
+    for (size_t i = 0; i < a->h_probes.size(); ++i) {
+        const TProbe &x = a->h_probes[i], &y = b->h_probes[i];
+        if (x.probe != y.probe || x.pcode != y.pcode || x.pmoff != y.pmoff || x.pmask != y.pmask) return false;
+    }
+    return true;
+}
+
+static int upload_values(Ctx *c, FusedPlan *old, FusedPlan *nw) {
+    MOSSQ_HIP(hipMemcpyAsync(old->d_pool, nw->h_pool.data(), nw->h_pool.size() * sizeof(c128), hipMemcpyHostToDevice,
+                             c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(old->d_diag, nw->h_diag.data(), nw->h_diag.size() * sizeof(TDiag), hipMemcpyHostToDevice,
+                             c->stream));
+    MOSSQ_HIP(hipMemcpyAsync(old->d_probes, nw->h_probes.data(), nw->h_probes.size() * sizeof(TProbe),
+                             hipMemcpyHostToDevice, c->stream));
+    old->h_pool.swap(nw->h_pool);
+    old->h_diag.swap(nw->h_diag);
+    old->h_probes.swap(nw->h_probes);
+    return MOSSQ_OK;
+}
+
+bool fused_run_launches_step(const Program *p, int i) {
+    if (!p->fused) return true;
+    for (const PlanItem &it : p->fused->items)
+        if (!it.sweep && it.step == i) return true;
+    return false;
+}
+
+int fused_plan_update(Program *p, const Program *q) {
+    MOSSQ_CHECK(p->retired.empty(), MOSSQ_EINVAL, "a captured graph still uses a plan the autotuner replaced");
+    std::string err;
+    std::vector<std::pair<FusedPlan *, FusedPlan *>> pairs;  // (old, rebuilt)
+    DensityAdjointPlan *ndp = nullptr;
+    auto cleanup = [&](int code, const std::string &m) {
+        for (auto &pr : pairs) delete pr.second;
+        if (ndp) dm_plan_delete(ndp);
+        return fail(code, m);
+    };
+    for (FusedPlan *old : {p->fused, p->fused_rev}) {
+        if (!old) continue;
+        std::vector<POp> ops;
+        if (old->adjoint)
+            lower_steps_rev(q, ops);
+        else
+            lower_steps(q, ops);
+        FusedPlan *nw = plan_with_tail(q, ops, old->min_tail, old->adjoint, old->kron, old->rb, &err);
+        if (!nw) return cleanup(MOSSQ_EINVAL, "rebuilding a plan failed: " + err);
+        pairs.push_back({old, nw});
+        if (!same_tables(old, nw)) return cleanup(MOSSQ_EINVAL, "the new values change the program's fused plan");
+    }
+    if (p->dm_adj) {
+        ndp = dm_plan_build_host(q, &err);
+        if (!ndp) return cleanup(MOSSQ_EINVAL, "rebuilding the density adjoint plans failed: " + err);
+        const std::vector<FusedPlan *> a = dm_plans(p->dm_adj), b = dm_plans(ndp);
+        if (a.size() != b.size()) return cleanup(MOSSQ_EINVAL, "the new values change the density adjoint plans");
+        for (size_t i = 0; i < a.size(); ++i)
+            if (!same_tables(a[i], b[i])) return cleanup(MOSSQ_EINVAL, "the new values change the density adjoint plans");
+        for (size_t i = 0; i < a.size(); ++i) pairs.push_back({a[i], b[i]});
+    }
+    int st = MOSSQ_OK;
+    for (auto &pr : pairs)
+        if (st == MOSSQ_OK) st = upload_values(p->ctx, pr.first, pr.second);
+    if (st == MOSSQ_OK && hipStreamSynchronize(p->ctx->stream) != hipSuccess)
+        st = fail(MOSSQ_EDEVICE, "hipStreamSynchronize failed");
+    for (auto &pr : pairs) delete pr.second;
+    if (ndp) {
+        for (auto *v : {&ndp->fwd, &ndp->rev, &ndp->revp, &ndp->segfwd, &ndp->segrev}) v->clear();
+        dm_plan_delete(ndp);
+    }
+    return st;
+}
+
 }  // namespace mossq
 
 using namespace mossq;

@@ -313,6 +313,59 @@ def batched_adjoint(circuits, observables, nqubits, ctx: _lib.Context | None = N
     return vals, out
 
 
+class BatchAdjointGraph:
+    """One batched adjoint run of device programs (the launches of
+    mossq_batch_adjoint: tiled sweeps, observe, gate-by-gate items on the
+    second stream) recorded as a HIP graph by mossq_batch_capture.  `run()`
+    replays the whole batch with one launch call and returns (values[B],
+    [slot gradient vector per item]).  Rebinding an item's program with
+    Program.update between runs evaluates its new parameters with the same
+    graph.  Holds the programs and observables so they outlive the graph."""
+
+    def __init__(self, programs, observables, ctx: _lib.Context | None = None):
+        self.ctx = ctx or programs[0].ctx
+        self.programs, self.observables = list(programs), list(observables)
+        B = len(self.programs)
+        if B == 0:
+            raise ValueError("a batch graph needs at least one program")
+        self.offs = np.zeros(B, dtype=np.int64)
+        tot = 0
+        for i, p in enumerate(self.programs):
+            self.offs[i] = tot
+            tot += p.lowered.nderiv
+        self.total = tot
+        ph = (C.c_void_p * B)(*[p.h.value for p in self.programs])
+        oh = (C.c_void_p * B)(*[o.h.value for o in self.observables])
+        h = C.c_void_p()
+        check(self.ctx.lib.mossq_batch_capture(self.ctx.h, B, ph, oh,
+                                               self.offs.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(h)))
+        self.h = h
+
+    def run(self):
+        vals = np.zeros(len(self.programs))
+        grads = np.zeros(max(self.total, 1))
+        check(self.ctx.lib.mossq_batch_graph_run(self.ctx.h, self.h, _lib.dptr(vals), _lib.dptr(grads)))
+        return vals, [p.lowered.slot_gradients(grads[o:o + p.lowered.nderiv])
+                      for p, o in zip(self.programs, self.offs)]
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.mossq_graph_destroy(self.h)
+            self.h = None
+        except Exception:
+            pass
+
+
+def batched_adjoint_graph(circuits, observables, nqubits, ctx: _lib.Context | None = None, fuse: bool = True):
+    """batched_adjoint's programs captured once (BatchAdjointGraph); rebind
+    item i with graph.programs[i].update(circuit) and replay with run()."""
+    ctx = ctx or _lib.default_context()
+    progs = [Program(c, n, False, ctx, fuse=fuse) for c, n in zip(circuits, nqubits)]
+    obs = [DeviceObservable(o, n, ctx) for o, n in zip(observables, nqubits)]
+    return BatchAdjointGraph(progs, obs, ctx)
+
+
 _torch_rule = None
 
 
