@@ -202,6 +202,8 @@ def bench_batched(ctx, B=512, layers=6, seed=20260300, cpu=True):
             out.update({"per_circuit_ms": plain * 1e3, "per_circuit_grads_per_s": B / plain,
                         "max_abs_grad_diff_vs_per_circuit": float(np.max(np.abs(grads - pg))),
                         "max_abs_value_diff_vs_per_circuit": float(np.max(np.abs(vals - pv)))})
+            # last on these programs: it rebinds them to new angles
+            out.update(bench_batch_graph(ctx, progs, ph, oh, offs, tot, vals, grads, sizes, layers, rng, reps))
         del progs
     wall, dev_ms, vals, grads = results["tiled"]
     swall, sdev, svals, sgrads = results["stepped"]
@@ -230,6 +232,67 @@ def bench_batched(ctx, B=512, layers=6, seed=20260300, cpu=True):
                              "time sums the per-size times over the batch's sizes")
         log(f"cpu estimate: {est:.1f} s for the batch")
     return out
+
+
+def bench_batch_graph(ctx, progs, ph, oh, offs, tot, vals, grads, sizes, layers, rng, reps):
+    """The batch captured once as a HIP graph (mossq_batch_capture) and
+    replayed (mossq_batch_graph_run): replays at the same angles must equal
+    the engine call bit for bit; then every program is rebound to new angles
+    (mossq_program_update, lowering done beforehand) and the graph replayed,
+    which must equal an engine call on the rebound programs."""
+    import ctypes as C
+    from mossquantum_amd import _lib, lower
+    from mossquantum_amd.workloads import hea_circuit
+    B = len(progs)
+    optr = offs.ctypes.data_as(C.POINTER(C.c_int64))
+    gh = C.c_void_p()
+    t0 = time.perf_counter()
+    _lib.check(ctx.lib.mossq_batch_capture(ctx.h, B, ph, oh, optr, C.byref(gh)))
+    cap_ms = (time.perf_counter() - t0) * 1e3
+    gv, gg = np.zeros(B), np.zeros(max(tot, 1))
+
+    def replay():
+        _lib.check(ctx.lib.mossq_batch_graph_run(ctx.h, gh, _lib.dptr(gv), _lib.dptr(gg)))
+
+    try:
+        replay()  # warm-up
+        ctx.sync()
+        w0 = time.perf_counter()
+        ctx.record(6)
+        for _ in range(reps):
+            replay()
+        ctx.record(7)
+        wall = (time.perf_counter() - w0) / reps
+        dev = ctx.elapsed_ms(6, 7) / reps
+        same = bool(np.array_equal(gv, vals) and np.array_equal(gg[:tot], grads[:tot]))
+        log(f"batch graph: capture {cap_ms:.1f} ms, replay {wall * 1e3:.1f} ms wall, {dev:.1f} ms device; "
+            f"equal to the engine call: {same}")
+        # rebinding to new angles: host lowering first (not timed), then updates + one replay
+        new = []
+        for p, n in zip(progs, sizes):
+            lw = lower(hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers)), n, False)
+            recs, pool = lw.arrays()
+            new.append((p, lw, recs, pool))
+        w0 = time.perf_counter()
+        for p, lw, recs, pool in new:
+            _lib.check(ctx.lib.mossq_program_update(ctx.h, p.h, len(lw.records), recs,
+                                                    pool.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)),
+                                                    pool.size))
+        upd = time.perf_counter() - w0
+        replay()
+        rebind = time.perf_counter() - w0
+        for p, lw, _, _ in new:
+            p.lowered = lw
+        ev, eg = np.zeros(B), np.zeros(max(tot, 1))
+        _lib.check(ctx.lib.mossq_batch_adjoint(ctx.h, B, ph, oh, optr, _lib.dptr(ev), _lib.dptr(eg)))
+        same_new = bool(np.array_equal(gv, ev) and np.array_equal(gg, eg))
+        log(f"rebind {B} programs: {upd * 1e3:.1f} ms, + replay {rebind * 1e3:.1f} ms; "
+            f"equal to the engine call: {same_new}")
+    finally:
+        _lib.check(ctx.lib.mossq_graph_destroy(gh))
+    return {"graph_capture_ms": cap_ms, "graph_ms_per_batch": wall * 1e3, "graph_device_ms": dev,
+            "graph_grads_per_s": B / wall, "graph_equals_engine": same, "rebind_update_ms": upd * 1e3,
+            "rebind_and_replay_ms": rebind * 1e3, "rebind_replay_equals_engine": same_new}
 
 
 def bench_density(ctx, n=13, layers=8, seed=20260213, reps=2):

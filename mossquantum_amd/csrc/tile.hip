@@ -934,6 +934,7 @@ struct FusedPlan {
     int rb = 4;            // register bits of every layout (4: 256 threads per tile, 3: 512)
     int min_tail = 0;      // planner variant the plan came from (rebuilt alike by
     bool kron = true;      // fused_plan_update)
+    std::vector<int> reg_trace;  // every register set the scheduler picked, in order (rb each)
     bool adjoint = false;  // reverse plan of the pure-state adjoint (probes, no general diagonals)
     bool reverse_steps = false;  // unfused items apply the steps' reverse matrices
     bool shared_device = false;  // device arrays live in a block owned elsewhere
@@ -1543,9 +1544,19 @@ static u64 choose_tile_bits(const std::vector<POp> &ops, const std::vector<char>
     return T;
 }
 
+// With `hint` (a plan of the same program structure), the tile bits of each
+// sweep and every register set come from the hint instead of their searches,
+// which are most of the planning time; the values (and everything built from
+// them) are planned afresh, so the tables match the hint's whenever the
+// structure does.
 static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, int min_tail, bool adjoint, bool kron,
-                                 int rb, std::string *err) {
+                                 int rb, std::string *err, const FusedPlan *hint = nullptr) {
     FusedPlan *fp = new FusedPlan();
+    std::vector<const PlanItem *> hint_sweeps;
+    if (hint)
+        for (const PlanItem &it : hint->items)
+            if (it.sweep) hint_sweeps.push_back(&it);
+    size_t hint_reg = 0;
     fp->adjoint = adjoint;
     fp->rb = rb;
     fp->min_tail = min_tail;
@@ -1569,7 +1580,13 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
             done[first] = 1;
             continue;
         }
-        u64 T = choose_tile_bits(ops, done, first, p->nbits);
+        u64 T = 0;
+        if (hint) {
+            if (size_t(fp->nsweeps) >= hint_sweeps.size()) return failp("the plan has more sweeps than its hint");
+            for (int q = 0; q < TM; ++q) T |= u64(1) << hint_sweeps[fp->nsweeps]->args.tbits[q];
+        } else {
+            T = choose_tile_bits(ops, done, first, p->nbits);
+        }
         std::vector<int> chosen;
         sweep_ops(ops, done, first, p->nbits, T, &chosen);
         for (int i : chosen) done[i] = 1;
@@ -1634,6 +1651,13 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
         // greedily with the bits of further ready ops; the candidate that
         // schedules the most work wins (ties: fewer bits changed).
         auto next_reg = [&](int *reg) {
+            if (hint) {
+                if (hint_reg + rb > hint->reg_trace.size()) return false;
+                std::copy(hint->reg_trace.begin() + hint_reg, hint->reg_trace.begin() + hint_reg + rb, reg);
+                hint_reg += rb;
+                fp->reg_trace.insert(fp->reg_trace.end(), reg, reg + rb);
+                return true;
+            }
             std::vector<int> best;
             int best_score = -1, best_keep = -1;
             int seeds = 0;
@@ -1672,6 +1696,7 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                 std::adjacent_find(best.begin(), best.end()) != best.end())
                 return false;
             for (int i = 0; i < rb; ++i) reg[i] = best[i];
+            fp->reg_trace.insert(fp->reg_trace.end(), reg, reg + rb);
             return true;
         };
         int reg0[kRegMax];
@@ -3061,7 +3086,7 @@ int fused_plan_update(Program *p, const Program *q) {
             lower_steps_rev(q, ops);
         else
             lower_steps(q, ops);
-        FusedPlan *nw = plan_with_tail(q, ops, old->min_tail, old->adjoint, old->kron, old->rb, &err);
+        FusedPlan *nw = plan_with_tail(q, ops, old->min_tail, old->adjoint, old->kron, old->rb, &err, old);
         if (!nw) return cleanup(MOSSQ_EINVAL, "rebuilding a plan failed: " + err);
         pairs.push_back({old, nw});
         if (!same_tables(old, nw)) return cleanup(MOSSQ_EINVAL, "the new values change the program's fused plan");
