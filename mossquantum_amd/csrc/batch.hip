@@ -107,6 +107,19 @@ __device__ __forceinline__ c128 observe_rows(const BItemDev &it, u64 i0, u64 i1,
                                              const c128 *mats, const c128 *seed) {
     const c128 *__restrict__ psi = it.psi;
     c128 acc = mk(0, 0);
+    if (it.seed_twice || it.value_only) {
+        // one sum: the seed is exactly 2 (M psi)_i (scaling by 2 is exact, so
+        // this equals the two-sum form bit for bit)
+        for (u64 i = i0 + threadIdx.x; i < i1; i += kBT) {
+            c128 hp = mk(0, 0), dh = mk(0, 0), u0 = mk(0, 0), u1 = mk(0, 0);
+            obs_row<false>(i, psi, it.G, groups, mats, mats, hp, dh, u0, u1);
+            const c128 x0 = psi[i];
+            hp = cfma(dh, x0, hp);
+            if (!it.value_only) it.lam[i] = mk(2 * hp.re, 2 * hp.im);
+            acc = cfmaconj(x0, hp, acc);
+        }
+        return acc;
+    }
     for (u64 i = i0 + threadIdx.x; i < i1; i += kBT) {
         c128 hp = mk(0, 0), sd = mk(0, 0), dh = mk(0, 0), ds = mk(0, 0);
         obs_row<true>(i, psi, it.G, groups, mats, seed, hp, dh, sd, ds);
@@ -128,7 +141,7 @@ __device__ __forceinline__ void observe_chunk(const BItemDev &it, int c, c128 *r
     if (nm <= kObsMatsLds) {
         for (int j = threadIdx.x; j < nm; j += kBT) {
             s_mats[j] = it.mats[j];
-            s_seed[j] = it.seed[j];
+            if (!it.seed_twice && !it.value_only) s_seed[j] = it.seed[j];
         }
         __syncthreads();
         acc = observe_rows(it, u64(c) * per, u64(c + 1) * per, const_table(it.groups), s_mats, s_seed);
@@ -408,6 +421,7 @@ static int batch_prepare(BatchRun &R, int B, const Program *const *progs, const 
         d.vslot = b;
         d.value_only = value_only ? 1 : 0;
         d.preset = (init && init[b]) ? 1 : 0;
+        d.seed_twice = obs[b]->seed_twice ? 1 : 0;
         ch[k] += d.nchunk;
         if (d.preset) {
             R.presets.push_back({d.psi, init[b]});

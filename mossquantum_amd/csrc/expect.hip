@@ -62,18 +62,21 @@ int reduce_partials(Ctx *c, const c128 *part, int nparts, c128 *dev_out) {
 // (A psi)_i = sum_g sum_x A_g[r, r ^ x] psi[i ^ flip_g(x)] (r = local index of
 // i on g's sites) loads only the flip patterns in xmask, and every group's
 // diagonal term shares the one load of psi_i.  Each workgroup writes the
-// partial sum of conj(phi_i) (M psi)_i over its rows; with WRITE it also
-// stores out_i = (W psi)_i (W = M + M^dagger: the adjoint seed).  Group
+// partial sum of conj(phi_i) (M psi)_i over its rows; with OUT = 1 it also
+// stores out_i = (W psi)_i (W = M + M^dagger: the adjoint seed), and with
+// OUT = 2 (every W entry exactly 2 M: a Hermitian observable) out_i =
+// 2 (M psi)_i from the one sum, equal to the W form bit for bit.  Group
 // records are read as uniform (scalar) loads; the matrices, gathered at each
 // row's local index, are staged in LDS when they fit.
 // (The reference sums one pass per site group, observables/expectation.jl:12-33.)
 constexpr int kObsLdsMats = 512;
 
-template <bool WRITE>
+template <int OUT>
 __global__ __launch_bounds__(kB) void k_observe(const c128 *__restrict__ phi, const c128 *__restrict__ psi, u64 len,
                                                 int G, const ObsGroup *__restrict__ groups,
                                                 const c128 *__restrict__ mats, const c128 *__restrict__ wmats,
                                                 c128 *__restrict__ out, c128 *__restrict__ part) {
+    constexpr bool WRITE = OUT == 1;
     __shared__ c128 s_m[kObsLdsMats], s_w[WRITE ? kObsLdsMats : 1];
     const ObsGroup last = groups[G - 1];
     const int nm = last.moff + (1 << (2 * last.k));
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(kB) void k_observe(const c128 *__restrict__ phi, co
     // blockIdx.y: the column of a batch (consecutive vectors of len entries)
     phi += u64(blockIdx.y) * len;
     psi += u64(blockIdx.y) * len;
-    if (WRITE) out += u64(blockIdx.y) * len;
+    if (OUT) out += u64(blockIdx.y) * len;
     c128 acc = mk(0, 0);
     for (u64 i = u64(blockIdx.x) * kB + threadIdx.x; i < len; i += u64(gridDim.x) * kB) {
         c128 hp = mk(0, 0), dh = mk(0, 0), wp = mk(0, 0), dw = mk(0, 0);
@@ -98,7 +101,8 @@ __global__ __launch_bounds__(kB) void k_observe(const c128 *__restrict__ phi, co
         const c128 x0 = psi[i];
         hp = cfma(dh, x0, hp);
         acc = cfmaconj(phi[i], hp, acc);
-        if (WRITE) out[i] = cfma(dw, x0, wp);
+        if (OUT == 1) out[i] = cfma(dw, x0, wp);
+        if (OUT == 2) out[i] = mk(2 * hp.re, 2 * hp.im);
     }
     const c128 s = block_sum(acc);
     if (threadIdx.x == 0) part[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
@@ -107,11 +111,14 @@ __global__ __launch_bounds__(kB) void k_observe(const c128 *__restrict__ phi, co
 int observe_dev(Ctx *c, const Obs *o, const c128 *phi, const c128 *psi, c128 *seed, c128 *dev_out, c128 *part) {
     const u64 len = u64(1) << o->n;
     const unsigned nb = blocks_for(len, 4, kObsParts);
-    if (seed)
-        hipLaunchKernelGGL(k_observe<true>, dim3(nb), dim3(kB), 0, c->stream, phi, psi, len, o->G, o->d_groups,
+    if (seed && o->seed_twice)
+        hipLaunchKernelGGL(k_observe<2>, dim3(nb), dim3(kB), 0, c->stream, phi, psi, len, o->G, o->d_groups,
+                           o->d_mats, o->d_mats, seed, part);
+    else if (seed)
+        hipLaunchKernelGGL(k_observe<1>, dim3(nb), dim3(kB), 0, c->stream, phi, psi, len, o->G, o->d_groups,
                            o->d_mats, o->d_mats_seed, seed, part);
     else
-        hipLaunchKernelGGL(k_observe<false>, dim3(nb), dim3(kB), 0, c->stream, phi, psi, len, o->G, o->d_groups,
+        hipLaunchKernelGGL(k_observe<0>, dim3(nb), dim3(kB), 0, c->stream, phi, psi, len, o->G, o->d_groups,
                            o->d_mats, o->d_mats, nullptr, part);
     MOSSQ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kB), 0, c->stream, part, int(nb), int(nb), dev_out);
@@ -363,6 +370,9 @@ int mossq_obs_create(mossq_ctx *ctx, int n, int ngroups, const int32_t *group_k,
         mo += D * D;
         o->h_groups.push_back(gr);
     }
+    o->seed_twice = true;
+    for (size_t j = 0; j < o->h_seed.size(); ++j)
+        if (o->h_seed[j].re != 2 * o->h_mats[j].re || o->h_seed[j].im != 2 * o->h_mats[j].im) o->seed_twice = false;
     Ctx *c = o->ctx;
     int st = cached_alloc(c, sizeof(ObsGroup) * ngroups, (void **)&o->d_groups);
     if (st == MOSSQ_OK) st = cached_alloc(c, sizeof(c128) * o->h_mats.size(), (void **)&o->d_mats);
@@ -466,7 +476,7 @@ int mossq_sv_expectation_batch(mossq_ctx *ctx, const mossq_obs *obs, const void 
     MOSSQ_TRY(get_scratch(c, (u64(B) * nb + B) * sizeof(c128), &scr));
     c128 *part = (c128 *)scr, *res = part + size_t(B) * nb;
     const c128 *v = (const c128 *)psi;
-    hipLaunchKernelGGL(k_observe<false>, dim3(nb, B), dim3(kB), 0, c->stream, v, v, len, o->G, o->d_groups,
+    hipLaunchKernelGGL(k_observe<0>, dim3(nb, B), dim3(kB), 0, c->stream, v, v, len, o->G, o->d_groups,
                        o->d_mats, o->d_mats, nullptr, part);
     MOSSQ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_rows, dim3(B), dim3(kB), 0, c->stream, part, int(nb), int(nb), res);

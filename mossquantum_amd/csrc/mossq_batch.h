@@ -37,6 +37,7 @@ struct BItemDev {
     int gslot0, vslot;
     int value_only;  // stop after the observe step (no seed, no reverse pass)
     int preset;      // psi already holds the initial state (k_binit leaves it)
+    int seed_twice;  // the observable's seed matrices are 2 M_g: seed = 2 (M psi)
 };
 
 // Host-side description of one item.

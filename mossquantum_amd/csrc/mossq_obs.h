@@ -96,6 +96,9 @@ struct Obs {
     c128 *d_mats = nullptr;
     c128 *d_mats_seed = nullptr;
     c128 *d_mats_adj = nullptr;
+    // Every seed entry is exactly twice its M_g entry (a Hermitian observable):
+    // the seed (M + M^dagger) psi is then 2 (M psi), computed from one sum.
+    bool seed_twice = false;
 };
 
 // Partial sums the one-pass observation kernels write (at most).

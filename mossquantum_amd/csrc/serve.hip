@@ -314,6 +314,8 @@ static int rebuild_table(Server *s) {
         d.gslot0 = r->slot * kMaxGrads;
         d.vslot = r->slot;
         d.value_only = 0;
+        d.preset = 0;
+        d.seed_twice = r->obs->seed_twice ? 1 : 0;
         ch += r->nchunk;
     }
     s->nchunks = ch;
