@@ -61,20 +61,31 @@ __device__ __forceinline__ void obs_row(u64 i, const c128 *__restrict__ psi, int
             dh = cadd(dh, m[D * r]);
             if (TWO) dw = cadd(dw, w[D * r]);
         }
+        // A row whose entry is zero skips its partner load (the sum is
+        // unchanged): XX + YY pairs have nonzero entries on half the rows, and
+        // for sites at or above the wave's 64 rows whole waves skip.
         if (g.nf >= 0) {
 #pragma unroll
             for (int q = 0; q < kObsFlips; ++q)
                 if (q < g.nf) {
-                    const c128 y = psi[i ^ g.fm[q]];
-                    hp = cfma(m[D * (r ^ g.fx[q])], y, hp);
-                    if (TWO) wp = cfma(w[D * (r ^ g.fx[q])], y, wp);
+                    const c128 mq = m[D * (r ^ g.fx[q])];
+                    const c128 wq = TWO ? w[D * (r ^ g.fx[q])] : mq;
+                    if (mq.re != 0.0 || mq.im != 0.0 || (TWO && (wq.re != 0.0 || wq.im != 0.0))) {
+                        const c128 y = psi[i ^ g.fm[q]];
+                        hp = cfma(mq, y, hp);
+                        if (TWO) wp = cfma(wq, y, wp);
+                    }
                 }
         } else {
             for (int x = 1; x < D; ++x)
                 if ((g.xmask >> x) & 1u) {
-                    const c128 y = psi[i ^ group_flip(g, x)];
-                    hp = cfma(m[D * (r ^ x)], y, hp);
-                    if (TWO) wp = cfma(w[D * (r ^ x)], y, wp);
+                    const c128 mq = m[D * (r ^ x)];
+                    const c128 wq = TWO ? w[D * (r ^ x)] : mq;
+                    if (mq.re != 0.0 || mq.im != 0.0 || (TWO && (wq.re != 0.0 || wq.im != 0.0))) {
+                        const c128 y = psi[i ^ group_flip(g, x)];
+                        hp = cfma(mq, y, hp);
+                        if (TWO) wp = cfma(wq, y, wp);
+                    }
                 }
         }
     }
