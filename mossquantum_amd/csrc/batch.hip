@@ -343,12 +343,20 @@ struct BatchRun {
 // run the gate-by-gate engine micro-program.
 // value_only: forward and observe steps only (no lambda vectors, no gradients);
 // init[b], when given, is item b's initial state (else |0...0>).
-static int batch_prepare(BatchRun &R, int B, const Program *const *progs, const Obs *const *obs,
+static int batch_prepare(BatchRun &R, int B, const Program *const *progs_in, const Obs *const *obs,
                          const int64_t *grad_offset, bool value_only, const void *const *init) {
     Ctx *c = R.c;
     R.B = B;
     R.value_only = value_only;
-    R.progs.assign(progs, progs + B);
+    // A program of fewer than 12 qubits with a 12-qubit form (Program::pad)
+    // runs as that form's tile sweeps, from |0...0> (an item with its own
+    // initial state keeps the gate-by-gate engine: its vector has 2^n entries).
+    R.progs.assign(progs_in, progs_in + B);
+    for (int b = 0; b < B; ++b) {
+        const Program *pd = progs_in[b]->pad;
+        if (pd && !(init && init[b]) && fused_plan_batchable(pd, !value_only)) R.progs[b] = pd;
+    }
+    const Program *const *progs = R.progs.data();
     R.grad_offset.assign(B, 0);
     if (!value_only) R.grad_offset.assign(grad_offset, grad_offset + B);
     std::vector<BItemHost> items(B);

@@ -50,6 +50,11 @@ struct Program {
     // monomial entries as kernel arguments (not rewritten by an update)
     mutable int live_step_graphs = 0;
     std::vector<FusedPlan *> retired;
+    // A fused pure-state program of fewer than 12 qubits also carries itself
+    // as a 12-qubit program (the same steps; the extra qubits stay |0>) with
+    // fused plans, so the batched engine runs it as tile sweeps with the
+    // others.  It shares d_pool with this program.
+    Program *pad = nullptr;
 };
 
 struct BatchRun;  // a prepared batch of the batched engine (batch.hip)

@@ -584,6 +584,19 @@ int mossq_program_update(mossq_ctx *ctx, mossq_program *prog, int nops, const mo
     for (size_t i = 0; i < moved.size() && p->live_step_graphs > 0; ++i)
         MOSSQ_CHECK(!(moved[i] && fused_run_launches_step(p, int(i))), MOSSQ_EINVAL,
                     "a program graph launches a step whose entries would change: destroy it first");
+    if (p->pad) {
+        // the 12-qubit form first: a program below 12 qubits has no plans of
+        // its own, so once the form takes the values nothing else can refuse
+        Program qp;
+        qp.ctx = c;
+        qp.n = qp.nbits = p->pad->n;
+        qp.nderiv = q.nderiv;
+        qp.steps = q.steps;
+        qp.h_pool = q.h_pool;
+        MOSSQ_TRY(fused_plan_update(p->pad, &qp));
+        p->pad->steps = q.steps;
+        p->pad->h_pool = q.h_pool;
+    }
     MOSSQ_TRY(fused_plan_update(p, &q));
     p->steps.swap(q.steps);
     p->h_pool.swap(q.h_pool);

@@ -2338,8 +2338,29 @@ int fused_plan_upload(Ctx *c, FusedPlan *fp) {
 }
 
 // Forward plan, and for pure states the adjoint (reverse) plan.
+// The 12-qubit form of a small pure-state program (Program::pad); none when
+// its plans are not all sweeps (the engine then runs it gate by gate).
+static void pad_build(Program *p) {
+    auto *q = new Program();
+    q->ctx = p->ctx;
+    q->n = q->nbits = TM;
+    q->nderiv = p->nderiv;
+    q->steps = p->steps;
+    q->h_pool = p->h_pool;
+    q->d_pool = p->d_pool;
+    if (fused_plan_build(q) == MOSSQ_OK && fused_plan_batchable(q, q->nderiv > 0)) {
+        p->pad = q;
+        return;
+    }
+    fused_plan_free(q);
+    delete q;
+}
+
 int fused_plan_build(Program *p) {
-    if (p->nbits < TM) return MOSSQ_OK;  // too small to tile: run unfused
+    if (p->nbits < TM) {  // too small to tile: run unfused (and padded in batches)
+        if (!p->is_density && !p->pad) pad_build(p);
+        return MOSSQ_OK;
+    }
     if (p->is_density && p->nderiv > 0) {
         int s = density_adjoint_plan_build(p);
         if (s != MOSSQ_OK) return s;
@@ -2385,6 +2406,11 @@ void fused_plan_free(Program *p) {
     free_plan(p->ctx, p->fused_rev);
     fused_plan_release_retired(p);
     density_adjoint_plan_free(p);
+    if (p->pad) {
+        fused_plan_free(p->pad);
+        delete p->pad;  // its d_pool is this program's
+        p->pad = nullptr;
+    }
 }
 
 int fused_plan_sweeps(const Program *p) {
