@@ -764,11 +764,6 @@ __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__re
         }
     }
     const int slot = int(tile) * TGeo<RB>::WPT + (t >> 6);
-    // The sweep's groups are consecutive; each iteration loads the next
-    // group's record before its own op, so the record's round trip overlaps
-    // the op's matrix load.
-    const int glast = segs[seg1 - 1].g1 - 1;
-    TGroup gnext = groups[sg.g0];
     for (int s = seg0; s < seg1; ++s) {
         sg = segs[s];
         if (sg.transpose) {
