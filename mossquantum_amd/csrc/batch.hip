@@ -153,21 +153,31 @@ __device__ __forceinline__ void observe_chunk(const BItemDev &it, int c, c128 *r
     if (threadIdx.x == 0) part[it.chunk0 + c] = s;
 }
 
+// Workgroups are dealt to the 8 XCDs round robin (workgroup b runs on XCD
+// b mod 8).  The chunk of workgroup b: XCD x takes a contiguous range of the
+// nb chunks (q or q + 1 of them), so an item's chunks, whose partner rows are
+// each other's, share one XCD's L2 instead of each XCD fetching them.
+__device__ __forceinline__ int xcd_chunk(int b, int nb) {
+    const int x = b & 7, k = b >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // The observe step alone, for a table whose items are all at it (the tiled
 // items of a batch): without the gate micro-ops' registers it runs at a
 // higher occupancy, which hides more of the partner loads.
 __global__ __launch_bounds__(kBT) void k_bobserve(const BItemDev *__restrict__ items, int nitems,
                                                   c128 *__restrict__ part) {
     __shared__ c128 red[kBT / 64];
+    const int cb = xcd_chunk(int(blockIdx.x), int(gridDim.x));
     int lo = 0, hi = nitems - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (items[mid].chunk0 <= int(blockIdx.x))
+        if (items[mid].chunk0 <= cb)
             lo = mid;
         else
             hi = mid - 1;
     }
-    observe_chunk(items[lo], int(blockIdx.x) - items[lo].chunk0, red, part);
+    observe_chunk(items[lo], cb - items[lo].chunk0, red, part);
 }
 
 // One engine iteration for every active item: a workgroup owns one chunk of
