@@ -340,6 +340,9 @@ int mossq_obs_create(mossq_ctx *ctx, int n, int ngroups, const int32_t *group_k,
                 return fail(MOSSQ_EINVAL, "observable sites must be increasing and < n");
             }
         }
+        gr.rshift = gr.k == 0 ? 0 : gr.sites[0];
+        for (int j = 1; j < gr.k; ++j)
+            if (gr.sites[j] != gr.sites[0] + j) gr.rshift = -1;
         so += gr.k;
         int D = 1 << gr.k;
         gr.moff = mo;

@@ -18,6 +18,9 @@ struct ObsGroup {
     // or nf = -1 when there are more than kObsFlips.
     int nf;
     int fx[kObsFlips];
+    // Sites s0, s0 + 1, ..., s0 + k - 1 (a chain's neighbour pairs): rshift =
+    // s0 and the local index is one shift and mask; -1 for other site sets.
+    int rshift;
     u64 fm[kObsFlips];
 };
 
@@ -36,6 +39,7 @@ __host__ __device__ __forceinline__ u64 group_flip(const ObsGroup &g, int x) {
 }
 
 __host__ __device__ __forceinline__ int group_row(const ObsGroup &g, u64 i) {
+    if (g.rshift >= 0) return int(i >> g.rshift) & ((1 << g.k) - 1);
     int r = 0;
 #pragma unroll
     for (int j = 0; j < kMaxObsSites; ++j)
@@ -51,6 +55,10 @@ __host__ __device__ __forceinline__ int group_row(const ObsGroup &g, u64 i) {
 template <bool TWO>
 __device__ __forceinline__ void obs_row(u64 i, const c128 *__restrict__ psi, int G, const ObsGroup *groups,
                                         const c128 *M, const c128 *W, c128 &hp, c128 &dh, c128 &wp, c128 &dw) {
+    // psi is a global-memory vector: typed so, its loads are global (not
+    // flat) loads, which wait on the memory counter alone
+    typedef const __attribute__((address_space(1))) c128 *gptr;
+    const gptr gpsi = (gptr)psi;
     for (int gi = 0; gi < G; ++gi) {
         const ObsGroup g = groups[gi];
         const int D = 1 << g.k;
@@ -71,7 +79,8 @@ __device__ __forceinline__ void obs_row(u64 i, const c128 *__restrict__ psi, int
                     const c128 mq = m[D * (r ^ g.fx[q])];
                     const c128 wq = TWO ? w[D * (r ^ g.fx[q])] : mq;
                     if (mq.re != 0.0 || mq.im != 0.0 || (TWO && (wq.re != 0.0 || wq.im != 0.0))) {
-                        const c128 y = psi[i ^ g.fm[q]];
+                        const u64 pi = i ^ g.fm[q];
+                        const c128 y = mk(gpsi[pi].re, gpsi[pi].im);
                         hp = cfma(mq, y, hp);
                         if (TWO) wp = cfma(wq, y, wp);
                     }
@@ -82,7 +91,8 @@ __device__ __forceinline__ void obs_row(u64 i, const c128 *__restrict__ psi, int
                     const c128 mq = m[D * (r ^ x)];
                     const c128 wq = TWO ? w[D * (r ^ x)] : mq;
                     if (mq.re != 0.0 || mq.im != 0.0 || (TWO && (wq.re != 0.0 || wq.im != 0.0))) {
-                        const c128 y = psi[i ^ group_flip(g, x)];
+                        const u64 pi = i ^ group_flip(g, x);
+                        const c128 y = mk(gpsi[pi].re, gpsi[pi].im);
                         hp = cfma(mq, y, hp);
                         if (TWO) wp = cfma(wq, y, wp);
                     }
