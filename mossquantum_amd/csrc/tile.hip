@@ -743,7 +743,7 @@ template <int RB>
 __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__restrict__ ls, const TSweepArgs &sw,
                                                const TSeg *__restrict__ segs, const TGroup *__restrict__ groups,
                                                const c128 *__restrict__ pool, const TProbe *__restrict__ probes,
-                                               c128 *__restrict__ part, int row, u64 tile) {
+                                               c128 *__restrict__ part, int row, u64 tile, bool last = false) {
     constexpr int TN = TGeo<RB>::TN;
     __shared__ c128 lds[1 << TM];
     char *ldsb = reinterpret_cast<char *>(lds);
@@ -801,7 +801,7 @@ __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__re
             }
         }
     }
-    {
+    if (!last) {  // after an item's last reverse sweep only its gradient partials are read
         const u64 g = base | seg_gthread<RB>(t, sg);
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
@@ -865,6 +865,7 @@ struct TBItem {
     u64 segs, groups, diag, pool, probes;  // plan tables (device addresses)
     int row;
     int tile0;
+    int last;  // adjoint: the item's last reverse sweep (y and lambda are not stored)
     TSweepArgs args;
 };
 
@@ -885,7 +886,7 @@ struct TBItem {
         const TBItem &it = items[e >> 16];                                                                          \
         tile_adj_sweep<RB>(vec + it.x, vec + it.l, it.args, table<TSeg>(it.segs), table<TGroup>(it.groups),         \
                            table<c128>(it.pool), table<TProbe>(it.probes), part + it.part, it.row,                  \
-                           u64(e & 0xFFFF));                                                                        \
+                           u64(e & 0xFFFF), it.last != 0);                                                          \
     }
 MOSSQ_BTILE_KERNELS(4, 256, 2)
 MOSSQ_BTILE_KERNELS(3, 512, 4)
@@ -2568,6 +2569,7 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
             it.probes = reinterpret_cast<u64>(fp->d_probes);
             it.row = adjoint ? row[b] : 0;
             it.tile0 = int(tiles);
+            it.last = (adjoint && s + 1 == fp->items.size()) ? 1 : 0;
             it.args = fp->items[s].args;
             MOSSQ_CHECK(tiles + nt <= u64(INT32_MAX), MOSSQ_EINVAL, "batched sweep launch too large");
             MOSSQ_CHECK(x[b] + len <= vec_len && (!adjoint || l[b] + len <= vec_len), MOSSQ_EINVAL,
