@@ -196,7 +196,9 @@ int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state);
  * unchanged when the records describe another program or a plan would change;
  * also when a live program graph launches a monomial step on its own (an
  * unfused step: its entries are kernel arguments) whose entries change, and
- * after an autotune replaced a plan that a live graph still uses. */
+ * after an autotune replaced a plan that a live graph still uses.  Updates of
+ * different programs of one context may run on several host threads at once
+ * (host replanning, then copies on the context's stream). */
 int mossq_program_update(mossq_ctx *ctx, mossq_program *prog, int nops, const mossq_op *ops, const double *pool,
                          uint64_t pool_len);
 /* Autotune a fused program: plan every forward-plan variant (tail thresholds

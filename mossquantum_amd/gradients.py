@@ -14,6 +14,7 @@ plays that role for torch.autograd.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -340,6 +341,47 @@ class BatchAdjointGraph:
         check(self.ctx.lib.mossq_batch_capture(self.ctx.h, B, ph, oh,
                                                self.offs.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(h)))
         self.h = h
+
+    def rebind(self, items, workers: int | None = None):
+        """New parameters for every program: items[i] is a circuit with
+        program i's structure, or a pool for its current lowering (for
+        instance `program.lowered.pool_with_slots(values)`).  Circuits are
+        lowered here; the library updates (host replanning and uploads,
+        mossq_program_update) run on `workers` host threads, as updates of
+        different programs are independent.  Raises if any program refuses;
+        the others keep their new values."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        if len(items) != len(self.programs):
+            raise ValueError("one circuit or pool per program")
+        jobs = []
+        for p, it in zip(self.programs, items):
+            if isinstance(it, QCircuit):
+                lw = lower(it, p.n, p.is_density)
+                recs, pool = lw.arrays()
+            else:
+                lw = p.lowered
+                recs, _ = lw.arrays()
+                pool = np.ascontiguousarray(it, dtype=np.complex128)
+            jobs.append((p, lw, recs, pool))
+
+        def one(job):
+            p, lw, recs, pool = job
+            st = self.ctx.lib.mossq_program_update(self.ctx.h, p.h, len(lw.records), recs,
+                                                   pool.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)),
+                                                   pool.size)
+            msg = self.ctx.lib.mossq_last_error() if st else b""  # thread-local: read on this thread
+            return st, msg
+
+        workers = workers or min(16, len(jobs), os.cpu_count() or 1)
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            results = list(ex.map(one, jobs))
+        for (p, lw, _, _), (st, _) in zip(jobs, results):
+            if st == _lib.MOSSQ_OK:
+                p.lowered = lw
+        for st, msg in results:
+            if st != _lib.MOSSQ_OK:
+                raise _lib.MossqError(st, msg.decode(errors="replace"))
 
     def run(self):
         vals = np.zeros(len(self.programs))

@@ -7,8 +7,8 @@ through the C ABI):
     src/api/parameter_binding.jl:29) over a program's device tables; a plain
     run, the pure-state and density adjoints and captured graphs then give the
     oracle's results for the new angles;
-  * mossq_batch_capture records one batched adjoint run (tiled sweeps and the
-    gate-by-gate items on the second stream); each replay, with eager work and
+  * mossq_batch_capture records one batched adjoint run (tile sweeps of every
+    item, those below 12 qubits in their 12-qubit form); each replay, with eager work and
     synchronisations in between, equals an eager mossq_batch_adjoint call, and
     after rebinding items the replay equals an eager call on the new circuits;
   * refusals leave the program as it was: records of another program, and a
@@ -42,7 +42,7 @@ def _same(a, b):
 
 
 def test_batch_graph_replays_equal_eager(ctx):
-    """Tiled items (12-16 qubits) and gate-by-gate items (10, 11) in one graph:
+    """Items of 10-16 qubits in one graph (those below 12 in their 12-qubit form):
     three replays with unrelated eager work between them each equal the eager
     engine bit for bit (fixed-order reductions), and the oracle on small items."""
     rng = np.random.default_rng(301)
@@ -92,6 +92,27 @@ def test_batch_graph_rebind(ctx):
         ref = first if i in back else second
         assert abs(third[0][i] - ref[0][i]) < EXP, i
         assert np.max(np.abs(third[1][i] - ref[1][i])) < GRAD, i
+    del graph
+
+
+def test_batch_graph_parallel_rebind(ctx):
+    """Rebinding every program on several host threads at once (circuits for
+    some items, pools from the lowering's slot rebinding for the others) gives
+    the same replay as the eager engine on the new circuits."""
+    rng = np.random.default_rng(304)
+    sizes = [12, 10, 13, 11, 14, 12, 15, 10]
+    c1 = _batch(rng, sizes)
+    angles = [_angles(rng, n, 2) for n in sizes]
+    c2 = [hea_circuit(n, 2, a) for n, a in zip(sizes, angles)]
+    obs = [xxz(n) for n in sizes]
+    graph = mq.batched_adjoint_graph(c1, obs, sizes, ctx, fuse=True)
+    graph.run()
+    items = [c if i % 2 == 0 else graph.programs[i].lowered.pool_with_slots(angles[i])
+             for i, c in enumerate(c2)]
+    graph.rebind(items, workers=6)
+    _same(graph.run(), mq.batched_adjoint(c2, obs, sizes, ctx, fuse=True))
+    with pytest.raises(Exception):
+        graph.rebind(c2[:-1])
     del graph
 
 
