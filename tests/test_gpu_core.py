@@ -559,6 +559,29 @@ def test_batched_expectation_tiled_with_initial_states(ctx):
     assert np.max(np.abs(vals - np.array(want))) < EXP
 
 
+@pytest.mark.parametrize("sizes", [[10, 11, 1, 12, 3], [5]])
+def test_batched_small_fused_programs_as_tiles(ctx, sizes):
+    """Fused programs below 12 qubits run in the batched engine as their
+    12-qubit form (the extra qubits idle in |0>): values from |0...0> (value
+    only and with gradients) match the oracle, in batches with and without
+    12-qubit items."""
+    rng = np.random.default_rng(19 + len(sizes))
+    B = len(sizes)
+    circs = [hea_circuit(n, 3, rng.uniform(0, 2 * np.pi, 6 * n)) for n in sizes]
+    obs = [tfim(n) for n in sizes]
+    progs = [mq.Program(c, n, False, ctx, fuse=True) for c, n in zip(circs, sizes)]
+    dobs = [mq.DeviceObservable(o, n, ctx) for o, n in zip(obs, sizes)]
+    ph = (C.c_void_p * B)(*[p.h.value for p in progs])
+    oh = (C.c_void_p * B)(*[o.h.value for o in dobs])
+    vals = np.zeros(B)
+    mq._lib.check(ctx.lib.mossq_batch_expectation(ctx.h, B, ph, oh, None, mq._lib.dptr(vals)))
+    v_adj, g_adj = mq.batched_adjoint(circs, obs, sizes, ctx, fuse=True)
+    for i, n in enumerate(sizes):
+        v, g = oracle.adjoint(lower(circs[i], n, False), obs[i].groups())
+        assert abs(vals[i] - v) < EXP and abs(v_adj[i] - v) < EXP, i
+        assert np.max(np.abs(g_adj[i] - g)) < GRAD, i
+
+
 def test_unsupported_before_payload(ctx):
     sv = mq.StateVector.zeros(3)
     big = mq.QubitsOperator([mq.QubitsTerm({1: "X", 2: "X", 3: "X"})])
