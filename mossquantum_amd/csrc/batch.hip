@@ -363,7 +363,7 @@ static int batch_prepare(BatchRun &R, int B, const Program *const *progs_in, con
     // initial state keeps the gate-by-gate engine: its vector has 2^n entries).
     R.progs.assign(progs_in, progs_in + B);
     for (int b = 0; b < B; ++b) {
-        const Program *pd = progs_in[b]->pad;
+        const Program *pd = fused_pad(progs_in[b]);
         if (pd && !(init && init[b]) && fused_plan_batchable(pd, !value_only)) R.progs[b] = pd;
     }
     const Program *const *progs = R.progs.data();

@@ -53,8 +53,11 @@ struct Program {
     // A fused pure-state program of fewer than 12 qubits also carries itself
     // as a 12-qubit program (the same steps; the extra qubits stay |0>) with
     // fused plans, so the batched engine runs it as tile sweeps with the
-    // others.  It shares d_pool with this program.
-    Program *pad = nullptr;
+    // others.  Built on the first batched call (fused_pad); it shares d_pool
+    // with this program.
+    bool fuse_requested = false;
+    mutable bool pad_tried = false;
+    mutable Program *pad = nullptr;
 };
 
 struct BatchRun;  // a prepared batch of the batched engine (batch.hip)
@@ -101,6 +104,9 @@ int fused_plan_update(Program *p, const Program *q);
 // the fused plan, or every step of an unfused program), passing a monomial
 // step's entries as kernel arguments.
 bool fused_run_launches_step(const Program *p, int i);
+// The 12-qubit form of a fused pure-state program below 12 qubits
+// (Program::pad), built on first use; null when there is none.
+const Program *fused_pad(const Program *p);
 // Releases the plans kept for captured graphs (call with no graph alive).
 void fused_plan_release_retired(Program *p);
 // Reverse pass of the pure-state adjoint on (y, l) with the adjoint plan;

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "mossq_program.h"
 
@@ -2352,11 +2353,21 @@ static void pad_build(Program *p) {
     delete q;
 }
 
-int fused_plan_build(Program *p) {
-    if (p->nbits < TM) {  // too small to tile: run unfused (and padded in batches)
-        if (!p->is_density && !p->pad) pad_build(p);
-        return MOSSQ_OK;
+static std::mutex g_pad_mu;  // first-use builds of Program::pad
+
+const Program *fused_pad(const Program *p) {
+    if (!p->fuse_requested || p->is_density || p->nbits >= TM) return nullptr;
+    std::lock_guard<std::mutex> lk(g_pad_mu);
+    if (!p->pad_tried) {
+        pad_build(const_cast<Program *>(p));
+        p->pad_tried = true;
     }
+    return p->pad;
+}
+
+int fused_plan_build(Program *p) {
+    p->fuse_requested = true;
+    if (p->nbits < TM) return MOSSQ_OK;  // too small to tile: run unfused (padded in batches, fused_pad)
     if (p->is_density && p->nderiv > 0) {
         int s = density_adjoint_plan_build(p);
         if (s != MOSSQ_OK) return s;
