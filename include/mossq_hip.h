@@ -306,10 +306,11 @@ int mossq_adjoint_noalloc(mossq_ctx *ctx, const mossq_program *prog, const mossq
 
 /* Batched adjoint gradients of B independent programs (uneven qubit counts,
  * own parameters, own observable), all started from |0...0>.  grads of item b
- * start at grads[grad_offset[b]].  Programs created with fuse=1 of 12 or more
- * qubits (whose fused plans are all sweeps) run their forward and reverse
- * passes as batched LDS-tile sweeps, one launch per sweep index for the whole
- * batch; the other items (gates on at most two qubits) run gate by gate.
+ * start at grads[grad_offset[b]].  Programs created with fuse=1 (whose fused
+ * plans are all sweeps; below 12 qubits a 12-qubit form with the extra qubits
+ * idle, built on first use) run their forward and reverse passes as batched
+ * LDS-tile sweeps, one launch per sweep index for the whole batch; the other
+ * items (gates on at most two qubits) run gate by gate.
  * New capability: the reference's batch reverse is per column with one
  * shared circuit (src/differentiation/batch.jl:35-66). */
 int mossq_batch_adjoint(mossq_ctx *ctx, int B, mossq_program *const *progs,
@@ -329,7 +330,7 @@ int mossq_batch_adjoint_init(mossq_ctx *ctx, int B, mossq_program *const *progs,
 /* Record one mossq_batch_adjoint call (items from |0...0>) as a HIP graph: the
  * launch tables, item tables and scratch vectors are built once and held by
  * the graph; mossq_batch_graph_run replays the whole batch (tiled sweeps,
- * observe, gate-by-gate items on the second stream) with one launch call and
+ * observe, any gate-by-gate items on the second stream) with one launch call and
  * copies the values and gradients out as mossq_batch_adjoint does.  With
  * mossq_program_update between runs, the same graph evaluates new parameters
  * (an optimizer's repeated batch gradients).  The programs and observables

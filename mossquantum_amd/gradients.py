@@ -316,7 +316,7 @@ def batched_adjoint(circuits, observables, nqubits, ctx: _lib.Context | None = N
 
 class BatchAdjointGraph:
     """One batched adjoint run of device programs (the launches of
-    mossq_batch_adjoint: tiled sweeps, observe, gate-by-gate items on the
+    mossq_batch_adjoint: tile sweeps, observe, any gate-by-gate items on the
     second stream) recorded as a HIP graph by mossq_batch_capture.  `run()`
     replays the whole batch with one launch call and returns (values[B],
     [slot gradient vector per item]).  Rebinding an item's program with
