@@ -110,7 +110,8 @@ const Program *fused_pad(const Program *p);
 // Releases the plans kept for captured graphs (call with no graph alive).
 void fused_plan_release_retired(Program *p);
 // Reverse pass of the pure-state adjoint on (y, l) with the adjoint plan;
-// gradient partials into rows of `row` entries (zeroed by the caller).
+// gradient partials into rows of `row` entries (zeroed by the caller).  y and
+// l are left undefined (the last sweep does not store them).
 int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, int row);
 // Partials per gradient row the adjoint plan writes (0 without one).
 int fused_adjoint_parts(const Program *p);

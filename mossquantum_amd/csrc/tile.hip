@@ -833,8 +833,8 @@ __device__ __forceinline__ void tile_adj_sweep(c128 *__restrict__ ys, c128 *__re
     __global__ __launch_bounds__(TTHREADS, WAVES) void k_tile_adj##RB(                                               \
         c128 *__restrict__ ys, c128 *__restrict__ ls, TSweepArgs sw, const TSeg *__restrict__ segs,                  \
         const TGroup *__restrict__ groups, const c128 *__restrict__ pool, const TProbe *__restrict__ probes,         \
-        c128 *__restrict__ part, int row) {                                                                          \
-        tile_adj_sweep<RB>(ys, ls, sw, segs, groups, pool, probes, part, row, blockIdx.x);                          \
+        c128 *__restrict__ part, int row, int last) {                                                                \
+        tile_adj_sweep<RB>(ys, ls, sw, segs, groups, pool, probes, part, row, blockIdx.x, last != 0);               \
     }
 MOSSQ_TILE_KERNELS(4, 256, 2)
 MOSSQ_TILE_KERNELS(3, 512, 4)
@@ -2495,17 +2495,20 @@ int fused_adjoint_parts(const Program *p) {
 int fused_adjoint_run(Ctx *c, const Program *p, c128 *y, c128 *l, c128 *part, int row) {
     const FusedPlan *fp = p->fused_rev;
     const unsigned ntiles = unsigned(u64(1) << (p->nbits - TM));
-    for (const PlanItem &it : fp->items) {
+    for (size_t i = 0; i < fp->items.size(); ++i) {
+        const PlanItem &it = fp->items[i];
         if (!it.sweep) {
             MOSSQ_TRY(adjoint_rev_step(c, p, it.step, y, l, part, row));
             continue;
         }
+        // after the plan's last item only the gradient partials are read
+        const int last = i + 1 == fp->items.size() ? 1 : 0;
         if (fp->rb == 3)
             hipLaunchKernelGGL(k_tile_adj3, dim3(ntiles), dim3(TGeo<3>::TT), 0, c->stream, y, l, it.args,
-                               fp->d_segs, fp->d_groups, fp->d_pool, fp->d_probes, part, row);
+                               fp->d_segs, fp->d_groups, fp->d_pool, fp->d_probes, part, row, last);
         else
             hipLaunchKernelGGL(k_tile_adj4, dim3(ntiles), dim3(TGeo<4>::TT), 0, c->stream, y, l, it.args,
-                               fp->d_segs, fp->d_groups, fp->d_pool, fp->d_probes, part, row);
+                               fp->d_segs, fp->d_groups, fp->d_pool, fp->d_probes, part, row, last);
         MOSSQ_HIP(hipGetLastError());
     }
     return MOSSQ_OK;
