@@ -110,7 +110,22 @@ __device__ __forceinline__ c128 observe_rows(const BItemDev &it, u64 i0, u64 i1,
     if (it.seed_twice || it.value_only) {
         // one sum: the seed is exactly 2 (M psi)_i (scaling by 2 is exact, so
         // this equals the two-sum form bit for bit)
-        for (u64 i = i0 + threadIdx.x; i < i1; i += kBT) {
+        u64 i = i0 + threadIdx.x;
+        for (; i + kBT < i1; i += 2 * kBT) {  // two rows at a time: their loads overlap
+            const u64 ib = i + kBT;
+            c128 hpa = mk(0, 0), dha = mk(0, 0), hpb = mk(0, 0), dhb = mk(0, 0);
+            obs_row_pair(i, ib, psi, it.G, groups, mats, hpa, dha, hpb, dhb);
+            const c128 xa = psi[i], xb = psi[ib];
+            hpa = cfma(dha, xa, hpa);
+            hpb = cfma(dhb, xb, hpb);
+            if (!it.value_only) {
+                it.lam[i] = mk(2 * hpa.re, 2 * hpa.im);
+                it.lam[ib] = mk(2 * hpb.re, 2 * hpb.im);
+            }
+            acc = cfmaconj(xa, hpa, acc);
+            acc = cfmaconj(xb, hpb, acc);
+        }
+        for (; i < i1; i += kBT) {
             c128 hp = mk(0, 0), dh = mk(0, 0), u0 = mk(0, 0), u1 = mk(0, 0);
             obs_row<false>(i, psi, it.G, groups, mats, mats, hp, dh, u0, u1);
             const c128 x0 = psi[i];

@@ -7,6 +7,9 @@ namespace mossq {
 
 constexpr int kMaxObsSites = 5;  // reference LARGEST_SUPPORTED_NTERMS, backends/cpu/operators/generic.jl:1
 constexpr int kObsFlips = 4;     // nonzero flip patterns listed per group
+// A global-memory c128 vector: its loads are global (not flat) loads, which
+// wait on the memory counter alone.
+typedef const __attribute__((address_space(1))) c128 *gptr_c128;
 
 struct ObsGroup {
     int k;                      // number of sites (0..5)
@@ -55,10 +58,7 @@ __host__ __device__ __forceinline__ int group_row(const ObsGroup &g, u64 i) {
 template <bool TWO>
 __device__ __forceinline__ void obs_row(u64 i, const c128 *__restrict__ psi, int G, const ObsGroup *groups,
                                         const c128 *M, const c128 *W, c128 &hp, c128 &dh, c128 &wp, c128 &dw) {
-    // psi is a global-memory vector: typed so, its loads are global (not
-    // flat) loads, which wait on the memory counter alone
-    typedef const __attribute__((address_space(1))) c128 *gptr;
-    const gptr gpsi = (gptr)psi;
+    const gptr_c128 gpsi = (gptr_c128)psi;
     for (int gi = 0; gi < G; ++gi) {
         const ObsGroup g = groups[gi];
         const int D = 1 << g.k;
@@ -97,6 +97,48 @@ __device__ __forceinline__ void obs_row(u64 i, const c128 *__restrict__ psi, int
                         if (TWO) wp = cfma(wq, y, wp);
                     }
                 }
+        }
+    }
+}
+
+// Rows ia and ib of the group sum (one sum, matrices M), interleaved so the
+// two rows' partner loads are in flight together.  A flip pattern is skipped
+// when no lane of the wave has a nonzero entry for it in either row (for
+// sites at or above the wave's 64 rows whole waves skip); otherwise every
+// lane loads its partners and adds entry x partner (a zero entry adds zero).
+__device__ __forceinline__ void obs_pair_term(const gptr_c128 gpsi, u64 ia, u64 ib, u64 fm, c128 qa, c128 qb,
+                                              c128 &hpa, c128 &hpb) {
+    const bool nz = qa.re != 0.0 || qa.im != 0.0 || qb.re != 0.0 || qb.im != 0.0;
+    if (__any(nz)) {
+        const u64 pa = ia ^ fm, pb = ib ^ fm;
+        const c128 ya = mk(gpsi[pa].re, gpsi[pa].im), yb = mk(gpsi[pb].re, gpsi[pb].im);
+        hpa = cfma(qa, ya, hpa);
+        hpb = cfma(qb, yb, hpb);
+    }
+}
+
+__device__ __forceinline__ void obs_row_pair(u64 ia, u64 ib, const c128 *__restrict__ psi, int G,
+                                             const ObsGroup *groups, const c128 *M, c128 &hpa, c128 &dha,
+                                             c128 &hpb, c128 &dhb) {
+    const gptr_c128 gpsi = (gptr_c128)psi;
+    for (int gi = 0; gi < G; ++gi) {
+        const ObsGroup g = groups[gi];
+        const int D = 1 << g.k;
+        const int ra = group_row(g, ia), rb = group_row(g, ib);
+        const c128 *ma = M + g.moff + ra, *mb = M + g.moff + rb;
+        if (g.xmask & 1u) {
+            dha = cadd(dha, ma[D * ra]);
+            dhb = cadd(dhb, mb[D * rb]);
+        }
+        if (g.nf >= 0) {
+#pragma unroll
+            for (int q = 0; q < kObsFlips; ++q)
+                if (q < g.nf)
+                    obs_pair_term(gpsi, ia, ib, g.fm[q], ma[D * (ra ^ g.fx[q])], mb[D * (rb ^ g.fx[q])], hpa, hpb);
+        } else {
+            for (int x = 1; x < D; ++x)
+                if ((g.xmask >> x) & 1u)
+                    obs_pair_term(gpsi, ia, ib, group_flip(g, x), ma[D * (ra ^ x)], mb[D * (rb ^ x)], hpa, hpb);
         }
     }
 }
