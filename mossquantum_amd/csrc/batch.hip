@@ -359,6 +359,13 @@ struct BatchRun {
     BatchSweeps fwd, rev;
     std::vector<std::pair<c128 *, const void *>> presets;  // (psi, init) copies of each call
     std::vector<u64> preset_len;
+    // inputs of the reverse launch tables when their build is deferred until the
+    // forward launches are enqueued (rev_deferred)
+    bool rev_deferred = false;
+    std::vector<const Program *> tprogs;
+    std::vector<u64> tx, tl, tpart;
+    std::vector<int> trow;
+    u64 vec_len = 0;
     explicit BatchRun(Ctx *cc) : c(cc), al(cc) {}
 };
 
@@ -369,7 +376,8 @@ struct BatchRun {
 // value_only: forward and observe steps only (no lambda vectors, no gradients);
 // init[b], when given, is item b's initial state (else |0...0>).
 static int batch_prepare(BatchRun &R, int B, const Program *const *progs_in, const Obs *const *obs,
-                         const int64_t *grad_offset, bool value_only, const void *const *init) {
+                         const int64_t *grad_offset, bool value_only, const void *const *init,
+                         bool defer_rev = false) {
     Ctx *c = R.c;
     R.B = B;
     R.value_only = value_only;
@@ -479,9 +487,18 @@ static int batch_prepare(BatchRun &R, int B, const Program *const *progs_in, con
         const u64 vec_len = u64(total_amps) * (value_only ? 1 : 2);
         MOSSQ_TRY(batch_sweeps_build(c, al, nt, tprogs.data(), vec, vec_len, tx.data(), nullptr, nullptr, 0, nullptr,
                                      nullptr, false, &R.fwd));
-        if (!value_only)
+        if (!value_only && defer_rev) {
+            R.rev_deferred = true;
+            R.tprogs.swap(tprogs);
+            R.tx.swap(tx);
+            R.tl.swap(tl);
+            R.tpart.swap(tpart);
+            R.trow.swap(trow);
+            R.vec_len = vec_len;
+        } else if (!value_only) {
             MOSSQ_TRY(batch_sweeps_build(c, al, nt, tprogs.data(), vec, vec_len, tx.data(), tl.data(), R.rpart,
                                          u64(R.rev_parts), tpart.data(), trow.data(), true, &R.rev));
+        }
     }
     R.nrows = int(rows.size());
     if (!rows.empty()) {
@@ -538,6 +555,14 @@ static int batch_enqueue(BatchRun &R) {
         MOSSQ_HIP(hipGetLastError());
     }
     if (R.ntab[0] && !R.value_only) {
+        if (R.rev_deferred) {
+            // built now, while the device runs the forward launches; the table
+            // copies queue behind them on the stream
+            MOSSQ_TRY(batch_sweeps_build(c, R.al, int(R.tprogs.size()), R.tprogs.data(), R.vec, R.vec_len,
+                                         R.tx.data(), R.tl.data(), R.rpart, u64(R.rev_parts), R.tpart.data(),
+                                         R.trow.data(), true, &R.rev, false));
+            R.rev_deferred = false;
+        }
         MOSSQ_TRY(zero_fill(c, R.rpart, u64(R.rev_parts)));
         MOSSQ_TRY(batch_sweeps_run(c, R.rev, true));
         if (R.np) {
@@ -575,7 +600,7 @@ static int batch_adjoint_engine(Ctx *c, int B, const Program *const *progs, cons
                                 const int64_t *grad_offset, double *values, double *grads, bool value_only,
                                 const void *const *init) {
     BatchRun R(c);
-    MOSSQ_TRY(batch_prepare(R, B, progs, obs, grad_offset, value_only, init));
+    MOSSQ_TRY(batch_prepare(R, B, progs, obs, grad_offset, value_only, init, true));
     MOSSQ_TRY(batch_enqueue(R));
     return batch_finish(R, values, grads);
 }

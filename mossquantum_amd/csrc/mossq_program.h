@@ -152,15 +152,18 @@ struct BatchSweeps {
     unsigned *d_wg = nullptr; // workgroup -> item of its launch << 16 | tile
     c128 *vec = nullptr, *part = nullptr;
     std::vector<int> item0, wg0, tiles, rb;  // per launch (rb: register bits of its plans)
+    std::vector<char> h_items;                // host tables of an upload not waited for
+    std::vector<unsigned> h_wg;
 };
 // Builds and uploads the launch tables into al's memory (synchronises c's
-// stream).  Every vector lies in the allocation vec (vec_len entries), given
+// stream; with sync false the copies are only enqueued there, and the host
+// tables stay in *out until it is destroyed).  Every vector lies in the allocation vec (vec_len entries), given
 // by its offset: forward, x[b] is program b's state; adjoint, x[b] and l[b]
 // are y and lambda, and the gradient partials go to rows of row[b] entries
 // at part + part_off[b] (part holds part_len entries).
 int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c128 *vec, u64 vec_len,
                        const u64 *x, const u64 *l, c128 *part, u64 part_len, const u64 *part_off, const int *row,
-                       bool adjoint, BatchSweeps *out);
+                       bool adjoint, BatchSweeps *out, bool sync = true);
 int batch_sweeps_run(Ctx *c, const BatchSweeps &bs, bool adjoint);
 // Zero `count` complex entries with a kernel (safe inside captured graphs).
 int zero_fill(Ctx *c, c128 *v, u64 count);

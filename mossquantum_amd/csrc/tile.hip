@@ -2544,7 +2544,7 @@ bool fused_plan_batchable(const Program *p, bool adjoint) {
 
 int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c128 *vec, u64 vec_len,
                        const u64 *x, const u64 *l, c128 *part, u64 part_len, const u64 *part_off, const int *row,
-                       bool adjoint, BatchSweeps *out) {
+                       bool adjoint, BatchSweeps *out, bool sync) {
     std::vector<TBItem> items;
     out->vec = vec;
     out->part = part;
@@ -2610,6 +2610,16 @@ int batch_sweeps_build(Ctx *c, Allocs &al, int m, const Program *const *progs, c
     if (items.empty()) return MOSSQ_OK;
     MOSSQ_TRY(al.get(items.size() * sizeof(TBItem), &out->d_items));
     MOSSQ_TRY(al.get(wg.size() * sizeof(unsigned), (void **)&out->d_wg));
+    if (!sync) {  // the copies read the host tables after this returns: keep them
+        out->h_items.assign(reinterpret_cast<const char *>(items.data()),
+                            reinterpret_cast<const char *>(items.data() + items.size()));
+        out->h_wg.swap(wg);
+        MOSSQ_HIP(hipMemcpyAsync(out->d_items, out->h_items.data(), out->h_items.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        MOSSQ_HIP(hipMemcpyAsync(out->d_wg, out->h_wg.data(), out->h_wg.size() * sizeof(unsigned),
+                                 hipMemcpyHostToDevice, c->stream));
+        return MOSSQ_OK;
+    }
     MOSSQ_HIP(hipMemcpyAsync(out->d_items, items.data(), items.size() * sizeof(TBItem), hipMemcpyHostToDevice,
                              c->stream));
     MOSSQ_HIP(hipMemcpyAsync(out->d_wg, wg.data(), wg.size() * sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
