@@ -118,8 +118,12 @@ class BatchExpectationObjective:
 def _objective_program(objective, flat, state, fuse):
     """The device program of the objective's template bound to slot values
     `flat`: the template is lowered once per (qubit count, representation)
-    and later bindings only rewrite the parameterized gates' matrices."""
+    and later bindings only rewrite the parameterized gates' matrices.  The
+    program itself is kept too and rebound in place (Program.update: its
+    plans stay) when the values change; a rebinding the library refuses (new
+    values that change a plan) builds a new program."""
     is_dm = isinstance(state, DensityMatrix) or objective.template.has_channel()
+    ctx = state.storage.ctx
     cache = objective.__dict__.setdefault("_lowered", {})
     key = (state.n, is_dm)
     lw = cache.get(key)
@@ -129,13 +133,39 @@ def _objective_program(objective, flat, state, fuse):
         pool = None
     else:
         pool = lw.pool_with_slots(flat)
-    return Program(None, state.n, is_dm, state.storage.ctx, fuse=fuse, lowered=lw, pool=pool)
+    progs = objective.__dict__.setdefault("_programs", {})
+    pkey = (state.n, is_dm, bool(fuse))
+    ent = progs.get(pkey)
+    if ent is not None and ent[0].ctx is ctx and ent[0].lowered is lw:
+        prog, bound = ent
+        if np.array_equal(bound, flat):
+            return prog
+        try:
+            prog.update(pool=pool if pool is not None else lw.arrays()[1])
+            progs[pkey] = (prog, flat.copy())
+            return prog
+        except _lib.MossqError:
+            pass
+    prog = Program(None, state.n, is_dm, ctx, fuse=fuse, lowered=lw, pool=pool)
+    progs[pkey] = (prog, flat.copy())
+    return prog
 
 
-def _adjoint_one(b: HIPBackend, prog: Program, observable, state):
-    """(value, slot gradients) of real(<obs>) for one program on one state."""
+def _objective_observable(objective, n, ctx):
+    """The objective's observable on the device, created once per (qubit
+    count, context)."""
+    obs = objective.__dict__.setdefault("_device_obs", {})
+    ent = obs.get(n)
+    if ent is None or ent.ctx is not ctx:
+        ent = obs[n] = DeviceObservable(objective.observable, n, ctx)
+    return ent
+
+
+def _adjoint_one(b: HIPBackend, prog: Program, observable, state, dob=None):
+    """(value, slot gradients) of real(<obs>) for one program on one state
+    (dob: the observable already on the device)."""
     ctx = state.storage.ctx
-    dob = DeviceObservable(observable, state.n, ctx)
+    dob = dob or DeviceObservable(observable, state.n, ctx)
     nd = prog.lowered.nderiv
     grads = np.zeros(max(nd, 1))
     value = C.c_double(0)
@@ -144,11 +174,11 @@ def _adjoint_one(b: HIPBackend, prog: Program, observable, state):
     return value.value, prog.lowered.slot_gradients(grads[:nd])
 
 
-def _adjoint_columns(prog: Program, observable, batch: BatchStateVector):
+def _adjoint_columns(prog: Program, observable, batch: BatchStateVector, dob=None):
     """(values[B], slot gradients per column) of real(<obs>) for one program
     on every column of a state-vector batch (mossq_batch_adjoint_init)."""
     ctx = batch.storage.ctx
-    dob = DeviceObservable(observable, batch.n, ctx)
+    dob = dob or DeviceObservable(observable, batch.n, ctx)
     B, nd = batch.B, prog.lowered.nderiv
     offs = np.arange(B, dtype=np.int64) * nd
     vals = np.zeros(B)
@@ -215,19 +245,22 @@ def value_and_gradient(method, objective, values, rng=None):
             # below the tiled size) go column by column below
             try:
                 vals, slot_grads = _adjoint_columns(_objective_program(objective, flat, init.columns[0], b.fuse),
-                                                    objective.observable, init)
+                                                    objective.observable, init,
+                                                    _objective_observable(objective, init.n, init.storage.ctx))
                 grad = sum(wi * pm.accumulate(g) for wi, g in zip(w, slot_grads))
                 return float(np.dot(w, vals)), grad
             except _lib.MossqUnsupported:
                 pass
         total, grad = 0.0, np.zeros(pm.count)
         for wi, col in zip(w, init.columns):
-            v, g = _adjoint_one(b, _objective_program(objective, flat, col, b.fuse), objective.observable, col)
+            v, g = _adjoint_one(b, _objective_program(objective, flat, col, b.fuse), objective.observable, col,
+                                _objective_observable(objective, col.n, col.storage.ctx))
             total += wi * v
             grad += wi * pm.accumulate(g)
         return total, grad
     st = objective.initial_state
-    v, g = _adjoint_one(b, _objective_program(objective, flat, st, b.fuse), objective.observable, st)
+    v, g = _adjoint_one(b, _objective_program(objective, flat, st, b.fuse), objective.observable, st,
+                        _objective_observable(objective, st.n, st.storage.ctx))
     return v, pm.accumulate(g)
 
 
