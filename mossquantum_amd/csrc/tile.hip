@@ -1147,13 +1147,6 @@ static int perm_kind(const c128 *M) {
 
 static bool is_perm(const POp &o) { return o.kind == 0 && o.nb == 2 && perm_kind(o.m.data()) >= 0; }
 
-static bool commutes(const POp &a, const POp &b) {
-    if (a.kind == 2 || b.kind == 2) return false;  // unfusable steps never move
-    if (a.fixed || b.fixed) return false;
-    if ((a.mask() & b.mask()) == 0) return true;
-    return a.kind == 1 && b.kind == 1;
-}
-
 static int popc(u64 x) { return __builtin_popcountll(x); }
 
 // diag(D) * M for a diagonal D on bits Dbits and dense M on Mbits (Dbits subset of Mbits).
@@ -1606,40 +1599,82 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
         }
         it.args.ncbits = nc;
         // Schedule the chosen ops into register segments.
-        // Op ci waits for every earlier chosen op it does not commute with:
-        // pending[ci] counts those not yet scheduled.
+        // Op ci waits for every earlier chosen op it does not commute with
+        // (two ops commute when their bits are disjoint or both are diagonal;
+        // unfusable steps and fixed ops never move past another op).  The
+        // waits are kept as a transitive reduction with the same closure: per
+        // bit, an op waits for the last non-diagonal op on that bit before it
+        // and, if it is not diagonal itself, for the diagonals since then
+        // (everything earlier on the bit precedes that last non-diagonal op).
+        // Unfusable and fixed ops act as non-diagonal ops on one extra bit
+        // that every other op holds as a diagonal.  pending[ci] counts the
+        // waits not yet scheduled, so an op is ready exactly when every op
+        // it does not commute with has been scheduled.
         std::vector<char> sched(chosen.size(), 0);
         std::vector<std::vector<int>> succ(chosen.size());
         std::vector<int> pending(chosen.size(), 0);
-        for (size_t ci = 0; ci < chosen.size(); ++ci)
-            for (size_t pj = 0; pj < ci; ++pj)
-                if (!commutes(ops[chosen[pj]], ops[chosen[ci]])) {
+        std::vector<unsigned char> cdiag(chosen.size());
+        std::vector<unsigned> lmask(chosen.size());  // tile-local bits; bit 31: a bit outside the tile
+        {
+            const int kAll = 64;  // the extra bit
+            int last_nd[kAll + 1];
+            std::vector<int> diags_since[kAll + 1];
+            std::vector<int> stamp(chosen.size(), -1);
+            for (int b = 0; b <= kAll; ++b) last_nd[b] = -1;
+            for (size_t ci = 0; ci < chosen.size(); ++ci) {
+                const POp &o = ops[chosen[ci]];
+                const bool any = o.kind == 2 || o.fixed;
+                cdiag[ci] = o.kind == 1;
+                unsigned lm = 0;
+                for (int q = 0; q < o.nb; ++q) lm |= local_of[o.bits[q]] >= 0 ? 1u << local_of[o.bits[q]] : 1u << 31;
+                lmask[ci] = lm;
+                auto wait_for = [&](int pj) {
+                    if (stamp[pj] == int(ci)) return;
+                    stamp[pj] = int(ci);
                     succ[pj].push_back(int(ci));
                     pending[ci]++;
-                }
+                };
+                auto on_bit = [&](int b, bool nd) {
+                    if (last_nd[b] >= 0) wait_for(last_nd[b]);
+                    if (nd) {
+                        for (int pj : diags_since[b]) wait_for(pj);
+                        diags_since[b].clear();
+                        last_nd[b] = int(ci);
+                    } else {
+                        diags_since[b].push_back(int(ci));
+                    }
+                };
+                for (int q = 0; q < o.nb; ++q) on_bit(o.bits[q], !cdiag[ci] || any);
+                on_bit(kAll, any);
+            }
+        }
         auto mark = [&](size_t ci, std::vector<char> &s, std::vector<int> &pend) {
             s[ci] = 1;
             for (int x : succ[ci]) pend[x]--;
         };
         size_t nsched = 0;
         auto ready = [&](size_t ci) { return pending[ci] == 0; };
-        auto fits = [&](size_t ci, const int *reg) {
-            const POp &o = ops[chosen[ci]];
-            if (o.kind == 1) return true;
-            for (int q = 0; q < o.nb; ++q)
-                if (std::find(reg, reg + rb, local_of[o.bits[q]]) == reg + rb) return false;
-            return true;
+        auto reg_bits = [&](const int *reg) {
+            unsigned r = 0;
+            for (int i = 0; i < rb; ++i) r |= reg[i] >= 0 ? 1u << reg[i] : 1u << 31;
+            return r;
         };
+        // every bit of a non-diagonal op is one of the register bits
+        auto fits = [&](size_t ci, unsigned rmask) { return cdiag[ci] || (lmask[ci] & ~rmask) == 0; };
         // Ops a segment with register set `reg` would schedule from the current state.
+        // the chosen ops not yet scheduled, in order (compacted after each segment)
+        std::vector<int> todo(chosen.size());
+        for (size_t ci = 0; ci < chosen.size(); ++ci) todo[ci] = int(ci);
         auto simulate = [&](const int *reg) {
             std::vector<char> s2(sched);
             std::vector<int> p2(pending);
+            const unsigned rm = reg_bits(reg);
             int count = 0;
             bool progress = true;
             while (progress) {
                 progress = false;
-                for (size_t ci = 0; ci < chosen.size(); ++ci) {
-                    if (s2[ci] || p2[ci] || !fits(ci, reg)) continue;
+                for (int ci : todo) {
+                    if (s2[ci] || p2[ci] || !fits(ci, rm)) continue;
                     mark(ci, s2, p2);
                     count += ops[chosen[ci]].kind == 1 ? 1 : 2;
                     progress = true;
@@ -1661,7 +1696,8 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
             std::vector<int> best;
             int best_score = -1, best_keep = -1;
             int seeds = 0;
-            for (size_t c0 = 0; c0 < chosen.size() && seeds < 16; ++c0) {
+            for (int c0 : todo) {
+                if (seeds >= 16) break;
                 if (sched[c0] || (ops[chosen[c0]].kind != 0 && ops[chosen[c0]].kind != 3) || !ready(c0)) continue;
                 ++seeds;
                 std::vector<int> want;
@@ -1674,8 +1710,10 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
                     if (want.size() + add.size() <= size_t(rb)) want.insert(want.end(), add.begin(), add.end());
                 };
                 try_add(ops[chosen[c0]]);
-                for (size_t ci = 0; ci < chosen.size() && want.size() < size_t(rb); ++ci)
+                for (int ci : todo) {
+                    if (want.size() >= size_t(rb)) break;
                     if (!sched[ci] && ci != c0 && ops[chosen[ci]].kind != 1) try_add(ops[chosen[ci]]);
+                }
                 for (int i = 0; i < rb && want.size() < size_t(rb); ++i)  // keep current bits
                     if (std::find(want.begin(), want.end(), reg[i]) == want.end()) want.push_back(reg[i]);
                 std::sort(want.begin(), want.end());
@@ -1713,19 +1751,21 @@ static FusedPlan *plan_with_tail(const Program *p, const std::vector<POp> &ops, 
             SegPlan sp;
             memcpy(sp.reg, reg0, sizeof(reg0));
             sp.work = 0;
+            const unsigned rm0 = reg_bits(reg0);
             bool progress = true;
             while (progress) {
                 progress = false;
-                for (size_t ci = 0; ci < chosen.size(); ++ci) {
-                    if (sched[ci] || !fits(ci, reg0) || !ready(ci)) continue;
+                for (int ci : todo) {
+                    if (sched[ci] || !fits(ci, rm0) || !ready(ci)) continue;
                     mark(ci, sched, pending);
                     ++nsched;
-                    sp.idx.push_back(int(ci));
+                    sp.idx.push_back(ci);
                     sp.work += ops[chosen[ci]].kind == 1 ? 1 : 2;
                     progress = true;
                 }
             }
             if (sp.idx.empty()) return failp("fusion planner made no progress");
+            todo.erase(std::remove_if(todo.begin(), todo.end(), [&](int ci) { return sched[ci] != 0; }), todo.end());
             plan.push_back(sp);
             if (nsched >= chosen.size()) break;
             if (!next_reg(reg0)) return failp("fusion planner produced an invalid register set");
