@@ -276,20 +276,15 @@ def bench_batch_graph(ctx, progs, ph, oh, offs, tot, vals, grads, sizes, layers,
             lw = lower(hea_circuit(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers)), n, False)
             recs, pool = lw.arrays()
             new.append((p, lw, recs, pool))
-        from concurrent.futures import ThreadPoolExecutor
-
-        def update(job):  # updates of different programs run on host threads at once
-            p, lw, recs, pool = job
-            return ctx.lib.mossq_program_update(ctx.h, p.h, len(lw.records), recs,
-                                                pool.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)),
-                                                pool.size)
-
+        # one library call: replanning on host threads, one upload of all values
+        ph = (C.c_void_p * B)(*[p.h for p, _, _, _ in new])
+        nops = (C.c_int * B)(*[len(lw.records) for _, lw, _, _ in new])
+        rp = (C.c_void_p * B)(*[C.addressof(r) for _, _, r, _ in new])
+        pp = (C.c_void_p * B)(*[pl.ctypes.data for _, _, _, pl in new])
+        lens = (C.c_uint64 * B)(*[pl.size for _, _, _, pl in new])
         w0 = time.perf_counter()
-        with ThreadPoolExecutor(max_workers=REBIND_THREADS) as ex:
-            sts = list(ex.map(update, new))
+        _lib.check(ctx.lib.mossq_program_update_many(ctx.h, B, ph, nops, rp, pp, lens, REBIND_THREADS, None))
         upd = time.perf_counter() - w0
-        for st in sts:
-            _lib.check(st)
         replay()
         rebind = time.perf_counter() - w0
         for p, lw, _, _ in new:
@@ -299,11 +294,15 @@ def bench_batch_graph(ctx, progs, ph, oh, offs, tot, vals, grads, sizes, layers,
         same_new = bool(np.array_equal(gv, ev) and np.array_equal(gg, eg))
         log(f"rebind {B} programs: {upd * 1e3:.1f} ms, + replay {rebind * 1e3:.1f} ms; "
             f"equal to the engine call: {same_new}")
+        w0 = time.perf_counter()  # the same rebinding on one host thread
+        _lib.check(ctx.lib.mossq_program_update_many(ctx.h, B, ph, nops, rp, pp, lens, 1, None))
+        upd1 = time.perf_counter() - w0
+        log(f"rebind on one host thread: {upd1 * 1e3:.1f} ms")
     finally:
         _lib.check(ctx.lib.mossq_graph_destroy(gh))
     return {"graph_capture_ms": cap_ms, "graph_ms_per_batch": wall * 1e3, "graph_device_ms": dev,
             "graph_grads_per_s": B / wall, "graph_equals_engine": same, "rebind_update_ms": upd * 1e3,
-            "rebind_threads": REBIND_THREADS,
+            "rebind_threads": REBIND_THREADS, "rebind_update_1thread_ms": upd1 * 1e3,
             "rebind_and_replay_ms": rebind * 1e3, "rebind_replay_equals_engine": same_new}
 
 

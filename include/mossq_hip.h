@@ -201,6 +201,22 @@ int mossq_program_run(mossq_ctx *ctx, const mossq_program *prog, void *state);
  * (host replanning, then copies on the context's stream). */
 int mossq_program_update(mossq_ctx *ctx, mossq_program *prog, int nops, const mossq_op *ops, const double *pool,
                          uint64_t pool_len);
+/* mossq_program_update of m distinct programs of one context at once (an
+ * optimizer rebinding a batch before the next mossq_batch_graph_run): program
+ * i takes nops[i] records ops[i] and pool pools[i] of pool_lens[i] complex
+ * entries.
+ * The host replanning runs on `threads` host threads (0: one per hardware
+ * thread, at most 64); the new values of all programs then go to the device
+ * in one host-to-device copy and one scatter launch on the context's stream,
+ * and the call returns after them.  A refused program keeps its old values,
+ * the others take their new ones.  status (NULL, or m entries) receives each
+ * program's status; the call returns the first refused program's (its index
+ * in the message).  Calls on one context must not overlap (they share its
+ * pinned staging).  No counterpart in the reference (its parameters are
+ * rebound per circuit). */
+int mossq_program_update_many(mossq_ctx *ctx, int m, mossq_program *const *progs, const int *nops,
+                              const mossq_op *const *ops, const double *const *pools, const uint64_t *pool_lens,
+                              int threads, int *status);
 /* Autotune a fused program: plan every forward-plan variant (tail thresholds
  * 0/3/5/8, with and without pairing one-qubit gates into 4x4 ops, with 4 or 3
  * register bits per layout: 256- or 512-thread tiles), time each

@@ -99,6 +99,7 @@ SIGNATURES = [
     ("mossq_plan_emulate_density_adjoint", _I,
      [_I, _I, C.POINTER(OpRecord), _DP, _U64, _I, _DP, _DP, _DP, C.POINTER(C.c_int)]),
     ("mossq_program_update", _I, [_P, _P, _I, C.POINTER(OpRecord), _DP, _U64]),
+    ("mossq_program_update_many", _I, [_P, _I, _P, _P, _P, _P, _P, _I, C.POINTER(C.c_int)]),
     ("mossq_program_autotune", _I, [_P, _P, _I, C.POINTER(C.c_int), _DP]),
     ("mossq_program_capture", _I, [_P, _P, _P, C.POINTER(_P)]),
     ("mossq_graph_launch", _I, [_P, _P]),

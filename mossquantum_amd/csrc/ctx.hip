@@ -244,6 +244,19 @@ int get_pinned(Ctx *c, u64 bytes, double **p) {
     return MOSSQ_OK;
 }
 
+int get_update_pinned(Ctx *c, u64 bytes, unsigned char **p) {
+    if (c->upd_pinned_bytes < bytes) {
+        if (c->upd_pinned) MOSSQ_HIP(hipHostFree(c->upd_pinned));
+        c->upd_pinned = nullptr;
+        c->upd_pinned_bytes = 0;
+        const u64 want = std::max<u64>(bytes + bytes / 4, 1 << 20);
+        MOSSQ_HIP(hipHostMalloc((void **)&c->upd_pinned, want, hipHostMallocDefault));
+        c->upd_pinned_bytes = want;
+    }
+    *p = c->upd_pinned;
+    return MOSSQ_OK;
+}
+
 }  // namespace mossq
 
 using namespace mossq;
@@ -327,6 +340,7 @@ int mossq_ctx_destroy(mossq_ctx *ctx) {
     c->allocs.clear();
     group_release(c, freed);
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+    if (c->upd_pinned) (void)hipHostFree(c->upd_pinned);
     for (int i = 0; i < kNumEvents; ++i)
         if (c->events[i]) (void)hipEventDestroy(c->events[i]);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);

@@ -162,6 +162,8 @@ struct Ctx {
     u64 mat_stage_bytes = 0;
     double *host_pinned = nullptr;  // pinned host staging for results
     u64 host_pinned_bytes = 0;
+    unsigned char *upd_pinned = nullptr;  // pinned host staging of batched parameter updates
+    u64 upd_pinned_bytes = 0;
     // second in-order stream (default priority) for work that overlaps the
     // main stream's, with two ordering events; created on first use
     hipStream_t aux = nullptr;
@@ -205,6 +207,9 @@ int get_scratch(Ctx *c, u64 bytes, void **p);
 // Upload `count` complex numbers to the context's matrix staging area.
 int stage_matrix(Ctx *c, const c128 *host, u64 count, const c128 **dev);
 int get_pinned(Ctx *c, u64 bytes, double **p);
+// Pinned host staging of at least `bytes` for batched parameter updates
+// (its previous contents consumed: every such update waits for its copy).
+int get_update_pinned(Ctx *c, u64 bytes, unsigned char **p);
 
 // A device table that a kernel only reads, as a constant-memory pointer: a
 // pointer read from memory is otherwise generic and loads through the flat

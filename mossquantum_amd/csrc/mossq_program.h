@@ -95,11 +95,29 @@ int fused_run(Ctx *c, const Program *p, c128 *state);
 // Time every forward-plan variant on a scratch state (reps runs each) and keep
 // the fastest as p->fused; tried = distinct variants, best_ms = its run time.
 int fused_plan_autotune(Program *p, int reps, int *tried, double *best_ms);
-// New matrix values (q: the program rebuilt from the new records): every plan
-// of p is rebuilt with the variant it came from and must keep its tables; the
-// values are then written over p's device arrays in place (MOSSQ_EINVAL, p
-// unchanged, when a plan's structure would change).
-int fused_plan_update(Program *p, const Program *q);
+// New matrix values (q: the program rebuilt from the new records), in three
+// steps.  prepare (host work only): every plan of p rebuilt with the variant
+// it came from, which must keep its tables (MOSSQ_EINVAL, p unchanged,
+// otherwise).  copies: the new values as host-to-device copies onto the old
+// plans' device arrays.  commit: the new values into the old plans' host
+// tables (the copies' sources move with them and stay valid).
+struct PlanUpdate {
+    std::vector<std::pair<FusedPlan *, FusedPlan *>> pairs;  // (old, rebuilt)
+    DensityAdjointPlan *ndp = nullptr;                       // rebuilt density adjoint plans
+    bool dm_paired = false;                                  // ndp's plans are in pairs
+    PlanUpdate() = default;
+    PlanUpdate(const PlanUpdate &) = delete;
+    PlanUpdate &operator=(const PlanUpdate &) = delete;
+    ~PlanUpdate();  // frees the rebuilt plans
+};
+struct DevCopy {
+    void *dst;
+    const void *src;
+    u64 bytes;
+};
+int fused_plan_update_prepare(const Program *p, const Program *q, PlanUpdate *u);
+void fused_plan_update_copies(const PlanUpdate &u, std::vector<DevCopy> &out);
+void fused_plan_update_commit(PlanUpdate *u);
 // Whether the forward run of p launches step i on its own (an unfused item of
 // the fused plan, or every step of an unfused program), passing a monomial
 // step's entries as kernel arguments.

@@ -9,9 +9,14 @@
 //   density matrix channel: S, S^dagger                         on bits (pos, pos+n)
 // so every launch is one sweep of a dense or monomial block kernel.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_set>
 
 #include "mossq_program.h"
 
@@ -500,6 +505,120 @@ int program_build_host(Program *p, int n, int is_density, int nops, const mossq_
     return MOSSQ_OK;
 }
 
+// One parameter update, prepared on the host (program_update_prepare), then
+// committed: the new steps and pool and every plan's new values on the host,
+// with the device copies listed in `copies`.
+struct ProgramUpdate {
+    Program q;       // the program rebuilt from the new records
+    PlanUpdate pad;  // the 12-qubit form's plans
+    PlanUpdate main;
+    std::vector<DevCopy> copies;
+};
+
+// A table segment of a batched update: `bytes` at `src` in the staging block
+// go to device address `dst`.
+struct ScatterSeg {
+    u64 src, bytes, dst, unused;
+};
+
+// Copies every segment from the staging block to its table (32-bit words;
+// each workgroup takes whole segments).
+__global__ void __launch_bounds__(256) k_scatter(const unsigned char *stage, const ScatterSeg *segs, u64 nseg) {
+    for (u64 s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const ScatterSeg g = segs[s];
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(stage + g.src);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(g.dst);
+        const u64 words = g.bytes / 4;
+        for (u64 k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
+    }
+}
+
+// f(i) for i in [0, m) on nt threads (this one included).
+template <class F>
+static void run_on_threads(int nt, int m, F f) {
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i = next++; i < m; i = next++) f(i);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+}
+
+static bool same_steps(const Program &a, const Program &b, std::vector<char> &moved) {
+    moved.assign(a.steps.size(), 0);
+    if (a.steps.size() != b.steps.size() || a.h_pool.size() != b.h_pool.size() || a.nderiv != b.nderiv) return false;
+    bool m = false;
+    auto same_info = [&](const MatInfo &x, const MatInfo &y) {
+        if (x.monomial != y.monomial || x.nchanged != y.nchanged) return false;
+        for (int j = 0; x.monomial && j < x.nchanged; ++j) {
+            if (x.rows[j] != y.rows[j] || x.src[j] != y.src[j]) return false;
+            if (x.val[j].re != y.val[j].re || x.val[j].im != y.val[j].im) m = true;
+        }
+        return true;
+    };
+    for (size_t i = 0; i < a.steps.size(); ++i) {
+        const Step &x = a.steps[i], &y = b.steps[i];
+        m = false;
+        if (x.channel != y.channel || x.k != y.k || x.kb != y.kb || memcmp(x.bits, y.bits, sizeof(x.bits)) != 0 ||
+            x.fwd != y.fwd || x.rev != y.rev || x.grad != y.grad || x.grad_index != y.grad_index ||
+            x.u_src != y.u_src || x.split != y.split || x.fwd_cols != y.fwd_cols || x.rev_cols != y.rev_cols)
+            return false;
+        if (!same_info(x.ifwd, y.ifwd) || !same_info(x.irev, y.irev) || !same_info(x.ifwd_cols, y.ifwd_cols) ||
+            !same_info(x.irev_cols, y.irev_cols))
+            return false;
+        moved[i] = m ? 1 : 0;
+    }
+    return true;
+}
+
+// Host half of a parameter update of p (no device calls; updates of
+// different programs may be prepared on different threads): the program
+// rebuilt from the new records, its plans and those of its 12-qubit form
+// rebuilt and checked, and the device copies listed.  Refusals leave p as it
+// was.
+static int program_update_prepare(Program *p, int nops, const mossq_op *ops, const double *pool, uint64_t pool_len,
+                                  ProgramUpdate *u) {
+    Program &q = u->q;
+    MOSSQ_TRY(program_build_host(&q, p->n, p->is_density, nops, ops, pool, pool_len));
+    std::vector<char> moved;
+    MOSSQ_CHECK(same_steps(*p, q, moved), MOSSQ_EINVAL, "the records describe another program");
+    // A program graph recorded the entries of the monomial steps its run
+    // launches on their own (kernel arguments); those must not move under it.
+    for (size_t i = 0; i < moved.size() && p->live_step_graphs > 0; ++i)
+        MOSSQ_CHECK(!(moved[i] && fused_run_launches_step(p, int(i))), MOSSQ_EINVAL,
+                    "a program graph launches a step whose entries would change: destroy it first");
+    if (p->pad) {
+        Program qp;
+        qp.ctx = p->ctx;
+        qp.n = qp.nbits = p->pad->n;
+        qp.nderiv = q.nderiv;
+        qp.steps = q.steps;
+        qp.h_pool = q.h_pool;
+        MOSSQ_TRY(fused_plan_update_prepare(p->pad, &qp, &u->pad));
+        fused_plan_update_copies(u->pad, u->copies);
+    }
+    MOSSQ_TRY(fused_plan_update_prepare(p, &q, &u->main));
+    fused_plan_update_copies(u->main, u->copies);
+    // (the 12-qubit form shares d_pool)
+    u->copies.push_back({p->d_pool, q.h_pool.data(), q.h_pool.size() * sizeof(c128)});
+    return MOSSQ_OK;
+}
+
+// The new values into p's host tables (and its 12-qubit form's); the listed
+// copies write them to the device.
+static void program_update_commit(Program *p, ProgramUpdate *u) {
+    if (p->pad) {
+        fused_plan_update_commit(&u->pad);
+        p->pad->steps = u->q.steps;
+        p->pad->h_pool = u->q.h_pool;
+    }
+    fused_plan_update_commit(&u->main);
+    p->steps.swap(u->q.steps);
+    p->h_pool.swap(u->q.h_pool);
+}
+
 }  // namespace mossq
 
 using namespace mossq;
@@ -541,33 +660,6 @@ int mossq_program_create(mossq_ctx *ctx, int n, int is_density, int nops, const 
 // Same steps up to matrix values: kinds, bits, pool offsets and, for
 // monomial matrices, the pattern of changed rows.  moved[i] reports whether
 // monomial step i's entries differ.
-static bool same_steps(const Program &a, const Program &b, std::vector<char> &moved) {
-    moved.assign(a.steps.size(), 0);
-    if (a.steps.size() != b.steps.size() || a.h_pool.size() != b.h_pool.size() || a.nderiv != b.nderiv) return false;
-    bool m = false;
-    auto same_info = [&](const MatInfo &x, const MatInfo &y) {
-        if (x.monomial != y.monomial || x.nchanged != y.nchanged) return false;
-        for (int j = 0; x.monomial && j < x.nchanged; ++j) {
-            if (x.rows[j] != y.rows[j] || x.src[j] != y.src[j]) return false;
-            if (x.val[j].re != y.val[j].re || x.val[j].im != y.val[j].im) m = true;
-        }
-        return true;
-    };
-    for (size_t i = 0; i < a.steps.size(); ++i) {
-        const Step &x = a.steps[i], &y = b.steps[i];
-        m = false;
-        if (x.channel != y.channel || x.k != y.k || x.kb != y.kb || memcmp(x.bits, y.bits, sizeof(x.bits)) != 0 ||
-            x.fwd != y.fwd || x.rev != y.rev || x.grad != y.grad || x.grad_index != y.grad_index ||
-            x.u_src != y.u_src || x.split != y.split || x.fwd_cols != y.fwd_cols || x.rev_cols != y.rev_cols)
-            return false;
-        if (!same_info(x.ifwd, y.ifwd) || !same_info(x.irev, y.irev) || !same_info(x.ifwd_cols, y.ifwd_cols) ||
-            !same_info(x.irev_cols, y.irev_cols))
-            return false;
-        moved[i] = m ? 1 : 0;
-    }
-    return true;
-}
-
 int mossq_program_update(mossq_ctx *ctx, mossq_program *prog, int nops, const mossq_op *ops, const double *pool,
                          uint64_t pool_len) {
     MOSSQ_TRY(ctx_check(ctx));
@@ -575,35 +667,99 @@ int mossq_program_update(mossq_ctx *ctx, mossq_program *prog, int nops, const mo
     Program *p = reinterpret_cast<Program *>(prog);
     Ctx *c = C(ctx);
     MOSSQ_CHECK(p->ctx == c, MOSSQ_EINVAL, "program belongs to another context");
-    Program q;
-    MOSSQ_TRY(program_build_host(&q, p->n, p->is_density, nops, ops, pool, pool_len));
-    std::vector<char> moved;
-    MOSSQ_CHECK(same_steps(*p, q, moved), MOSSQ_EINVAL, "the records describe another program");
-    // A program graph recorded the entries of the monomial steps its run
-    // launches on their own (kernel arguments); those must not move under it.
-    for (size_t i = 0; i < moved.size() && p->live_step_graphs > 0; ++i)
-        MOSSQ_CHECK(!(moved[i] && fused_run_launches_step(p, int(i))), MOSSQ_EINVAL,
-                    "a program graph launches a step whose entries would change: destroy it first");
-    if (p->pad) {
-        // the 12-qubit form first: a program below 12 qubits has no plans of
-        // its own, so once the form takes the values nothing else can refuse
-        Program qp;
-        qp.ctx = c;
-        qp.n = qp.nbits = p->pad->n;
-        qp.nderiv = q.nderiv;
-        qp.steps = q.steps;
-        qp.h_pool = q.h_pool;
-        MOSSQ_TRY(fused_plan_update(p->pad, &qp));
-        p->pad->steps = q.steps;
-        p->pad->h_pool = q.h_pool;
-    }
-    MOSSQ_TRY(fused_plan_update(p, &q));
-    p->steps.swap(q.steps);
-    p->h_pool.swap(q.h_pool);
-    if (!p->h_pool.empty())
-        MOSSQ_HIP(hipMemcpyAsync(p->d_pool, p->h_pool.data(), p->h_pool.size() * sizeof(c128),
-                                 hipMemcpyHostToDevice, c->stream));
+    ProgramUpdate u;
+    MOSSQ_TRY(program_update_prepare(p, nops, ops, pool, pool_len, &u));
+    program_update_commit(p, &u);
+    for (const DevCopy &cp : u.copies)
+        if (cp.bytes) MOSSQ_HIP(hipMemcpyAsync(cp.dst, cp.src, cp.bytes, hipMemcpyHostToDevice, c->stream));
     MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    return MOSSQ_OK;
+}
+
+int mossq_program_update_many(mossq_ctx *ctx, int m, mossq_program *const *progs, const int *nops,
+                              const mossq_op *const *ops, const double *const *pools, const uint64_t *pool_lens,
+                              int threads, int *status) {
+    MOSSQ_TRY(ctx_check(ctx));
+    MOSSQ_CHECK(m >= 0 && (m == 0 || (progs && nops && ops && pools && pool_lens)), MOSSQ_EINVAL, "bad arguments");
+    for (int i = 0; status && i < m; ++i) status[i] = MOSSQ_EINVAL;
+    Ctx *c = C(ctx);
+    {
+        std::unordered_set<const void *> seen;
+        for (int i = 0; i < m; ++i) {
+            MOSSQ_CHECK(progs[i], MOSSQ_EINVAL, "null program");
+            MOSSQ_CHECK(reinterpret_cast<Program *>(progs[i])->ctx == c, MOSSQ_EINVAL,
+                        "program belongs to another context");
+            MOSSQ_CHECK(seen.insert(progs[i]).second, MOSSQ_EINVAL, "a program appears twice");
+        }
+    }
+    if (m == 0) return MOSSQ_OK;
+    std::vector<std::unique_ptr<ProgramUpdate>> ups(m);
+    std::vector<int> st(m, MOSSQ_OK);
+    std::vector<std::string> msg(m);
+    const int nt = std::max(1, std::min({threads > 0 ? threads : int(std::thread::hardware_concurrency()), 64, m}));
+    // host phase on nt threads: prepare every update
+    run_on_threads(nt, m, [&](int i) {
+        ups[i].reset(new ProgramUpdate());
+        st[i] = program_update_prepare(reinterpret_cast<Program *>(progs[i]), nops[i], ops[i], pools[i],
+                                       pool_lens[i], ups[i].get());
+        if (st[i] != MOSSQ_OK) {
+            msg[i] = mossq_last_error();
+            ups[i].reset();
+        }
+    });
+    // one staging layout: every copy's bytes at 16-byte aligned offsets,
+    // then the segment table
+    std::vector<u64> first(m + 1, 0);  // first segment of each update
+    u64 data = 0;
+    for (int i = 0; i < m; ++i) {
+        first[i + 1] = first[i];
+        if (!ups[i]) continue;
+        for (const DevCopy &cp : ups[i]->copies) {
+            if (!cp.bytes) continue;
+            MOSSQ_CHECK(cp.bytes % 4 == 0, MOSSQ_EINVAL, "table sizes are whole 32-bit words");
+            data += (cp.bytes + 15) & ~u64(15);
+            ++first[i + 1];
+        }
+    }
+    const u64 nseg = first[m];
+    const u64 total = data + nseg * sizeof(ScatterSeg);
+    if (nseg) {
+        unsigned char *host = nullptr, *dev = nullptr;
+        MOSSQ_TRY(get_update_pinned(c, total, &host));
+        Allocs al(c);
+        MOSSQ_TRY(al.get(total, (void **)&dev));
+        ScatterSeg *segs = reinterpret_cast<ScatterSeg *>(host + data);
+        // offsets of each update's first byte
+        std::vector<u64> at(m + 1, 0);
+        for (int i = 0; i < m; ++i) {
+            at[i + 1] = at[i];
+            if (ups[i])
+                for (const DevCopy &cp : ups[i]->copies) at[i + 1] += cp.bytes ? (cp.bytes + 15) & ~u64(15) : 0;
+        }
+        // host phase on nt threads: stage the values, fill the segments, commit
+        run_on_threads(nt, m, [&](int i) {
+            if (!ups[i]) return;
+            u64 o = at[i], sg = first[i];
+            for (const DevCopy &cp : ups[i]->copies) {
+                if (!cp.bytes) continue;
+                memcpy(host + o, cp.src, cp.bytes);
+                segs[sg++] = ScatterSeg{o, cp.bytes, reinterpret_cast<u64>(cp.dst), 0};
+                o += (cp.bytes + 15) & ~u64(15);
+            }
+            program_update_commit(reinterpret_cast<Program *>(progs[i]), ups[i].get());
+        });
+        MOSSQ_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, c->stream));
+        const unsigned grid = unsigned(std::min<u64>(nseg, 8192));
+        hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, c->stream, dev,
+                           reinterpret_cast<const ScatterSeg *>(dev + data), nseg);
+        MOSSQ_HIP(hipGetLastError());
+        // the staging block returns to the cache behind the scatter; the wait
+        // also keeps the pinned staging from being rewritten under the copy
+        MOSSQ_HIP(hipStreamSynchronize(c->stream));
+    }
+    for (int i = 0; status && i < m; ++i) status[i] = st[i];
+    for (int i = 0; i < m; ++i)
+        if (st[i] != MOSSQ_OK) return fail(st[i], "program " + std::to_string(i) + ": " + msg[i]);
     return MOSSQ_OK;
 }
 

@@ -3112,7 +3112,9 @@ static void emulate_density_adjoint(const Program *p, const DensityAdjointPlan *
 // rebuilt from q (the program rebuilt from the new records) with the variant
 // it came from, and must come out with the same tables but for matrix values,
 // diagonal values and probe factors, which are then written over the old
-// plan's device arrays in place (captured graphs keep reading them).
+// plan's device arrays in place (captured graphs keep reading them).  The
+// rebuild is host work only, so updates of different programs can be
+// prepared on several threads and their copies issued together.
 static bool same_tables(const FusedPlan *a, const FusedPlan *b) {
     if (a->rb != b->rb || a->adjoint != b->adjoint || a->items.size() != b->items.size() ||
         a->h_segs.size() != b->h_segs.size() || a->h_groups.size() != b->h_groups.size() ||
@@ -3139,19 +3141,6 @@ static bool same_tables(const FusedPlan *a, const FusedPlan *b) {
     return true;
 }
 
-static int upload_values(Ctx *c, FusedPlan *old, FusedPlan *nw) {
-    MOSSQ_HIP(hipMemcpyAsync(old->d_pool, nw->h_pool.data(), nw->h_pool.size() * sizeof(c128), hipMemcpyHostToDevice,
-                             c->stream));
-    MOSSQ_HIP(hipMemcpyAsync(old->d_diag, nw->h_diag.data(), nw->h_diag.size() * sizeof(TDiag), hipMemcpyHostToDevice,
-                             c->stream));
-    MOSSQ_HIP(hipMemcpyAsync(old->d_probes, nw->h_probes.data(), nw->h_probes.size() * sizeof(TProbe),
-                             hipMemcpyHostToDevice, c->stream));
-    old->h_pool.swap(nw->h_pool);
-    old->h_diag.swap(nw->h_diag);
-    old->h_probes.swap(nw->h_probes);
-    return MOSSQ_OK;
-}
-
 bool fused_run_launches_step(const Program *p, int i) {
     if (!p->fused) return true;
     for (const PlanItem &it : p->fused->items)
@@ -3159,16 +3148,19 @@ bool fused_run_launches_step(const Program *p, int i) {
     return false;
 }
 
-int fused_plan_update(Program *p, const Program *q) {
+PlanUpdate::~PlanUpdate() {
+    for (auto &pr : pairs) delete pr.second;
+    if (ndp) {
+        // paired density plans are deleted above, not with ndp
+        if (dm_paired)
+            for (auto *v : {&ndp->fwd, &ndp->rev, &ndp->revp, &ndp->segfwd, &ndp->segrev}) v->clear();
+        dm_plan_delete(ndp);
+    }
+}
+
+int fused_plan_update_prepare(const Program *p, const Program *q, PlanUpdate *u) {
     MOSSQ_CHECK(p->retired.empty(), MOSSQ_EINVAL, "a captured graph still uses a plan the autotuner replaced");
     std::string err;
-    std::vector<std::pair<FusedPlan *, FusedPlan *>> pairs;  // (old, rebuilt)
-    DensityAdjointPlan *ndp = nullptr;
-    auto cleanup = [&](int code, const std::string &m) {
-        for (auto &pr : pairs) delete pr.second;
-        if (ndp) dm_plan_delete(ndp);
-        return fail(code, m);
-    };
     for (FusedPlan *old : {p->fused, p->fused_rev}) {
         if (!old) continue;
         std::vector<POp> ops;
@@ -3177,30 +3169,38 @@ int fused_plan_update(Program *p, const Program *q) {
         else
             lower_steps(q, ops);
         FusedPlan *nw = plan_with_tail(q, ops, old->min_tail, old->adjoint, old->kron, old->rb, &err, old);
-        if (!nw) return cleanup(MOSSQ_EINVAL, "rebuilding a plan failed: " + err);
-        pairs.push_back({old, nw});
-        if (!same_tables(old, nw)) return cleanup(MOSSQ_EINVAL, "the new values change the program's fused plan");
+        if (!nw) return fail(MOSSQ_EINVAL, "rebuilding a plan failed: " + err);
+        u->pairs.push_back({old, nw});
+        if (!same_tables(old, nw)) return fail(MOSSQ_EINVAL, "the new values change the program's fused plan");
     }
     if (p->dm_adj) {
-        ndp = dm_plan_build_host(q, &err);
-        if (!ndp) return cleanup(MOSSQ_EINVAL, "rebuilding the density adjoint plans failed: " + err);
-        const std::vector<FusedPlan *> a = dm_plans(p->dm_adj), b = dm_plans(ndp);
-        if (a.size() != b.size()) return cleanup(MOSSQ_EINVAL, "the new values change the density adjoint plans");
+        u->ndp = dm_plan_build_host(q, &err);
+        if (!u->ndp) return fail(MOSSQ_EINVAL, "rebuilding the density adjoint plans failed: " + err);
+        const std::vector<FusedPlan *> a = dm_plans(p->dm_adj), b = dm_plans(u->ndp);
+        if (a.size() != b.size()) return fail(MOSSQ_EINVAL, "the new values change the density adjoint plans");
         for (size_t i = 0; i < a.size(); ++i)
-            if (!same_tables(a[i], b[i])) return cleanup(MOSSQ_EINVAL, "the new values change the density adjoint plans");
-        for (size_t i = 0; i < a.size(); ++i) pairs.push_back({a[i], b[i]});
+            if (!same_tables(a[i], b[i])) return fail(MOSSQ_EINVAL, "the new values change the density adjoint plans");
+        for (size_t i = 0; i < a.size(); ++i) u->pairs.push_back({a[i], b[i]});
+        u->dm_paired = true;
     }
-    int st = MOSSQ_OK;
-    for (auto &pr : pairs)
-        if (st == MOSSQ_OK) st = upload_values(p->ctx, pr.first, pr.second);
-    if (st == MOSSQ_OK && hipStreamSynchronize(p->ctx->stream) != hipSuccess)
-        st = fail(MOSSQ_EDEVICE, "hipStreamSynchronize failed");
-    for (auto &pr : pairs) delete pr.second;
-    if (ndp) {
-        for (auto *v : {&ndp->fwd, &ndp->rev, &ndp->revp, &ndp->segfwd, &ndp->segrev}) v->clear();
-        dm_plan_delete(ndp);
+    return MOSSQ_OK;
+}
+
+void fused_plan_update_copies(const PlanUpdate &u, std::vector<DevCopy> &out) {
+    for (const auto &pr : u.pairs) {
+        const FusedPlan *old = pr.first, *nw = pr.second;
+        out.push_back({old->d_pool, nw->h_pool.data(), nw->h_pool.size() * sizeof(c128)});
+        out.push_back({old->d_diag, nw->h_diag.data(), nw->h_diag.size() * sizeof(TDiag)});
+        out.push_back({old->d_probes, nw->h_probes.data(), nw->h_probes.size() * sizeof(TProbe)});
     }
-    return st;
+}
+
+void fused_plan_update_commit(PlanUpdate *u) {
+    for (auto &pr : u->pairs) {
+        pr.first->h_pool.swap(pr.second->h_pool);
+        pr.first->h_diag.swap(pr.second->h_diag);
+        pr.first->h_probes.swap(pr.second->h_probes);
+    }
 }
 
 }  // namespace mossq

@@ -379,16 +379,17 @@ class BatchAdjointGraph:
         """New parameters for every program: items[i] is a circuit with
         program i's structure, or a pool for its current lowering (for
         instance `program.lowered.pool_with_slots(values)`).  Circuits are
-        lowered here; the library updates (host replanning and uploads,
-        mossq_program_update) run on `workers` host threads, as updates of
-        different programs are independent.  Raises if any program refuses;
-        the others keep their new values."""
-        from concurrent.futures import ThreadPoolExecutor
-
+        lowered here; one library call (mossq_program_update_many) replans
+        the programs on `workers` host threads (default: one per hardware
+        thread, at most 16) and uploads all new values at once.  Raises if any
+        program refuses; the others keep their new values."""
         if len(items) != len(self.programs):
             raise ValueError("one circuit or pool per program")
-        jobs = []
-        for p, it in zip(self.programs, items):
+        m = len(items)
+        lws, keep = [], []
+        progs, nops = (C.c_void_p * m)(), (C.c_int * m)()
+        recp, poolp, lens = (C.c_void_p * m)(), (C.c_void_p * m)(), (C.c_uint64 * m)()
+        for i, (p, it) in enumerate(zip(self.programs, items)):
             if isinstance(it, QCircuit):
                 lw = lower(it, p.n, p.is_density)
                 recs, pool = lw.arrays()
@@ -396,25 +397,19 @@ class BatchAdjointGraph:
                 lw = p.lowered
                 recs, _ = lw.arrays()
                 pool = np.ascontiguousarray(it, dtype=np.complex128)
-            jobs.append((p, lw, recs, pool))
-
-        def one(job):
-            p, lw, recs, pool = job
-            st = self.ctx.lib.mossq_program_update(self.ctx.h, p.h, len(lw.records), recs,
-                                                   pool.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)),
-                                                   pool.size)
-            msg = self.ctx.lib.mossq_last_error() if st else b""  # thread-local: read on this thread
-            return st, msg
-
-        workers = workers or min(16, len(jobs), os.cpu_count() or 1)
-        with ThreadPoolExecutor(max_workers=workers) as ex:
-            results = list(ex.map(one, jobs))
-        for (p, lw, _, _), (st, _) in zip(jobs, results):
-            if st == _lib.MOSSQ_OK:
+            lws.append(lw)
+            keep.append((recs, pool))  # alive through the call
+            progs[i], nops[i] = p.h, len(lw.records)
+            recp[i], poolp[i], lens[i] = C.addressof(recs), pool.ctypes.data, pool.size
+        workers = workers or min(16, os.cpu_count() or 1)
+        status = (C.c_int * m)()
+        st = self.ctx.lib.mossq_program_update_many(self.ctx.h, m, progs, nops, recp, poolp, lens, workers, status)
+        msg = self.ctx.lib.mossq_last_error() if st else b""
+        for p, lw, s in zip(self.programs, lws, status):
+            if s == _lib.MOSSQ_OK:
                 p.lowered = lw
-        for st, msg in results:
-            if st != _lib.MOSSQ_OK:
-                raise _lib.MossqError(st, msg.decode(errors="replace"))
+        if st != _lib.MOSSQ_OK:
+            raise _lib.MossqError(st, msg.decode(errors="replace"))
 
     def run(self):
         vals = np.zeros(len(self.programs))

@@ -208,3 +208,44 @@ def test_program_update_refusals(ctx):
     want2[0] = 1
     oracle.run_program(want2, lower(c2, n, False))
     assert np.max(np.abs(run() - want2)) < AMP
+
+
+def test_update_many_statuses(ctx):
+    """mossq_program_update_many on the fused programs of a captured batch
+    (9-15 qubits; the 9-qubit one also as its 12-qubit form): the refused
+    program (records of another structure) keeps its old values, every other
+    program takes its new ones, each reports its own status, and the replay
+    equals the eager engine on the resulting circuits."""
+    import ctypes as C
+    rng = np.random.default_rng(308)
+    sizes = [13, 9, 14, 12, 15]
+    c1 = _batch(rng, sizes)
+    c2 = _batch(rng, sizes)
+    c2[3] = hea_circuit(12, 3, _angles(rng, 12, 3))  # another structure: refused
+    obs = [xxz(n) for n in sizes]
+    graph = mq.batched_adjoint_graph(c1, obs, sizes, ctx, fuse=True)
+    graph.run()
+    progs, m = graph.programs, len(sizes)
+    keep = [lower(c, n, False).arrays() for c, n in zip(c2, sizes)]
+    ph = (C.c_void_p * m)(*[p.h for p in progs])
+    nops = (C.c_int * m)(*[len(lower(c, n, False).records) for c, n in zip(c2, sizes)])
+    rp = (C.c_void_p * m)(*[C.addressof(r) for r, _ in keep])
+    pp = (C.c_void_p * m)(*[pl.ctypes.data for _, pl in keep])
+    lens = (C.c_uint64 * m)(*[pl.size for _, pl in keep])
+    status = (C.c_int * m)()
+    st = ctx.lib.mossq_program_update_many(ctx.h, m, ph, nops, rp, pp, lens, 3, status)
+    assert st != 0 and list(status) == [0, 0, 0, st, 0]
+    c3 = [c1[i] if i == 3 else c2[i] for i in range(m)]
+    for i, (p, n) in enumerate(zip(progs, sizes)):
+        sv = mq.StateVector.zeros(n, ctx)
+        p.run(sv.storage)
+        want = np.zeros(1 << n, dtype=np.complex128)
+        want[0] = 1
+        oracle.run_program(want, lower(c3[i], n, False))
+        assert np.max(np.abs(sv.to_host() - want)) < AMP, i
+    _same(graph.run(), mq.batched_adjoint(c3, obs, sizes, ctx, fuse=True))
+    for i in (1, 3):
+        v0, g0 = oracle.adjoint(lower(c3[i], sizes[i], False), obs[i].groups())
+        v, g = mq.batched_adjoint([c3[i]], [obs[i]], [sizes[i]], ctx, fuse=True)
+        assert abs(v[0] - v0) < EXP and np.max(np.abs(g[0] - g0)) < GRAD, i
+    del graph
