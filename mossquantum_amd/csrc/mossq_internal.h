@@ -260,8 +260,8 @@ struct Allocs {
 struct MatInfo {
     bool monomial = false;  // exactly one nonzero per row and column
     int nchanged = 0;       // rows with a non-identity action (monomial only)
-    int rows[32];           // changed rows
-    int src[32];            // source column of each changed row
+    uint8_t rows[32];       // changed rows
+    uint8_t src[32];        // source column of each changed row
     c128 val[32];           // matrix entry of each changed row
 };
 MatInfo classify(const c128 *m, int dim);

@@ -1,9 +1,17 @@
 // Lowered circuits: launch plans over a device matrix pool.
 #pragma once
 
+#include <memory>
+
 #include "mossq_obs.h"
 
 namespace mossq {
+
+// Classifications of a split step's column-pass matrices (kept out of Step,
+// whose other instances do not need them).
+struct SplitInfo {
+    MatInfo fwd, rev;
+};
 
 // One unfused instruction, expressed on the bits of the stored vector
 // (state vector: the qubit bits; density matrix: the 2n bits of vec(rho)).
@@ -22,7 +30,7 @@ struct Step {
     // density gates wider than 2 qubits: applied as row pass then column pass
     bool split = false;
     int fwd_cols = -1, rev_cols = -1;
-    MatInfo ifwd_cols, irev_cols;
+    std::shared_ptr<const SplitInfo> cols;  // split steps: the column passes' classifications
 };
 
 struct FusedPlan;           // LDS-tile sweep plan (tile.hip)

@@ -82,7 +82,7 @@ static void dm_grad_matrix(const c128 *D, int k, std::vector<c128> &out) {
 static int apply_step_fwd(Ctx *c, const Program *p, const Step &s, c128 *v) {
     if (s.split) {
         MOSSQ_TRY(launch_dense(c, v, p->nbits, s.k, s.bits, p->d_pool + s.fwd, s.ifwd));
-        return launch_dense(c, v, p->nbits, s.k, s.bits + s.k, p->d_pool + s.fwd_cols, s.ifwd_cols);
+        return launch_dense(c, v, p->nbits, s.k, s.bits + s.k, p->d_pool + s.fwd_cols, s.cols->fwd);
     }
     return launch_dense(c, v, p->nbits, s.kb, s.bits, p->d_pool + s.fwd, s.ifwd);
 }
@@ -90,7 +90,7 @@ static int apply_step_fwd(Ctx *c, const Program *p, const Step &s, c128 *v) {
 static int apply_step_rev(Ctx *c, const Program *p, const Step &s, c128 *v) {
     if (s.split) {
         MOSSQ_TRY(launch_dense(c, v, p->nbits, s.k, s.bits, p->d_pool + s.rev, s.irev));
-        return launch_dense(c, v, p->nbits, s.k, s.bits + s.k, p->d_pool + s.rev_cols, s.irev_cols);
+        return launch_dense(c, v, p->nbits, s.k, s.bits + s.k, p->d_pool + s.rev_cols, s.cols->rev);
     }
     return launch_dense(c, v, p->nbits, s.kb, s.bits, p->d_pool + s.rev, s.irev);
 }
@@ -404,6 +404,7 @@ int program_build_host(Program *p, int n, int is_density, int nops, const mossq_
     p->is_density = is_density ? 1 : 0;
     p->nbits = is_density ? 2 * n : n;
     auto bad = [&](int code, const std::string &m) { return fail(code, m); };
+    p->steps.reserve(size_t(nops));
     for (int i = 0; i < nops; ++i) {
         const mossq_op &op = ops[i];
         const int k = op.k;
@@ -493,12 +494,14 @@ int program_build_host(Program *p, int n, int is_density, int nops, const mossq_
             }
             st.fwd = push(p->h_pool, U, size_t(d) * d);
             st.ifwd = classify(U, d);
+            auto cols = std::make_shared<SplitInfo>();
             st.fwd_cols = push(p->h_pool, Uc.data(), Uc.size());
-            st.ifwd_cols = classify(Uc.data(), d);
+            cols->fwd = classify(Uc.data(), d);
             st.rev = push(p->h_pool, Ud.data(), Ud.size());
             st.irev = classify(Ud.data(), d);
             st.rev_cols = push(p->h_pool, Udc.data(), Udc.size());
-            st.irev_cols = classify(Udc.data(), d);
+            cols->rev = classify(Udc.data(), d);
+            st.cols = cols;
         }
         p->steps.push_back(st);
     }
@@ -565,8 +568,10 @@ static bool same_steps(const Program &a, const Program &b, std::vector<char> &mo
             x.fwd != y.fwd || x.rev != y.rev || x.grad != y.grad || x.grad_index != y.grad_index ||
             x.u_src != y.u_src || x.split != y.split || x.fwd_cols != y.fwd_cols || x.rev_cols != y.rev_cols)
             return false;
-        if (!same_info(x.ifwd, y.ifwd) || !same_info(x.irev, y.irev) || !same_info(x.ifwd_cols, y.ifwd_cols) ||
-            !same_info(x.irev_cols, y.irev_cols))
+        if (!same_info(x.ifwd, y.ifwd) || !same_info(x.irev, y.irev))
+            return false;
+        if (x.split && (!x.cols || !y.cols || !same_info(x.cols->fwd, y.cols->fwd) ||
+                        !same_info(x.cols->rev, y.cols->rev)))
             return false;
         moved[i] = m ? 1 : 0;
     }
