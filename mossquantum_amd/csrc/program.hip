@@ -730,6 +730,9 @@ int mossq_program_update_many(mossq_ctx *ctx, int m, mossq_program *const *progs
     const u64 total = data + nseg * sizeof(ScatterSeg);
     if (nseg) {
         unsigned char *host = nullptr, *dev = nullptr;
+        // an earlier call that returned on an error may have left its copy
+        // from the pinned staging in flight
+        MOSSQ_HIP(hipStreamSynchronize(c->stream));
         MOSSQ_TRY(get_update_pinned(c, total, &host));
         Allocs al(c);
         MOSSQ_TRY(al.get(total, (void **)&dev));
