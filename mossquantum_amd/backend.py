@@ -252,6 +252,25 @@ class ProgramGraph:
             pass
 
 
+def update_many(programs, pools, ctx, threads: int = 0):
+    """mossq_program_update_many: program i takes pools[i] for the records of
+    its current lowering, all in one library call (host replanning on
+    `threads` threads, 0 = one per hardware thread; one upload).  Returns each
+    program's status (0: rebound; otherwise it kept its old values)."""
+    m = len(programs)
+    if m == 0:
+        return []
+    keep = [(p.lowered.arrays()[0], np.ascontiguousarray(pl, dtype=np.complex128)) for p, pl in zip(programs, pools)]
+    ph = (C.c_void_p * m)(*[p.h.value for p in programs])
+    nops = (C.c_int * m)(*[len(p.lowered.records) for p in programs])
+    rp = (C.c_void_p * m)(*[C.addressof(r) for r, _ in keep])
+    pp = (C.c_void_p * m)(*[pl.ctypes.data for _, pl in keep])
+    lens = (C.c_uint64 * m)(*[pl.size for _, pl in keep])
+    status = (C.c_int * m)()
+    ctx.lib.mossq_program_update_many(ctx.h, m, ph, nops, rp, pp, lens, int(threads), status)
+    return list(status)
+
+
 def plan_emulate(circuit: QCircuit, n: int, is_density: bool, state: np.ndarray):
     """Run the fused plan of `circuit` on a host vector with the tile kernel's
     index arithmetic (no device).  state None: plan statistics only.

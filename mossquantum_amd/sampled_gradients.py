@@ -19,7 +19,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check
-from .backend import DeviceObservable, Program, expectation
+from .backend import DeviceObservable, Program, expectation, update_many
 from .circuit import lower
 from .gates import CPHASEGate, CRxGate, CRyGate, CRzGate, PHASEGate, RxGate, RyGate, RzGate
 from .gradients import AbstractGradientMethod, BatchExpectationObjective, ExpectationObjective, UnsupportedGradient
@@ -159,15 +159,41 @@ class _Evaluator:
             out[j] = float(np.dot(self.weights, vals))
         return out
 
+    def _programs(self, pools):
+        """Unfused programs holding `pools` (this lowering's records; fused
+        ones measured slower here, their replanning costing more than their
+        sweeps save).  The objective keeps them between calls: programs from an
+        earlier call with the same records are rebound in one library call
+        (update_many), and only the missing or refused ones are created."""
+        fuse = False
+
+        def make(pool):
+            return Program(None, self.n, False, self.ctx, fuse, lowered=self.lw, pool=pool)
+
+        recs, _ = self.lw.arrays()
+        key = (self.n, bytes(recs), id(self.ctx), fuse)  # (the kept programs hold the context)
+        kept = getattr(self.obj, "_shift_programs", None)
+        progs = list(kept[1]) if kept is not None and kept[0] == key else []
+        reuse = progs[:len(pools)]
+        status = update_many(reuse, pools[:len(reuse)], self.ctx)
+        for i, st in enumerate(status):
+            if st == 0:
+                reuse[i].lowered = self.lw
+            else:  # refused (the new values change a step's kind): a program of its own
+                reuse[i] = make(pools[i])
+        out = reuse + [make(p) for p in pools[len(reuse):]]
+        self.obj._shift_programs = (key, out if len(out) >= len(progs) else out + progs[len(out):])
+        return out
+
     def _engine_values(self, pools):
         nc = len(self.columns)
         per_item = (1 << self.n) * 16
         chunk = max(1, _ENGINE_BYTES // (per_item * nc))
         dob = DeviceObservable(self.obj.observable, self.n, self.ctx)
         out = np.zeros(len(pools))
+        every = self._programs(pools)
         for lo in range(0, len(pools), chunk):
-            progs = [Program(None, self.n, False, self.ctx, False, lowered=self.lw, pool=p)
-                     for p in pools[lo:lo + chunk]]
+            progs = every[lo:lo + chunk]
             items = [(p, col) for p in progs for col in self.columns]
             B = len(items)
             ph = (C.c_void_p * B)(*[p.h.value for p, _ in items])

@@ -38,6 +38,30 @@ def test_parameter_shift_matches_adjoint(ctx, n):
     assert np.array_equal(g, est.gradient)
 
 
+def test_parameter_shift_keeps_programs(ctx):
+    """Repeated parameter-shift gradients of one objective rebind the shifted
+    programs kept from the previous call (one mossq_program_update_many); angles
+    at pi/2, whose shifts turn a rotation into a permutation, make some of them
+    refuse and be rebuilt.  Every call equals the adjoint at its angles, and a
+    call at earlier angles repeats that call's gradient bit for bit."""
+    n = 9
+    tpl = _mixed(n, 21)
+    rng = np.random.default_rng(21)
+    th1 = rng.uniform(0, 2 * np.pi, tpl.nparameters())
+    th2 = th1 + 0.3
+    th2[[0, 3, 5]] = np.pi / 2
+    obj = mq.ExpectationObjective(tpl, xxz(n), mq.StateVector.zeros(n))
+    first = mq.estimate_gradient(mq.ParameterShift(), obj, th1)
+    kept = list(obj._shift_programs[1])
+    for th in (th2, th1):
+        est = mq.estimate_gradient(mq.ParameterShift(), obj, th)
+        v_adj, g_adj = mq.value_and_gradient(mq.AdjointGradient(), obj, th)
+        assert abs(est.value - v_adj) < EXP and np.max(np.abs(est.gradient - g_adj)) < GRAD
+    assert np.array_equal(est.gradient, first.gradient)
+    now = obj._shift_programs[1]
+    assert len(now) == len(kept) and sum(a is b for a, b in zip(now, kept)) >= len(kept) - 12
+
+
 def test_parameter_shift_per_program_paths(ctx):
     """A three-qubit gate (per-evaluation programs) and a noisy density matrix."""
     n = 6
